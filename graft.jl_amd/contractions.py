@@ -1,0 +1,575 @@
+"""Environments and effective Hamiltonians on the device — the reference's Contractions.
+
+Restates lib/GraftContractions/src/Contractions/envcache.jl (EnvCache,
+env!, _build_env_spec, invalidation) and effective.jl (_h1_spec, _h2_spec,
+_h0_spec, eff_h1 / eff_h2 / eff_h0, two_site_tensor, split_two_site!) with
+0-based indices.  The label bookkeeping is the reference's; execution is a
+compiled plan in libgraft_heff.so.  Pivotal twists (envcache.jl:883-905) are
+the identity for trivial and bosonic abelian sectors, which is every space
+this engine handles, so none are applied.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from . import backend as B
+from .planning import ContractionSpec, plan_contraction, slot_dims
+from .tensors import BlockStructure, DTensor, TensorProto
+from .trees import childslot, nchildren, neighbors, nnodes, subtree_nodes
+
+
+# ---------------------------------------------------------------------------
+# Native plans
+# ---------------------------------------------------------------------------
+
+
+class NativePlan:
+    """A ContractionPlan compiled into libgraft_heff.so for fixed operand structures."""
+
+    def __init__(self, plan, operands, out_structure=None, out_dims=None):
+        self.plan = plan
+        self.ninputs = plan.ninputs
+        ctx = nat.default_context()
+        descs = [op.desc() for op in operands]
+        arr = (nat.TensorDescC * len(descs))(*[d.c for d in descs])
+        steps = (nat.PairStepC * len(plan.steps))(*[nat.make_step(s) for s in plan.steps])
+        self._outdesc = out_structure.desc() if out_structure is not None else None
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().graft_plan_create(
+            ctx.handle, steps, len(plan.steps), plan.nslots, plan.output_slot, arr, len(descs),
+            ctypes.byref(self._outdesc.c) if self._outdesc is not None else None, ctypes.byref(h)))
+        self.handle = h
+        ws = ctypes.c_uint64()
+        macs = ctypes.c_double()
+        outn = ctypes.c_uint64()
+        nat.check(nat.lib().graft_plan_info(h, ctypes.byref(ws), ctypes.byref(macs), ctypes.byref(outn)))
+        self.workspace_bytes = ws.value
+        self.macs = macs.value          # complex multiply-adds over the executed blocks
+        self.output_elements = outn.value
+        self.out_dims = out_dims
+        self.out_structure = out_structure
+
+    def step_info(self):
+        out = []
+        for i in range(len(self.plan.steps)):
+            mkn = (ctypes.c_int64 * 3)()
+            nb = ctypes.c_int64()
+            macs = ctypes.c_double()
+            cfg = ctypes.c_int32()
+            nat.check(nat.lib().graft_plan_step_info(self.handle, i, mkn, ctypes.byref(nb),
+                                                     ctypes.byref(macs), ctypes.byref(cfg)))
+            out.append(dict(m=mkn[0], k=mkn[1], n=mkn[2], blocks=nb.value, macs=macs.value,
+                            config=cfg.value))
+        return out
+
+    def execute(self, operands, out, precision=nat.PREC_FP64):
+        ctx = nat.default_context()
+        ins = nat.ptr_array([op.ptr for op in operands])
+        nat.check(nat.lib().graft_plan_execute(ctx.handle, self.handle, ins,
+                                               ctypes.c_void_p(out.ptr), int(precision)))
+        return out
+
+    def execute_timed(self, operands, out, precision=nat.PREC_FP64):
+        """Execute with HIP events around every step; returns per-step milliseconds."""
+        ctx = nat.default_context()
+        ins = nat.ptr_array([op.ptr for op in operands])
+        ms = (ctypes.c_float * len(self.plan.steps))()
+        nat.check(nat.lib().graft_plan_execute_timed(ctx.handle, self.handle, ins,
+                                                     ctypes.c_void_p(out.ptr), int(precision), ms))
+        return list(ms)
+
+    def autotune(self, operands, out, precision=nat.PREC_FP64, repeats=2):
+        ctx = nat.default_context()
+        ins = nat.ptr_array([op.ptr for op in operands])
+        nat.check(nat.lib().graft_plan_autotune(ctx.handle, self.handle, ins,
+                                                ctypes.c_void_p(out.ptr), int(precision), repeats))
+
+    def set_tile_config(self, step, config):
+        nat.check(nat.lib().graft_plan_set_tile_config(self.handle, step, config))
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and nat._lib is not None:
+            nat.lib().graft_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _legs_of(t):
+    return None if t.blocks is None else t.blocks.legs
+
+
+# ---------------------------------------------------------------------------
+# EnvCache — envcache.jl:61-109, 812-844
+# ---------------------------------------------------------------------------
+
+
+class EnvCache:
+    """Directed-edge environment cache with explicit invalidation events."""
+
+    def __init__(self, topo):
+        self.topo = topo
+        self.envs = {}
+        self.node_generations = [0] * nnodes(topo)
+        self.plans = {}
+        self.rootcaps = {}
+        self._sides = {}
+        self.env_builds = 0
+        self.plan_hits = 0
+        self.plan_misses = 0
+
+    def _side(self, u, v):
+        s = self._sides.get((u, v))
+        if s is None:
+            s = frozenset(subtree_nodes(self.topo, u, v))
+            self._sides[(u, v)] = s
+        return s
+
+    def invalidate_node_(self, n):
+        """invalidate_node! — envcache.jl:812-822."""
+        self.node_generations[n] += 1
+        for key in [k for k in self.envs if n in self._side(*k)]:
+            del self.envs[key]
+        return self
+
+    def invalidate_edge_(self, n, m):
+        """invalidate_edge! — envcache.jl:830-844."""
+        self.node_generations[n] += 1
+        if n != m:
+            self.node_generations[m] += 1
+        for key in [k for k in self.envs if n in self._side(*k) or m in self._side(*k)]:
+            del self.envs[key]
+        return self
+
+    def get_plan(self, kind, spec, operands, optimize=True, memory_weight=1.0, sector_aware=True,
+                 memory_cap_bytes=None, out_structure=None):
+        """_cache_get_or_plan! — shape-only plan cache keyed by spec and operand structures."""
+        okey = out_structure.key() if out_structure is not None else None
+        key = (kind, spec.key(), tuple(op.structure_key() for op in operands), optimize,
+               memory_weight, sector_aware, memory_cap_bytes, okey)
+        p = self.plans.get(key)
+        if p is not None:
+            self.plan_hits += 1
+            return p
+        self.plan_misses += 1
+        dims = [list(op.dims) for op in operands]
+        legs = None
+        if any(op.blocks is not None for op in operands):
+            legs = [_legs_of(op) for op in operands]
+        plan = plan_contraction(spec, dims, legs, optimize=optimize, memory_weight=memory_weight,
+                                sector_aware=sector_aware, memory_cap_bytes=memory_cap_bytes)
+        out_dims = slot_dims(plan, dims)[plan.output_slot]
+        p = NativePlan(plan, operands, out_structure, out_dims)
+        self.plans[key] = p
+        return p
+
+    def root_cap(self, dims):
+        """_root_cap! — envcache.jl:626-655 (ones tensor closing unit legs)."""
+        key = tuple(dims)
+        c = self.rootcaps.get(key)
+        if c is None:
+            c = B.ones_tensor(dims)
+            self.rootcaps[key] = c
+        return c
+
+
+def _stateleg(t, hp, u, w):
+    """envcache.jl:852-853 (0-based)."""
+    return nchildren(t, u) + (1 if hp else 0) if t.parent[u] == w else childslot(t, u, w)
+
+
+def _opleg(t, hp, u, w):
+    """envcache.jl:855-857 (0-based)."""
+    return nchildren(t, u) + (2 if hp else 0) if t.parent[u] == w else childslot(t, u, w)
+
+
+def _build_env_spec(cache, ket, O, bra, u, v):
+    """envcache.jl:1010-1101: operands (ket, envs..., cap?, W, bra) with conj(bra)."""
+    t = ket.topo
+    hp = ket.hasphys[u]
+    A = ket.tensors[u]
+    Bt = bra.tensors[u]
+    W = O.tensors[u]
+    if O.hasphys[u] != hp:
+        raise ValueError(f"TTNO/TTNS physical-leg mismatch at node {t.ids[u]}")
+    aidx = [0] * A.rank
+    bidx = [0] * Bt.rank
+    widx = [0] * W.rank
+    operands = [A]
+    labels = [aidx]
+    conjs = [False]
+    envslots, caps = [], []
+    nxt = [0]
+
+    def fresh():
+        nxt[0] += 1
+        return nxt[0]
+
+    if hp:
+        K = nchildren(t, u)
+        pin, pout = fresh(), fresh()
+        aidx[K] = pin
+        widx[K + 1] = pin
+        widx[K] = pout
+        bidx[K] = pout
+    for w in neighbors(t, u):
+        la = _stateleg(t, hp, u, w)
+        if w == v:
+            aidx[la] = -1
+            widx[_opleg(t, hp, u, w)] = -2
+            bidx[la] = -3
+        else:
+            E = cache.envs[(w, u)]
+            e = [fresh(), fresh(), fresh()]
+            aidx[la] = e[0]
+            widx[_opleg(t, hp, u, w)] = e[1]
+            bidx[la] = e[2]
+            operands.append(E)
+            labels.append(e)
+            conjs.append(False)
+            envslots.append(len(labels) - 1)
+    if t.parent[u] < 0:
+        ka, ko, kb = fresh(), fresh(), fresh()
+        aidx[-1] = ka
+        bidx[-1] = kb
+        widx[-1] = ko
+        operands.append(cache.root_cap((Bt.dims[-1], W.dims[-1], A.dims[-1])))
+        labels.append([kb, ko, ka])
+        conjs.append(False)
+        caps.append(len(labels) - 1)
+    operands.append(W)
+    labels.append(widx)
+    conjs.append(False)
+    wslot = len(labels) - 1
+    operands.append(Bt)
+    labels.append(bidx)
+    conjs.append(True)
+    braslot = len(labels) - 1
+    preferred = [0] + envslots + [wslot] + caps + [braslot]
+    nopen = 0 if v is None else 3
+    spec = ContractionSpec(labels, conjs, nopen, (nopen, 0), None, preferred)
+    return spec, operands
+
+
+def env_(cache, ket, O, u, v, bra=None):
+    """env!(cache, ket, O, bra, u, v) — envcache.jl:1440-1476; returns env(u -> v)."""
+    bra = ket if bra is None else bra
+    key = (u, v)
+    E = cache.envs.get(key)
+    if E is not None:
+        return E
+    for w in neighbors(ket.topo, u):
+        if w != v:
+            env_(cache, ket, O, w, u, bra)
+    spec, operands = _build_env_spec(cache, ket, O, bra, u, v)
+    plan = cache.get_plan("env_ket_op_bra", spec, operands)
+    out = DTensor.empty(plan.out_dims, 3)
+    plan.execute(operands, out)
+    cache.envs[key] = out
+    cache.env_builds += 1
+    return out
+
+
+def expect(psi, H, cache=None):
+    """<psi|H|psi> closed at the orthogonality centre — expectation.jl:32-41."""
+    cache = EnvCache(psi.topo) if cache is None else cache
+    n = psi.center
+    for w in neighbors(psi.topo, n):
+        env_(cache, psi, H, w, n)
+    spec, operands = _build_env_spec(cache, psi, H, psi, n, None)
+    plan = cache.get_plan("env_closure", spec, operands)
+    out = DTensor.empty((), 0)
+    plan.execute(operands, out)
+    return complex(out.data.cpu().numpy()[0])
+
+
+# ---------------------------------------------------------------------------
+# Effective maps — effective.jl:627-1034, types.jl:402-445
+# ---------------------------------------------------------------------------
+
+
+class EffectiveMap:
+    """Callable x -> y = H_eff x (the reference's EffectiveMap / WorkspaceMap).
+
+    x is never modified; y is a fresh device tensor (executor.jl:122-124)."""
+
+    def __init__(self, native, statics, xproto, kind, precision="fp64"):
+        self.native = native
+        self.statics = list(statics)
+        self.xdims = xproto.dims
+        self.ncod = xproto.ncod
+        self.blocks = xproto.blocks
+        self.kind = kind
+        self.precision = precision
+        self.napplies = 0
+
+    @property
+    def plan(self):
+        return self.native.plan
+
+    @property
+    def flops(self):
+        """Real floating-point operations per apply: 8 per complex multiply-add."""
+        return 8.0 * self.native.macs
+
+    def __call__(self, x, out=None):
+        if x.dims != self.xdims:
+            raise ValueError(f"effective map expects dims {self.xdims}, got {x.dims}")
+        y = DTensor.empty(self.xdims, self.ncod, self.blocks) if out is None else out
+        ctx = nat.default_context()
+        st = nat.ptr_array([s.ptr for s in self.statics])
+        nat.check(nat.lib().graft_heff_apply(ctx.handle, self.native.handle, ctypes.c_void_p(x.ptr), st,
+                                             ctypes.c_void_p(y.ptr),
+                                             nat.PRECISIONS[self.precision]))
+        self.napplies += 1
+        return y
+
+    def operands(self, x):
+        return [x] + self.statics
+
+    def autotune(self, x, repeats=2):
+        y = DTensor.empty(self.xdims, self.ncod, self.blocks)
+        self.native.autotune(self.operands(x), y, nat.PRECISIONS[self.precision], repeats)
+        return y
+
+    def with_precision(self, precision):
+        """The same map with another product arithmetic (fp64, fp32, bf16x3)."""
+        return EffectiveMap(self.native, self.statics, TensorProto(self.xdims, self.ncod, self.blocks),
+                            self.kind, precision)
+
+
+def with_workspace_map(effective, f):
+    """with_workspace_map (effective.jl:300-316): the device map already owns its workspace."""
+    return f(effective)
+
+
+def _h1_spec(cache, psi, H, n):
+    """effective.jl:627-686."""
+    t = psi.topo
+    x = psi.tensors[n]
+    hp = psi.hasphys[n]
+    W = H.tensors[n]
+    envlist = [(w, env_(cache, psi, H, w, n)) for w in neighbors(t, n)]
+    isroot = t.parent[n] < 0
+    K = nchildren(t, n)
+    Nx = x.rank
+    xidx = [0] * Nx
+    widx = [0] * W.rank
+    labels = [xidx, widx]
+    conjs = [False, False]
+    statics = [W]
+    children, parents, caps = [], [], []
+    nxt = [0]
+
+    def fresh():
+        nxt[0] += 1
+        return nxt[0]
+
+    if hp:
+        pin = fresh()
+        xidx[K] = pin
+        widx[K + 1] = pin
+        widx[K] = -(K + 1)
+    for w, E in envlist:
+        lx = _stateleg(t, hp, n, w)
+        kk, oo = fresh(), fresh()
+        xidx[lx] = kk
+        widx[_opleg(t, hp, n, w)] = oo
+        labels.append([kk, oo, -(lx + 1)])
+        conjs.append(False)
+        statics.append(E)
+        (parents if t.parent[n] == w else children).append(len(labels) - 1)
+    if isroot:
+        ka, ko = fresh(), fresh()
+        xidx[-1] = ka
+        widx[-1] = ko
+        labels.append([ka, ko, -Nx])
+        conjs.append(False)
+        statics.append(cache.root_cap((x.dims[-1], W.dims[-1], x.dims[-1])))
+        caps.append(len(labels) - 1)
+    preferred = children + [1] + parents + caps
+    spec = ContractionSpec(labels, conjs, Nx, (Nx - 1, 1), 0, preferred)
+    return spec, statics
+
+
+def eff_h1(cache, psi, H, n, optimize=True, memory_weight=1.0, sector_aware=True,
+           memory_cap_bytes=None, precision="fp64"):
+    """One-site effective Hamiltonian at node n — effective.jl:709-730."""
+    spec, statics = _h1_spec(cache, psi, H, n)
+    x = psi.tensors[n]
+    native = cache.get_plan("h1", spec, [x] + statics, optimize, memory_weight, sector_aware,
+                            memory_cap_bytes, out_structure=x.blocks)
+    return EffectiveMap(native, statics, x, "h1", precision)
+
+
+def _h0_spec(cache, psi, H, n, m):
+    """effective.jl:739-749."""
+    if psi.topo.parent[n] != m:
+        raise ValueError("eff_h0: m must be the parent of n")
+    En = env_(cache, psi, H, n, m)
+    Em = env_(cache, psi, H, m, n)
+    spec = ContractionSpec([[1, 2], [1, 3, -1], [2, 3, -2]], [False] * 3, 2, (1, 1), 0, [1, 2])
+    return spec, [En, Em]
+
+
+def eff_h0(cache, psi, H, n, m, optimize=True, memory_weight=1.0, sector_aware=True,
+           memory_cap_bytes=None, precision="fp64"):
+    """Zero-site (link) effective Hamiltonian on edge (n child, m parent) — effective.jl:762-797."""
+    spec, statics = _h0_spec(cache, psi, H, n, m)
+    if psi.center not in (n, m):
+        raise ValueError("eff_h0: orthogonality center must lie on the active edge")
+    En, Em = statics
+    proto = TensorProto((En.dims[0], Em.dims[0]), 1, None)
+    native = cache.get_plan("h0", spec, [proto] + statics, optimize, memory_weight, sector_aware,
+                            memory_cap_bytes)
+    return EffectiveMap(native, statics, proto, "h0", precision)
+
+
+def two_site_dims(psi, n, m):
+    t = psi.topo
+    k = childslot(t, m, n)
+    A, Bm = psi.tensors[n], psi.tensors[m]
+    return tuple(A.dims[:A.ncod]) + tuple(d for j, d in enumerate(Bm.dims) if j != k)
+
+
+def two_site_tensor(psi, n, m):
+    """Theta = A_n contracted with A_m over their edge — effective.jl:827-843."""
+    t = psi.topo
+    if t.parent[n] != m:
+        raise ValueError("two_site_tensor: m must be the parent of n")
+    A, Bm = psi.tensors[n], psi.tensors[m]
+    k = childslot(t, m, n)
+    pn = A.ncod
+    bopen = tuple(j for j in range(Bm.rank) if j != k)
+    nopen = pn + len(bopen)
+    return B.contract_pair(A, tuple(range(pn)), (pn,), False, Bm, (k,), bopen, False,
+                           (tuple(range(nopen)), ()))
+
+
+def split_two_site_(psi, theta, n, m, trunc=B.NO_TRUNCATION, center_on="n"):
+    """Truncated SVD of Theta back into psi[n], psi[m] — effective.jl:852-876.
+
+    Returns (psi, info) with the discarded norm and the full singular spectrum."""
+    t = psi.topo
+    k = childslot(t, m, n)
+    pn = psi.tensors[n].ncod
+    Km = psi.tensors[m].ncod
+    M = B.matrix_view(theta, pn)
+    U, S, Vh, disc, spectrum = B.split_svd(M, trunc)
+    r = U.shape[1]
+    Sc = S.to(U.dtype)
+    if center_on == "n":
+        An = U * Sc[None, :]
+        Rm = Vh
+    else:
+        An = U
+        Rm = Sc[:, None] * Vh
+    ndims = theta.dims[:pn] + (r,)
+    mdims = (r,) + theta.dims[pn:]
+    An_t = B.from_matrix(An, ndims, pn)
+    Rm_t = B.from_matrix(Rm, mdims, 1)
+    # Rm legs: (Y, m legs except slot k in order); move Y to slot k.
+    p1 = tuple(0 if j == k else 1 + (j if j < k else j - 1) for j in range(Km))
+    Am = B.permute(Rm_t, p1 + (Rm_t.rank - 1,), Km)
+    psi.tensors[n] = An_t
+    psi.tensors[m] = Am
+    psi.center = n if center_on == "n" else m
+    return psi, dict(discarded=disc, spectrum=spectrum, kept=r)
+
+
+def _h2_spec(cache, psi, H, n, m):
+    """effective.jl:878-976."""
+    t = psi.topo
+    if t.parent[n] != m:
+        raise ValueError("eff_h2: m must be the parent of n")
+    hpn, hpm = psi.hasphys[n], psi.hasphys[m]
+    Wn, Wm = H.tensors[n], H.tensors[m]
+    k = childslot(t, m, n)
+    Kn, Km = nchildren(t, n), nchildren(t, m)
+    envn = [(w, env_(cache, psi, H, w, n)) for w in neighbors(t, n) if w != m]
+    envm = [(w, env_(cache, psi, H, w, m)) for w in neighbors(t, m) if w != n]
+    isroot = t.parent[m] < 0
+    pn = Kn + (1 if hpn else 0)
+    xdims = two_site_dims(psi, n, m)
+    Nx = len(xdims)
+    xidx = [0] * Nx
+    wnidx = [0] * Wn.rank
+    wmidx = [0] * Wm.rank
+    labels = [xidx, wnidx, wmidx]
+    conjs = [False, False, False]
+    statics = [Wn, Wm]
+    envn_slots, envm_children, envm_parents, caps = [], [], [], []
+    nxt = [0]
+
+    def fresh():
+        nxt[0] += 1
+        return nxt[0]
+
+    def mpos(j):
+        return pn + (j if j < k else j - 1)
+
+    ob = fresh()
+    wnidx[-1] = ob
+    wmidx[k] = ob
+    if hpn:
+        pin = fresh()
+        xidx[Kn] = pin
+        wnidx[Kn + 1] = pin
+        wnidx[Kn] = -(Kn + 1)
+    if hpm:
+        pin = fresh()
+        xidx[mpos(Km)] = pin
+        wmidx[Km + 1] = pin
+        wmidx[Km] = -(mpos(Km) + 1)
+    for w, E in envn:
+        lx = childslot(t, n, w)
+        kk, oo = fresh(), fresh()
+        xidx[lx] = kk
+        wnidx[lx] = oo
+        labels.append([kk, oo, -(lx + 1)])
+        conjs.append(False)
+        statics.append(E)
+        envn_slots.append(len(labels) - 1)
+    for w, E in envm:
+        if t.parent[m] == w:
+            lw, lx = Wm.rank - 1, Nx - 1
+        else:
+            lw = childslot(t, m, w)
+            lx = mpos(lw)
+        kk, oo = fresh(), fresh()
+        xidx[lx] = kk
+        wmidx[lw] = oo
+        labels.append([kk, oo, -(lx + 1)])
+        conjs.append(False)
+        statics.append(E)
+        (envm_parents if t.parent[m] == w else envm_children).append(len(labels) - 1)
+    if isroot:
+        ka, ko = fresh(), fresh()
+        xidx[Nx - 1] = ka
+        wmidx[-1] = ko
+        labels.append([ka, ko, -Nx])
+        conjs.append(False)
+        statics.append(cache.root_cap((xdims[-1], Wm.dims[-1], xdims[-1])))
+        caps.append(len(labels) - 1)
+    preferred = envn_slots + [1] + envm_children + [2] + envm_parents + caps
+    spec = ContractionSpec(labels, conjs, Nx, (Nx, 0), 0, preferred)
+    return spec, statics, xdims
+
+
+def eff_h2(cache, psi, H, n, m, optimize=True, memory_weight=1.0, sector_aware=True,
+           memory_cap_bytes=None, precision="fp64", out_structure=None):
+    """Two-site effective Hamiltonian on bond (n child, m parent) — effective.jl:986-1034."""
+    spec, statics, xdims = _h2_spec(cache, psi, H, n, m)
+    proto = TensorProto(xdims, len(xdims), out_structure)
+    native = cache.get_plan("h2", spec, [proto] + statics, optimize, memory_weight, sector_aware,
+                            memory_cap_bytes, out_structure=out_structure)
+    return EffectiveMap(native, statics, proto, "h2", precision)

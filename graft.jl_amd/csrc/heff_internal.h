@@ -1,0 +1,74 @@
+// Internal structures shared by the runtime (heff_runtime.hip) and the
+// kernels (zgemm_kernels.hip, vector_kernels.hip).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/graft_heff.h"
+
+namespace graft {
+
+// Maximum number of merged legs in one GEMM index group (m, n or k).
+constexpr int GL = 8;
+
+// One contribution to an output block: C_blk += op(A_blk) * op(B_blk),
+// contracted over K = prod(k dims).  Offsets are in complex elements.
+struct GemmSeg {
+  int64_t a_off;            // A block start inside the A buffer
+  int64_t b_off;            // B block start inside the B buffer
+  int32_t K;                // contracted extent
+  int32_t koff_idx;         // koff[koff_idx + q] = A offset, koff[koff_idx + K + q] = B offset
+  int32_t a_kfast;          // A's unit-stride leg is contracted (load along k)
+  int32_t b_kfast;          // B's unit-stride leg is contracted (load along k)
+  uint32_t a_mstr[GL];      // A element strides of the item's m legs
+  uint32_t b_nstr[GL];      // B element strides of the item's n legs
+};
+
+// One output block and its tile grid.  m legs come from A, n legs from B.
+struct GemmItem {
+  int64_t c_off;
+  int32_t M, N;
+  int32_t mrank, nrank;
+  int32_t mdims[GL];
+  int32_t ndims[GL];
+  uint32_t c_mstr[GL];
+  uint32_t c_nstr[GL];
+  int32_t seg_begin, seg_count;
+  int32_t tile_begin, tiles_n;
+};
+
+// Tile configurations of the complex-FP64 MFMA GEMM.
+struct TileConfig {
+  int bm, bn, bk, wm, wn;
+};
+constexpr int kNumTileConfigs = 7;
+extern const TileConfig kTileConfigs[kNumTileConfigs];
+
+// Launch one grouped strided complex GEMM (precision GRAFT_PREC_*).
+hipError_t launch_zgemm(int config, int precision, const void* A, const void* B, void* C,
+                        const GemmItem* items, int nitems, const GemmSeg* segs,
+                        const uint32_t* koff, int total_tiles, int conjA, int conjB,
+                        hipStream_t stream);
+int tiles_for(int config, int M, int N);
+
+// Krylov vector kernels (vector_kernels.hip).
+hipError_t launch_dotc_partials(int64_t n, const void* x, const void* y, double* partials,
+                                int nblocks, hipStream_t stream);
+hipError_t launch_reduce_partials(const double* partials, int nblocks, int k, double* out,
+                                  hipStream_t stream);
+hipError_t launch_multi_dotc_partials(int64_t n, int k, const void* const* V, const void* y,
+                                      double* partials, int nblocks, hipStream_t stream);
+hipError_t launch_axpby(int64_t n, double ar, double ai, const void* x, double br, double bi,
+                        void* y, hipStream_t stream);
+hipError_t launch_multi_axpy(int64_t n, int k, const void* const* V, const double* c, void* y,
+                             hipStream_t stream);
+constexpr int kReduceBlocks = 1024;
+constexpr int kMaxMultiVec = 64;
+
+void set_error(const std::string& msg);
+
+}  // namespace graft

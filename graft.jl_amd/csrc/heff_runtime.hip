@@ -1,0 +1,1021 @@
+// Runtime of the effective-Hamiltonian engine: contexts, the device block
+// store, plan compilation (pairwise steps -> grouped strided GEMM launches
+// with a liveness-coloured workspace arena), execution, the tile autotuner
+// and the Krylov vector entry points.  Everything exported here is declared
+// in include/graft_heff.h.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "heff_internal.h"
+
+namespace graft {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+#define GRAFT_CHECK_HIP(expr)                                                          \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess) {                                                            \
+      set_error(std::string("HIP error in ") + #expr + ": " + hipGetErrorString(_e));  \
+      return GRAFT_E_HIP;                                                              \
+    }                                                                                  \
+  } while (0)
+
+#define GRAFT_REQUIRE(cond, msg)  \
+  do {                            \
+    if (!(cond)) {                \
+      set_error(msg);             \
+      return GRAFT_E_INVALID;     \
+    }                             \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Tensor structure: legs with sector sizes, blocks with sector indices,
+// offsets, dims and strides (elements).  Dense = one sector per leg, one block.
+// ---------------------------------------------------------------------------
+struct Block {
+  std::vector<int32_t> sec;
+  std::vector<int64_t> dims;
+  std::vector<int64_t> strides;
+  int64_t offset = 0;
+  int64_t size = 1;
+};
+
+struct TStruct {
+  int rank = 0;
+  std::vector<std::vector<int64_t>> secdims;  // per leg
+  std::vector<Block> blocks;
+  std::map<std::vector<int32_t>, int> index;
+  int64_t total = 0;  // elements spanned by the buffer
+};
+
+static int build_struct(const graft_tensor_desc& d, TStruct& t, std::string& err) {
+  if (d.rank < 0 || d.rank > GRAFT_MAX_RANK) {
+    err = "tensor rank out of range";
+    return 1;
+  }
+  t.rank = d.rank;
+  t.secdims.assign(d.rank, {});
+  const bool dense = (d.sector_dims == nullptr);
+  int64_t pos = 0;
+  for (int l = 0; l < d.rank; ++l) {
+    if (dense) {
+      if (d.dims[l] <= 0) {
+        err = "non-positive leg dimension";
+        return 1;
+      }
+      t.secdims[l].push_back(d.dims[l]);
+    } else {
+      if (d.nsectors[l] <= 0) {
+        err = "leg without sectors";
+        return 1;
+      }
+      int64_t s = 0;
+      for (int q = 0; q < d.nsectors[l]; ++q) {
+        t.secdims[l].push_back(d.sector_dims[pos + q]);
+        s += d.sector_dims[pos + q];
+      }
+      pos += d.nsectors[l];
+      if (s != d.dims[l]) {
+        err = "sector sizes do not sum to the leg dimension";
+        return 1;
+      }
+    }
+  }
+  const int64_t nb = dense ? 1 : d.nblocks;
+  if (nb < 0 || (!dense && nb > 0 && d.block_sectors == nullptr)) {
+    err = "sector tensor without block list";
+    return 1;
+  }
+  int64_t next = 0;
+  t.total = 0;
+  for (int64_t b = 0; b < nb; ++b) {
+    Block blk;
+    blk.sec.resize(d.rank);
+    blk.dims.resize(d.rank);
+    blk.strides.resize(d.rank);
+    int64_t stride = 1;
+    for (int l = 0; l < d.rank; ++l) {
+      const int32_t s = dense ? 0 : d.block_sectors[b * d.rank + l];
+      if (s < 0 || s >= (int32_t)t.secdims[l].size()) {
+        err = "block sector index out of range";
+        return 1;
+      }
+      blk.sec[l] = s;
+      blk.dims[l] = t.secdims[l][s];
+      if (d.block_strides != nullptr && !dense) {
+        blk.strides[l] = d.block_strides[b * d.rank + l];
+      } else {
+        blk.strides[l] = stride;
+      }
+      stride *= blk.dims[l];
+    }
+    blk.size = stride;
+    blk.offset = (d.block_offsets != nullptr && !dense) ? d.block_offsets[b] : next;
+    next = blk.offset + blk.size;
+    int64_t span = 1;
+    for (int l = 0; l < d.rank; ++l) span += (blk.dims[l] - 1) * blk.strides[l];
+    t.total = std::max(t.total, blk.offset + span);
+    if (t.index.count(blk.sec)) {
+      err = "duplicate block";
+      return 1;
+    }
+    t.index[blk.sec] = (int)t.blocks.size();
+    t.blocks.push_back(std::move(blk));
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Compiled step / plan
+// ---------------------------------------------------------------------------
+struct StepExec {
+  int a = 0, b = 0, dst = 0;
+  int conjA = 0, conjB = 0;
+  int config = 0;
+  int pinned = -1;
+  std::vector<GemmItem> items;
+  std::vector<GemmSeg> segs;
+  std::vector<uint32_t> koff;
+  int64_t maxM = 0, maxK = 0, maxN = 0;
+  double macs = 0.0;
+  // device copies
+  GemmItem* d_items = nullptr;
+  GemmSeg* d_segs = nullptr;
+  uint32_t* d_koff = nullptr;
+  int total_tiles = 0;
+};
+
+struct graft_plan_impl {
+  int nslots = 0, output_slot = 0, ninputs = 0;
+  std::vector<TStruct> slots;
+  std::vector<StepExec> steps;
+  std::vector<int64_t> ws_offset;  // per slot byte offset in workspace (-1 = not in workspace)
+  uint64_t ws_bytes = 0;
+  double macs = 0.0;
+  void* d_meta = nullptr;  // one allocation holding every step's items/segs/koff
+  int device = 0;
+};
+
+}  // namespace graft
+
+struct graft_plan {
+  graft::graft_plan_impl impl;
+};
+
+struct graft_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* ws = nullptr;
+  uint64_t ws_size = 0;
+  uint64_t ws_cap = 0;
+  double* d_partials = nullptr;  // reduction scratch
+  double* d_result = nullptr;
+  double* h_result = nullptr;    // pinned
+  void** d_ptrs = nullptr;       // pointer table for multi-vector ops
+  double* d_coef = nullptr;
+};
+
+namespace graft {
+
+static int set_device(graft_ctx* ctx) {
+  GRAFT_CHECK_HIP(hipSetDevice(ctx->device));
+  return GRAFT_OK;
+}
+
+// Merge adjacent legs of an index group when they are contiguous in every
+// tensor addressing that group.  dims/strides: [tensor][leg].
+static void merge_group(std::vector<int64_t>& dims, std::vector<std::vector<int64_t>*> strides) {
+  std::vector<int64_t> nd;
+  std::vector<std::vector<int64_t>> ns(strides.size());
+  for (size_t l = 0; l < dims.size(); ++l) {
+    if (dims[l] == 1) continue;
+    bool merge = !nd.empty();
+    if (merge) {
+      for (size_t t = 0; t < strides.size(); ++t) {
+        if ((*strides[t])[l] != ns[t].back() * nd.back()) {
+          merge = false;
+          break;
+        }
+      }
+    }
+    if (merge) {
+      nd.back() *= dims[l];
+    } else {
+      nd.push_back(dims[l]);
+      for (size_t t = 0; t < strides.size(); ++t) ns[t].push_back((*strides[t])[l]);
+    }
+  }
+  dims = nd;
+  for (size_t t = 0; t < strides.size(); ++t) *strides[t] = ns[t];
+}
+
+static int choose_config(int64_t M, int64_t N) {
+  if (N <= 16) return 3;
+  if (M <= 16) return 6;
+  if (M <= 32 && N <= 32) return 5;
+  if (M >= 128 && N >= 128) return 0;
+  if (N <= 64 && M >= 128) return 1;
+  if (M <= 64 && N >= 128) return 4;
+  return 2;
+}
+
+static int compile_step(const graft_pair_step& st, const TStruct& A, const TStruct& B,
+                        const TStruct* fixed_out, TStruct& C, StepExec& ex, std::string& err) {
+  const int nOA = st.n_oindA, nCA = st.n_cindA, nOB = st.n_oindB, nCB = st.n_cindB;
+  if (nCA != nCB || nOA + nCA != A.rank || nOB + nCB != B.rank) {
+    err = "pair step leg counts do not match operand ranks";
+    return 1;
+  }
+  const int nOut = st.n_out_codomain + st.n_out_domain;
+  if (nOut != nOA + nOB) {
+    err = "output partition does not cover the open legs";
+    return 1;
+  }
+  std::vector<int> seenA(A.rank, 0), seenB(B.rank, 0), seenO(nOut, 0);
+  for (int i = 0; i < nOA; ++i) seenA.at(st.oindA[i])++;
+  for (int i = 0; i < nCA; ++i) seenA.at(st.cindA[i])++;
+  for (int i = 0; i < nOB; ++i) seenB.at(st.oindB[i])++;
+  for (int i = 0; i < nCB; ++i) seenB.at(st.cindB[i])++;
+  for (int i = 0; i < nOut; ++i) seenO.at(st.out[i])++;
+  for (int v : seenA) if (v != 1) { err = "A legs are not a partition"; return 1; }
+  for (int v : seenB) if (v != 1) { err = "B legs are not a partition"; return 1; }
+  for (int v : seenO) if (v != 1) { err = "output permutation is not a permutation"; return 1; }
+  for (int i = 0; i < nCA; ++i) {
+    if (A.secdims[st.cindA[i]] != B.secdims[st.cindB[i]]) {
+      err = "contracted legs have different dimensions or sectors";
+      return 1;
+    }
+  }
+  // Output leg q takes intermediate position out[q]; position p < nOA is A leg oindA[p].
+  C.rank = nOut;
+  C.secdims.assign(nOut, {});
+  std::vector<int> pos_of_out(nOut);  // inverse permutation: intermediate pos -> output leg
+  for (int q = 0; q < nOut; ++q) {
+    const int p = st.out[q];
+    pos_of_out[p] = q;
+    C.secdims[q] = p < nOA ? A.secdims[st.oindA[p]] : B.secdims[st.oindB[p - nOA]];
+  }
+  // Enumerate contributions: group B blocks by contracted sector tuple.
+  std::map<std::vector<int32_t>, std::vector<int>> bByC;
+  for (int bi = 0; bi < (int)B.blocks.size(); ++bi) {
+    std::vector<int32_t> key(nCB);
+    for (int i = 0; i < nCB; ++i) key[i] = B.blocks[bi].sec[st.cindB[i]];
+    bByC[key].push_back(bi);
+  }
+  std::map<std::vector<int32_t>, std::vector<std::pair<int, int>>> contrib;
+  for (int ai = 0; ai < (int)A.blocks.size(); ++ai) {
+    std::vector<int32_t> key(nCA);
+    for (int i = 0; i < nCA; ++i) key[i] = A.blocks[ai].sec[st.cindA[i]];
+    auto f = bByC.find(key);
+    if (f == bByC.end()) continue;
+    for (int bi : f->second) {
+      std::vector<int32_t> csec(nOut);
+      for (int p = 0; p < nOA; ++p) csec[pos_of_out[p]] = A.blocks[ai].sec[st.oindA[p]];
+      for (int p = 0; p < nOB; ++p) csec[pos_of_out[nOA + p]] = B.blocks[bi].sec[st.oindB[p]];
+      contrib[csec].push_back({ai, bi});
+    }
+  }
+  // Output blocks.
+  if (fixed_out != nullptr) {
+    if (fixed_out->rank != nOut) {
+      err = "output descriptor rank does not match the plan";
+      return 1;
+    }
+    for (int q = 0; q < nOut; ++q)
+      if (fixed_out->secdims[q] != C.secdims[q]) {
+        err = "output descriptor legs do not match the plan";
+        return 1;
+      }
+    for (auto& kv : contrib)
+      if (!fixed_out->index.count(kv.first)) {
+        err = "contraction produces a block outside the output structure";
+        return 1;
+      }
+    C = *fixed_out;
+  } else {
+    int64_t off = 0;
+    C.blocks.clear();
+    C.index.clear();
+    for (auto& kv : contrib) {
+      Block blk;
+      blk.sec = kv.first;
+      blk.dims.resize(nOut);
+      blk.strides.resize(nOut);
+      int64_t s = 1;
+      for (int q = 0; q < nOut; ++q) {
+        blk.dims[q] = C.secdims[q][kv.first[q]];
+        blk.strides[q] = s;
+        s *= blk.dims[q];
+      }
+      blk.size = s;
+      blk.offset = off;
+      off += s;
+      C.index[blk.sec] = (int)C.blocks.size();
+      C.blocks.push_back(blk);
+    }
+    C.total = off;
+    if (C.blocks.empty()) {
+      err = "contraction step has no matching blocks";
+      return 1;
+    }
+  }
+  if ((uint64_t)A.total >= (1ull << 32) || (uint64_t)B.total >= (1ull << 32) ||
+      (uint64_t)C.total >= (1ull << 32)) {
+    err = "tensor exceeds 2^32 elements (32-bit tile offsets)";
+    return 1;
+  }
+  ex.a = st.a;
+  ex.b = st.b;
+  ex.dst = st.dst;
+  ex.conjA = st.conjA;
+  ex.conjB = st.conjB;
+  ex.items.clear();
+  ex.segs.clear();
+  ex.koff.clear();
+  ex.macs = 0.0;
+  // Build items (one per output block) and segments.
+  for (int cb = 0; cb < (int)C.blocks.size(); ++cb) {
+    const Block& cblk = C.blocks[cb];
+    GemmItem it;
+    std::memset(&it, 0, sizeof(it));
+    it.c_off = cblk.offset;
+    it.seg_begin = (int)ex.segs.size();
+    auto f = contrib.find(cblk.sec);
+    const std::vector<std::pair<int, int>> empty;
+    const auto& pairs = (f == contrib.end()) ? empty : f->second;
+    // m/n dims are fixed by the output block; strides per contribution.
+    std::vector<int64_t> mdims0, ndims0;
+    std::vector<int64_t> cm0, cn0;
+    for (int p = 0; p < nOA; ++p) {
+      const int q = pos_of_out[p];
+      mdims0.push_back(cblk.dims[q]);
+      cm0.push_back(cblk.strides[q]);
+    }
+    for (int p = 0; p < nOB; ++p) {
+      const int q = pos_of_out[nOA + p];
+      ndims0.push_back(cblk.dims[q]);
+      cn0.push_back(cblk.strides[q]);
+    }
+    int64_t M = 1, N = 1;
+    for (auto d : mdims0) M *= d;
+    for (auto d : ndims0) N *= d;
+    // Merge decisions must hold for every contribution; compute a common
+    // merged layout by merging over (C, every A block, every B block).
+    std::vector<std::vector<int64_t>> am(pairs.size()), bn(pairs.size());
+    for (size_t s = 0; s < pairs.size(); ++s) {
+      const Block& ab = A.blocks[pairs[s].first];
+      const Block& bb = B.blocks[pairs[s].second];
+      for (int p = 0; p < nOA; ++p) am[s].push_back(ab.strides[st.oindA[p]]);
+      for (int p = 0; p < nOB; ++p) bn[s].push_back(bb.strides[st.oindB[p]]);
+    }
+    std::vector<int64_t> md = mdims0, nd = ndims0;
+    {
+      std::vector<std::vector<int64_t>*> grp{&cm0};
+      for (auto& v : am) grp.push_back(&v);
+      merge_group(md, grp);
+      std::vector<std::vector<int64_t>*> grpn{&cn0};
+      for (auto& v : bn) grpn.push_back(&v);
+      merge_group(nd, grpn);
+    }
+    if ((int)md.size() > GL || (int)nd.size() > GL) {
+      err = "too many unmerged legs in one GEMM index group";
+      return 1;
+    }
+    it.M = (int32_t)M;
+    it.N = (int32_t)N;
+    if (M >= (1ll << 31) || N >= (1ll << 31)) {
+      err = "GEMM extent exceeds 2^31";
+      return 1;
+    }
+    it.mrank = (int)md.size();
+    it.nrank = (int)nd.size();
+    for (int l = 0; l < it.mrank; ++l) {
+      it.mdims[l] = (int32_t)md[l];
+      it.c_mstr[l] = (uint32_t)cm0[l];
+    }
+    for (int l = 0; l < it.nrank; ++l) {
+      it.ndims[l] = (int32_t)nd[l];
+      it.c_nstr[l] = (uint32_t)cn0[l];
+    }
+    ex.maxM = std::max(ex.maxM, M);
+    ex.maxN = std::max(ex.maxN, N);
+    for (size_t s = 0; s < pairs.size(); ++s) {
+      const Block& ab = A.blocks[pairs[s].first];
+      const Block& bb = B.blocks[pairs[s].second];
+      GemmSeg sg;
+      std::memset(&sg, 0, sizeof(sg));
+      sg.a_off = ab.offset;
+      sg.b_off = bb.offset;
+      for (int l = 0; l < it.mrank; ++l) sg.a_mstr[l] = (uint32_t)am[s][l];
+      for (int l = 0; l < it.nrank; ++l) sg.b_nstr[l] = (uint32_t)bn[s][l];
+      // contracted legs
+      std::vector<int64_t> kd, ka, kb;
+      for (int i = 0; i < nCA; ++i) {
+        kd.push_back(ab.dims[st.cindA[i]]);
+        ka.push_back(ab.strides[st.cindA[i]]);
+        kb.push_back(bb.strides[st.cindB[i]]);
+      }
+      std::vector<std::vector<int64_t>*> grpk{&ka, &kb};
+      merge_group(kd, grpk);
+      int64_t K = 1;
+      for (auto d : kd) K *= d;
+      if (K >= (1ll << 31)) {
+        err = "contracted extent exceeds 2^31";
+        return 1;
+      }
+      sg.K = (int32_t)K;
+      sg.koff_idx = (int32_t)ex.koff.size();
+      ex.koff.resize(ex.koff.size() + 2 * K);
+      for (int64_t q = 0; q < K; ++q) {
+        int64_t idx = q, oa = 0, ob = 0;
+        for (size_t l = 0; l < kd.size(); ++l) {
+          const int64_t i = idx % kd[l];
+          idx /= kd[l];
+          oa += i * ka[l];
+          ob += i * kb[l];
+        }
+        ex.koff[sg.koff_idx + q] = (uint32_t)oa;
+        ex.koff[sg.koff_idx + K + q] = (uint32_t)ob;
+      }
+      // unit-stride leg decides the load direction
+      int64_t minm = INT64_MAX, mink = INT64_MAX, minn = INT64_MAX, minkb = INT64_MAX;
+      for (int l = 0; l < it.mrank; ++l) minm = std::min(minm, am[s][l]);
+      for (int l = 0; l < it.nrank; ++l) minn = std::min(minn, bn[s][l]);
+      for (size_t l = 0; l < kd.size(); ++l) {
+        mink = std::min(mink, ka[l]);
+        minkb = std::min(minkb, kb[l]);
+      }
+      sg.a_kfast = (mink < minm) ? 1 : 0;
+      sg.b_kfast = (minkb < minn) ? 1 : 0;
+      ex.segs.push_back(sg);
+      ex.maxK = std::max(ex.maxK, K);
+      ex.macs += (double)M * (double)K * (double)N;
+    }
+    it.seg_count = (int)ex.segs.size() - it.seg_begin;
+    ex.items.push_back(it);
+  }
+  return 0;
+}
+
+static void assign_tiles(StepExec& ex) {
+  const TileConfig& c = kTileConfigs[ex.config];
+  int t = 0;
+  for (auto& it : ex.items) {
+    it.tile_begin = t;
+    it.tiles_n = (it.N + c.bn - 1) / c.bn;
+    t += ((it.M + c.bm - 1) / c.bm) * it.tiles_n;
+  }
+  ex.total_tiles = t;
+}
+
+static int upload_step_meta(graft_plan_impl& p) {
+  // one allocation: per step [items | segs | koff], 256-byte aligned pieces
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  size_t total = 0;
+  for (auto& ex : p.steps) {
+    total += al(ex.items.size() * sizeof(GemmItem));
+    total += al(std::max<size_t>(1, ex.segs.size()) * sizeof(GemmSeg));
+    total += al(std::max<size_t>(1, ex.koff.size()) * sizeof(uint32_t));
+  }
+  if (p.d_meta) {
+    (void)hipFree(p.d_meta);
+    p.d_meta = nullptr;
+  }
+  GRAFT_CHECK_HIP(hipMalloc(&p.d_meta, total));
+  std::vector<char> host(total, 0);
+  size_t off = 0;
+  for (auto& ex : p.steps) {
+    size_t s1 = ex.items.size() * sizeof(GemmItem);
+    std::memcpy(host.data() + off, ex.items.data(), s1);
+    ex.d_items = (GemmItem*)((char*)p.d_meta + off);
+    off += al(s1);
+    size_t s2 = ex.segs.size() * sizeof(GemmSeg);
+    if (s2) std::memcpy(host.data() + off, ex.segs.data(), s2);
+    ex.d_segs = (GemmSeg*)((char*)p.d_meta + off);
+    off += al(std::max<size_t>(1, ex.segs.size()) * sizeof(GemmSeg));
+    size_t s3 = ex.koff.size() * sizeof(uint32_t);
+    if (s3) std::memcpy(host.data() + off, ex.koff.data(), s3);
+    ex.d_koff = (uint32_t*)((char*)p.d_meta + off);
+    off += al(std::max<size_t>(1, ex.koff.size()) * sizeof(uint32_t));
+  }
+  GRAFT_CHECK_HIP(hipMemcpy(p.d_meta, host.data(), total, hipMemcpyHostToDevice));
+  return GRAFT_OK;
+}
+
+static int ensure_workspace(graft_ctx* ctx, uint64_t bytes) {
+  if (bytes <= ctx->ws_size) return GRAFT_OK;
+  if (ctx->ws_cap && bytes > ctx->ws_cap) {
+    set_error("plan workspace of " + std::to_string(bytes) + " bytes exceeds the cap of " +
+              std::to_string(ctx->ws_cap) + " bytes");
+    return GRAFT_E_NOMEM;
+  }
+  GRAFT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->ws) {
+    GRAFT_CHECK_HIP(hipFree(ctx->ws));
+    ctx->ws = nullptr;
+    ctx->ws_size = 0;
+  }
+  hipError_t e = hipMalloc(&ctx->ws, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    ctx->ws = nullptr;
+    set_error("workspace allocation of " + std::to_string(bytes) + " bytes failed: " +
+              hipGetErrorString(e));
+    return GRAFT_E_NOMEM;
+  }
+  ctx->ws_size = bytes;
+  return GRAFT_OK;
+}
+
+static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void* const* inputs,
+                    void* output, int precision) {
+  auto ptr = [&](int slot) -> void* {
+    if (slot < p.ninputs) return const_cast<void*>(inputs[slot]);
+    if (slot == p.output_slot) return output;
+    return (char*)ctx->ws + p.ws_offset[slot];
+  };
+  void* A = ptr(ex.a);
+  void* B = ptr(ex.b);
+  void* C = ptr(ex.dst);
+  if (!A || !B || !C) {
+    set_error("null operand pointer");
+    return GRAFT_E_INVALID;
+  }
+  hipError_t e = launch_zgemm(ex.config, precision, A, B, C, ex.d_items, (int)ex.items.size(),
+                              ex.d_segs, ex.d_koff, ex.total_tiles, ex.conjA, ex.conjB,
+                              ctx->stream);
+  if (e == hipErrorNotSupported) {
+    set_error("precision not supported by this build");
+    return GRAFT_E_UNSUPPORTED;
+  }
+  GRAFT_CHECK_HIP(e);
+  return GRAFT_OK;
+}
+
+static int execute_plan(graft_ctx* ctx, graft_plan_impl& p, const void* const* inputs,
+                        void* output, int precision) {
+  if (!inputs || !output) {
+    set_error("null inputs/output");
+    return GRAFT_E_INVALID;
+  }
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  rc = ensure_workspace(ctx, p.ws_bytes);
+  if (rc) return rc;
+  for (auto& ex : p.steps) {
+    rc = run_step(ctx, p, ex, inputs, output, precision);
+    if (rc) return rc;
+  }
+  return GRAFT_OK;
+}
+
+}  // namespace graft
+
+using namespace graft;
+
+extern "C" {
+
+const char* graft_last_error(void) { return g_last_error.c_str(); }
+const char* graft_version(void) { return "graft-heff 0.1 (gfx950)"; }
+
+int graft_ctx_create(int device, void* hip_stream, graft_ctx** out) {
+  GRAFT_REQUIRE(out != nullptr, "null output handle");
+  std::unique_ptr<graft_ctx> c(new graft_ctx());
+  c->device = device;
+  c->stream = (hipStream_t)hip_stream;
+  GRAFT_CHECK_HIP(hipSetDevice(device));
+  GRAFT_CHECK_HIP(hipMalloc((void**)&c->d_partials,
+                            sizeof(double) * 2 * kReduceBlocks * kMaxMultiVec));
+  GRAFT_CHECK_HIP(hipMalloc((void**)&c->d_result, sizeof(double) * 2 * kMaxMultiVec));
+  GRAFT_CHECK_HIP(hipHostMalloc((void**)&c->h_result, sizeof(double) * 2 * kMaxMultiVec));
+  GRAFT_CHECK_HIP(hipMalloc((void**)&c->d_ptrs, sizeof(void*) * kMaxMultiVec));
+  GRAFT_CHECK_HIP(hipMalloc((void**)&c->d_coef, sizeof(double) * 2 * kMaxMultiVec));
+  *out = c.release();
+  return GRAFT_OK;
+}
+
+int graft_ctx_destroy(graft_ctx* ctx) {
+  if (!ctx) return GRAFT_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->ws) (void)hipFree(ctx->ws);
+  (void)hipFree(ctx->d_partials);
+  (void)hipFree(ctx->d_result);
+  (void)hipHostFree(ctx->h_result);
+  (void)hipFree(ctx->d_ptrs);
+  (void)hipFree(ctx->d_coef);
+  delete ctx;
+  return GRAFT_OK;
+}
+
+int graft_ctx_set_stream(graft_ctx* ctx, void* hip_stream) {
+  GRAFT_REQUIRE(ctx != nullptr, "null context");
+  ctx->stream = (hipStream_t)hip_stream;
+  return GRAFT_OK;
+}
+
+int graft_ctx_set_workspace_cap(graft_ctx* ctx, uint64_t bytes) {
+  GRAFT_REQUIRE(ctx != nullptr, "null context");
+  ctx->ws_cap = bytes;
+  return GRAFT_OK;
+}
+
+int graft_ctx_workspace_bytes(graft_ctx* ctx, uint64_t* bytes) {
+  GRAFT_REQUIRE(ctx != nullptr && bytes != nullptr, "null argument");
+  *bytes = ctx->ws_size;
+  return GRAFT_OK;
+}
+
+int graft_ctx_release_workspace(graft_ctx* ctx) {
+  GRAFT_REQUIRE(ctx != nullptr, "null context");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  GRAFT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->ws) GRAFT_CHECK_HIP(hipFree(ctx->ws));
+  ctx->ws = nullptr;
+  ctx->ws_size = 0;
+  return GRAFT_OK;
+}
+
+int graft_synchronize(graft_ctx* ctx) {
+  GRAFT_REQUIRE(ctx != nullptr, "null context");
+  GRAFT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return GRAFT_OK;
+}
+
+int graft_malloc(graft_ctx* ctx, uint64_t bytes, void** dptr) {
+  GRAFT_REQUIRE(ctx != nullptr && dptr != nullptr, "null argument");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    set_error(std::string("device allocation failed: ") + hipGetErrorString(e));
+    return GRAFT_E_NOMEM;
+  }
+  return GRAFT_OK;
+}
+
+int graft_free(graft_ctx* ctx, void* dptr) {
+  GRAFT_REQUIRE(ctx != nullptr, "null context");
+  if (dptr) GRAFT_CHECK_HIP(hipFree(dptr));
+  return GRAFT_OK;
+}
+
+int graft_upload(graft_ctx* ctx, void* dst, const void* host_src, uint64_t bytes) {
+  GRAFT_REQUIRE(ctx != nullptr && dst != nullptr && host_src != nullptr, "null argument");
+  GRAFT_CHECK_HIP(hipMemcpyAsync(dst, host_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  GRAFT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return GRAFT_OK;
+}
+
+int graft_download(graft_ctx* ctx, void* host_dst, const void* src, uint64_t bytes) {
+  GRAFT_REQUIRE(ctx != nullptr && host_dst != nullptr && src != nullptr, "null argument");
+  GRAFT_CHECK_HIP(hipMemcpyAsync(host_dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  GRAFT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return GRAFT_OK;
+}
+
+int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nsteps,
+                      int32_t nslots, int32_t output_slot, const graft_tensor_desc* input_descs,
+                      int32_t ninputs, const graft_tensor_desc* out_desc, graft_plan** out) {
+  GRAFT_REQUIRE(ctx != nullptr && out != nullptr, "null argument");
+  GRAFT_REQUIRE(steps != nullptr && nsteps >= 1, "plan without steps");
+  GRAFT_REQUIRE(input_descs != nullptr && ninputs >= 1, "plan without inputs");
+  GRAFT_REQUIRE(nslots == ninputs + nsteps, "nslots must equal ninputs + nsteps");
+  GRAFT_REQUIRE(output_slot >= ninputs && output_slot < nslots, "output slot out of range");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  std::unique_ptr<graft_plan> plan(new graft_plan());
+  graft_plan_impl& p = plan->impl;
+  p.nslots = nslots;
+  p.output_slot = output_slot;
+  p.ninputs = ninputs;
+  p.device = ctx->device;
+  p.slots.resize(nslots);
+  std::vector<int> defined(nslots, 0), consumed(nslots, 0), born(nslots, -1), dies(nslots, -1);
+  std::string err;
+  for (int i = 0; i < ninputs; ++i) {
+    if (build_struct(input_descs[i], p.slots[i], err)) {
+      set_error("input " + std::to_string(i) + ": " + err);
+      return GRAFT_E_INVALID;
+    }
+    defined[i] = 1;
+  }
+  TStruct outfixed;
+  bool have_out = false;
+  if (out_desc) {
+    if (build_struct(*out_desc, outfixed, err)) {
+      set_error("output descriptor: " + err);
+      return GRAFT_E_INVALID;
+    }
+    have_out = true;
+  }
+  p.steps.resize(nsteps);
+  for (int s = 0; s < nsteps; ++s) {
+    const graft_pair_step& st = steps[s];
+    GRAFT_REQUIRE(st.a >= 0 && st.a < nslots && st.b >= 0 && st.b < nslots && st.dst >= 0 &&
+                      st.dst < nslots,
+                  "step slot out of range");
+    GRAFT_REQUIRE(defined[st.a] && defined[st.b], "step " + std::to_string(s) +
+                                                      " references an undefined or released slot");
+    GRAFT_REQUIRE(!defined[st.dst] && st.dst >= ninputs, "step destination already defined");
+    GRAFT_REQUIRE(st.a != st.b, "step contracts a slot with itself");
+    GRAFT_REQUIRE(st.n_oindA >= 0 && st.n_cindA >= 0 && st.n_oindB >= 0 && st.n_cindB >= 0 &&
+                      st.n_oindA <= GRAFT_MAX_RANK && st.n_oindB <= GRAFT_MAX_RANK &&
+                      st.n_cindA <= GRAFT_MAX_RANK && st.n_cindB <= GRAFT_MAX_RANK &&
+                      st.n_out_codomain >= 0 && st.n_out_domain >= 0 &&
+                      st.n_out_codomain + st.n_out_domain <= GRAFT_MAX_RANK,
+                  "step leg counts out of range");
+    const TStruct* fixed = (st.dst == output_slot && have_out) ? &outfixed : nullptr;
+    TStruct C;
+    if (compile_step(st, p.slots[st.a], p.slots[st.b], fixed, C, p.steps[s], err)) {
+      set_error("step " + std::to_string(s) + ": " + err);
+      return GRAFT_E_INVALID;
+    }
+    p.slots[st.dst] = std::move(C);
+    defined[st.dst] = 1;
+    born[st.dst] = s;
+    if (st.a >= ninputs) dies[st.a] = s;
+    if (st.b >= ninputs) dies[st.b] = s;
+    defined[st.a] = (st.a < ninputs);  // inputs stay valid; intermediates are consumed
+    defined[st.b] = (st.b < ninputs);
+    StepExec& ex = p.steps[s];
+    ex.config = choose_config(ex.maxM, ex.maxN);
+    assign_tiles(ex);
+    p.macs += ex.macs;
+  }
+  GRAFT_REQUIRE(defined[output_slot], "output slot never produced");
+  for (int s = ninputs; s < nslots; ++s)
+    GRAFT_REQUIRE(s == output_slot || dies[s] >= 0, "an intermediate is never consumed");
+  // Workspace: first-fit over liveness intervals [born, dies].
+  p.ws_offset.assign(nslots, -1);
+  struct Live { int64_t off, size; int b, d; };
+  std::vector<Live> placed;
+  uint64_t peak = 0;
+  for (int s = ninputs; s < nslots; ++s) {
+    if (s == output_slot) continue;
+    const int64_t size = ((p.slots[s].total * 16 + 255) / 256) * 256;
+    const int b = born[s], d = dies[s];
+    std::vector<std::pair<int64_t, int64_t>> busy;
+    for (auto& l : placed)
+      if (!(l.d < b || d < l.b)) busy.push_back({l.off, l.off + l.size});
+    std::sort(busy.begin(), busy.end());
+    int64_t off = 0;
+    for (auto& iv : busy) {
+      if (off + size <= iv.first) break;
+      off = std::max(off, iv.second);
+    }
+    placed.push_back({off, size, b, d});
+    p.ws_offset[s] = off;
+    peak = std::max<uint64_t>(peak, off + size);
+  }
+  p.ws_bytes = peak;
+  rc = upload_step_meta(p);
+  if (rc) return rc;
+  *out = plan.release();
+  return GRAFT_OK;
+}
+
+int graft_plan_destroy(graft_plan* plan) {
+  if (!plan) return GRAFT_OK;
+  if (plan->impl.d_meta) {
+    (void)hipSetDevice(plan->impl.device);
+    (void)hipFree(plan->impl.d_meta);
+  }
+  delete plan;
+  return GRAFT_OK;
+}
+
+int graft_plan_info(graft_plan* plan, uint64_t* workspace_bytes, double* macs,
+                    uint64_t* output_elements) {
+  GRAFT_REQUIRE(plan != nullptr, "null plan");
+  if (workspace_bytes) *workspace_bytes = plan->impl.ws_bytes;
+  if (macs) *macs = plan->impl.macs;
+  if (output_elements) *output_elements = (uint64_t)plan->impl.slots[plan->impl.output_slot].total;
+  return GRAFT_OK;
+}
+
+int graft_plan_step_info(graft_plan* plan, int32_t step, int64_t* mkn3, int64_t* nblocks,
+                         double* macs, int32_t* tile_config) {
+  GRAFT_REQUIRE(plan != nullptr, "null plan");
+  GRAFT_REQUIRE(step >= 0 && step < (int32_t)plan->impl.steps.size(), "step out of range");
+  const StepExec& ex = plan->impl.steps[step];
+  if (mkn3) {
+    mkn3[0] = ex.maxM;
+    mkn3[1] = ex.maxK;
+    mkn3[2] = ex.maxN;
+  }
+  if (nblocks) *nblocks = (int64_t)ex.items.size();
+  if (macs) *macs = ex.macs;
+  if (tile_config) *tile_config = ex.config;
+  return GRAFT_OK;
+}
+
+int graft_num_tile_configs(void) { return kNumTileConfigs; }
+
+int graft_plan_set_tile_config(graft_plan* plan, int32_t step, int32_t config) {
+  GRAFT_REQUIRE(plan != nullptr, "null plan");
+  GRAFT_REQUIRE(step >= 0 && step < (int32_t)plan->impl.steps.size(), "step out of range");
+  GRAFT_REQUIRE(config >= -1 && config < kNumTileConfigs, "tile configuration out of range");
+  StepExec& ex = plan->impl.steps[step];
+  ex.pinned = config;
+  ex.config = config >= 0 ? config : choose_config(ex.maxM, ex.maxN);
+  assign_tiles(ex);
+  return upload_step_meta(plan->impl);
+}
+
+int graft_plan_execute(graft_ctx* ctx, graft_plan* plan, const void* const* inputs, void* output,
+                       int32_t precision) {
+  GRAFT_REQUIRE(ctx != nullptr && plan != nullptr, "null argument");
+  return execute_plan(ctx, plan->impl, inputs, output, precision);
+}
+
+int graft_plan_execute_timed(graft_ctx* ctx, graft_plan* plan, const void* const* inputs,
+                             void* output, int32_t precision, float* step_ms) {
+  GRAFT_REQUIRE(ctx != nullptr && plan != nullptr && step_ms != nullptr, "null argument");
+  graft_plan_impl& p = plan->impl;
+  GRAFT_REQUIRE(inputs != nullptr && output != nullptr, "null inputs/output");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  rc = ensure_workspace(ctx, p.ws_bytes);
+  if (rc) return rc;
+  const size_t ns = p.steps.size();
+  std::vector<hipEvent_t> ev(ns + 1);
+  for (auto& e : ev) GRAFT_CHECK_HIP(hipEventCreate(&e));
+  GRAFT_CHECK_HIP(hipEventRecord(ev[0], ctx->stream));
+  for (size_t s = 0; s < ns && !rc; ++s) {
+    rc = run_step(ctx, p, p.steps[s], inputs, output, precision);
+    if (!rc) GRAFT_CHECK_HIP(hipEventRecord(ev[s + 1], ctx->stream));
+  }
+  if (!rc) {
+    GRAFT_CHECK_HIP(hipEventSynchronize(ev[ns]));
+    for (size_t s = 0; s < ns; ++s) GRAFT_CHECK_HIP(hipEventElapsedTime(&step_ms[s], ev[s], ev[s + 1]));
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return rc;
+}
+
+int graft_plan_autotune(graft_ctx* ctx, graft_plan* plan, const void* const* inputs, void* output,
+                        int32_t precision, int32_t repeats) {
+  GRAFT_REQUIRE(ctx != nullptr && plan != nullptr, "null argument");
+  graft_plan_impl& p = plan->impl;
+  int rc = execute_plan(ctx, p, inputs, output, precision);  // fills every slot once
+  if (rc) return rc;
+  if (repeats < 1) repeats = 1;
+  hipEvent_t e0, e1;
+  GRAFT_CHECK_HIP(hipEventCreate(&e0));
+  GRAFT_CHECK_HIP(hipEventCreate(&e1));
+  for (auto& ex : p.steps) {
+    if (ex.pinned >= 0) continue;
+    int best = ex.config;
+    float best_ms = 1e30f;
+    for (int c = 0; c < kNumTileConfigs; ++c) {
+      ex.config = c;
+      assign_tiles(ex);
+      rc = upload_step_meta(p);
+      if (rc) break;
+      // warm-up, then time
+      rc = run_step(ctx, p, ex, inputs, output, precision);
+      if (rc) break;
+      GRAFT_CHECK_HIP(hipEventRecord(e0, ctx->stream));
+      for (int r = 0; r < repeats && !rc; ++r) rc = run_step(ctx, p, ex, inputs, output, precision);
+      GRAFT_CHECK_HIP(hipEventRecord(e1, ctx->stream));
+      GRAFT_CHECK_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      GRAFT_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (!rc && ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+    if (rc) break;
+    ex.config = best;
+    assign_tiles(ex);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  rc = upload_step_meta(p);
+  if (rc) return rc;
+  // Re-run so the output holds the plan's result.
+  return execute_plan(ctx, p, inputs, output, precision);
+}
+
+int graft_heff_apply(graft_ctx* ctx, graft_plan* plan, const void* x, const void* const* statics,
+                     void* y, int32_t precision) {
+  GRAFT_REQUIRE(ctx != nullptr && plan != nullptr, "null argument");
+  const int n = plan->impl.ninputs;
+  GRAFT_REQUIRE(n >= 1 && x != nullptr && (n == 1 || statics != nullptr), "null operand");
+  std::vector<const void*> inputs(n);
+  inputs[0] = x;
+  for (int i = 1; i < n; ++i) inputs[i] = statics[i - 1];
+  return execute_plan(ctx, plan->impl, inputs.data(), y, precision);
+}
+
+int graft_env_update(graft_ctx* ctx, graft_plan* plan, const void* const* operands,
+                     void* env_out) {
+  GRAFT_REQUIRE(ctx != nullptr && plan != nullptr, "null argument");
+  return execute_plan(ctx, plan->impl, operands, env_out, GRAFT_PREC_FP64);
+}
+
+int graft_contract_pair(graft_ctx* ctx, const graft_pair_step* step,
+                        const graft_tensor_desc* descA, const void* A,
+                        const graft_tensor_desc* descB, const void* B, void* C) {
+  GRAFT_REQUIRE(ctx && step && descA && descB && A && B && C, "null argument");
+  graft_pair_step st = *step;
+  st.a = 0;
+  st.b = 1;
+  st.dst = 2;
+  graft_tensor_desc descs[2] = {*descA, *descB};
+  graft_plan* plan = nullptr;
+  int rc = graft_plan_create(ctx, &st, 1, 3, 2, descs, 2, nullptr, &plan);
+  if (rc) return rc;
+  const void* ins[2] = {A, B};
+  rc = graft_plan_execute(ctx, plan, ins, C, GRAFT_PREC_FP64);
+  if (rc == GRAFT_OK) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      set_error(std::string("HIP error: ") + hipGetErrorString(e));
+      rc = GRAFT_E_HIP;
+    }
+  }
+  graft_plan_destroy(plan);
+  return rc;
+}
+
+int graft_vec_dotc(graft_ctx* ctx, int64_t n, const void* x, const void* y, double* out2) {
+  GRAFT_REQUIRE(ctx && x && y && out2 && n >= 0, "bad argument");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  GRAFT_CHECK_HIP(launch_dotc_partials(n, x, y, ctx->d_partials, kReduceBlocks, ctx->stream));
+  GRAFT_CHECK_HIP(launch_reduce_partials(ctx->d_partials, kReduceBlocks, 1, ctx->d_result, ctx->stream));
+  GRAFT_CHECK_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, 2 * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+  GRAFT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  out2[0] = ctx->h_result[0];
+  out2[1] = ctx->h_result[1];
+  return GRAFT_OK;
+}
+
+int graft_vec_nrm2(graft_ctx* ctx, int64_t n, const void* x, double* out) {
+  double d[2];
+  int rc = graft_vec_dotc(ctx, n, x, x, d);
+  if (rc) return rc;
+  *out = std::sqrt(std::max(d[0], 0.0));
+  return GRAFT_OK;
+}
+
+int graft_vec_axpby(graft_ctx* ctx, int64_t n, const double* alpha, const void* x,
+                    const double* beta, void* y) {
+  GRAFT_REQUIRE(ctx && alpha && beta && x && y && n >= 0, "bad argument");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  GRAFT_CHECK_HIP(launch_axpby(n, alpha[0], alpha[1], x, beta[0], beta[1], y, ctx->stream));
+  return GRAFT_OK;
+}
+
+int graft_vec_multi_dotc(graft_ctx* ctx, int64_t n, int32_t k, const void* const* V,
+                         const void* y, double* out2k) {
+  GRAFT_REQUIRE(ctx && V && y && out2k && n >= 0, "bad argument");
+  GRAFT_REQUIRE(k >= 1 && k <= kMaxMultiVec, "vector count out of range");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  GRAFT_CHECK_HIP(hipMemcpyAsync(ctx->d_ptrs, V, sizeof(void*) * k, hipMemcpyHostToDevice,
+                                 ctx->stream));
+  GRAFT_CHECK_HIP(launch_multi_dotc_partials(n, k, (const void* const*)ctx->d_ptrs, y,
+                                             ctx->d_partials, kReduceBlocks, ctx->stream));
+  GRAFT_CHECK_HIP(launch_reduce_partials(ctx->d_partials, kReduceBlocks, k, ctx->d_result,
+                                         ctx->stream));
+  GRAFT_CHECK_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, 2 * k * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+  GRAFT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(out2k, ctx->h_result, 2 * k * sizeof(double));
+  return GRAFT_OK;
+}
+
+int graft_vec_multi_axpy(graft_ctx* ctx, int64_t n, int32_t k, const void* const* V,
+                         const double* c2k, void* y) {
+  GRAFT_REQUIRE(ctx && V && c2k && y && n >= 0, "bad argument");
+  GRAFT_REQUIRE(k >= 1 && k <= kMaxMultiVec, "vector count out of range");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  // The pointer/coefficient tables are reused by the next call, so the
+  // upload is synchronous with respect to earlier kernels on the stream.
+  GRAFT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  GRAFT_CHECK_HIP(hipMemcpy(ctx->d_ptrs, V, sizeof(void*) * k, hipMemcpyHostToDevice));
+  GRAFT_CHECK_HIP(hipMemcpy(ctx->d_coef, c2k, sizeof(double) * 2 * k, hipMemcpyHostToDevice));
+  GRAFT_CHECK_HIP(launch_multi_axpy(n, k, (const void* const*)ctx->d_ptrs, ctx->d_coef, y,
+                                    ctx->stream));
+  return GRAFT_OK;
+}
+
+}  // extern "C"

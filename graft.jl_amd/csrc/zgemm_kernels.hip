@@ -1,0 +1,308 @@
+// Grouped, strided, permute-fused complex GEMM on CDNA4 (gfx950) MFMA.
+//
+// One launch executes one pairwise contraction step of an effective-
+// Hamiltonian or environment plan: for every output block (GemmItem) and
+// every (A block, B block) contribution to it (GemmSeg),
+//     C[m, n] = sum_seg sum_k op(A)[m, k] * op(B)[k, n].
+// Legs are never permuted in memory.  Each operand element address is the
+// sum of a row part and a column part, because the open and the contracted
+// legs of a tensor are disjoint: offset(m, k) = rowoff(m) + koff(k).  Row
+// offsets are decoded once per tile from the merged leg dims and strides;
+// contracted offsets come from a per-step table.  That folds the permutes
+// TensorOperations performs around its zgemm into the tile loads/stores.
+//
+// Complex arithmetic on real MFMA: the 16x16x4 f64 instruction consumes four
+// real k values per lane group.  Two complex k values are laid out as
+// (k0.re, k0.im, k1.re, k1.im), so with A in that interleaved form,
+//     Cr = A_ri . [ Br; -Bi ],   Ci = A_ri . [ Bi; Br ],
+// i.e. two MFMAs per 16x16 complex block and complex k pair (4 real
+// products per complex product).  The B tile is staged in LDS as three
+// planes (Br, Bi, -Bi) so that every lane reads its operand directly.
+// Conjugation flags are folded into the LDS stores.
+#include "heff_internal.h"
+
+namespace graft {
+
+const TileConfig kTileConfigs[kNumTileConfigs] = {
+    {128, 128, 16, 64, 64},  // 0: large m and n
+    {128, 64, 16, 64, 32},   // 1
+    {64, 64, 16, 32, 32},    // 2
+    {256, 16, 16, 64, 16},   // 3: skinny n (TTNO absorb)
+    {64, 128, 16, 32, 64},   // 4
+    {32, 32, 16, 16, 16},    // 5: small blocks
+    {16, 256, 16, 16, 64},   // 6: skinny m (environment closure)
+};
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+static constexpr uint32_t kBadRow = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t decode_offset(uint32_t idx, int rank, const int32_t* dims,
+                                                  const uint32_t* strides) {
+  uint32_t off = 0;
+  for (int l = 0; l < rank; ++l) {
+    uint32_t d = (uint32_t)dims[l];
+    uint32_t q = idx / d;
+    off += (idx - q * d) * strides[l];
+    idx = q;
+  }
+  return off;
+}
+
+template <int BM, int BN, int BK, int WM, int WN>
+__global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
+    zgemm_f64_kernel(const double2* __restrict__ Abase, const double2* __restrict__ Bbase,
+                     double2* __restrict__ Cbase, const GemmItem* __restrict__ items, int nitems,
+                     const GemmSeg* __restrict__ segs, const uint32_t* __restrict__ koff,
+                     int total_tiles, int conjA, int conjB) {
+  constexpr int NWN = BN / WN;
+  constexpr int NT = 64 * (BM / WM) * NWN;
+  constexpr int EA = BM * BK / NT;
+  constexpr int EB = BN * BK / NT;
+  constexpr int FM = WM / 16;
+  constexpr int FN = WN / 16;
+  constexpr int ASL = 2 * BM + 1;  // LDS slab (doubles) per k: re plane, im plane, pad
+  constexpr int BSL = 3 * BN + 1;  // re plane, im plane, -im plane, pad
+  static_assert(NT % BM == 0 && NT % BN == 0 && NT % BK == 0, "tile/thread mismatch");
+  static_assert(EA >= 1 && EB >= 1, "tile too small for the thread count");
+
+  __shared__ double As[BK * ASL];
+  __shared__ double Bs[BK * BSL];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm0 = (wave / NWN) * WM;
+  const int wn0 = (wave % NWN) * WN;
+  const int li = lane & 15;
+  const int kr = lane >> 4;
+  const int kcoff = kr >> 1;
+  const int part = kr & 1;
+
+  // XCD-aware tile order: hardware dispatches block b to XCD b % 8, so give
+  // each XCD a contiguous range of logical tiles (tiles sharing an A panel
+  // then share that XCD's L2).
+  int tile;
+  {
+    const int b = blockIdx.x;
+    const int x = b & 7, i = b >> 3;
+    const int q = total_tiles >> 3, r = total_tiles & 7;
+    tile = x * q + (x < r ? x : r) + i;
+  }
+  int lo = 0, hi = nitems - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
+  }
+  const GemmItem* it = items + lo;
+  const int M = it->M, N = it->N;
+  const int mrank = it->mrank, nrank = it->nrank;
+  const int local = tile - it->tile_begin;
+  const int m0 = (local / it->tiles_n) * BM;
+  const int n0 = (local % it->tiles_n) * BN;
+
+  dbl4 accR[FM][FN], accI[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b) {
+      accR[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+      accI[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+    }
+
+  const double sA = conjA ? -1.0 : 1.0;
+  const double sB = conjB ? -1.0 : 1.0;
+
+  const int seg_end = it->seg_begin + it->seg_count;
+  for (int s = it->seg_begin; s < seg_end; ++s) {
+    const GemmSeg* sg = segs + s;
+    const double2* A = Abase + sg->a_off;
+    const double2* B = Bbase + sg->b_off;
+    const int K = sg->K;
+    const uint32_t* kA = koff + sg->koff_idx;
+    const uint32_t* kB = kA + K;
+    const bool akf = sg->a_kfast != 0;
+    const bool bkf = sg->b_kfast != 0;
+
+    // Row offsets of this thread's A elements and column offsets of its B elements.
+    uint32_t rowA[EA], colB[EB];
+    if (akf) {
+#pragma unroll
+      for (int e = 0; e < EA; ++e) {
+        const int p = tid / BK + e * (NT / BK);
+        const int m = m0 + p;
+        rowA[e] = (m < M) ? decode_offset((uint32_t)m, mrank, it->mdims, sg->a_mstr) : kBadRow;
+      }
+    } else {
+      const int m = m0 + tid % BM;
+      rowA[0] = (m < M) ? decode_offset((uint32_t)m, mrank, it->mdims, sg->a_mstr) : kBadRow;
+    }
+    if (bkf) {
+#pragma unroll
+      for (int e = 0; e < EB; ++e) {
+        const int p = tid / BK + e * (NT / BK);
+        const int n = n0 + p;
+        colB[e] = (n < N) ? decode_offset((uint32_t)n, nrank, it->ndims, sg->b_nstr) : kBadRow;
+      }
+    } else {
+      const int n = n0 + tid % BN;
+      colB[0] = (n < N) ? decode_offset((uint32_t)n, nrank, it->ndims, sg->b_nstr) : kBadRow;
+    }
+
+    double2 ra[EA], rb[EB];
+    auto fetch = [&](int k0) {
+      if (akf) {
+        const int q = k0 + tid % BK;
+        const uint32_t ko = (q < K) ? kA[q] : 0u;
+#pragma unroll
+        for (int e = 0; e < EA; ++e)
+          ra[e] = (q < K && rowA[e] != kBadRow) ? A[(uint64_t)rowA[e] + ko] : double2{0.0, 0.0};
+      } else {
+#pragma unroll
+        for (int e = 0; e < EA; ++e) {
+          const int q = k0 + tid / BM + e * (NT / BM);
+          ra[e] = (q < K && rowA[0] != kBadRow) ? A[(uint64_t)rowA[0] + kA[q]] : double2{0.0, 0.0};
+        }
+      }
+      if (bkf) {
+        const int q = k0 + tid % BK;
+        const uint32_t ko = (q < K) ? kB[q] : 0u;
+#pragma unroll
+        for (int e = 0; e < EB; ++e)
+          rb[e] = (q < K && colB[e] != kBadRow) ? B[(uint64_t)colB[e] + ko] : double2{0.0, 0.0};
+      } else {
+#pragma unroll
+        for (int e = 0; e < EB; ++e) {
+          const int q = k0 + tid / BN + e * (NT / BN);
+          rb[e] = (q < K && colB[0] != kBadRow) ? B[(uint64_t)colB[0] + kB[q]] : double2{0.0, 0.0};
+        }
+      }
+    };
+    auto stage = [&]() {
+      if (akf) {
+#pragma unroll
+        for (int e = 0; e < EA; ++e) {
+          const int q = tid % BK, p = tid / BK + e * (NT / BK);
+          As[q * ASL + p] = ra[e].x;
+          As[q * ASL + BM + p] = sA * ra[e].y;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < EA; ++e) {
+          const int p = tid % BM, q = tid / BM + e * (NT / BM);
+          As[q * ASL + p] = ra[e].x;
+          As[q * ASL + BM + p] = sA * ra[e].y;
+        }
+      }
+      if (bkf) {
+#pragma unroll
+        for (int e = 0; e < EB; ++e) {
+          const int q = tid % BK, p = tid / BK + e * (NT / BK);
+          const double im = sB * rb[e].y;
+          Bs[q * BSL + p] = rb[e].x;
+          Bs[q * BSL + BN + p] = im;
+          Bs[q * BSL + 2 * BN + p] = -im;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < EB; ++e) {
+          const int p = tid % BN, q = tid / BN + e * (NT / BN);
+          const double im = sB * rb[e].y;
+          Bs[q * BSL + p] = rb[e].x;
+          Bs[q * BSL + BN + p] = im;
+          Bs[q * BSL + 2 * BN + p] = -im;
+        }
+      }
+    };
+
+    const int nkt = (K + BK - 1) / BK;
+    fetch(0);
+    for (int kt = 0; kt < nkt; ++kt) {
+      __syncthreads();
+      stage();
+      __syncthreads();
+      if (kt + 1 < nkt) fetch((kt + 1) * BK);
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 2) {
+        const int kq = kk + kcoff;
+        const double* ap = As + kq * ASL + part * BM + wm0 + li;
+        const double* brp = Bs + kq * BSL + (part ? 2 * BN : 0) + wn0 + li;
+        const double* bip = Bs + kq * BSL + (part ? 0 : BN) + wn0 + li;
+        double av[FM], bre[FN], bim[FN];
+#pragma unroll
+        for (int a = 0; a < FM; ++a) av[a] = ap[a * 16];
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+          bre[b] = brp[b * 16];
+          bim[b] = bip[b * 16];
+        }
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+          for (int b = 0; b < FN; ++b) {
+            accR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bre[b], accR[a][b], 0, 0, 0);
+            accI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bim[b], accI[a][b], 0, 0, 0);
+          }
+      }
+    }
+  }
+
+  // Epilogue: v_mfma_f64_16x16x4f64 leaves D[kr + 4*r][li] in accumulator
+  // element r of lane (16*kr + li).
+  double2* C = Cbase + it->c_off;
+  uint32_t coln[FN];
+#pragma unroll
+  for (int b = 0; b < FN; ++b) {
+    const int n = n0 + wn0 + b * 16 + li;
+    coln[b] = (n < N) ? decode_offset((uint32_t)n, nrank, it->ndims, it->c_nstr) : kBadRow;
+  }
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + wm0 + a * 16 + kr + 4 * r;
+      if (m >= M) continue;
+      const uint32_t ro = decode_offset((uint32_t)m, mrank, it->mdims, it->c_mstr);
+#pragma unroll
+      for (int b = 0; b < FN; ++b) {
+        if (coln[b] == kBadRow) continue;
+        C[(uint64_t)ro + coln[b]] = double2{accR[a][b][r], accI[a][b][r]};
+      }
+    }
+}
+
+int tiles_for(int config, int M, int N) {
+  const TileConfig& c = kTileConfigs[config];
+  return ((M + c.bm - 1) / c.bm) * ((N + c.bn - 1) / c.bn);
+}
+
+template <int BM, int BN, int BK, int WM, int WN>
+static hipError_t launch_f64(const void* A, const void* B, void* C, const GemmItem* items,
+                             int nitems, const GemmSeg* segs, const uint32_t* koff,
+                             int total_tiles, int conjA, int conjB, hipStream_t stream) {
+  constexpr int NT = 64 * (BM / WM) * (BN / WN);
+  hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, BK, WM, WN>), dim3(total_tiles), dim3(NT), 0,
+                     stream, (const double2*)A, (const double2*)B, (double2*)C, items, nitems,
+                     segs, koff, total_tiles, conjA, conjB);
+  return hipGetLastError();
+}
+
+hipError_t launch_zgemm(int config, int precision, const void* A, const void* B, void* C,
+                        const GemmItem* items, int nitems, const GemmSeg* segs,
+                        const uint32_t* koff, int total_tiles, int conjA, int conjB,
+                        hipStream_t stream) {
+  if (total_tiles <= 0) return hipSuccess;
+  if (precision != GRAFT_PREC_FP64) return hipErrorNotSupported;
+  switch (config) {
+    case 0: return launch_f64<128, 128, 16, 64, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, stream);
+    case 1: return launch_f64<128, 64, 16, 64, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, stream);
+    case 2: return launch_f64<64, 64, 16, 32, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, stream);
+    case 3: return launch_f64<256, 16, 16, 64, 16>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, stream);
+    case 4: return launch_f64<64, 128, 16, 32, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, stream);
+    case 5: return launch_f64<32, 32, 16, 16, 16>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, stream);
+    case 6: return launch_f64<16, 256, 16, 16, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace graft

@@ -1,0 +1,59 @@
+"""Ground-state sweeps — lib/GraftGroundState/src/GroundState/GroundState.jl.
+
+``dmrg2_`` is dmrg2! (GroundState.jl:102-143): post-order bond sweep and its
+reverse, a Lanczos ground state of the two-site effective Hamiltonian on every
+bond, and a truncated split.  The local solve goes through
+``groundstate_eigsolve_backend`` (GroundState.jl:25-26), here the device
+Lanczos of ``krylov.eigsolve``.
+"""
+from __future__ import annotations
+
+import time
+
+from . import backend as B
+from . import krylov
+from .contractions import EnvCache, eff_h2, split_two_site_, two_site_tensor
+from .networks import move_center_
+from .trees import postorder
+
+
+def groundstate_eigsolve_backend(effective, x, howmany, which, **kw):
+    """Hook point of GroundState.jl:25-26."""
+    return krylov.eigsolve(effective, x, howmany, which, **kw)
+
+
+def _energy_converged(energies, tol):
+    return len(energies) >= 2 and abs(energies[-1] - energies[-2]) < tol
+
+
+def dmrg2_(psi, H, trunc=B.NO_TRUNCATION, nsweeps=10, tol=1e-10, krylovdim=20, eig_tol=1e-12,
+           precision="fp64", verbose=False, callback=None):
+    """Two-site DMRG; returns (psi, energies) with one energy per sweep."""
+    if not H.ishermitian:
+        raise ValueError("dmrg2_: DMRG requires ishermitian(H) == true")
+    t = psi.topo
+    cache = EnvCache(t)
+    energies = []
+    bonds = [n for n in postorder(t) if t.parent[n] >= 0]
+    schedule = [(n, "m") for n in bonds] + [(n, "n") for n in reversed(bonds)]
+    for sweep in range(1, nsweeps + 1):
+        t0 = time.perf_counter()
+        E = float("nan")
+        for n, center_on in schedule:
+            m = t.parent[n]
+            move_center_(psi, n, cache)
+            theta = two_site_tensor(psi, n, m)
+            h2 = eff_h2(cache, psi, H, n, m, precision=precision)
+            vals, vecs, info = groundstate_eigsolve_backend(
+                h2, theta, 1, "SR", ishermitian=True, krylovdim=krylovdim, tol=eig_tol)
+            E = float(vals[0].real)
+            cache.invalidate_edge_(n, m)
+            split_two_site_(psi, vecs[0], n, m, trunc=trunc, center_on=center_on)
+            if callback is not None:
+                callback(sweep, n, m, E, info)
+        energies.append(E)
+        if verbose:
+            print(f"dmrg2_ sweep {sweep}: E = {E:.14f} ({time.perf_counter() - t0:.2f} s)")
+        if _energy_converged(energies, tol):
+            break
+    return psi, energies

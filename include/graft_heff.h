@@ -1,0 +1,181 @@
+/*
+ * graft_heff.h — C ABI of the MI355X effective-Hamiltonian engine.
+ *
+ * The engine executes the pairwise contraction plans that the reference's
+ * planner produces for eff_h1 / eff_h2 / eff_h0 applies and for the
+ * environment update, on device-resident complex tensors, with hand-written
+ * CDNA4 (gfx950) MFMA kernels.  Every entry point takes plain pointers and
+ * sizes, returns an int status (GRAFT_OK == 0) and records a message that
+ * graft_last_error() returns (per calling thread).
+ *
+ * Tensor storage follows TensorKit's block layout for a TensorMap with
+ * trivial sectors: one dense block, flat legs (codomain legs, then domain
+ * legs), first leg fastest, complex128 interleaved (re, im).  Abelian
+ * sector tensors are a list of strided dense blocks addressed by one sector
+ * index per leg (graft_tensor_desc below), which covers the layout TensorKit
+ * uses for each coupled-sector block of a U(1) x U(1) TensorMap.
+ *
+ * Reference interfaces replaced (paths relative to the Graft.jl repo):
+ *   graft_plan_create   <- Planning.PairStep / ContractionPlan
+ *                          lib/GraftPlanning/src/Planning/types.jl:54-65, 94-115
+ *   graft_plan_execute  <- Planning.execute / _execute_workspace
+ *                          lib/GraftPlanning/src/Planning/executor.jl:16-45, 99-143
+ *   graft_heff_apply    <- (::EffectiveMap)(x) / (::WorkspaceMap)(x)
+ *                          lib/GraftPlanning/src/Planning/types.jl:440-445, 565-574
+ *   graft_env_update    <- Contractions._build_env_value -> Planning.execute
+ *                          lib/GraftContractions/src/Contractions/envcache.jl:1127-1132
+ *   graft_contract_pair <- Backend.contract_pair / contract_pair!
+ *                          lib/GraftFoundation/src/Backend/Backend.jl:166-222
+ *   graft_vec_*         <- the vector operations KrylovKit performs inside
+ *                          eigsolve / exponentiate on the map's output
+ *                          (lib/GraftGroundState/src/GroundState/GroundState.jl:25-39,
+ *                           lib/GraftEvolution/src/Evolution/Evolution.jl:134-148)
+ */
+#ifndef GRAFT_HEFF_H
+#define GRAFT_HEFF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GRAFT_MAX_RANK 12
+
+enum graft_status {
+  GRAFT_OK = 0,
+  GRAFT_E_INVALID = 1,      /* malformed plan, descriptor or argument        */
+  GRAFT_E_NOMEM = 2,        /* device allocation or byte-cap admission failed */
+  GRAFT_E_HIP = 3,          /* a HIP runtime call failed                     */
+  GRAFT_E_UNSUPPORTED = 4   /* valid request outside what this build supports */
+};
+
+/* Arithmetic of the complex products (the Krylov algebra stays FP64). */
+enum graft_precision {
+  GRAFT_PREC_FP64 = 0,      /* complex FP64 on v_mfma_f64_16x16x4f64          */
+  GRAFT_PREC_FP32 = 1,      /* operands rounded to FP32, FP32 MFMA, FP32 acc.  */
+  GRAFT_PREC_BF16X3 = 2     /* FP32 split into bf16 hi+lo, 3 bf16 MFMA products,
+                               FP32 accumulation                               */
+};
+
+typedef struct graft_ctx graft_ctx;
+typedef struct graft_plan graft_plan;
+
+/*
+ * One binary contraction, the reference's PairStep with 0-based indices:
+ * C = contract(A = slot[a] (conj if conjA), B = slot[b] (conj if conjB)),
+ * contracting A legs cindA[i] with B legs cindB[i]; the result's flat legs
+ * are out[0..n_out-1] indexing the concatenation (A legs oindA..., B legs
+ * oindB...).  n_out_codomain only records the TensorMap partition.
+ */
+typedef struct graft_pair_step {
+  int32_t a, b, dst;
+  int32_t n_oindA, n_cindA, n_oindB, n_cindB;
+  int32_t oindA[GRAFT_MAX_RANK];
+  int32_t cindA[GRAFT_MAX_RANK];
+  int32_t oindB[GRAFT_MAX_RANK];
+  int32_t cindB[GRAFT_MAX_RANK];
+  int32_t conjA, conjB;
+  int32_t n_out_codomain, n_out_domain;
+  int32_t out[GRAFT_MAX_RANK];
+} graft_pair_step;
+
+/*
+ * Structure of one tensor.  Dense tensors: rank and dims, all pointers NULL,
+ * nblocks = 1.  Sector tensors: nsectors[l] sectors on leg l with sizes
+ * sector_dims (concatenated leg by leg); nblocks blocks, block b covering
+ * sector indices block_sectors[b*rank + l]; its data starts at element
+ * block_offsets[b] of the buffer with per-leg element strides
+ * block_strides[b*rank + l] (NULL = packed, first leg fastest, blocks in
+ * order).  Sector indices are matched by position between contracted legs;
+ * the caller supplies charge-conserving block lists.
+ */
+typedef struct graft_tensor_desc {
+  int32_t rank;
+  int64_t dims[GRAFT_MAX_RANK];
+  int32_t nsectors[GRAFT_MAX_RANK];
+  const int64_t* sector_dims;
+  int64_t nblocks;
+  const int32_t* block_sectors;
+  const int64_t* block_offsets;
+  const int64_t* block_strides;
+} graft_tensor_desc;
+
+/* ---- context: one device, one stream, one workspace arena ------------- */
+int graft_ctx_create(int device, void* hip_stream, graft_ctx** out);
+int graft_ctx_destroy(graft_ctx* ctx);
+int graft_ctx_set_stream(graft_ctx* ctx, void* hip_stream);
+/* Hard cap on workspace bytes (0 = no cap); exceeding it fails with
+ * GRAFT_E_NOMEM before any launch (mirrors memory_cap_bytes). */
+int graft_ctx_set_workspace_cap(graft_ctx* ctx, uint64_t bytes);
+int graft_ctx_workspace_bytes(graft_ctx* ctx, uint64_t* bytes);
+int graft_ctx_release_workspace(graft_ctx* ctx);
+int graft_synchronize(graft_ctx* ctx);
+const char* graft_last_error(void);
+const char* graft_version(void);
+
+/* ---- device block store ------------------------------------------------ */
+int graft_malloc(graft_ctx* ctx, uint64_t bytes, void** dptr);
+int graft_free(graft_ctx* ctx, void* dptr);
+int graft_upload(graft_ctx* ctx, void* dst, const void* host_src, uint64_t bytes);
+int graft_download(graft_ctx* ctx, void* host_dst, const void* src, uint64_t bytes);
+
+/* ---- plans ------------------------------------------------------------- */
+/* input_descs[0..ninputs-1] describe slots 0..ninputs-1 (slot 0 is the
+ * Krylov vector x for effective maps); out_desc describes the result
+ * (NULL = derive it from the plan).  Shapes are fixed at creation. */
+int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nsteps,
+                      int32_t nslots, int32_t output_slot,
+                      const graft_tensor_desc* input_descs, int32_t ninputs,
+                      const graft_tensor_desc* out_desc, graft_plan** out);
+int graft_plan_destroy(graft_plan* plan);
+/* Workspace bytes the plan needs, algorithmic complex MACs (sum m*k*n over
+ * all GEMM blocks), and the number of output elements. */
+int graft_plan_info(graft_plan* plan, uint64_t* workspace_bytes, double* macs,
+                    uint64_t* output_elements);
+/* Per-step GEMM profile: m, k, n of the largest block, block count and MACs. */
+int graft_plan_step_info(graft_plan* plan, int32_t step, int64_t* mkn3, int64_t* nblocks,
+                         double* macs, int32_t* tile_config);
+/* Pin the tile configuration of one step (-1 = heuristic). */
+int graft_plan_set_tile_config(graft_plan* plan, int32_t step, int32_t config);
+int graft_num_tile_configs(void);
+/* Time every tile configuration of every step on the given operands and
+ * keep the fastest (the tile autotuner).  Operands are left unchanged. */
+int graft_plan_autotune(graft_ctx* ctx, graft_plan* plan, const void* const* inputs,
+                        void* output, int32_t precision, int32_t repeats);
+int graft_plan_execute(graft_ctx* ctx, graft_plan* plan, const void* const* inputs,
+                       void* output, int32_t precision);
+/* Same, with a HIP event pair on the context's stream around every step's
+ * launch; step_ms[s] receives that step's duration (waits for completion). */
+int graft_plan_execute_timed(graft_ctx* ctx, graft_plan* plan, const void* const* inputs,
+                             void* output, int32_t precision, float* step_ms);
+
+/* eff_h1 / eff_h2 / eff_h0 apply: inputs = (x, statics...). */
+int graft_heff_apply(graft_ctx* ctx, graft_plan* plan, const void* x,
+                     const void* const* statics, void* y, int32_t precision);
+/* env(u -> v) from (ket, envs..., cap?, W, bra). */
+int graft_env_update(graft_ctx* ctx, graft_plan* plan, const void* const* operands,
+                     void* env_out);
+/* One dense binary contraction (two_site_tensor, absorb_on_leg). */
+int graft_contract_pair(graft_ctx* ctx, const graft_pair_step* step,
+                        const graft_tensor_desc* descA, const void* A,
+                        const graft_tensor_desc* descB, const void* B, void* C);
+
+/* ---- Krylov vector operations on complex128 vectors of length n ------- */
+int graft_vec_dotc(graft_ctx* ctx, int64_t n, const void* x, const void* y, double* out2);
+int graft_vec_nrm2(graft_ctx* ctx, int64_t n, const void* x, double* out);
+/* y = alpha*x + beta*y (alpha, beta complex: re, im pairs) */
+int graft_vec_axpby(graft_ctx* ctx, int64_t n, const double* alpha, const void* x,
+                    const double* beta, void* y);
+/* out[j] = <V_j, y> for j < k (one pass over y) */
+int graft_vec_multi_dotc(graft_ctx* ctx, int64_t n, int32_t k, const void* const* V,
+                         const void* y, double* out2k);
+/* y += sum_j c_j V_j (one pass over y) */
+int graft_vec_multi_axpy(graft_ctx* ctx, int64_t n, int32_t k, const void* const* V,
+                         const double* c2k, void* y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRAFT_HEFF_H */

@@ -1,0 +1,235 @@
+"""Pin the numpy oracle against closed forms, invariants and the reference's known answers.
+
+Each check is chosen so that a plausible mistake (wrong Kronecker order,
+wrong leg label, missing conjugation, wrong coefficient) fails it.  CPU only.
+"""
+import numpy as np
+import pytest
+
+from oracle import heff_oracle as ref
+
+from graft_amd import operators as O
+from graft_amd import trees as Tr
+
+
+def _product_tree(t):
+    """Oracle Tree with the same node order as a product TreeTopology."""
+    return ref.Tree(list(t.ids), list(t.parent), [list(c) for c in t.children])
+
+
+# --------------------------------------------------------------------------
+# dense Hamiltonians
+# --------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("n,J,g", [(2, 1.0, 0.0), (3, 0.0, 0.7), (6, 1.0, 0.9), (7, 0.8, 1.3)])
+def test_tfi_chain_matches_free_fermions(n, J, g):
+    t = ref.chain_tree(n)
+    # chain edges run through consecutive sites in any rooting
+    H = ref.dense_hamiltonian(n, ref.tfi_terms(t, J, g))
+    E0 = np.linalg.eigvalsh(H)[0]
+    assert abs(E0 - ref.tfi_chain_exact_energy(n, J, g)) < 1e-10
+
+
+def test_heisenberg_two_site_singlet_triplet():
+    t = ref.chain_tree(2)
+    H = ref.dense_hamiltonian(2, ref.heisenberg_terms(t, J=1.3))
+    w = np.sort(np.linalg.eigvalsh(H))
+    assert np.allclose(w, [-0.75 * 1.3, 0.25 * 1.3, 0.25 * 1.3, 0.25 * 1.3], atol=1e-13)
+
+
+def test_kron_order_first_site_fastest():
+    H = ref.dense_hamiltonian(3, [(1.0, [(0, ref.SZ)])])
+    diag = np.real(np.diag(H))
+    assert np.allclose(diag, [1 - 2 * (j % 2) for j in range(8)])
+    H2 = ref.dense_hamiltonian(3, [(1.0, [(2, ref.SZ)])])
+    assert np.allclose(np.real(np.diag(H2)), [1 - 2 * ((j // 4) % 2) for j in range(8)])
+
+
+# --------------------------------------------------------------------------
+# TTNO builders (product, host side) against the dense Hamiltonian
+# --------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("topo_fn", [lambda: Tr.star_topology(3, 1), lambda: Tr.binary_topology(2),
+                                     lambda: Tr.mps_topology(4), lambda: Tr.star_topology(2, 2)])
+def test_ttno_builders_equal_dense(topo_fn):
+    topo = topo_fn()
+    t = _product_tree(topo)
+    n = t.n
+    for W, terms in [(O.tfi_host(topo, 1.0, 0.9), ref.tfi_terms(t, 1.0, 0.9)),
+                     (O.heisenberg_host(topo, 1.0), ref.heisenberg_terms(t, 1.0))]:
+        Hd = ref.dense_hamiltonian(n, terms)
+        Ht = ref.ttno_to_dense(t, W)
+        assert np.abs(Ht - Hd).max() < 1e-12
+    rng = np.random.default_rng(5)
+    Js = rng.uniform(0.5, 1.5, n)
+    hs = rng.uniform(0.5, 1.5, n)
+    Hd = ref.dense_hamiltonian(n, ref.tfi_terms_random(t, Js, hs))
+    assert np.abs(ref.ttno_to_dense(t, O.tfi_random_host(topo, Js, hs)) - Hd).max() < 1e-12
+
+
+def test_ttno_bond_dims():
+    topo = Tr.binary_topology(2)
+    assert O.heisenberg_host(topo)[1].shape[-1] == 5
+    assert O.tfi_host(topo)[1].shape[-1] == 3
+
+
+# --------------------------------------------------------------------------
+# states and environments
+# --------------------------------------------------------------------------
+
+
+def _setup(topo_fn=lambda: ref.star_tree(3, 2), D=3, seed=3, center=0):
+    t = topo_fn()
+    rng = np.random.default_rng(seed)
+    psi = ref.random_ttns(rng, t, 2, D, center=center)
+    phys = [True] * t.n
+    return t, psi, phys
+
+
+def test_random_ttns_canonical():
+    t, psi, phys = _setup()
+    v = ref.ttns_to_dense(t, psi, phys)
+    assert abs(np.linalg.norm(v) - 1) < 1e-12
+    for n in range(1, t.n):
+        A = psi[n].reshape(-1, psi[n].shape[-1], order="F")
+        assert np.allclose(A.conj().T @ A, np.eye(A.shape[1]), atol=1e-12)
+
+
+def test_move_center_preserves_state():
+    t, psi, phys = _setup()
+    v = ref.ttns_to_dense(t, psi, phys)
+    psi2 = ref.move_center(t, psi, 0, 5)
+    assert np.linalg.norm(ref.ttns_to_dense(t, psi2, phys) - v) < 1e-12
+    A = np.moveaxis(psi2[0], 2, -1)  # centre left node 0; node 0 must be isometric towards leg 2 (child b3_1)
+    M = A.reshape(-1, A.shape[-1], order="F")
+    assert np.allclose(M.conj().T @ M, np.eye(M.shape[1]), atol=1e-12)
+
+
+def test_env_closure_is_expectation():
+    t, psi, phys = _setup(D=4)
+    W = O.tfi_host(Tr.star_topology(3, 2), 1.0, 0.9)
+    Hd = ref.dense_hamiltonian(t.n, ref.tfi_terms(t, 1.0, 0.9))
+    v = ref.ttns_to_dense(t, psi, phys)
+    envs = ref.all_envs(t, psi, W)
+    e_dense = np.vdot(v, Hd @ v)
+    for u in range(t.n):
+        assert abs(ref.env_closure(t, psi, W, envs, u) - e_dense) < 1e-12
+
+
+def _matrix_of(apply, shape):
+    size = int(np.prod(shape))
+    cols = []
+    for j in range(size):
+        e = np.zeros(size, dtype=complex)
+        e[j] = 1
+        cols.append(apply(e.reshape(shape, order="F")).reshape(-1, order="F"))
+    return np.stack(cols, axis=1)
+
+
+@pytest.mark.parametrize("center", [0, 1, 2])
+def test_h1_tangent_identity(center):
+    t, psi, phys = _setup(D=3, center=center)
+    W = O.heisenberg_host(Tr.star_topology(3, 2), 1.0)
+    Hd = ref.dense_hamiltonian(t.n, ref.heisenberg_terms(t, 1.0))
+    envs = ref.all_envs(t, psi, W)
+    h = _matrix_of(lambda x: ref.h1_apply(t, W, envs, center, x), psi[center].shape)
+    P = ref.embedding_h1(t, psi, phys, center)
+    assert np.allclose(P.conj().T @ P, np.eye(P.shape[1]), atol=1e-12)
+    assert np.abs(h - P.conj().T @ Hd @ P).max() < 1e-12
+    assert np.abs(h - h.conj().T).max() < 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 2, 4])
+def test_h2_tangent_identity(n):
+    t, psi, phys = _setup(D=3, center=n)
+    m = t.parent[n]
+    W = O.tfi_host(Tr.star_topology(3, 2), 1.0, 0.9)
+    Hd = ref.dense_hamiltonian(t.n, ref.tfi_terms(t, 1.0, 0.9))
+    envs = ref.all_envs(t, psi, W)
+    theta = ref.two_site_tensor(t, psi, n, m)
+    h = _matrix_of(lambda x: ref.h2_apply(t, W, envs, n, m, x), theta.shape)
+    P = ref.embedding_h2(t, psi, phys, n, m)
+    assert np.allclose(P.conj().T @ P, np.eye(P.shape[1]), atol=1e-12)
+    assert np.abs(h - P.conj().T @ Hd @ P).max() < 1e-12
+    # the state itself: Theta is the embedding preimage of psi
+    v = ref.ttns_to_dense(t, psi, phys)
+    assert np.linalg.norm(P @ theta.reshape(-1, order="F") - v) < 1e-12
+
+
+def test_h2_binary_root_edge_identity():
+    t = ref.binary_tree(2)
+    rng = np.random.default_rng(4)
+    psi = ref.random_ttns(rng, t, 2, 3, center=1)
+    phys = [True] * t.n
+    W = O.heisenberg_host(Tr.binary_topology(2))
+    Hd = ref.dense_hamiltonian(t.n, ref.heisenberg_terms(t))
+    envs = ref.all_envs(t, psi, W)
+    theta = ref.two_site_tensor(t, psi, 1, 0)
+    h = _matrix_of(lambda x: ref.h2_apply(t, W, envs, 1, 0, x), theta.shape)
+    P = ref.embedding_h2(t, psi, phys, 1, 0)
+    assert np.abs(h - P.conj().T @ Hd @ P).max() < 1e-12
+
+
+def test_h0_tangent_identity():
+    t, psi, phys = _setup(D=3, center=1)
+    n, m = 1, 0
+    # split the centre at n: psi[n] = Q C, Q isometric towards its parent leg
+    Q, C = ref.left_orth(psi[n])
+    split = list(psi)
+    split[n] = Q
+    W = O.tfi_host(Tr.star_topology(3, 2), 1.0, 0.9)
+    Hd = ref.dense_hamiltonian(t.n, ref.tfi_terms(t, 1.0, 0.9))
+    envs = ref.all_envs(t, split, W)
+    h = _matrix_of(lambda x: ref.h0_apply(envs, n, m, x), C.shape)
+    P = ref.embedding_h0(t, split, phys, n, m)
+    assert np.allclose(P.conj().T @ P, np.eye(P.shape[1]), atol=1e-12)
+    assert np.abs(h - P.conj().T @ Hd @ P).max() < 1e-12
+
+
+# --------------------------------------------------------------------------
+# cost model known answers (lib/GraftPlanning/test/runtests.jl:121-204)
+# --------------------------------------------------------------------------
+
+
+def test_dense_cost_fixture():
+    m = ref.dense_cost([-1, 1], [2, 3], [1, -2], [3, 4])
+    assert m["oindA"] == [0] and m["cindA"] == [1] and m["oindB"] == [1] and m["cindB"] == [0]
+    assert m["labels"] == [-1, -2] and m["dims"] == [2, 4] and m["peak_elements"] == 8
+    assert m["flops"] > 0
+
+
+def test_u1_chain_known_answers():
+    left = ({0: 1, 1: 1}, {0: 1, 1: 2})
+    middle = ({0: 1, 1: 2}, {0: 1, 1: 4})
+    right = ({0: 1, 1: 4}, {0: 2, 1: 1})
+    assert ref.u1_chain_plan_costs([left, middle, right], "left") == (60.0, 30.0)
+    assert ref.u1_chain_plan_costs([left, middle, right], "right") == (63.0, 28.0)
+
+
+def test_execute_pair_steps_equals_ncon():
+    rng = np.random.default_rng(9)
+    A = ref.random_complex(rng, (3, 4, 5))
+    B = ref.random_complex(rng, (5, 6, 3))
+    C = ref.random_complex(rng, (6, 2))
+    want = ref.ncon([A, B, C], [[1, -1, 2], [2, 3, 1], [3, -2]], [False, True, False])
+    # (A . conj B) then . C
+    steps = [(0, 1, 3, (1,), (0, 2), (1,), (2, 0), False, True, (0, 1)),
+             (3, 2, 4, (0,), (1,), (1,), (0,), False, False, (0, 1))]
+    got = ref.execute_pair_steps(steps, [A, B, C], 5, 4)
+    assert np.abs(got - want).max() < 1e-12
+
+
+def test_exact_evolve_unitary_and_groundstate():
+    t = ref.chain_tree(4)
+    H = ref.dense_hamiltonian(4, ref.tfi_terms(t, 1.0, 0.9))
+    E0, v0 = ref.exact_groundstate(H)
+    assert np.linalg.norm(H @ v0 - E0 * v0) < 1e-10
+    rng = np.random.default_rng(2)
+    v = ref.random_complex(rng, 16)
+    v /= np.linalg.norm(v)
+    w = ref.exact_evolve(H, v, -0.3j)
+    assert abs(np.linalg.norm(w) - 1) < 1e-12
+    back = ref.exact_evolve(H, w, 0.3j)
+    assert np.linalg.norm(back - v) < 1e-12
