@@ -166,7 +166,8 @@ class EnvCache:
         if any(op.blocks is not None for op in operands):
             legs = [_legs_of(op) for op in operands]
         plan = plan_contraction(spec, dims, legs, optimize=optimize, memory_weight=memory_weight,
-                                sector_aware=sector_aware, memory_cap_bytes=memory_cap_bytes)
+                                sector_aware=sector_aware, memory_cap_bytes=memory_cap_bytes,
+                                fused_permutes=True)
         out_dims = slot_dims(plan, dims)[plan.output_slot]
         p = NativePlan(plan, operands, out_structure, out_dims)
         self.plans[key] = p

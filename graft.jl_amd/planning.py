@@ -407,8 +407,13 @@ def _is_identity(order, n):
 
 
 def compile_plan(tree, spec, dims, legs=None, strategy="env_first", canonical=False,
-                 scalar_bytes=16):
-    """Walk a binary tree into PairSteps with dense and sector metrics."""
+                 scalar_bytes=16, fused_permutes=False):
+    """Walk a binary tree into PairSteps with dense and sector metrics.
+
+    ``fused_permutes=True`` models an executor that folds every permutation
+    and conjugation into its GEMM loads/stores (the device engine): no
+    transform temporaries are charged to the live-memory model.  False is the
+    reference's TensorOperations model (cost.jl:82-122)."""
     n = len(dims)
     structural = legs is not None
     st = dict(nextslot=n - 1, steps=[], flops=0.0, peak=max(_prod(d) for d in dims),
@@ -463,8 +468,13 @@ def compile_plan(tree, spec, dims, legs=None, strategy="env_first", canonical=Fa
         if not _is_identity(flat, len(m["labels"])):
             perm_d += dense_out
             perm_s += sector_out
-        st["tmp_dense"].append(perm_d + (dna if ca else 0.0) + (dnb if cb else 0.0))
-        st["tmp_sector"].append(perm_s + (sca if ca else 0.0) + (scb if cb else 0.0))
+        if fused_permutes:
+            perm_d = perm_s = 0.0
+            st["tmp_dense"].append(0.0)
+            st["tmp_sector"].append(0.0)
+        else:
+            st["tmp_dense"].append(perm_d + (dna if ca else 0.0) + (dnb if cb else 0.0))
+            st["tmp_sector"].append(perm_s + (sca if ca else 0.0) + (scb if cb else 0.0))
         st["perm_dense"].append(perm_d)
         st["perm_sector"].append(perm_s)
         st["steps"].append(PairStep(a, b, dst, tuple(m["oindA"]), tuple(m["cindA"]),
@@ -545,11 +555,12 @@ def _sector_nontrivial(legs_list):
 
 
 def plan_contraction(spec, dims, legs=None, optimize=True, memory_weight=1.0, sector_aware=True,
-                     scalar_bytes=16, memory_cap_bytes=None):
+                     scalar_bytes=16, memory_cap_bytes=None, fused_permutes=False):
     """plan_contraction — planner.jl:584-700.
 
     ``dims[i]`` are the flat leg dimensions of slot i; ``legs[i]`` (optional)
-    its abelian SectorLeg list."""
+    its abelian SectorLeg list.  ``fused_permutes`` selects the device
+    executor's live-memory model (see ``compile_plan``)."""
     if not (math.isfinite(memory_weight) and memory_weight >= 0):
         raise ValueError("memory_weight must be finite and nonnegative")
     cap = math.inf if memory_cap_bytes is None else float(memory_cap_bytes)
@@ -564,7 +575,7 @@ def plan_contraction(spec, dims, legs=None, optimize=True, memory_weight=1.0, se
     structural = optimize and legs is not None and _sector_nontrivial(legs)
     slegs = legs if structural else None
     heuristic = compile_plan(_heuristic_tree(spec), spec, dims, slegs, "env_first",
-                             scalar_bytes=scalar_bytes)
+                             scalar_bytes=scalar_bytes, fused_permutes=fused_permutes)
     if not optimize:
         if not _within_cap(heuristic, cap):
             raise ValueError("env-first contraction plan requires at least "
@@ -576,17 +587,17 @@ def plan_contraction(spec, dims, legs=None, optimize=True, memory_weight=1.0, se
     if use_sector:
         for t in _sector_dp_trees(spec, dims, legs):
             candidates.append(compile_plan(t, spec, dims, slegs, "sector_exact", canonical=True,
-                                           scalar_bytes=scalar_bytes))
+                                           scalar_bytes=scalar_bytes, fused_permutes=fused_permutes))
     if len(spec.labels) <= EXACT_TENSOR_LIMIT:
         t = _optimal_tree(spec, dims)
         if t is not None:
             candidates.append(compile_plan(t, spec, dims, slegs, "dense_optimal",
-                                           scalar_bytes=scalar_bytes))
+                                           scalar_bytes=scalar_bytes, fused_permutes=fused_permutes))
         candidates.append(compile_plan(_greedy_tree(spec, dims), spec, dims, slegs, "memory_greedy",
-                                       scalar_bytes=scalar_bytes))
+                                       scalar_bytes=scalar_bytes, fused_permutes=fused_permutes))
     else:
         candidates.append(compile_plan(_greedy_tree(spec, dims), spec, dims, slegs, "dense_greedy",
-                                       scalar_bytes=scalar_bytes))
+                                       scalar_bytes=scalar_bytes, fused_permutes=fused_permutes))
     best, best_score = None, math.inf
     for c in candidates:
         if not _within_floors(c, heuristic) or not _within_cap(c, cap):
