@@ -89,6 +89,7 @@ def lib():
         "graft_plan_step_info": [vp, i32, ctypes.POINTER(i64), ctypes.POINTER(i64), dp,
                                  ctypes.POINTER(i32)],
         "graft_plan_set_tile_config": [vp, i32, i32],
+        "graft_plan_set_orientation": [vp, i32, i32],
         "graft_plan_autotune": [vp, vp, pp, vp, i32, i32],
         "graft_plan_execute": [vp, vp, pp, vp, i32],
         "graft_plan_execute_timed": [vp, vp, pp, vp, i32, ctypes.POINTER(ctypes.c_float)],

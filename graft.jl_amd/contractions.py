@@ -92,6 +92,10 @@ class NativePlan:
     def set_tile_config(self, step, config):
         nat.check(nat.lib().graft_plan_set_tile_config(self.handle, step, config))
 
+    def set_orientation(self, step, orientation):
+        """-1 automatic, 0 normal, 1 transposed MFMA orientation."""
+        nat.check(nat.lib().graft_plan_set_orientation(self.handle, step, orientation))
+
     def close(self):
         if getattr(self, "handle", None) is not None and nat._lib is not None:
             nat.lib().graft_plan_destroy(self.handle)

@@ -49,10 +49,14 @@ constexpr int kNumTileConfigs = 7;
 extern const TileConfig kTileConfigs[kNumTileConfigs];
 
 // Launch one grouped strided complex GEMM (precision GRAFT_PREC_*).
+// ksplit > 1 (single-block items only) splits K over ksplit blocks per
+// tile; partial must hold ksplit * M * N complex values.  tr selects the
+// transposed MFMA orientation (coalesced stores when C's unit-stride leg is
+// an m leg).
 hipError_t launch_zgemm(int config, int precision, const void* A, const void* B, void* C,
                         const GemmItem* items, int nitems, const GemmSeg* segs,
                         const uint32_t* koff, int total_tiles, int conjA, int conjB,
-                        hipStream_t stream);
+                        int ksplit, void* partial, bool tr, hipStream_t stream);
 int tiles_for(int config, int M, int N);
 
 // Krylov vector kernels (vector_kernels.hip).

@@ -151,6 +151,11 @@ struct StepExec {
   GemmSeg* d_segs = nullptr;
   uint32_t* d_koff = nullptr;
   int total_tiles = 0;
+  int ksplit = 1;             // split-K factor (single-block steps with few tiles)
+  bool tr = false;            // transposed MFMA orientation in use
+  bool tr_auto = false;       // orientation from the output's unit-stride leg
+  int tr_pinned = -1;         // -1 = automatic
+  uint64_t partial_bytes = 0; // workspace for the split-K partial tiles
 };
 
 struct graft_plan_impl {
@@ -158,7 +163,8 @@ struct graft_plan_impl {
   std::vector<TStruct> slots;
   std::vector<StepExec> steps;
   std::vector<int64_t> ws_offset;  // per slot byte offset in workspace (-1 = not in workspace)
-  uint64_t ws_bytes = 0;
+  uint64_t ws_slots = 0;           // bytes of the intermediate slots
+  uint64_t ws_bytes = 0;           // slots + split-K partial region
   double macs = 0.0;
   void* d_meta = nullptr;  // one allocation holding every step's items/segs/koff
   int device = 0;
@@ -462,8 +468,23 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
     it.seg_count = (int)ex.segs.size() - it.seg_begin;
     ex.items.push_back(it);
   }
+  // Orientation: decided by the largest output block's unit-stride leg.
+  {
+    size_t big = 0;
+    for (size_t i = 1; i < ex.items.size(); ++i)
+      if ((int64_t)ex.items[i].M * ex.items[i].N > (int64_t)ex.items[big].M * ex.items[big].N) big = i;
+    const GemmItem& it = ex.items[big];
+    uint32_t mm = UINT32_MAX, mn = UINT32_MAX;
+    for (int l = 0; l < it.mrank; ++l) mm = std::min(mm, it.c_mstr[l]);
+    for (int l = 0; l < it.nrank; ++l) mn = std::min(mn, it.c_nstr[l]);
+    ex.tr_auto = mm < mn;
+    ex.tr = ex.tr_auto;
+  }
   return 0;
 }
+
+// Target number of workgroups for one launch (two per CU on 256 CUs).
+static constexpr int kTargetBlocks = 512;
 
 static void assign_tiles(StepExec& ex) {
   const TileConfig& c = kTileConfigs[ex.config];
@@ -474,6 +495,26 @@ static void assign_tiles(StepExec& ex) {
     t += ((it.M + c.bm - 1) / c.bm) * it.tiles_n;
   }
   ex.total_tiles = t;
+  // Split K when one output block has too few tiles to fill the chip and a
+  // long contraction (environment closures contract D*D*d legs at once).
+  ex.ksplit = 1;
+  ex.partial_bytes = 0;
+  if (ex.items.size() == 1 && ex.segs.size() == 1 && t > 0 && t < kTargetBlocks) {
+    const int64_t K = ex.segs[0].K;
+    const int64_t ktiles = (K + c.bk - 1) / c.bk;
+    int64_t ks = (kTargetBlocks + t - 1) / t;
+    ks = std::min<int64_t>(ks, ktiles / 8);  // keep at least 8 k-tiles per chunk
+    if (ks >= 2) {
+      ex.ksplit = (int)ks;
+      ex.partial_bytes = (uint64_t)ks * ex.items[0].M * ex.items[0].N * 16;
+    }
+  }
+}
+
+static void update_workspace_size(graft_plan_impl& p) {
+  uint64_t part = 0;
+  for (auto& ex : p.steps) part = std::max(part, ex.partial_bytes);
+  p.ws_bytes = ((p.ws_slots + 255) / 256) * 256 + part;
 }
 
 static int upload_step_meta(graft_plan_impl& p) {
@@ -549,9 +590,10 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     set_error("null operand pointer");
     return GRAFT_E_INVALID;
   }
+  void* partial = ex.ksplit > 1 ? (char*)ctx->ws + ((p.ws_slots + 255) / 256) * 256 : nullptr;
   hipError_t e = launch_zgemm(ex.config, precision, A, B, C, ex.d_items, (int)ex.items.size(),
                               ex.d_segs, ex.d_koff, ex.total_tiles, ex.conjA, ex.conjB,
-                              ctx->stream);
+                              ex.ksplit, partial, ex.tr, ctx->stream);
   if (e == hipErrorNotSupported) {
     set_error("precision not supported by this build");
     return GRAFT_E_UNSUPPORTED;
@@ -778,7 +820,8 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
     p.ws_offset[s] = off;
     peak = std::max<uint64_t>(peak, off + size);
   }
-  p.ws_bytes = peak;
+  p.ws_slots = peak;
+  update_workspace_size(p);
   rc = upload_step_meta(p);
   if (rc) return rc;
   *out = plan.release();
@@ -830,7 +873,18 @@ int graft_plan_set_tile_config(graft_plan* plan, int32_t step, int32_t config) {
   ex.pinned = config;
   ex.config = config >= 0 ? config : choose_config(ex.maxM, ex.maxN);
   assign_tiles(ex);
+  update_workspace_size(plan->impl);
   return upload_step_meta(plan->impl);
+}
+
+int graft_plan_set_orientation(graft_plan* plan, int32_t step, int32_t orientation) {
+  GRAFT_REQUIRE(plan != nullptr, "null plan");
+  GRAFT_REQUIRE(step >= 0 && step < (int32_t)plan->impl.steps.size(), "step out of range");
+  GRAFT_REQUIRE(orientation >= -1 && orientation <= 1, "orientation must be -1, 0 or 1");
+  StepExec& ex = plan->impl.steps[step];
+  ex.tr_pinned = orientation;
+  ex.tr = orientation < 0 ? ex.tr_auto : orientation == 1;
+  return GRAFT_OK;
 }
 
 int graft_plan_execute(graft_ctx* ctx, graft_plan* plan, const void* const* inputs, void* output,
@@ -875,31 +929,43 @@ int graft_plan_autotune(graft_ctx* ctx, graft_plan* plan, const void* const* inp
   GRAFT_CHECK_HIP(hipEventCreate(&e0));
   GRAFT_CHECK_HIP(hipEventCreate(&e1));
   for (auto& ex : p.steps) {
-    if (ex.pinned >= 0) continue;
+    if (ex.pinned >= 0 && ex.tr_pinned >= 0) continue;
     int best = ex.config;
+    bool best_tr = ex.tr;
     float best_ms = 1e30f;
-    for (int c = 0; c < kNumTileConfigs; ++c) {
+    for (int c = 0; c < kNumTileConfigs && !rc; ++c) {
+      if (ex.pinned >= 0 && c != ex.pinned) continue;
       ex.config = c;
       assign_tiles(ex);
+      update_workspace_size(p);
+      rc = ensure_workspace(ctx, p.ws_bytes);
+      if (rc) break;
       rc = upload_step_meta(p);
       if (rc) break;
-      // warm-up, then time
-      rc = run_step(ctx, p, ex, inputs, output, precision);
-      if (rc) break;
-      GRAFT_CHECK_HIP(hipEventRecord(e0, ctx->stream));
-      for (int r = 0; r < repeats && !rc; ++r) rc = run_step(ctx, p, ex, inputs, output, precision);
-      GRAFT_CHECK_HIP(hipEventRecord(e1, ctx->stream));
-      GRAFT_CHECK_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      GRAFT_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (!rc && ms < best_ms) {
-        best_ms = ms;
-        best = c;
+      for (int o = 0; o < 2 && !rc; ++o) {
+        if (ex.tr_pinned >= 0 && o != ex.tr_pinned) continue;
+        ex.tr = (o == 1);
+        // warm-up, then time
+        rc = run_step(ctx, p, ex, inputs, output, precision);
+        if (rc) break;
+        GRAFT_CHECK_HIP(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < repeats && !rc; ++r) rc = run_step(ctx, p, ex, inputs, output, precision);
+        GRAFT_CHECK_HIP(hipEventRecord(e1, ctx->stream));
+        GRAFT_CHECK_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        GRAFT_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (!rc && ms < best_ms) {
+          best_ms = ms;
+          best = c;
+          best_tr = ex.tr;
+        }
       }
     }
     if (rc) break;
     ex.config = best;
+    ex.tr = best_tr;
     assign_tiles(ex);
+    update_workspace_size(p);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -49,12 +49,16 @@ __device__ __forceinline__ uint32_t decode_offset(uint32_t idx, int rank, const 
   return off;
 }
 
-template <int BM, int BN, int BK, int WM, int WN>
+// TR (transposed output orientation): issue the MFMAs as B^T A^T so that the
+// 16 lanes of an accumulator column walk the m index.  Chosen when the
+// output's unit-stride leg is an m leg, which makes the stores coalesced.
+template <int BM, int BN, int BK, int WM, int WN, bool TR>
 __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
     zgemm_f64_kernel(const double2* __restrict__ Abase, const double2* __restrict__ Bbase,
                      double2* __restrict__ Cbase, const GemmItem* __restrict__ items, int nitems,
                      const GemmSeg* __restrict__ segs, const uint32_t* __restrict__ koff,
-                     int total_tiles, int conjA, int conjB) {
+                     int total_tiles, int conjA, int conjB, int ksplit,
+                     double2* __restrict__ partial) {
   constexpr int NWN = BN / WN;
   constexpr int NT = 64 * (BM / WM) * NWN;
   constexpr int EA = BM * BK / NT;
@@ -82,12 +86,17 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
   // XCD-aware tile order: hardware dispatches block b to XCD b % 8, so give
   // each XCD a contiguous range of logical tiles (tiles sharing an A panel
   // then share that XCD's L2).
-  int tile;
+  // With split-K (ksplit > 1, single-block items only) each logical block
+  // handles one k-chunk of one tile and writes a partial tile.
+  int tile, kc = 0;
   {
+    const int nblk = total_tiles * ksplit;
     const int b = blockIdx.x;
     const int x = b & 7, i = b >> 3;
-    const int q = total_tiles >> 3, r = total_tiles & 7;
-    tile = x * q + (x < r ? x : r) + i;
+    const int q = nblk >> 3, r = nblk & 7;
+    const int lin = x * q + (x < r ? x : r) + i;
+    tile = lin / ksplit;
+    kc = lin - tile * ksplit;
   }
   int lo = 0, hi = nitems - 1;
   while (lo < hi) {
@@ -123,6 +132,13 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
     const uint32_t* kB = kA + K;
     const bool akf = sg->a_kfast != 0;
     const bool bkf = sg->b_kfast != 0;
+    // k range of this block: all of K, or one BK-aligned chunk under split-K.
+    int kbeg = 0, kend = K;
+    if (ksplit > 1) {
+      const int chunk = ((K + BK - 1) / BK + ksplit - 1) / ksplit * BK;
+      kbeg = kc * chunk;
+      kend = min(K, kbeg + chunk);
+    }
 
     // Row offsets of this thread's A elements and column offsets of its B elements.
     uint32_t rowA[EA], colB[EB];
@@ -153,28 +169,28 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
     auto fetch = [&](int k0) {
       if (akf) {
         const int q = k0 + tid % BK;
-        const uint32_t ko = (q < K) ? kA[q] : 0u;
+        const uint32_t ko = (q < kend) ? kA[q] : 0u;
 #pragma unroll
         for (int e = 0; e < EA; ++e)
-          ra[e] = (q < K && rowA[e] != kBadRow) ? A[(uint64_t)rowA[e] + ko] : double2{0.0, 0.0};
+          ra[e] = (q < kend && rowA[e] != kBadRow) ? A[(uint64_t)rowA[e] + ko] : double2{0.0, 0.0};
       } else {
 #pragma unroll
         for (int e = 0; e < EA; ++e) {
           const int q = k0 + tid / BM + e * (NT / BM);
-          ra[e] = (q < K && rowA[0] != kBadRow) ? A[(uint64_t)rowA[0] + kA[q]] : double2{0.0, 0.0};
+          ra[e] = (q < kend && rowA[0] != kBadRow) ? A[(uint64_t)rowA[0] + kA[q]] : double2{0.0, 0.0};
         }
       }
       if (bkf) {
         const int q = k0 + tid % BK;
-        const uint32_t ko = (q < K) ? kB[q] : 0u;
+        const uint32_t ko = (q < kend) ? kB[q] : 0u;
 #pragma unroll
         for (int e = 0; e < EB; ++e)
-          rb[e] = (q < K && colB[e] != kBadRow) ? B[(uint64_t)colB[e] + ko] : double2{0.0, 0.0};
+          rb[e] = (q < kend && colB[e] != kBadRow) ? B[(uint64_t)colB[e] + ko] : double2{0.0, 0.0};
       } else {
 #pragma unroll
         for (int e = 0; e < EB; ++e) {
           const int q = k0 + tid / BN + e * (NT / BN);
-          rb[e] = (q < K && colB[0] != kBadRow) ? B[(uint64_t)colB[0] + kB[q]] : double2{0.0, 0.0};
+          rb[e] = (q < kend && colB[0] != kBadRow) ? B[(uint64_t)colB[0] + kB[q]] : double2{0.0, 0.0};
         }
       }
     };
@@ -215,9 +231,9 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
       }
     };
 
-    const int nkt = (K + BK - 1) / BK;
-    fetch(0);
-    for (int kt = 0; kt < nkt; ++kt) {
+    const int nkt = (kend + BK - 1) / BK;
+    fetch(kbeg);
+    for (int kt = kbeg / BK; kt < nkt; ++kt) {
       __syncthreads();
       stage();
       __syncthreads();
@@ -240,35 +256,81 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
         for (int a = 0; a < FM; ++a)
 #pragma unroll
           for (int b = 0; b < FN; ++b) {
-            accR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bre[b], accR[a][b], 0, 0, 0);
-            accI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bim[b], accI[a][b], 0, 0, 0);
+            if constexpr (TR) {
+              accR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bre[b], av[a], accR[a][b], 0, 0, 0);
+              accI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bim[b], av[a], accI[a][b], 0, 0, 0);
+            } else {
+              accR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bre[b], accR[a][b], 0, 0, 0);
+              accI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bim[b], accI[a][b], 0, 0, 0);
+            }
           }
       }
     }
   }
 
   // Epilogue: v_mfma_f64_16x16x4f64 leaves D[kr + 4*r][li] in accumulator
-  // element r of lane (16*kr + li).
-  double2* C = Cbase + it->c_off;
-  uint32_t coln[FN];
+  // element r of lane (16*kr + li); D is the (m, n) block, or (n, m) under TR.
+  auto m_of = [&](int a, int r) { return m0 + wm0 + a * 16 + (TR ? li : kr + 4 * r); };
+  auto n_of = [&](int b, int r) { return n0 + wn0 + b * 16 + (TR ? kr + 4 * r : li); };
+  if (ksplit > 1) {
+    // partial tile of k-chunk kc, element (m, n) at partial[(kc * N + n) * M + m]
+    double2* P = partial + (size_t)kc * N * M;
 #pragma unroll
-  for (int b = 0; b < FN; ++b) {
-    const int n = n0 + wn0 + b * 16 + li;
-    coln[b] = (n < N) ? decode_offset((uint32_t)n, nrank, it->ndims, it->c_nstr) : kBadRow;
-  }
+    for (int a = 0; a < FM; ++a)
 #pragma unroll
-  for (int a = 0; a < FM; ++a)
+      for (int r = 0; r < 4; ++r) {
+        const int m = m_of(a, r);
+        if (m >= M) continue;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int m = m0 + wm0 + a * 16 + kr + 4 * r;
-      if (m >= M) continue;
-      const uint32_t ro = decode_offset((uint32_t)m, mrank, it->mdims, it->c_mstr);
-#pragma unroll
-      for (int b = 0; b < FN; ++b) {
-        if (coln[b] == kBadRow) continue;
-        C[(uint64_t)ro + coln[b]] = double2{accR[a][b][r], accI[a][b][r]};
+        for (int b = 0; b < FN; ++b) {
+          const int n = n_of(b, r);
+          if (n < N) P[(size_t)n * M + m] = double2{accR[a][b][r], accI[a][b][r]};
+        }
       }
+    return;
+  }
+  double2* C = Cbase + it->c_off;
+  if constexpr (TR) {
+    uint32_t rowm[FM];
+#pragma unroll
+    for (int a = 0; a < FM; ++a) {
+      const int m = m_of(a, 0);
+      rowm[a] = (m < M) ? decode_offset((uint32_t)m, mrank, it->mdims, it->c_mstr) : kBadRow;
     }
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n_of(b, r);
+        if (n >= N) continue;
+        const uint32_t co = decode_offset((uint32_t)n, nrank, it->ndims, it->c_nstr);
+#pragma unroll
+        for (int a = 0; a < FM; ++a) {
+          if (rowm[a] == kBadRow) continue;
+          C[(uint64_t)rowm[a] + co] = double2{accR[a][b][r], accI[a][b][r]};
+        }
+      }
+  } else {
+    uint32_t coln[FN];
+#pragma unroll
+    for (int b = 0; b < FN; ++b) {
+      const int n = n_of(b, 0);
+      coln[b] = (n < N) ? decode_offset((uint32_t)n, nrank, it->ndims, it->c_nstr) : kBadRow;
+    }
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m_of(a, r);
+        if (m >= M) continue;
+        const uint32_t ro = decode_offset((uint32_t)m, mrank, it->mdims, it->c_mstr);
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+          if (coln[b] == kBadRow) continue;
+          C[(uint64_t)ro + coln[b]] = double2{accR[a][b][r], accI[a][b][r]};
+        }
+      }
+  }
 }
 
 int tiles_for(int config, int M, int N) {
@@ -279,28 +341,80 @@ int tiles_for(int config, int M, int N) {
 template <int BM, int BN, int BK, int WM, int WN>
 static hipError_t launch_f64(const void* A, const void* B, void* C, const GemmItem* items,
                              int nitems, const GemmSeg* segs, const uint32_t* koff,
-                             int total_tiles, int conjA, int conjB, hipStream_t stream) {
+                             int total_tiles, int conjA, int conjB, int ksplit, void* partial,
+                             bool tr, hipStream_t stream) {
   constexpr int NT = 64 * (BM / WM) * (BN / WN);
-  hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, BK, WM, WN>), dim3(total_tiles), dim3(NT), 0,
-                     stream, (const double2*)A, (const double2*)B, (double2*)C, items, nitems,
-                     segs, koff, total_tiles, conjA, conjB);
+  if (tr)
+    hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, BK, WM, WN, true>), dim3(total_tiles * ksplit),
+                       dim3(NT), 0, stream, (const double2*)A, (const double2*)B, (double2*)C,
+                       items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit,
+                       (double2*)partial);
+  else
+    hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, BK, WM, WN, false>), dim3(total_tiles * ksplit),
+                       dim3(NT), 0, stream, (const double2*)A, (const double2*)B, (double2*)C,
+                       items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit,
+                       (double2*)partial);
   return hipGetLastError();
+}
+
+hipError_t launch_zgemm_tiles(int config, const void* A, const void* B, void* C,
+                              const GemmItem* items, int nitems, const GemmSeg* segs,
+                              const uint32_t* koff, int total_tiles, int conjA, int conjB,
+                              int ksplit, void* partial, bool tr, hipStream_t stream);
+
+// Sum the k-chunk partials of a single-block item into its output layout.
+__global__ void __launch_bounds__(256)
+    splitk_reduce_kernel(const double2* __restrict__ partial, int ksplit,
+                         const GemmItem* __restrict__ item, double2* __restrict__ Cbase) {
+  const int M = item->M, N = item->N;
+  const int64_t total = (int64_t)M * N;
+  double2* C = Cbase + item->c_off;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < ksplit; ++s) {
+      const double2 v = partial[(size_t)s * total + e];
+      re += v.x;
+      im += v.y;
+    }
+    const int m = (int)(e % M), n = (int)(e / M);
+    const uint32_t off = decode_offset((uint32_t)m, item->mrank, item->mdims, item->c_mstr) +
+                         decode_offset((uint32_t)n, item->nrank, item->ndims, item->c_nstr);
+    C[off] = double2{re, im};
+  }
 }
 
 hipError_t launch_zgemm(int config, int precision, const void* A, const void* B, void* C,
                         const GemmItem* items, int nitems, const GemmSeg* segs,
                         const uint32_t* koff, int total_tiles, int conjA, int conjB,
-                        hipStream_t stream) {
+                        int ksplit, void* partial, bool tr, hipStream_t stream) {
   if (total_tiles <= 0) return hipSuccess;
   if (precision != GRAFT_PREC_FP64) return hipErrorNotSupported;
+  if (ksplit > 1) {
+    if (nitems != 1 || partial == nullptr) return hipErrorInvalidValue;
+    hipError_t e = launch_zgemm_tiles(config, A, B, C, items, nitems, segs, koff, total_tiles,
+                                      conjA, conjB, ksplit, partial, tr, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(1024), dim3(256), 0, stream,
+                       (const double2*)partial, ksplit, items, (double2*)C);
+    return hipGetLastError();
+  }
+  return launch_zgemm_tiles(config, A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB,
+                            1, nullptr, tr, stream);
+}
+
+hipError_t launch_zgemm_tiles(int config, const void* A, const void* B, void* C,
+                              const GemmItem* items, int nitems, const GemmSeg* segs,
+                              const uint32_t* koff, int total_tiles, int conjA, int conjB,
+                              int ksplit, void* partial, bool tr, hipStream_t stream) {
   switch (config) {
-    case 0: return launch_f64<128, 128, 16, 64, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, stream);
-    case 1: return launch_f64<128, 64, 16, 64, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, stream);
-    case 2: return launch_f64<64, 64, 16, 32, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, stream);
-    case 3: return launch_f64<256, 16, 16, 64, 16>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, stream);
-    case 4: return launch_f64<64, 128, 16, 32, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, stream);
-    case 5: return launch_f64<32, 32, 16, 16, 16>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, stream);
-    case 6: return launch_f64<16, 256, 16, 16, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, stream);
+    case 0: return launch_f64<128, 128, 16, 64, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+    case 1: return launch_f64<128, 64, 16, 64, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+    case 2: return launch_f64<64, 64, 16, 32, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+    case 3: return launch_f64<256, 16, 16, 64, 16>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+    case 4: return launch_f64<64, 128, 16, 32, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+    case 5: return launch_f64<32, 32, 16, 16, 16>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+    case 6: return launch_f64<16, 256, 16, 16, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
     default: return hipErrorInvalidValue;
   }
 }

@@ -443,6 +443,12 @@ def compile_plan(tree, spec, dims, legs=None, strategy="env_first", canonical=Fa
             out = _output_partition(m["labels"], spec)
         elif canonical:
             out = (tuple(_canonical_order(m["labels"])), ())
+        elif fused_permutes and _prod(db[i] for i in m["oindB"]) > _prod(da[i] for i in m["oindA"]):
+            # Device layout: the larger operand's open legs first, so the new
+            # intermediate keeps that operand's unit-stride leg and both the
+            # GEMM stores and the next step's loads stay coalesced.
+            na, nb = len(m["oindA"]), len(m["oindB"])
+            out = (tuple(range(na, na + nb)) + tuple(range(na)), ())
         else:
             out = (tuple(range(len(m["labels"]))), ())
         flat = list(out[0]) + list(out[1])

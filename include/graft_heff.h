@@ -140,8 +140,12 @@ int graft_plan_step_info(graft_plan* plan, int32_t step, int64_t* mkn3, int64_t*
 /* Pin the tile configuration of one step (-1 = heuristic). */
 int graft_plan_set_tile_config(graft_plan* plan, int32_t step, int32_t config);
 int graft_num_tile_configs(void);
-/* Time every tile configuration of every step on the given operands and
- * keep the fastest (the tile autotuner).  Operands are left unchanged. */
+/* Pin the MFMA orientation of one step: 0 = rows on lanes 16-group,
+ * 1 = transposed (B^T A^T), -1 = from the output's unit-stride leg. */
+int graft_plan_set_orientation(graft_plan* plan, int32_t step, int32_t orientation);
+/* Time every tile configuration and both MFMA orientations of every step on
+ * the given operands and keep the fastest (the tile autotuner).  Operands
+ * are left unchanged; the output holds the plan's result afterwards. */
 int graft_plan_autotune(graft_ctx* ctx, graft_plan* plan, const void* const* inputs,
                         void* output, int32_t precision, int32_t repeats);
 int graft_plan_execute(graft_ctx* ctx, graft_plan* plan, const void* const* inputs,

@@ -51,6 +51,9 @@ PAIR_CASES = [
     ((2, 300, 9), (9,), (2,), (0,), False, False, 5),
     ((1, 1, 6), (1, 6, 1), (0, 2), (0, 1), False, False, 6),
     ((3, 2, 2, 5), (3, 2, 5, 2, 5), (0, 1, 2), (0, 3, 1), False, True, 7),
+    # few output tiles and a long contraction: the split-K path
+    ((20, 600, 5), (5, 600, 30), (1, 2), (1, 0), True, False, 8),
+    ((300, 40, 9), (40, 9, 7), (1, 2), (0, 1), False, False, 9),
 ]
 
 
