@@ -102,7 +102,8 @@ def main():
     ap.add_argument("--D", type=int, default=256)
     ap.add_argument("--cpu-D", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the heuristic tile configurations (default: autotune every step)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -116,14 +117,16 @@ def main():
     t_setup = time.perf_counter()
     g, topo, psi, H, cache, h2, x = build_workload(args.D)
     native = h2.native
-    if args.autotune:
+    if not args.no_autotune:
+        t_at = time.perf_counter()
         h2.autotune(x)
+        print(f"# autotune {time.perf_counter() - t_at:.1f} s", file=sys.stderr, flush=True)
     print(f"# setup {time.perf_counter() - t_setup:.1f} s; plan {native.plan.strategy}, "
           f"{len(native.plan.steps)} steps, workspace {native.workspace_bytes / 2**30:.1f} GiB",
           file=sys.stderr, flush=True)
     for i, s in enumerate(native.step_info()):
-        print(f"#   step {i}: m={s['m']} k={s['k']} n={s['n']} MACs={s['macs']:.3e} tile={s['config']}",
-              file=sys.stderr, flush=True)
+        print(f"#   step {i}: m={s['m']} k={s['k']} n={s['n']} MACs={s['macs']:.3e} tile={s['config']} "
+              f"orient={s['orientation']} ksplit={s['ksplit']}", file=sys.stderr, flush=True)
 
     y = None
     for _ in range(args.warmup):

@@ -63,8 +63,12 @@ class NativePlan:
             cfg = ctypes.c_int32()
             nat.check(nat.lib().graft_plan_step_info(self.handle, i, mkn, ctypes.byref(nb),
                                                      ctypes.byref(macs), ctypes.byref(cfg)))
+            orient = ctypes.c_int32()
+            ks = ctypes.c_int32()
+            nat.check(nat.lib().graft_plan_step_launch(self.handle, i, ctypes.byref(cfg),
+                                                       ctypes.byref(orient), ctypes.byref(ks)))
             out.append(dict(m=mkn[0], k=mkn[1], n=mkn[2], blocks=nb.value, macs=macs.value,
-                            config=cfg.value))
+                            config=cfg.value, orientation=orient.value, ksplit=ks.value))
         return out
 
     def execute(self, operands, out, precision=nat.PREC_FP64):

@@ -44,8 +44,9 @@ struct GemmItem {
 // Tile configurations of the complex-FP64 MFMA GEMM.
 struct TileConfig {
   int bm, bn, bk, wm, wn;
+  int m3;  // three real products per complex product (Karatsuba form)
 };
-constexpr int kNumTileConfigs = 7;
+constexpr int kNumTileConfigs = 10;
 extern const TileConfig kTileConfigs[kNumTileConfigs];
 
 // Launch one grouped strided complex GEMM (precision GRAFT_PREC_*).
@@ -58,6 +59,22 @@ hipError_t launch_zgemm(int config, int precision, const void* A, const void* B,
                         const uint32_t* koff, int total_tiles, int conjA, int conjB,
                         int ksplit, void* partial, bool tr, hipStream_t stream);
 int tiles_for(int config, int M, int N);
+// FP32 / BF16x3 products with the same addressing (zgemm_lowp_kernels.hip).
+hipError_t launch_zgemm_lowp(int config, int precision, const void* A, const void* B, void* C,
+                             const GemmItem* items, int nitems, const GemmSeg* segs,
+                             const uint32_t* koff, int total_tiles, int conjA, int conjB,
+                             int ksplit, void* partial, bool tr, hipStream_t stream);
+
+// Streaming kernel for skinny steps (skinny_kernels.hip).  It is selected as
+// the virtual tile configuration kSkinnyConfig when one operand's open
+// extent is <= kSkinnyMaxS and K <= kSkinnyMaxK.
+constexpr int kSkinnyConfig = kNumTileConfigs;
+constexpr int kSkinnyMaxK = 16;
+constexpr int kSkinnyMaxS = 64;
+hipError_t launch_zskinny(bool big_is_a, const void* A, const void* B, void* C,
+                          const GemmItem* item, const GemmSeg* seg, const uint32_t* koff, int grid,
+                          int conjA, int conjB, hipStream_t stream);
+int skinny_grid(int64_t J);
 
 // Krylov vector kernels (vector_kernels.hip).
 hipError_t launch_dotc_partials(int64_t n, const void* x, const void* y, double* partials,

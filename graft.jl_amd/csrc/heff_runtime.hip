@@ -155,6 +155,7 @@ struct StepExec {
   bool tr = false;            // transposed MFMA orientation in use
   bool tr_auto = false;       // orientation from the output's unit-stride leg
   int tr_pinned = -1;         // -1 = automatic
+  int skinny = 0;             // streaming kernel eligible: 1 = A is the large operand, 2 = B
   uint64_t partial_bytes = 0; // workspace for the split-K partial tiles
 };
 
@@ -223,7 +224,9 @@ static void merge_group(std::vector<int64_t>& dims, std::vector<std::vector<int6
   for (size_t t = 0; t < strides.size(); ++t) *strides[t] = ns[t];
 }
 
-static int choose_config(int64_t M, int64_t N) {
+static int choose_config(int64_t M, int64_t N, int skinny) {
+  if (skinny == 1 && M >= 16384) return kSkinnyConfig;
+  if (skinny == 2 && N >= 16384) return kSkinnyConfig;
   if (N <= 16) return 3;
   if (M <= 16) return 6;
   if (M <= 32 && N <= 32) return 5;
@@ -480,6 +483,15 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
     ex.tr_auto = mm < mn;
     ex.tr = ex.tr_auto;
   }
+  // Streaming-kernel eligibility: one dense block, short K, one small open side.
+  ex.skinny = 0;
+  if (ex.items.size() == 1 && ex.segs.size() == 1 && ex.segs[0].K <= kSkinnyMaxK) {
+    const GemmItem& it = ex.items[0];
+    if (it.N <= kSkinnyMaxS && it.N * ex.segs[0].K <= kSkinnyMaxK * kSkinnyMaxS && it.M > it.N)
+      ex.skinny = 1;
+    else if (it.M <= kSkinnyMaxS && it.N > it.M)
+      ex.skinny = 2;
+  }
   return 0;
 }
 
@@ -487,6 +499,13 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
 static constexpr int kTargetBlocks = 512;
 
 static void assign_tiles(StepExec& ex) {
+  if (ex.config == kSkinnyConfig) {
+    const GemmItem& it = ex.items[0];
+    ex.total_tiles = skinny_grid(ex.skinny == 1 ? it.M : it.N);
+    ex.ksplit = 1;
+    ex.partial_bytes = 0;
+    return;
+  }
   const TileConfig& c = kTileConfigs[ex.config];
   int t = 0;
   for (auto& it : ex.items) {
@@ -589,6 +608,13 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
   if (!A || !B || !C) {
     set_error("null operand pointer");
     return GRAFT_E_INVALID;
+  }
+  if (ex.config == kSkinnyConfig) {
+    // HBM-bound: the streaming kernel always multiplies in FP64, whatever
+    // precision the compute-bound steps use.
+    GRAFT_CHECK_HIP(launch_zskinny(ex.skinny == 1, A, B, C, ex.d_items, ex.d_segs, ex.d_koff,
+                                   ex.total_tiles, ex.conjA, ex.conjB, ctx->stream));
+    return GRAFT_OK;
   }
   void* partial = ex.ksplit > 1 ? (char*)ctx->ws + ((p.ws_slots + 255) / 256) * 256 : nullptr;
   hipError_t e = launch_zgemm(ex.config, precision, A, B, C, ex.d_items, (int)ex.items.size(),
@@ -791,7 +817,7 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
     defined[st.a] = (st.a < ninputs);  // inputs stay valid; intermediates are consumed
     defined[st.b] = (st.b < ninputs);
     StepExec& ex = p.steps[s];
-    ex.config = choose_config(ex.maxM, ex.maxN);
+    ex.config = choose_config(ex.maxM, ex.maxN, ex.skinny);
     assign_tiles(ex);
     p.macs += ex.macs;
   }
@@ -863,15 +889,30 @@ int graft_plan_step_info(graft_plan* plan, int32_t step, int64_t* mkn3, int64_t*
   return GRAFT_OK;
 }
 
-int graft_num_tile_configs(void) { return kNumTileConfigs; }
+int graft_plan_step_launch(graft_plan* plan, int32_t step, int32_t* config, int32_t* orientation,
+                           int32_t* ksplit) {
+  GRAFT_REQUIRE(plan != nullptr, "null plan");
+  GRAFT_REQUIRE(step >= 0 && step < (int32_t)plan->impl.steps.size(), "step out of range");
+  const StepExec& ex = plan->impl.steps[step];
+  if (config) *config = ex.config;
+  if (orientation) *orientation = ex.tr ? 1 : 0;
+  if (ksplit) *ksplit = ex.ksplit;
+  return GRAFT_OK;
+}
+
+int graft_num_tile_configs(void) { return kNumTileConfigs + 1; }
 
 int graft_plan_set_tile_config(graft_plan* plan, int32_t step, int32_t config) {
   GRAFT_REQUIRE(plan != nullptr, "null plan");
   GRAFT_REQUIRE(step >= 0 && step < (int32_t)plan->impl.steps.size(), "step out of range");
-  GRAFT_REQUIRE(config >= -1 && config < kNumTileConfigs, "tile configuration out of range");
+  GRAFT_REQUIRE(config >= -1 && config <= kNumTileConfigs, "tile configuration out of range");
   StepExec& ex = plan->impl.steps[step];
+  if (config == kSkinnyConfig && ex.skinny == 0) {
+    set_error("the streaming kernel does not apply to this step");
+    return GRAFT_E_UNSUPPORTED;
+  }
   ex.pinned = config;
-  ex.config = config >= 0 ? config : choose_config(ex.maxM, ex.maxN);
+  ex.config = config >= 0 ? config : choose_config(ex.maxM, ex.maxN, ex.skinny);
   assign_tiles(ex);
   update_workspace_size(plan->impl);
   return upload_step_meta(plan->impl);
@@ -933,8 +974,9 @@ int graft_plan_autotune(graft_ctx* ctx, graft_plan* plan, const void* const* inp
     int best = ex.config;
     bool best_tr = ex.tr;
     float best_ms = 1e30f;
-    for (int c = 0; c < kNumTileConfigs && !rc; ++c) {
+    for (int c = 0; c <= kNumTileConfigs && !rc; ++c) {
       if (ex.pinned >= 0 && c != ex.pinned) continue;
+      if (c == kSkinnyConfig && ex.skinny == 0) continue;
       ex.config = c;
       assign_tiles(ex);
       update_workspace_size(p);
@@ -944,6 +986,7 @@ int graft_plan_autotune(graft_ctx* ctx, graft_plan* plan, const void* const* inp
       if (rc) break;
       for (int o = 0; o < 2 && !rc; ++o) {
         if (ex.tr_pinned >= 0 && o != ex.tr_pinned) continue;
+        if (c == kSkinnyConfig && o == 1) continue;
         ex.tr = (o == 1);
         // warm-up, then time
         rc = run_step(ctx, p, ex, inputs, output, precision);

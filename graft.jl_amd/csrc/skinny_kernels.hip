@@ -1,0 +1,125 @@
+// Streaming kernel for the HBM-bound TTNO-absorb steps (small K, one small
+// open dimension): C[j, s] = sum_k big[j, k] * small[k, s] for every j of the
+// large operand.  The small operand (K x S <= 16 x 64 complex) is staged once
+// per workgroup in LDS and read as a broadcast; each thread owns one j,
+// gathers its K inputs (consecutive threads read consecutive addresses when
+// the large operand's first open leg has unit stride) and writes its S
+// outputs (coalesced when the output's first leg is that same leg).  It
+// consumes the same item/segment/koff description as the tiled MFMA GEMM.
+#include "heff_internal.h"
+
+namespace graft {
+
+static constexpr int kSkinnyThreads = 256;
+
+__device__ __forceinline__ uint32_t sk_decode(uint32_t idx, int rank, const int32_t* dims,
+                                              const uint32_t* strides) {
+  uint32_t off = 0;
+  for (int l = 0; l < rank; ++l) {
+    const uint32_t d = (uint32_t)dims[l];
+    const uint32_t q = idx / d;
+    off += (idx - q * d) * strides[l];
+    idx = q;
+  }
+  return off;
+}
+
+// BIG_IS_A: A is the large operand (j = m, s = n); otherwise B is (j = n, s = m).
+template <bool BIG_IS_A>
+__global__ void __launch_bounds__(kSkinnyThreads)
+    zskinny_f64_kernel(const double2* __restrict__ Abase, const double2* __restrict__ Bbase,
+                       double2* __restrict__ Cbase, const GemmItem* __restrict__ item,
+                       const GemmSeg* __restrict__ seg, const uint32_t* __restrict__ koff,
+                       int conjA, int conjB) {
+  __shared__ double2 sm[kSkinnyMaxK * kSkinnyMaxS];
+  __shared__ uint32_t cso[kSkinnyMaxS];
+  const int M = item->M, N = item->N, K = seg->K;
+  const int S = BIG_IS_A ? N : M;
+  const int64_t J = BIG_IS_A ? M : N;
+  const double2* A = Abase + seg->a_off;
+  const double2* B = Bbase + seg->b_off;
+  const uint32_t* kA = koff + seg->koff_idx;
+  const uint32_t* kB = kA + K;
+  double2* C = Cbase + item->c_off;
+  const double sgA = conjA ? -1.0 : 1.0, sgB = conjB ? -1.0 : 1.0;
+
+  // Stage the small operand as sm[k * S + s], conjugation applied.
+  for (int e = threadIdx.x; e < S * K; e += blockDim.x) {
+    const int s = e % S, k = e / S;
+    double2 v;
+    if (BIG_IS_A) {
+      v = B[sk_decode((uint32_t)s, item->nrank, item->ndims, seg->b_nstr) + kB[k]];
+      v.y *= sgB;
+    } else {
+      v = A[sk_decode((uint32_t)s, item->mrank, item->mdims, seg->a_mstr) + kA[k]];
+      v.y *= sgA;
+    }
+    sm[k * S + s] = v;
+  }
+  for (int s = threadIdx.x; s < S; s += blockDim.x)
+    cso[s] = BIG_IS_A ? sk_decode((uint32_t)s, item->nrank, item->ndims, item->c_nstr)
+                      : sk_decode((uint32_t)s, item->mrank, item->mdims, item->c_mstr);
+  __syncthreads();
+
+  const double sgBig = BIG_IS_A ? sgA : sgB;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < J;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t bo, co;
+    const double2* big;
+    const uint32_t* kb;
+    if (BIG_IS_A) {
+      bo = sk_decode((uint32_t)j, item->mrank, item->mdims, seg->a_mstr);
+      co = sk_decode((uint32_t)j, item->mrank, item->mdims, item->c_mstr);
+      big = A;
+      kb = kA;
+    } else {
+      bo = sk_decode((uint32_t)j, item->nrank, item->ndims, seg->b_nstr);
+      co = sk_decode((uint32_t)j, item->nrank, item->ndims, item->c_nstr);
+      big = B;
+      kb = kB;
+    }
+    double2 x[kSkinnyMaxK];
+#pragma unroll
+    for (int k = 0; k < kSkinnyMaxK; ++k) {
+      if (k < K) {
+        x[k] = big[(uint64_t)bo + kb[k]];
+        x[k].y *= sgBig;
+      }
+    }
+    for (int s = 0; s < S; ++s) {
+      double re = 0.0, im = 0.0;
+#pragma unroll
+      for (int k = 0; k < kSkinnyMaxK; ++k) {
+        if (k < K) {
+          const double2 w = sm[k * S + s];
+          re += x[k].x * w.x - x[k].y * w.y;
+          im += x[k].x * w.y + x[k].y * w.x;
+        }
+      }
+      C[(uint64_t)co + cso[s]] = double2{re, im};
+    }
+  }
+}
+
+int skinny_grid(int64_t J) {
+  int64_t g = (J + kSkinnyThreads - 1) / kSkinnyThreads;
+  if (g > 16384) g = 16384;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+hipError_t launch_zskinny(bool big_is_a, const void* A, const void* B, void* C,
+                          const GemmItem* item, const GemmSeg* seg, const uint32_t* koff, int grid,
+                          int conjA, int conjB, hipStream_t stream) {
+  if (big_is_a)
+    hipLaunchKernelGGL(zskinny_f64_kernel<true>, dim3(grid), dim3(kSkinnyThreads), 0, stream,
+                       (const double2*)A, (const double2*)B, (double2*)C, item, seg, koff, conjA,
+                       conjB);
+  else
+    hipLaunchKernelGGL(zskinny_f64_kernel<false>, dim3(grid), dim3(kSkinnyThreads), 0, stream,
+                       (const double2*)A, (const double2*)B, (double2*)C, item, seg, koff, conjA,
+                       conjB);
+  return hipGetLastError();
+}
+
+}  // namespace graft

@@ -24,13 +24,16 @@
 namespace graft {
 
 const TileConfig kTileConfigs[kNumTileConfigs] = {
-    {128, 128, 16, 64, 64},  // 0: large m and n
-    {128, 64, 16, 64, 32},   // 1
-    {64, 64, 16, 32, 32},    // 2
-    {256, 16, 16, 64, 16},   // 3: skinny n (TTNO absorb)
-    {64, 128, 16, 32, 64},   // 4
-    {32, 32, 16, 16, 16},    // 5: small blocks
-    {16, 256, 16, 16, 64},   // 6: skinny m (environment closure)
+    {128, 128, 16, 64, 64, 0},  // 0: large m and n
+    {128, 64, 16, 64, 32, 0},   // 1
+    {64, 64, 16, 32, 32, 0},    // 2
+    {256, 16, 16, 64, 16, 0},   // 3: skinny n (TTNO absorb)
+    {64, 128, 16, 32, 64, 0},   // 4
+    {32, 32, 16, 16, 16, 0},    // 5: small blocks
+    {16, 256, 16, 16, 64, 0},   // 6: skinny m (environment closure)
+    {64, 64, 16, 32, 32, 1},    // 7: three-product form
+    {128, 64, 16, 64, 32, 1},   // 8: three-product form
+    {64, 128, 16, 32, 64, 1},   // 9: three-product form
 };
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -52,7 +55,12 @@ __device__ __forceinline__ uint32_t decode_offset(uint32_t idx, int rank, const 
 // TR (transposed output orientation): issue the MFMAs as B^T A^T so that the
 // 16 lanes of an accumulator column walk the m index.  Chosen when the
 // output's unit-stride leg is an m leg, which makes the stores coalesced.
-template <int BM, int BN, int BK, int WM, int WN, bool TR>
+//
+// M3 (three-product form): T1 = Ar Br, T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi),
+// Cr = T1 - T2, Ci = T3 - T1 - T2 — three real MFMAs per four complex k and
+// 16x16 block instead of four (Karatsuba / Gauss).  In FP64 its rounding
+// error stays O(eps (|Ar| + |Ai|)(|Br| + |Bi|)) per term.
+template <int BM, int BN, int BK, int WM, int WN, bool TR, bool M3>
 __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
     zgemm_f64_kernel(const double2* __restrict__ Abase, const double2* __restrict__ Bbase,
                      double2* __restrict__ Cbase, const GemmItem* __restrict__ items, int nitems,
@@ -65,8 +73,11 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
   constexpr int EB = BN * BK / NT;
   constexpr int FM = WM / 16;
   constexpr int FN = WN / 16;
-  constexpr int ASL = 2 * BM + 1;  // LDS slab (doubles) per k: re plane, im plane, pad
-  constexpr int BSL = 3 * BN + 1;  // re plane, im plane, -im plane, pad
+  // LDS slab (doubles) per k.  Four-product form: A (re, im), B (re, im, -im);
+  // three-product form: A (re, im, re+im), B (re, im, re+im).  +1 pad.
+  constexpr int ASL = (M3 ? 3 : 2) * BM + 1;
+  constexpr int BSL = 3 * BN + 1;
+  constexpr int NA = M3 ? 3 : 2;   // accumulators per 16x16 block
   static_assert(NT % BM == 0 && NT % BN == 0 && NT % BK == 0, "tile/thread mismatch");
   static_assert(EA >= 1 && EB >= 1, "tile too small for the thread count");
 
@@ -110,14 +121,13 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
   const int m0 = (local / it->tiles_n) * BM;
   const int n0 = (local % it->tiles_n) * BN;
 
-  dbl4 accR[FM][FN], accI[FM][FN];
+  dbl4 acc[NA][FM][FN];
 #pragma unroll
-  for (int a = 0; a < FM; ++a)
+  for (int t = 0; t < NA; ++t)
 #pragma unroll
-    for (int b = 0; b < FN; ++b) {
-      accR[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-      accI[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-    }
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b) acc[t][a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
 
   const double sA = conjA ? -1.0 : 1.0;
   const double sB = conjB ? -1.0 : 1.0;
@@ -201,6 +211,7 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
           const int q = tid % BK, p = tid / BK + e * (NT / BK);
           As[q * ASL + p] = ra[e].x;
           As[q * ASL + BM + p] = sA * ra[e].y;
+          if constexpr (M3) As[q * ASL + 2 * BM + p] = ra[e].x + sA * ra[e].y;
         }
       } else {
 #pragma unroll
@@ -208,6 +219,7 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
           const int p = tid % BM, q = tid / BM + e * (NT / BM);
           As[q * ASL + p] = ra[e].x;
           As[q * ASL + BM + p] = sA * ra[e].y;
+          if constexpr (M3) As[q * ASL + 2 * BM + p] = ra[e].x + sA * ra[e].y;
         }
       }
       if (bkf) {
@@ -217,7 +229,7 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
           const double im = sB * rb[e].y;
           Bs[q * BSL + p] = rb[e].x;
           Bs[q * BSL + BN + p] = im;
-          Bs[q * BSL + 2 * BN + p] = -im;
+          Bs[q * BSL + 2 * BN + p] = M3 ? rb[e].x + im : -im;
         }
       } else {
 #pragma unroll
@@ -226,7 +238,7 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
           const double im = sB * rb[e].y;
           Bs[q * BSL + p] = rb[e].x;
           Bs[q * BSL + BN + p] = im;
-          Bs[q * BSL + 2 * BN + p] = -im;
+          Bs[q * BSL + 2 * BN + p] = M3 ? rb[e].x + im : -im;
         }
       }
     };
@@ -238,32 +250,58 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
       stage();
       __syncthreads();
       if (kt + 1 < nkt) fetch((kt + 1) * BK);
+      if constexpr (M3) {
 #pragma unroll
-      for (int kk = 0; kk < BK; kk += 2) {
-        const int kq = kk + kcoff;
-        const double* ap = As + kq * ASL + part * BM + wm0 + li;
-        const double* brp = Bs + kq * BSL + (part ? 2 * BN : 0) + wn0 + li;
-        const double* bip = Bs + kq * BSL + (part ? 0 : BN) + wn0 + li;
-        double av[FM], bre[FN], bim[FN];
+        for (int kk = 0; kk < BK; kk += 4) {
+          const int kq = kk + kr;   // lane group kr supplies complex k = kk + kr
+          double av[3][FM], bv[3][FN];
 #pragma unroll
-        for (int a = 0; a < FM; ++a) av[a] = ap[a * 16];
+          for (int t = 0; t < 3; ++t) {
 #pragma unroll
-        for (int b = 0; b < FN; ++b) {
-          bre[b] = brp[b * 16];
-          bim[b] = bip[b * 16];
+            for (int a = 0; a < FM; ++a) av[t][a] = As[kq * ASL + t * BM + wm0 + a * 16 + li];
+#pragma unroll
+            for (int b = 0; b < FN; ++b) bv[t][b] = Bs[kq * BSL + t * BN + wn0 + b * 16 + li];
+          }
+#pragma unroll
+          for (int t = 0; t < 3; ++t)
+#pragma unroll
+            for (int a = 0; a < FM; ++a)
+#pragma unroll
+              for (int b = 0; b < FN; ++b) {
+                if constexpr (TR)
+                  acc[t][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[t][b], av[t][a], acc[t][a][b], 0, 0, 0);
+                else
+                  acc[t][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t][a], bv[t][b], acc[t][a][b], 0, 0, 0);
+              }
         }
+      } else {
 #pragma unroll
-        for (int a = 0; a < FM; ++a)
+        for (int kk = 0; kk < BK; kk += 2) {
+          const int kq = kk + kcoff;
+          const double* ap = As + kq * ASL + part * BM + wm0 + li;
+          const double* brp = Bs + kq * BSL + (part ? 2 * BN : 0) + wn0 + li;
+          const double* bip = Bs + kq * BSL + (part ? 0 : BN) + wn0 + li;
+          double av[FM], bre[FN], bim[FN];
+#pragma unroll
+          for (int a = 0; a < FM; ++a) av[a] = ap[a * 16];
 #pragma unroll
           for (int b = 0; b < FN; ++b) {
-            if constexpr (TR) {
-              accR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bre[b], av[a], accR[a][b], 0, 0, 0);
-              accI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bim[b], av[a], accI[a][b], 0, 0, 0);
-            } else {
-              accR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bre[b], accR[a][b], 0, 0, 0);
-              accI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bim[b], accI[a][b], 0, 0, 0);
-            }
+            bre[b] = brp[b * 16];
+            bim[b] = bip[b * 16];
           }
+#pragma unroll
+          for (int a = 0; a < FM; ++a)
+#pragma unroll
+            for (int b = 0; b < FN; ++b) {
+              if constexpr (TR) {
+                acc[0][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bre[b], av[a], acc[0][a][b], 0, 0, 0);
+                acc[1][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bim[b], av[a], acc[1][a][b], 0, 0, 0);
+              } else {
+                acc[0][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bre[b], acc[0][a][b], 0, 0, 0);
+                acc[1][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bim[b], acc[1][a][b], 0, 0, 0);
+              }
+            }
+        }
       }
     }
   }
@@ -271,6 +309,14 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
   // Epilogue: v_mfma_f64_16x16x4f64 leaves D[kr + 4*r][li] in accumulator
   // element r of lane (16*kr + li); D is the (m, n) block, or (n, m) under TR.
   auto m_of = [&](int a, int r) { return m0 + wm0 + a * 16 + (TR ? li : kr + 4 * r); };
+  auto cval = [&](int a, int b, int r) -> double2 {
+    if constexpr (M3) {
+      const double t1 = acc[0][a][b][r], t2 = acc[1][a][b][r], t3 = acc[2][a][b][r];
+      return double2{t1 - t2, t3 - t1 - t2};
+    } else {
+      return double2{acc[0][a][b][r], acc[1][a][b][r]};
+    }
+  };
   auto n_of = [&](int b, int r) { return n0 + wn0 + b * 16 + (TR ? kr + 4 * r : li); };
   if (ksplit > 1) {
     // partial tile of k-chunk kc, element (m, n) at partial[(kc * N + n) * M + m]
@@ -284,7 +330,7 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
 #pragma unroll
         for (int b = 0; b < FN; ++b) {
           const int n = n_of(b, r);
-          if (n < N) P[(size_t)n * M + m] = double2{accR[a][b][r], accI[a][b][r]};
+          if (n < N) P[(size_t)n * M + m] = cval(a, b, r);
         }
       }
     return;
@@ -307,7 +353,7 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
 #pragma unroll
         for (int a = 0; a < FM; ++a) {
           if (rowm[a] == kBadRow) continue;
-          C[(uint64_t)rowm[a] + co] = double2{accR[a][b][r], accI[a][b][r]};
+          C[(uint64_t)rowm[a] + co] = cval(a, b, r);
         }
       }
   } else {
@@ -327,7 +373,7 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
 #pragma unroll
         for (int b = 0; b < FN; ++b) {
           if (coln[b] == kBadRow) continue;
-          C[(uint64_t)ro + coln[b]] = double2{accR[a][b][r], accI[a][b][r]};
+          C[(uint64_t)ro + coln[b]] = cval(a, b, r);
         }
       }
   }
@@ -338,19 +384,19 @@ int tiles_for(int config, int M, int N) {
   return ((M + c.bm - 1) / c.bm) * ((N + c.bn - 1) / c.bn);
 }
 
-template <int BM, int BN, int BK, int WM, int WN>
+template <int BM, int BN, int BK, int WM, int WN, bool M3 = false>
 static hipError_t launch_f64(const void* A, const void* B, void* C, const GemmItem* items,
                              int nitems, const GemmSeg* segs, const uint32_t* koff,
                              int total_tiles, int conjA, int conjB, int ksplit, void* partial,
                              bool tr, hipStream_t stream) {
   constexpr int NT = 64 * (BM / WM) * (BN / WN);
   if (tr)
-    hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, BK, WM, WN, true>), dim3(total_tiles * ksplit),
+    hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, BK, WM, WN, true, M3>), dim3(total_tiles * ksplit),
                        dim3(NT), 0, stream, (const double2*)A, (const double2*)B, (double2*)C,
                        items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit,
                        (double2*)partial);
   else
-    hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, BK, WM, WN, false>), dim3(total_tiles * ksplit),
+    hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, BK, WM, WN, false, M3>), dim3(total_tiles * ksplit),
                        dim3(NT), 0, stream, (const double2*)A, (const double2*)B, (double2*)C,
                        items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit,
                        (double2*)partial);
@@ -389,18 +435,25 @@ hipError_t launch_zgemm(int config, int precision, const void* A, const void* B,
                         const uint32_t* koff, int total_tiles, int conjA, int conjB,
                         int ksplit, void* partial, bool tr, hipStream_t stream) {
   if (total_tiles <= 0) return hipSuccess;
-  if (precision != GRAFT_PREC_FP64) return hipErrorNotSupported;
+  if (precision != GRAFT_PREC_FP64 && precision != GRAFT_PREC_FP32 &&
+      precision != GRAFT_PREC_BF16X3)
+    return hipErrorNotSupported;
+  auto tiles = [&](int ks, void* part) {
+    if (precision == GRAFT_PREC_FP64)
+      return launch_zgemm_tiles(config, A, B, C, items, nitems, segs, koff, total_tiles, conjA,
+                                conjB, ks, part, tr, stream);
+    return launch_zgemm_lowp(config, precision, A, B, C, items, nitems, segs, koff, total_tiles,
+                             conjA, conjB, ks, part, tr, stream);
+  };
   if (ksplit > 1) {
     if (nitems != 1 || partial == nullptr) return hipErrorInvalidValue;
-    hipError_t e = launch_zgemm_tiles(config, A, B, C, items, nitems, segs, koff, total_tiles,
-                                      conjA, conjB, ksplit, partial, tr, stream);
+    hipError_t e = tiles(ksplit, partial);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(splitk_reduce_kernel, dim3(1024), dim3(256), 0, stream,
                        (const double2*)partial, ksplit, items, (double2*)C);
     return hipGetLastError();
   }
-  return launch_zgemm_tiles(config, A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB,
-                            1, nullptr, tr, stream);
+  return tiles(1, nullptr);
 }
 
 hipError_t launch_zgemm_tiles(int config, const void* A, const void* B, void* C,
@@ -415,6 +468,9 @@ hipError_t launch_zgemm_tiles(int config, const void* A, const void* B, void* C,
     case 4: return launch_f64<64, 128, 16, 32, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
     case 5: return launch_f64<32, 32, 16, 16, 16>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
     case 6: return launch_f64<16, 256, 16, 16, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+    case 7: return launch_f64<64, 64, 16, 32, 32, true>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+    case 8: return launch_f64<128, 64, 16, 64, 32, true>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+    case 9: return launch_f64<64, 128, 16, 32, 64, true>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
     default: return hipErrorInvalidValue;
   }
 }

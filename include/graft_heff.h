@@ -137,6 +137,10 @@ int graft_plan_info(graft_plan* plan, uint64_t* workspace_bytes, double* macs,
 /* Per-step GEMM profile: m, k, n of the largest block, block count and MACs. */
 int graft_plan_step_info(graft_plan* plan, int32_t step, int64_t* mkn3, int64_t* nblocks,
                          double* macs, int32_t* tile_config);
+/* Launch shape chosen for one step: tile configuration (the last index is
+ * the streaming kernel for skinny steps), MFMA orientation, split-K factor. */
+int graft_plan_step_launch(graft_plan* plan, int32_t step, int32_t* config, int32_t* orientation,
+                           int32_t* ksplit);
 /* Pin the tile configuration of one step (-1 = heuristic). */
 int graft_plan_set_tile_config(graft_plan* plan, int32_t step, int32_t config);
 int graft_num_tile_configs(void);

@@ -94,10 +94,47 @@ def test_every_tile_config_gives_the_same_result(g):
     want = ref.contract_pair(A, (0,), (1, 2), False, Bm, (0, 1), (2,), False, (0, 1))
     ncfg = nat.lib().graft_num_tile_configs()
     for cfg in range(ncfg):
-        np_.set_tile_config(0, cfg)
-        out = DTensor.empty((150, 70))
-        np_.execute([dA, dB], out)
-        assert _rel(out.to_numpy(), want) < TOL, cfg
+        for orient in (0, 1):
+            try:
+                np_.set_tile_config(0, cfg)
+            except nat.GraftError as e:
+                assert e.status == 4      # the streaming kernel does not apply here
+                continue
+            np_.set_orientation(0, orient)
+            out = DTensor.empty((150, 70))
+            np_.execute([dA, dB], out)
+            assert _rel(out.to_numpy(), want) < TOL, (cfg, orient)
+
+
+@pytest.mark.parametrize("big_is_a", [True, False])
+def test_streaming_kernel_matches_oracle(g, big_is_a):
+    """The skinny TTNO-absorb kernel: K <= 16, one open side <= 64, both orientations of the operands."""
+    from graft_amd.contractions import NativePlan
+    from graft_amd.planning import ContractionPlan, PairStep
+    from graft_amd.tensors import DTensor
+    rng = np.random.default_rng(12)
+    big = ref.random_complex(rng, (70, 2, 5, 3, 40))        # j legs (70, 3, 40), k legs (2, 5)
+    small = ref.random_complex(rng, (5, 7, 2, 4))           # k legs (5, 2), s legs (7, 4)
+    if big_is_a:
+        A, Bm, oa, ca, ob, cb, conjA, conjB = big, small, (0, 3, 4), (1, 2), (1, 3), (2, 0), True, False
+    else:
+        A, Bm, oa, ca, ob, cb, conjA, conjB = small, big, (1, 3), (2, 0), (0, 3, 4), (1, 2), False, True
+    _say("streaming kernel", big_is_a)
+    nout = len(oa) + len(ob)
+    out = tuple(range(nout))
+    step = PairStep(0, 1, 2, oa, ca, ob, cb, conjA, conjB, (out, ()))
+    dA, dB = DTensor.from_numpy(A), DTensor.from_numpy(Bm)
+    np_ = NativePlan(ContractionPlan(3, 2, [step]), [dA, dB])
+    np_.set_tile_config(0, nat_skinny_config())
+    want = ref.contract_pair(A, oa, ca, conjA, Bm, cb, ob, conjB, out)
+    C = DTensor.empty(want.shape)
+    np_.execute([dA, dB], C)
+    assert _rel(C.to_numpy(), want) < TOL
+
+
+def nat_skinny_config():
+    from graft_amd import _native as nat
+    return nat.lib().graft_num_tile_configs() - 1
 
 
 # --------------------------------------------------------------------------
@@ -173,6 +210,60 @@ def test_effective_map_is_hermitian(g):
     lhs = T.inner(a, h2(b))
     rhs = T.inner(h2(a), b)
     assert abs(lhs - rhs) < 1e-11 * abs(lhs)
+
+
+# --------------------------------------------------------------------------
+# reduced-precision products (FP32, split BF16x3) against the FP64 oracle
+# --------------------------------------------------------------------------
+
+PREC_TOL = {"fp32": 5e-6, "bf16x3": 2e-4}
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_reduced_precision_pair_every_config(g, prec):
+    from graft_amd import _native as nat
+    from graft_amd.contractions import NativePlan
+    from graft_amd.planning import ContractionPlan, PairStep
+    from graft_amd.tensors import DTensor
+    _say("reduced precision pair", prec)
+    rng = np.random.default_rng(21)
+    A = ref.random_complex(rng, (150, 40, 3))
+    Bm = ref.random_complex(rng, (40, 3, 70))
+    step = PairStep(0, 1, 2, (0,), (1, 2), (2,), (0, 1), True, False, ((0, 1), ()))
+    dA, dB = DTensor.from_numpy(A), DTensor.from_numpy(Bm)
+    np_ = NativePlan(ContractionPlan(3, 2, [step]), [dA, dB])
+    want = ref.contract_pair(A, (0,), (1, 2), True, Bm, (0, 1), (2,), False, (0, 1))
+    for cfg in range(nat.lib().graft_num_tile_configs()):
+        for orient in (0, 1):
+            try:
+                np_.set_tile_config(0, cfg)
+            except nat.GraftError:
+                continue
+            np_.set_orientation(0, orient)
+            out = DTensor.empty((150, 70))
+            np_.execute([dA, dB], out, nat.PRECISIONS[prec])
+            err = _rel(out.to_numpy(), want)
+            assert err < PREC_TOL[prec], (cfg, orient, err)
+            assert err > 1e-13 or cfg == nat.lib().graft_num_tile_configs() - 1  # really reduced
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_reduced_precision_h2(g, prec):
+    from graft_amd import contractions as C, trees as Tr
+    from graft_amd.tensors import DTensor
+    topo = Tr.binary_topology(3)
+    _say("reduced precision h2", prec)
+    psi, H, host, Wh, t = _network(g, topo, "heisenberg", 16, 5, 1)
+    envs = ref.all_envs(t, host, Wh)
+    cache = C.EnvCache(topo)
+    h2 = C.eff_h2(cache, psi, H, 1, 0, precision=prec)
+    rng = np.random.default_rng(1)
+    x = ref.random_complex(rng, h2.xdims)
+    want = ref.h2_apply(t, Wh, envs, 1, 0, x)
+    err = _rel(h2(DTensor.from_numpy(x)).to_numpy(), want)
+    assert err < PREC_TOL[prec], err
+    # the FP64 form of the same compiled plan stays at 1e-12
+    assert _rel(h2.with_precision("fp64")(DTensor.from_numpy(x)).to_numpy(), want) < TOL
 
 
 # --------------------------------------------------------------------------
