@@ -85,6 +85,34 @@ def cpu_baseline(D_cpu, depth=5, seed=7, budget_s=20.0):
     return dict(seconds_per_apply=sec, flops=8.0 * plan.flops, D=D_cpu, samples=len(times))
 
 
+def sweep_bench(D, precision, nsweeps=1, depth=5, seed=7, krylovdim=20):
+    """dmrg2_ sweeps on the depth-5 binary-tree Heisenberg model with maxdim D."""
+    import torch
+    import __graft_entry__
+    __graft_entry__.load_package()
+    from graft_amd import backend as B, groundstate as GS, networks as N, operators as O, trees as Tr
+    topo = Tr.binary_topology(depth)
+    rng = np.random.default_rng(seed)
+    psi = N.random_ttns(rng, topo, 2, D)
+    H = O.heisenberg_ttno(topo, 1.0)
+    times, counts = [], {"fp64": 0, "lowp": 0}
+
+    def cb(sweep, n, m, E, info):
+        counts["fp64"] += info.numops
+        counts["lowp"] += getattr(info, "lowp_ops", 0)
+
+    energies = []
+    for s in range(nsweeps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, Es = GS.dmrg2_(psi, H, trunc=B.TruncationScheme(maxdim=D), nsweeps=1, krylovdim=krylovdim,
+                          precision=precision, callback=cb)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        energies.append(Es[-1])
+    return dict(D=D, precision=precision, sweep_s=times, energies=energies, applies=counts)
+
+
 def blas_threads():
     try:
         from threadpoolctl import threadpool_info
@@ -104,6 +132,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the heuristic tile configurations (default: autotune every step)")
+    ap.add_argument("--mode", choices=["apply", "sweep"], default="apply")
+    ap.add_argument("--sweep-D", type=int, default=32)
+    ap.add_argument("--sweeps", type=int, default=2)
+    ap.add_argument("--precision", default="fp64")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -114,6 +146,13 @@ def main():
 
     import torch
     torch.cuda.set_device(0)
+    if args.mode == "sweep":
+        r = sweep_bench(args.sweep_D, args.precision, args.sweeps)
+        print(json.dumps({"metric": "dmrg2 sweep seconds", "value": r["sweep_s"][-1], "unit": "s",
+                          "higher_is_better": False, "n_gpus": 1, "config": {
+                              "workload": f"dmrg2_ sweep, depth-5 binary tree Heisenberg, maxdim {r['D']}",
+                              "precision": r["precision"]}, "detail": r}), flush=True)
+        return
     t_setup = time.perf_counter()
     g, topo, psi, H, cache, h2, x = build_workload(args.D)
     native = h2.native

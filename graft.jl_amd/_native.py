@@ -90,6 +90,8 @@ def lib():
                                  ctypes.POINTER(i32)],
         "graft_plan_set_tile_config": [vp, i32, i32],
         "graft_plan_set_orientation": [vp, i32, i32],
+        "graft_plan_output_blocks": [vp, ctypes.POINTER(i32), ctypes.POINTER(i64),
+                                     ctypes.POINTER(i32), ctypes.POINTER(i64)],
         "graft_plan_step_launch": [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                    ctypes.POINTER(i32)],
         "graft_plan_autotune": [vp, vp, pp, vp, i32, i32],

@@ -900,6 +900,24 @@ int graft_plan_step_launch(graft_plan* plan, int32_t step, int32_t* config, int3
   return GRAFT_OK;
 }
 
+int graft_plan_output_blocks(graft_plan* plan, int32_t* rank, int64_t* nblocks,
+                             int32_t* block_sectors, int64_t* block_offsets) {
+  GRAFT_REQUIRE(plan != nullptr && rank != nullptr && nblocks != nullptr, "null argument");
+  const TStruct& t = plan->impl.slots[plan->impl.output_slot];
+  *rank = t.rank;
+  if (block_sectors == nullptr || block_offsets == nullptr) {
+    *nblocks = (int64_t)t.blocks.size();
+    return GRAFT_OK;
+  }
+  GRAFT_REQUIRE(*nblocks >= (int64_t)t.blocks.size(), "block arrays too small");
+  for (size_t b = 0; b < t.blocks.size(); ++b) {
+    for (int l = 0; l < t.rank; ++l) block_sectors[b * t.rank + l] = t.blocks[b].sec[l];
+    block_offsets[b] = t.blocks[b].offset;
+  }
+  *nblocks = (int64_t)t.blocks.size();
+  return GRAFT_OK;
+}
+
 int graft_num_tile_configs(void) { return kNumTileConfigs + 1; }
 
 int graft_plan_set_tile_config(graft_plan* plan, int32_t step, int32_t config) {

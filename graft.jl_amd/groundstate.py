@@ -17,9 +17,14 @@ from .networks import move_center_
 from .trees import postorder
 
 
-def groundstate_eigsolve_backend(effective, x, howmany, which, **kw):
-    """Hook point of GroundState.jl:25-26."""
-    return krylov.eigsolve(effective, x, howmany, which, **kw)
+def groundstate_eigsolve_backend(effective, x, howmany, which, precision="fp64", **kw):
+    """Hook point of GroundState.jl:25-26.
+
+    ``precision`` other than "fp64" runs the two-phase solver: reduced-precision
+    products first, then FP64 refinement to the same tolerance."""
+    if precision == "fp64":
+        return krylov.eigsolve(effective, x, howmany, which, **kw)
+    return krylov.eigsolve_mixed(effective, x, precision, howmany, which, **kw)
 
 
 def _energy_converged(energies, tol):
@@ -28,7 +33,10 @@ def _energy_converged(energies, tol):
 
 def dmrg2_(psi, H, trunc=B.NO_TRUNCATION, nsweeps=10, tol=1e-10, krylovdim=20, eig_tol=1e-12,
            precision="fp64", verbose=False, callback=None):
-    """Two-site DMRG; returns (psi, energies) with one energy per sweep."""
+    """Two-site DMRG; returns (psi, energies) with one energy per sweep.
+
+    ``precision`` selects the product arithmetic of the local Lanczos
+    ("fp64", or "fp32" / "bf16x3" with FP64 refinement of every local solve)."""
     if not H.ishermitian:
         raise ValueError("dmrg2_: DMRG requires ishermitian(H) == true")
     t = psi.topo
@@ -43,9 +51,10 @@ def dmrg2_(psi, H, trunc=B.NO_TRUNCATION, nsweeps=10, tol=1e-10, krylovdim=20, e
             m = t.parent[n]
             move_center_(psi, n, cache)
             theta = two_site_tensor(psi, n, m)
-            h2 = eff_h2(cache, psi, H, n, m, precision=precision)
+            h2 = eff_h2(cache, psi, H, n, m)
             vals, vecs, info = groundstate_eigsolve_backend(
-                h2, theta, 1, "SR", ishermitian=True, krylovdim=krylovdim, tol=eig_tol)
+                h2, theta, 1, "SR", precision=precision, ishermitian=True, krylovdim=krylovdim,
+                tol=eig_tol)
             E = float(vals[0].real)
             cache.invalidate_edge_(n, m)
             split_two_site_(psi, vecs[0], n, m, trunc=trunc, center_on=center_on)

@@ -164,3 +164,25 @@ def exponentiate(A, dz, x, ishermitian=True, krylovdim=30, tol=1e-12, eager=True
             if numiter > maxiter:
                 raise RuntimeError("exponentiate: step size underflow")
     return y, ConvergenceInfo(1, worst, numiter, numops)
+
+
+LOWP_TOL = {"fp32": 1e-6, "bf16x3": 1e-4}
+
+
+def eigsolve_mixed(A, x0, lowp="bf16x3", howmany=1, which="SR", ishermitian=True, krylovdim=20,
+                   tol=1e-12, lowp_tol=None, maxiter=100, lowp_maxiter=4):
+    """Two-phase eigensolver: reduced-precision products, then FP64 refinement.
+
+    Phase 1 runs the thick-restart Lanczos on the map with FP32 or split-BF16x3
+    products until the Ritz residual reaches ``lowp_tol`` (the products' own
+    accuracy bounds it) or ``lowp_maxiter`` restarts pass.  Phase 2 restarts the same Lanczos in FP64 from that
+    Ritz vector and converges to ``tol``; the returned eigenvalue and vector
+    are those of the FP64 operator.  Returns (vals, vecs, info) with
+    info.numops counting FP64 applies and ``info.lowp_ops`` the reduced ones.
+    """
+    Alp = A.with_precision(lowp)
+    lowp_tol = LOWP_TOL[lowp] if lowp_tol is None else lowp_tol
+    _, v1, i1 = eigsolve(Alp, x0, howmany, which, ishermitian, krylovdim, lowp_tol, lowp_maxiter)
+    vals, vecs, i2 = eigsolve(A, v1[0], howmany, which, ishermitian, krylovdim, tol, maxiter)
+    i2.lowp_ops = i1.numops
+    return vals, vecs, i2

@@ -1,0 +1,222 @@
+"""Abelian sector tensors (U(1), U(1) x U(1)) on the host side of the device path.
+
+Device tensors with sectors are ``DTensor``s carrying a ``BlockStructure``
+(flat legs as ``SectorLeg`` lists, blocks = tuples of sector indices whose
+flat charges sum to zero, packed in block order).  The effective maps and
+environments run as grouped block GEMMs in libgraft_heff.so.  This module
+holds the gauge factorizations of such tensors (block QR, truncated block
+SVD) and their construction; the reference performs these in
+MatrixAlgebraKit through TensorKit (Backend.jl:466-609).  They run once per
+site move on small per-sector matrices and are outside the H_eff hot path.
+
+Charge conventions (TensorKit, flat-leg picture): a codomain leg carries the
+charges of its space V, a domain leg those of dual(V) (negated).  A TTNS
+tensor A :: (children) (x) P <- V_parent therefore has blocks with
+sum(children) + phys - parent = 0, so bond charges count the subtree below.
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+import torch
+
+from .planning import SectorLeg
+from .tensors import BlockStructure, DTensor
+
+
+def make_leg(charge_degs):
+    """SectorLeg from an ordered list of (charge tuple, degeneracy), dropping empty sectors."""
+    return SectorLeg([(tuple(q), int(d)) for q, d in charge_degs if d > 0])
+
+
+def invariant_blocks(legs):
+    return BlockStructure.invariant(legs).blocks
+
+
+class HostBlocks:
+    """Host copy of a sector tensor: legs, ncod and {sector tuple: ndarray}."""
+
+    def __init__(self, legs, blocks, ncod):
+        self.legs = list(legs)
+        self.blocks = dict(blocks)
+        self.ncod = ncod
+
+    @staticmethod
+    def from_device(t):
+        bs = t.blocks
+        host = t.data.detach().cpu().numpy()
+        blocks = {}
+        for i, b in enumerate(bs.blocks):
+            o = int(bs.offsets[i])
+            blocks[b] = host[o:o + bs.sizes[i]].reshape(bs.block_dims[i], order="F").copy()
+        return HostBlocks(bs.legs, blocks, t.ncod)
+
+    def to_device(self):
+        keys = sorted(self.blocks)
+        struct = BlockStructure(self.legs, keys)
+        return DTensor.from_blocks(struct, [self.blocks[k] for k in keys], self.ncod)
+
+    def dims_of(self, key):
+        return tuple(self.legs[l].sectors[s][1] for l, s in enumerate(key))
+
+    def dense(self):
+        out = np.zeros(tuple(l.dim for l in self.legs), dtype=np.complex128)
+        starts = [np.concatenate([[0], np.cumsum(l.degs())[:-1]]) for l in self.legs]
+        for key, blk in self.blocks.items():
+            sl = tuple(slice(int(starts[l][s]), int(starts[l][s]) + blk.shape[l]) for l, s in enumerate(key))
+            out[sl] = blk
+        return out
+
+    def permute(self, perm, ncod=None):
+        return HostBlocks([self.legs[p] for p in perm],
+                          {tuple(k[p] for p in perm): np.transpose(v, perm) for k, v in self.blocks.items()},
+                          self.ncod if ncod is None else ncod)
+
+
+def random_blocks(rng, legs, ncod):
+    """Invariant sector tensor with i.i.d. complex normal entries."""
+    blocks = {}
+    for key in invariant_blocks(legs):
+        shape = tuple(legs[l].sectors[s][1] for l, s in enumerate(key))
+        blocks[key] = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2)
+    return HostBlocks(legs, blocks, ncod)
+
+
+def _row_index(T, nrow):
+    """Group blocks by the sector tuple of legs nrow.. (the column part)."""
+    groups = {}
+    for key in T.blocks:
+        groups.setdefault(key[nrow:], []).append(key[:nrow])
+    return groups
+
+
+def qr_left(T):
+    """T (legs 0..N-2 | last leg) = Q . R per sector of the last leg.
+
+    Returns (Q, R) with Q's last leg the new bond (same charges as T's last
+    leg, reduced degeneracies) and R :: new <- old as a 2-leg tensor."""
+    N = len(T.legs)
+    last = T.legs[-1]
+    groups = _row_index(T, N - 1)
+    qblocks, rblocks, new_degs = {}, {}, {}
+    for col, rows in groups.items():
+        s = col[0]
+        rows = sorted(rows)
+        mats = [T.blocks[r + col].reshape(-1, last.sectors[s][1], order="F") for r in rows]
+        M = np.vstack(mats)
+        Q, R = np.linalg.qr(M, mode="reduced")
+        new_degs[s] = Q.shape[1]
+        off = 0
+        for r, m in zip(rows, mats):
+            qblocks[(r, s)] = Q[off:off + m.shape[0]]
+            off += m.shape[0]
+        rblocks[s] = R
+    order = [s for s in range(len(last.sectors)) if new_degs.get(s, 0) > 0]
+    remap = {s: i for i, s in enumerate(order)}
+    # the new leg has T's last-leg sectors (flat, i.e. dual charges) with reduced degeneracy
+    new_leg_flat = SectorLeg([(last.sectors[s][0], new_degs[s]) for s in order])
+    Qb = {}
+    for (r, s), q in qblocks.items():
+        if s in remap:
+            shape = T.dims_of(r + (s,))[:-1] + (q.shape[1],)
+            Qb[r + (remap[s],)] = q.reshape(shape, order="F")
+    Q = HostBlocks(T.legs[:-1] + [new_leg_flat], Qb, T.ncod)
+    # R :: V_new <- V_old; flat legs (dual of new_leg_flat, last)
+    Rb = {(remap[s], s): rblocks[s] for s in order}
+    R = HostBlocks([new_leg_flat.dual(), last], Rb, 1)
+    return Q, R
+
+
+def contract_leg(T, k, C, ck, cnew):
+    """Contract leg k of T with leg ck of the 2-leg tensor C; C's other leg cnew replaces leg k."""
+    out = {}
+    for key, blk in T.blocks.items():
+        for ckey, cblk in C.blocks.items():
+            if ckey[ck] != key[k]:
+                continue
+            m = np.tensordot(blk, cblk, axes=([k], [ck]))      # new leg last
+            m = np.moveaxis(m, -1, k)
+            nk = key[:k] + (ckey[cnew],) + key[k + 1:]
+            if nk in out:
+                out[nk] = out[nk] + m
+            else:
+                out[nk] = m
+    legs = list(T.legs)
+    legs[k] = C.legs[cnew]
+    return HostBlocks(legs, out, T.ncod)
+
+
+def svd_split(theta, nrow, trunc_rank):
+    """Truncated SVD of an all-codomain sector tensor across legs (0..nrow-1 | nrow..).
+
+    Singular values of all coupled sectors are ranked together and the
+    largest ``trunc_rank(spectrum)`` kept.  Returns (U, S dict, Vh, spectrum,
+    discarded norm, bond SectorLeg) with U's last leg and Vh's first leg the
+    new bond (charges of the row part's total for U)."""
+    rowsets, colsets = {}, {}
+    for key in theta.blocks:
+        r, c = key[:nrow], key[nrow:]
+        q = tuple(np.sum([theta.legs[l].sectors[s][0] for l, s in enumerate(r)], axis=0))
+        rowsets.setdefault(q, set()).add(r)
+        colsets.setdefault(q, set()).add(c)
+    pieces = []
+    for q in sorted(rowsets):
+        rows = sorted(rowsets[q])
+        cols = sorted(colsets[q])
+        rdims = [int(np.prod(theta.dims_of(r + cols[0])[:nrow])) for r in rows]
+        cdims = [int(np.prod(theta.dims_of(rows[0] + c)[nrow:])) for c in cols]
+        M = np.zeros((sum(rdims), sum(cdims)), dtype=np.complex128)
+        ro = np.concatenate([[0], np.cumsum(rdims)])
+        co = np.concatenate([[0], np.cumsum(cdims)])
+        for i, r in enumerate(rows):
+            for j, c in enumerate(cols):
+                b = theta.blocks.get(r + c)
+                if b is not None:
+                    M[ro[i]:ro[i + 1], co[j]:co[j + 1]] = b.reshape(rdims[i], cdims[j], order="F")
+        U, S, Vh = np.linalg.svd(M, full_matrices=False)
+        pieces.append((q, rows, cols, rdims, cdims, ro, co, U, S, Vh))
+    spectrum = np.sort(np.concatenate([p[8] for p in pieces]))[::-1]
+    keep_total = trunc_rank(spectrum)
+    thr = spectrum[keep_total - 1] if keep_total > 0 else np.inf
+    # keep the keep_total largest values (ties broken by sector order)
+    kept = 0
+    bond, Ub, Sb, Vb = [], {}, {}, {}
+    for q, rows, cols, rdims, cdims, ro, co, U, S, Vh in pieces:
+        r = int(np.sum(S > thr))
+        r += min(int(np.sum(S == thr)), keep_total - kept - r) if thr > 0 else 0
+        r = max(0, min(r, keep_total - kept))
+        kept += r
+        if r == 0:
+            continue
+        sidx = len(bond)
+        bond.append((q, r))
+        for i, rw in enumerate(rows):
+            Ub[rw + (sidx,)] = U[ro[i]:ro[i + 1], :r]
+        for j, c in enumerate(cols):
+            Vb[(sidx,) + c] = Vh[:r, co[j]:co[j + 1]]
+        Sb[sidx] = S[:r]
+    disc = float(np.sqrt(max(0.0, np.sum(spectrum ** 2) - sum(np.sum(x ** 2) for x in Sb.values()))))
+    bond_leg = SectorLeg(bond)                       # charges of the row part, +q
+    Ublocks = {}
+    for k, v in Ub.items():
+        shape = tuple(theta.legs[l].sectors[x][1] for l, x in enumerate(k[:-1])) + (v.shape[1],)
+        Ublocks[k] = v.reshape(shape, order="F")
+    Vblocks = {}
+    for k, v in Vb.items():
+        shape = (v.shape[0],) + tuple(theta.legs[nrow + l].sectors[x][1] for l, x in enumerate(k[1:]))
+        Vblocks[k] = v.reshape(shape, order="F")
+    U = HostBlocks(theta.legs[:nrow] + [bond_leg.dual()], Ublocks, nrow)
+    Vh = HostBlocks([bond_leg] + theta.legs[nrow:], Vblocks, 1)
+    return U, Sb, Vh, spectrum, disc, bond_leg
+
+
+def scale_bond(T, S, leg):
+    """Multiply T's leg ``leg`` (the bond) by the singular values S[sector]."""
+    out = {}
+    for k, v in T.blocks.items():
+        sv = S[k[leg]]
+        shape = [1] * v.ndim
+        shape[leg] = len(sv)
+        out[k] = v * sv.reshape(shape)
+    return HostBlocks(T.legs, out, T.ncod)

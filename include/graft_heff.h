@@ -141,6 +141,11 @@ int graft_plan_step_info(graft_plan* plan, int32_t step, int64_t* mkn3, int64_t*
  * the streaming kernel for skinny steps), MFMA orientation, split-K factor. */
 int graft_plan_step_launch(graft_plan* plan, int32_t step, int32_t* config, int32_t* orientation,
                            int32_t* ksplit);
+/* Block list of the plan's result (sector tensors): call with NULL arrays to
+ * get rank and nblocks, then with arrays of nblocks*rank sector indices and
+ * nblocks element offsets (packed, in this order). */
+int graft_plan_output_blocks(graft_plan* plan, int32_t* rank, int64_t* nblocks,
+                             int32_t* block_sectors, int64_t* block_offsets);
 /* Pin the tile configuration of one step (-1 = heuristic). */
 int graft_plan_set_tile_config(graft_plan* plan, int32_t step, int32_t config);
 int graft_num_tile_configs(void);
