@@ -27,6 +27,20 @@ sys.path.insert(0, ROOT)
 
 MI355X_FP64_MFMA_TFLOPS = 78.6
 MI355X_HBM_GBPS = 8000.0
+# HBM bytes per launch of the dominant step, from the two counter runs
+# (FETCH_SIZE and WRITE_SIZE, scaled by a 1 GiB copy) summarized in this file.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "zgemm_heff_apply_D256.md")
+
+
+def profiled_traffic(step, D):
+    """HBM bytes per launch of plan step ``step`` recorded by the counter runs, or None."""
+    import re
+    if D != 256 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    m = re.search(r"^traffic step (\d+): ([0-9.e+]+) bytes", open(TRAFFIC_PROFILE).read(), re.M)
+    if m is None or int(m.group(1)) != step:
+        return None
+    return float(m.group(2))
 
 
 def build_workload(D, depth=5, seed=7):
@@ -240,7 +254,10 @@ def main():
                    "mfma_fraction_whole_apply": apply_tflops / MI355X_FP64_MFMA_TFLOPS},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": MI355X_FP64_MFMA_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / MI355X_FP64_MFMA_TFLOPS,
-                     "traffic": None,
+                     "traffic": profiled_traffic(dom, args.D),
+                     "algorithmic_bytes": 16.0 * (info[dom]["m"] * info[dom]["k"]
+                                                  + info[dom]["k"] * info[dom]["n"]
+                                                  + info[dom]["m"] * info[dom]["n"]),
                      "kernel": f"zgemm_f64 step {dom} (m={info[dom]['m']}, k={info[dom]['k']}, "
                                f"n={info[dom]['n']})"},
         "cpu_baseline": cpu,

@@ -106,6 +106,16 @@ def lib():
         "graft_vec_axpby": [vp, i64, dp, vp, dp, vp],
         "graft_vec_multi_dotc": [vp, i64, i32, pp, vp, dp],
         "graft_vec_multi_axpy": [vp, i64, i32, pp, dp, vp],
+        "graft_vec_multi_dotc_dev": [vp, i64, i32, pp, vp, vp],
+        "graft_vec_multi_axpy_dev": [vp, i64, i32, pp, vp, ctypes.c_double, vp],
+        "graft_vec_nrm2_dev": [vp, i64, vp, vp],
+        "graft_vec_scale_inv_dev": [vp, i64, vp, vp],
+        "graft_vec_block_combine": [vp, i64, i32, pp, i32, dp, pp],
+        "graft_graph_capture_begin": [vp],
+        "graft_ctx_reserve_workspace": [vp, ctypes.c_uint64],
+        "graft_graph_capture_end": [vp, ctypes.POINTER(vp)],
+        "graft_graph_launch": [vp, vp],
+        "graft_graph_destroy": [vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -204,6 +214,9 @@ class Context:
         v = ctypes.c_uint64()
         check(lib().graft_ctx_workspace_bytes(self.handle, ctypes.byref(v)))
         return v.value
+
+    def reserve_workspace(self, nbytes):
+        check(lib().graft_ctx_reserve_workspace(self.handle, int(nbytes)))
 
     def release_workspace(self):
         check(lib().graft_ctx_release_workspace(self.handle))

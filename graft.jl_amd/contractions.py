@@ -18,7 +18,7 @@ import torch
 
 from . import _native as nat
 from . import backend as B
-from .planning import ContractionSpec, plan_contraction, slot_dims
+from .planning import ContractionSpec, conj_legs, plan_contraction, slot_dims
 from .tensors import BlockStructure, DTensor, TensorProto
 from .trees import childslot, nchildren, neighbors, nnodes, subtree_nodes
 
@@ -116,6 +116,29 @@ def _legs_of(t):
     return None if t.blocks is None else t.blocks.legs
 
 
+def _native_output_structure(native, plan, legs, conjs):
+    """BlockStructure of a sector plan's result: legs from the plan, blocks from the library."""
+    sl = [conj_legs(l, c) for l, c in zip(legs, conjs)] + [None] * len(plan.steps)
+    for s in plan.steps:
+        inter = [sl[s.a][i] for i in s.oindA] + [sl[s.b][i] for i in s.oindB]
+        sl[s.dst] = [inter[i] for i in list(s.out[0]) + list(s.out[1])]
+    out_legs = sl[plan.output_slot]
+    rank = ctypes.c_int32()
+    nb = ctypes.c_int64()
+    nat.check(nat.lib().graft_plan_output_blocks(native.handle, ctypes.byref(rank), ctypes.byref(nb),
+                                                 None, None))
+    secs = (ctypes.c_int32 * max(1, nb.value * max(1, rank.value)))()
+    offs = (ctypes.c_int64 * max(1, nb.value))()
+    nat.check(nat.lib().graft_plan_output_blocks(native.handle, ctypes.byref(rank), ctypes.byref(nb),
+                                                 secs, offs))
+    r = rank.value
+    blocks = [tuple(secs[b * r + l] for l in range(r)) for b in range(nb.value)]
+    st = BlockStructure(out_legs, blocks)
+    if not np.array_equal(st.offsets, np.array([offs[b] for b in range(nb.value)], dtype=np.int64)):
+        raise RuntimeError("block layout of a sector plan result disagrees with the library")
+    return st
+
+
 # ---------------------------------------------------------------------------
 # EnvCache — envcache.jl:61-109, 812-844
 # ---------------------------------------------------------------------------
@@ -178,15 +201,30 @@ class EnvCache:
                                 fused_permutes=True)
         out_dims = slot_dims(plan, dims)[plan.output_slot]
         p = NativePlan(plan, operands, out_structure, out_dims)
+        if out_structure is None and legs is not None:
+            p.out_structure = _native_output_structure(p, plan, legs, spec.conjs)
         self.plans[key] = p
         return p
 
-    def root_cap(self, dims):
-        """_root_cap! — envcache.jl:626-655 (ones tensor closing unit legs)."""
-        key = tuple(dims)
+    def root_cap(self, dims, legs=None):
+        """_root_cap! — envcache.jl:626-655 (ones tensor closing unit legs).
+
+        With sector ``legs`` the cap is the invariant tensor on those legs with
+        every stored entry one (the root's unit legs carry one sector each)."""
+        key = (tuple(dims), None if legs is None else tuple(legs))
         c = self.rootcaps.get(key)
         if c is None:
-            c = B.ones_tensor(dims)
+            if legs is None:
+                c = B.ones_tensor(dims)
+            else:
+                from .sectors import HostBlocks, invariant_blocks
+                blocks = {}
+                for b in invariant_blocks(legs):
+                    blocks[b] = np.ones(tuple(legs[l].sectors[s][1] for l, s in enumerate(b)),
+                                        dtype=np.complex128)
+                if not blocks:
+                    raise ValueError("root cap: the root legs admit no charge-conserving block")
+                c = HostBlocks(legs, blocks, len(legs)).to_device()
             self.rootcaps[key] = c
         return c
 
@@ -251,7 +289,11 @@ def _build_env_spec(cache, ket, O, bra, u, v):
         aidx[-1] = ka
         bidx[-1] = kb
         widx[-1] = ko
-        operands.append(cache.root_cap((Bt.dims[-1], W.dims[-1], A.dims[-1])))
+        cl = None
+        if A.blocks is not None:
+            # cap legs are the duals of the legs they close: conj(bra), W, ket
+            cl = [_legs_of(Bt)[-1], _legs_of(W)[-1].dual(), _legs_of(A)[-1].dual()]
+        operands.append(cache.root_cap((Bt.dims[-1], W.dims[-1], A.dims[-1]), cl))
         labels.append([kb, ko, ka])
         conjs.append(False)
         caps.append(len(labels) - 1)
@@ -281,7 +323,7 @@ def env_(cache, ket, O, u, v, bra=None):
             env_(cache, ket, O, w, u, bra)
     spec, operands = _build_env_spec(cache, ket, O, bra, u, v)
     plan = cache.get_plan("env_ket_op_bra", spec, operands)
-    out = DTensor.empty(plan.out_dims, 3)
+    out = DTensor.empty(plan.out_dims, 3, plan.out_structure)
     plan.execute(operands, out)
     cache.envs[key] = out
     cache.env_builds += 1
@@ -296,7 +338,7 @@ def expect(psi, H, cache=None):
         env_(cache, psi, H, w, n)
     spec, operands = _build_env_spec(cache, psi, H, psi, n, None)
     plan = cache.get_plan("env_closure", spec, operands)
-    out = DTensor.empty((), 0)
+    out = DTensor.empty((), 0, plan.out_structure)
     plan.execute(operands, out)
     return complex(out.data.cpu().numpy()[0])
 
@@ -403,7 +445,11 @@ def _h1_spec(cache, psi, H, n):
         widx[-1] = ko
         labels.append([ka, ko, -Nx])
         conjs.append(False)
-        statics.append(cache.root_cap((x.dims[-1], W.dims[-1], x.dims[-1])))
+        cl = None
+        if x.blocks is not None:
+            xl = _legs_of(x)[-1]
+            cl = [xl.dual(), _legs_of(W)[-1].dual(), xl]
+        statics.append(cache.root_cap((x.dims[-1], W.dims[-1], x.dims[-1]), cl))
         caps.append(len(labels) - 1)
     preferred = children + [1] + parents + caps
     spec = ContractionSpec(labels, conjs, Nx, (Nx - 1, 1), 0, preferred)
@@ -455,6 +501,9 @@ def two_site_tensor(psi, n, m):
     t = psi.topo
     if t.parent[n] != m:
         raise ValueError("two_site_tensor: m must be the parent of n")
+    if psi.tensors[n].blocks is not None:
+        from . import sectors
+        return sectors.two_site_tensor(psi, n, m)
     A, Bm = psi.tensors[n], psi.tensors[m]
     k = childslot(t, m, n)
     pn = A.ncod
@@ -468,6 +517,10 @@ def split_two_site_(psi, theta, n, m, trunc=B.NO_TRUNCATION, center_on="n"):
     """Truncated SVD of Theta back into psi[n], psi[m] — effective.jl:852-876.
 
     Returns (psi, info) with the discarded norm and the full singular spectrum."""
+    if theta.blocks is not None:
+        from . import sectors
+        return sectors.split_two_site_(psi, theta, n, m, lambda s: B.truncation_rank(s, trunc),
+                                       center_on)
     t = psi.topo
     k = childslot(t, m, n)
     pn = psi.tensors[n].ncod
@@ -567,7 +620,12 @@ def _h2_spec(cache, psi, H, n, m):
         wmidx[-1] = ko
         labels.append([ka, ko, -Nx])
         conjs.append(False)
-        statics.append(cache.root_cap((xdims[-1], Wm.dims[-1], xdims[-1])))
+        cl = None
+        Am = psi.tensors[m]
+        if Am.blocks is not None:
+            xl = _legs_of(Am)[-1]          # the root's parent leg is the last leg of Theta
+            cl = [xl.dual(), _legs_of(Wm)[-1].dual(), xl]
+        statics.append(cache.root_cap((xdims[-1], Wm.dims[-1], xdims[-1]), cl))
         caps.append(len(labels) - 1)
     preferred = envn_slots + [1] + envm_children + [2] + envm_parents + caps
     spec = ContractionSpec(labels, conjs, Nx, (Nx, 0), 0, preferred)

@@ -81,14 +81,23 @@ hipError_t launch_dotc_partials(int64_t n, const void* x, const void* y, double*
                                 int nblocks, hipStream_t stream);
 hipError_t launch_reduce_partials(const double* partials, int nblocks, int k, double* out,
                                   hipStream_t stream);
-hipError_t launch_multi_dotc_partials(int64_t n, int k, const void* const* V, const void* y,
+constexpr int kMaxMultiVec = 64;
+// Vector pointers passed by value, so launches stay capturable in graphs.
+struct PtrList {
+  const double2* p[kMaxMultiVec];
+};
+hipError_t launch_multi_dotc_partials(int64_t n, int k, const PtrList& V, const void* y,
                                       double* partials, int nblocks, hipStream_t stream);
 hipError_t launch_axpby(int64_t n, double ar, double ai, const void* x, double br, double bi,
                         void* y, hipStream_t stream);
-hipError_t launch_multi_axpy(int64_t n, int k, const void* const* V, const double* c, void* y,
-                             hipStream_t stream);
+hipError_t launch_multi_axpy(int64_t n, int k, const PtrList& V, const double* c, double sign,
+                             void* y, hipStream_t stream);
+hipError_t launch_sqrt_real(const double* dot2, double* out, hipStream_t stream);
+constexpr int kMaxCombine = 32;  // outputs of one block combine
+hipError_t launch_block_combine(int64_t n, int k, const PtrList& V, int p, const PtrList& O,
+                                const double* c, hipStream_t stream);
+hipError_t launch_scale_inv(int64_t n, const double* beta, void* x, hipStream_t stream);
 constexpr int kReduceBlocks = 1024;
-constexpr int kMaxMultiVec = 64;
 
 void set_error(const std::string& msg);
 

@@ -186,8 +186,16 @@ struct graft_ctx {
   double* d_partials = nullptr;  // reduction scratch
   double* d_result = nullptr;
   double* h_result = nullptr;    // pinned
-  void** d_ptrs = nullptr;       // pointer table for multi-vector ops
   double* d_coef = nullptr;
+  double* d_comb = nullptr;      // block-combine coefficients
+  bool capturing = false;        // a graph capture is open on the stream
+  uint64_t ws_generation = 0;    // bumped whenever the workspace moves
+};
+
+struct graft_graph {
+  hipGraphExec_t exec = nullptr;
+  graft_ctx* ctx = nullptr;
+  uint64_t ws_generation = 0;    // the workspace the captured launches address
 };
 
 namespace graft {
@@ -230,7 +238,10 @@ static int choose_config(int64_t M, int64_t N, int skinny) {
   if (N <= 16) return 3;
   if (M <= 16) return 6;
   if (M <= 32 && N <= 32) return 5;
-  if (M >= 128 && N >= 128) return 0;
+  // Measured on MI355X (autotuner, root-edge eff_h2 at D = 256): the 64x64
+  // three-product tiles beat the 128x128 four-product ones on every
+  // MFMA-bound step, so they are the default for large blocks.
+  if (M >= 64 && N >= 64) return 7;
   if (N <= 64 && M >= 128) return 1;
   if (M <= 64 && N >= 128) return 4;
   return 2;
@@ -572,6 +583,11 @@ static int upload_step_meta(graft_plan_impl& p) {
 
 static int ensure_workspace(graft_ctx* ctx, uint64_t bytes) {
   if (bytes <= ctx->ws_size) return GRAFT_OK;
+  if (ctx->capturing) {
+    set_error("the plan needs " + std::to_string(bytes) +
+              " workspace bytes but a graph capture is open: execute the plan once before capturing");
+    return GRAFT_E_INVALID;
+  }
   if (ctx->ws_cap && bytes > ctx->ws_cap) {
     set_error("plan workspace of " + std::to_string(bytes) + " bytes exceeds the cap of " +
               std::to_string(ctx->ws_cap) + " bytes");
@@ -592,6 +608,7 @@ static int ensure_workspace(graft_ctx* ctx, uint64_t bytes) {
     return GRAFT_E_NOMEM;
   }
   ctx->ws_size = bytes;
+  ++ctx->ws_generation;
   return GRAFT_OK;
 }
 
@@ -664,8 +681,9 @@ int graft_ctx_create(int device, void* hip_stream, graft_ctx** out) {
                             sizeof(double) * 2 * kReduceBlocks * kMaxMultiVec));
   GRAFT_CHECK_HIP(hipMalloc((void**)&c->d_result, sizeof(double) * 2 * kMaxMultiVec));
   GRAFT_CHECK_HIP(hipHostMalloc((void**)&c->h_result, sizeof(double) * 2 * kMaxMultiVec));
-  GRAFT_CHECK_HIP(hipMalloc((void**)&c->d_ptrs, sizeof(void*) * kMaxMultiVec));
   GRAFT_CHECK_HIP(hipMalloc((void**)&c->d_coef, sizeof(double) * 2 * kMaxMultiVec));
+  GRAFT_CHECK_HIP(
+      hipMalloc((void**)&c->d_comb, sizeof(double) * 2 * kMaxMultiVec * kMaxCombine));
   *out = c.release();
   return GRAFT_OK;
 }
@@ -678,14 +696,16 @@ int graft_ctx_destroy(graft_ctx* ctx) {
   (void)hipFree(ctx->d_partials);
   (void)hipFree(ctx->d_result);
   (void)hipHostFree(ctx->h_result);
-  (void)hipFree(ctx->d_ptrs);
   (void)hipFree(ctx->d_coef);
+  (void)hipFree(ctx->d_comb);
   delete ctx;
   return GRAFT_OK;
 }
 
 int graft_ctx_set_stream(graft_ctx* ctx, void* hip_stream) {
   GRAFT_REQUIRE(ctx != nullptr, "null context");
+  GRAFT_REQUIRE(!ctx->capturing || (hipStream_t)hip_stream == ctx->stream,
+                "cannot change the stream while a graph capture is open");
   ctx->stream = (hipStream_t)hip_stream;
   return GRAFT_OK;
 }
@@ -702,14 +722,23 @@ int graft_ctx_workspace_bytes(graft_ctx* ctx, uint64_t* bytes) {
   return GRAFT_OK;
 }
 
+int graft_ctx_reserve_workspace(graft_ctx* ctx, uint64_t bytes) {
+  GRAFT_REQUIRE(ctx != nullptr, "null context");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  return ensure_workspace(ctx, bytes);
+}
+
 int graft_ctx_release_workspace(graft_ctx* ctx) {
   GRAFT_REQUIRE(ctx != nullptr, "null context");
   int rc = set_device(ctx);
   if (rc) return rc;
+  GRAFT_REQUIRE(!ctx->capturing, "cannot release the workspace while a graph capture is open");
   GRAFT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   if (ctx->ws) GRAFT_CHECK_HIP(hipFree(ctx->ws));
   ctx->ws = nullptr;
   ctx->ws_size = 0;
+  ++ctx->ws_generation;
   return GRAFT_OK;
 }
 
@@ -1110,16 +1139,20 @@ int graft_vec_axpby(graft_ctx* ctx, int64_t n, const double* alpha, const void* 
   return GRAFT_OK;
 }
 
+static PtrList make_ptrs(int k, const void* const* V) {
+  PtrList L;
+  for (int j = 0; j < kMaxMultiVec; ++j) L.p[j] = j < k ? (const double2*)V[j] : nullptr;
+  return L;
+}
+
 int graft_vec_multi_dotc(graft_ctx* ctx, int64_t n, int32_t k, const void* const* V,
                          const void* y, double* out2k) {
   GRAFT_REQUIRE(ctx && V && y && out2k && n >= 0, "bad argument");
   GRAFT_REQUIRE(k >= 1 && k <= kMaxMultiVec, "vector count out of range");
   int rc = set_device(ctx);
   if (rc) return rc;
-  GRAFT_CHECK_HIP(hipMemcpyAsync(ctx->d_ptrs, V, sizeof(void*) * k, hipMemcpyHostToDevice,
-                                 ctx->stream));
-  GRAFT_CHECK_HIP(launch_multi_dotc_partials(n, k, (const void* const*)ctx->d_ptrs, y,
-                                             ctx->d_partials, kReduceBlocks, ctx->stream));
+  GRAFT_CHECK_HIP(launch_multi_dotc_partials(n, k, make_ptrs(k, V), y, ctx->d_partials,
+                                             kReduceBlocks, ctx->stream));
   GRAFT_CHECK_HIP(launch_reduce_partials(ctx->d_partials, kReduceBlocks, k, ctx->d_result,
                                          ctx->stream));
   GRAFT_CHECK_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, 2 * k * sizeof(double),
@@ -1135,13 +1168,109 @@ int graft_vec_multi_axpy(graft_ctx* ctx, int64_t n, int32_t k, const void* const
   GRAFT_REQUIRE(k >= 1 && k <= kMaxMultiVec, "vector count out of range");
   int rc = set_device(ctx);
   if (rc) return rc;
-  // The pointer/coefficient tables are reused by the next call, so the
-  // upload is synchronous with respect to earlier kernels on the stream.
+  // The coefficient table is reused by the next call, so the upload is
+  // ordered after earlier kernels on the stream.
   GRAFT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  GRAFT_CHECK_HIP(hipMemcpy(ctx->d_ptrs, V, sizeof(void*) * k, hipMemcpyHostToDevice));
   GRAFT_CHECK_HIP(hipMemcpy(ctx->d_coef, c2k, sizeof(double) * 2 * k, hipMemcpyHostToDevice));
-  GRAFT_CHECK_HIP(launch_multi_axpy(n, k, (const void* const*)ctx->d_ptrs, ctx->d_coef, y,
-                                    ctx->stream));
+  GRAFT_CHECK_HIP(launch_multi_axpy(n, k, make_ptrs(k, V), ctx->d_coef, 1.0, y, ctx->stream));
+  return GRAFT_OK;
+}
+
+int graft_vec_multi_dotc_dev(graft_ctx* ctx, int64_t n, int32_t k, const void* const* V,
+                             const void* y, double* d_out2k) {
+  GRAFT_REQUIRE(ctx && V && y && d_out2k && n >= 0, "bad argument");
+  GRAFT_REQUIRE(k >= 1 && k <= kMaxMultiVec, "vector count out of range");
+  GRAFT_CHECK_HIP(launch_multi_dotc_partials(n, k, make_ptrs(k, V), y, ctx->d_partials,
+                                             kReduceBlocks, ctx->stream));
+  GRAFT_CHECK_HIP(launch_reduce_partials(ctx->d_partials, kReduceBlocks, k, d_out2k, ctx->stream));
+  return GRAFT_OK;
+}
+
+int graft_vec_multi_axpy_dev(graft_ctx* ctx, int64_t n, int32_t k, const void* const* V,
+                             const double* d_coef2k, double sign, void* y) {
+  GRAFT_REQUIRE(ctx && V && d_coef2k && y && n >= 0, "bad argument");
+  GRAFT_REQUIRE(k >= 1 && k <= kMaxMultiVec, "vector count out of range");
+  GRAFT_CHECK_HIP(launch_multi_axpy(n, k, make_ptrs(k, V), d_coef2k, sign, y, ctx->stream));
+  return GRAFT_OK;
+}
+
+int graft_vec_nrm2_dev(graft_ctx* ctx, int64_t n, const void* x, double* d_out) {
+  GRAFT_REQUIRE(ctx && x && d_out && n >= 0, "bad argument");
+  GRAFT_CHECK_HIP(launch_dotc_partials(n, x, x, ctx->d_partials, kReduceBlocks, ctx->stream));
+  GRAFT_CHECK_HIP(launch_reduce_partials(ctx->d_partials, kReduceBlocks, 1, ctx->d_result,
+                                         ctx->stream));
+  GRAFT_CHECK_HIP(launch_sqrt_real(ctx->d_result, d_out, ctx->stream));
+  return GRAFT_OK;
+}
+
+int graft_vec_scale_inv_dev(graft_ctx* ctx, int64_t n, const double* d_beta, void* x) {
+  GRAFT_REQUIRE(ctx && d_beta && x && n >= 0, "bad argument");
+  GRAFT_CHECK_HIP(launch_scale_inv(n, d_beta, x, ctx->stream));
+  return GRAFT_OK;
+}
+
+int graft_vec_block_combine(graft_ctx* ctx, int64_t n, int32_t k, const void* const* V,
+                            int32_t p, const double* coef, void* const* O) {
+  GRAFT_REQUIRE(ctx && V && coef && O && n >= 0, "bad argument");
+  GRAFT_REQUIRE(k >= 1 && k <= kMaxMultiVec, "input vector count out of range");
+  GRAFT_REQUIRE(p >= 1 && p <= kMaxCombine, "output vector count out of range");
+  GRAFT_REQUIRE(!ctx->capturing, "block combine uploads coefficients and cannot be captured");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  // stream-ordered upload: kernels queued before this call finish reading
+  // the coefficient buffer before it is overwritten
+  GRAFT_CHECK_HIP(hipMemcpyAsync(ctx->d_comb, coef, sizeof(double) * 2 * k * p,
+                                 hipMemcpyHostToDevice, ctx->stream));
+  GRAFT_CHECK_HIP(launch_block_combine(n, k, make_ptrs(k, V), p, make_ptrs(p, (const void* const*)O),
+                                       ctx->d_comb, ctx->stream));
+  return GRAFT_OK;
+}
+
+int graft_graph_capture_begin(graft_ctx* ctx) {
+  GRAFT_REQUIRE(ctx != nullptr, "null context");
+  GRAFT_REQUIRE(ctx->stream != nullptr, "graph capture needs a non-default stream");
+  GRAFT_REQUIRE(!ctx->capturing, "a capture is already open on this context");
+  GRAFT_CHECK_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ctx->capturing = true;
+  return GRAFT_OK;
+}
+
+int graft_graph_capture_end(graft_ctx* ctx, graft_graph** out) {
+  GRAFT_REQUIRE(ctx != nullptr && out != nullptr, "null argument");
+  GRAFT_REQUIRE(ctx->capturing, "no capture is open on this context");
+  ctx->capturing = false;
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  GRAFT_CHECK_HIP(hipStreamEndCapture(ctx->stream, &g));
+  hipGraphExec_t ex = nullptr;
+  hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    set_error(std::string("graph instantiation failed: ") + hipGetErrorString(e));
+    return GRAFT_E_HIP;
+  }
+  graft_graph* gr = new graft_graph;
+  gr->exec = ex;
+  gr->ctx = ctx;
+  gr->ws_generation = ctx->ws_generation;
+  *out = gr;
+  return GRAFT_OK;
+}
+
+int graft_graph_launch(graft_ctx* ctx, graft_graph* graph) {
+  GRAFT_REQUIRE(ctx != nullptr && graph != nullptr, "null argument");
+  GRAFT_REQUIRE(graph->ctx == ctx, "the graph was captured on another context");
+  GRAFT_REQUIRE(graph->ws_generation == ctx->ws_generation,
+                "the context's workspace moved since the graph was captured: capture it again");
+  GRAFT_CHECK_HIP(hipGraphLaunch(graph->exec, ctx->stream));
+  return GRAFT_OK;
+}
+
+int graft_graph_destroy(graft_graph* graph) {
+  if (graph) {
+    (void)hipGraphExecDestroy(graph->exec);
+    delete graph;
+  }
   return GRAFT_OK;
 }
 

@@ -72,8 +72,8 @@ __global__ void __launch_bounds__(kVecThreads)
 
 template <int KMAX>
 __global__ void __launch_bounds__(kVecThreads)
-    multi_dotc_partials_kernel(int64_t n, int k, const double2* const* __restrict__ V,
-                               const double2* __restrict__ y, double* __restrict__ partials) {
+    multi_dotc_partials_kernel(int64_t n, int k, PtrList V, const double2* __restrict__ y,
+                               double* __restrict__ partials) {
   __shared__ double sh[2 * (kVecThreads / 64)];
   double re[KMAX], im[KMAX];
 #pragma unroll
@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(kVecThreads)
 #pragma unroll
     for (int j = 0; j < KMAX; ++j) {
       if (j < k) {
-        const double2 a = V[j][i];
+        const double2 a = V.p[j][i];
         re[j] += a.x * b.x + a.y * b.y;
         im[j] += a.x * b.y - a.y * b.x;
       }
@@ -114,18 +114,20 @@ __global__ void __launch_bounds__(kVecThreads)
   }
 }
 
+// y += sign * sum_j c_j V_j with the coefficients read from device memory
+// (so that a captured graph sees the values of the replay, not of capture).
 template <int KMAX>
 __global__ void __launch_bounds__(kVecThreads)
-    multi_axpy_kernel(int64_t n, int k, const double2* const* __restrict__ V,
-                      const double* __restrict__ c, double2* __restrict__ y) {
+    multi_axpy_kernel(int64_t n, int k, PtrList V, const double* __restrict__ c, double sign,
+                      double2* __restrict__ y) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     double2 acc = y[i];
 #pragma unroll
     for (int j = 0; j < KMAX; ++j) {
       if (j < k) {
-        const double2 v = V[j][i];
-        const double cr = c[2 * j], ci = c[2 * j + 1];
+        const double2 v = V.p[j][i];
+        const double cr = sign * c[2 * j], ci = sign * c[2 * j + 1];
         acc.x += cr * v.x - ci * v.y;
         acc.y += cr * v.y + ci * v.x;
       }
@@ -155,17 +157,17 @@ hipError_t launch_reduce_partials(const double* partials, int nblocks, int k, do
   return hipGetLastError();
 }
 
-hipError_t launch_multi_dotc_partials(int64_t n, int k, const void* const* V, const void* y,
+hipError_t launch_multi_dotc_partials(int64_t n, int k, const PtrList& V, const void* y,
                                       double* partials, int nblocks, hipStream_t stream) {
   if (k <= 8) {
     hipLaunchKernelGGL(multi_dotc_partials_kernel<8>, dim3(nblocks), dim3(kVecThreads), 0, stream,
-                       n, k, (const double2* const*)V, (const double2*)y, partials);
+                       n, k, V, (const double2*)y, partials);
   } else if (k <= 32) {
     hipLaunchKernelGGL(multi_dotc_partials_kernel<32>, dim3(nblocks), dim3(kVecThreads), 0,
-                       stream, n, k, (const double2* const*)V, (const double2*)y, partials);
+                       stream, n, k, V, (const double2*)y, partials);
   } else {
     hipLaunchKernelGGL(multi_dotc_partials_kernel<kMaxMultiVec>, dim3(nblocks), dim3(kVecThreads),
-                       0, stream, n, k, (const double2* const*)V, (const double2*)y, partials);
+                       0, stream, n, k, V, (const double2*)y, partials);
   }
   return hipGetLastError();
 }
@@ -177,17 +179,95 @@ hipError_t launch_axpby(int64_t n, double ar, double ai, const void* x, double b
   return hipGetLastError();
 }
 
-hipError_t launch_multi_axpy(int64_t n, int k, const void* const* V, const double* c, void* y,
-                             hipStream_t stream) {
+hipError_t launch_multi_axpy(int64_t n, int k, const PtrList& V, const double* c, double sign,
+                             void* y, hipStream_t stream) {
   if (k <= 8) {
     hipLaunchKernelGGL(multi_axpy_kernel<8>, dim3(grid_for(n)), dim3(kVecThreads), 0, stream, n, k,
-                       (const double2* const*)V, c, (double2*)y);
+                       V, c, sign, (double2*)y);
   } else if (k <= 32) {
     hipLaunchKernelGGL(multi_axpy_kernel<32>, dim3(grid_for(n)), dim3(kVecThreads), 0, stream, n,
-                       k, (const double2* const*)V, c, (double2*)y);
+                       k, V, c, sign, (double2*)y);
   } else {
     hipLaunchKernelGGL(multi_axpy_kernel<kMaxMultiVec>, dim3(grid_for(n)), dim3(kVecThreads), 0,
-                       stream, n, k, (const double2* const*)V, c, (double2*)y);
+                       stream, n, k, V, c, sign, (double2*)y);
+  }
+  return hipGetLastError();
+}
+
+// out = sqrt(Re <x, x>) from the reduced dot product, and x /= out.
+__global__ void sqrt_real_kernel(const double* __restrict__ dot2, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = sqrt(fmax(dot2[0], 0.0));
+}
+
+__global__ void __launch_bounds__(kVecThreads)
+    scale_inv_kernel(int64_t n, const double* __restrict__ beta, double2* __restrict__ x) {
+  const double s = 1.0 / beta[0];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double2 v = x[i];
+    v.x *= s;
+    v.y *= s;
+    x[i] = v;
+  }
+}
+
+hipError_t launch_sqrt_real(const double* dot2, double* out, hipStream_t stream) {
+  hipLaunchKernelGGL(sqrt_real_kernel, dim3(1), dim3(64), 0, stream, dot2, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_inv(int64_t n, const double* beta, void* x, hipStream_t stream) {
+  hipLaunchKernelGGL(scale_inv_kernel, dim3(grid_for(n)), dim3(kVecThreads), 0, stream, n, beta,
+                     (double2*)x);
+  return hipGetLastError();
+}
+
+// O_j = sum_l c[l + j k] V_l for j < p (column-major k x p coefficients).
+// Outputs may alias inputs: the thread that owns element i loads V_l[i] for
+// every l before it stores any O_j[i], and no other thread touches index i.
+template <int KMAX>
+__global__ void __launch_bounds__(kVecThreads)
+    block_combine_kernel(int64_t n, int k, PtrList V, int p, PtrList O,
+                         const double2* __restrict__ c) {
+  __shared__ double2 cs[kMaxMultiVec * kMaxCombine];
+  for (int t = threadIdx.x; t < k * p; t += blockDim.x) cs[t] = c[t];
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double2 x[KMAX];
+#pragma unroll
+    for (int l = 0; l < KMAX; ++l)
+      if (l < k) x[l] = V.p[l][i];
+    for (int j = 0; j < p; ++j) {
+      double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int l = 0; l < KMAX; ++l) {
+        if (l < k) {
+          const double2 cc = cs[l + j * k];
+          acc.x += cc.x * x[l].x - cc.y * x[l].y;
+          acc.y += cc.x * x[l].y + cc.y * x[l].x;
+        }
+      }
+      const_cast<double2*>(O.p[j])[i] = acc;
+    }
+  }
+}
+
+hipError_t launch_block_combine(int64_t n, int k, const PtrList& V, int p, const PtrList& O,
+                                const double* c, hipStream_t stream) {
+  const double2* cc = (const double2*)c;
+  if (k <= 8) {
+    hipLaunchKernelGGL(block_combine_kernel<8>, dim3(grid_for(n)), dim3(kVecThreads), 0, stream, n,
+                       k, V, p, O, cc);
+  } else if (k <= 16) {
+    hipLaunchKernelGGL(block_combine_kernel<16>, dim3(grid_for(n)), dim3(kVecThreads), 0, stream,
+                       n, k, V, p, O, cc);
+  } else if (k <= 32) {
+    hipLaunchKernelGGL(block_combine_kernel<32>, dim3(grid_for(n)), dim3(kVecThreads), 0, stream,
+                       n, k, V, p, O, cc);
+  } else {
+    hipLaunchKernelGGL(block_combine_kernel<kMaxMultiVec>, dim3(grid_for(n)), dim3(kVecThreads), 0,
+                       stream, n, k, V, p, O, cc);
   }
   return hipGetLastError();
 }

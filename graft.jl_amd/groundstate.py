@@ -51,7 +51,7 @@ def dmrg2_(psi, H, trunc=B.NO_TRUNCATION, nsweeps=10, tol=1e-10, krylovdim=20, e
             m = t.parent[n]
             move_center_(psi, n, cache)
             theta = two_site_tensor(psi, n, m)
-            h2 = eff_h2(cache, psi, H, n, m)
+            h2 = eff_h2(cache, psi, H, n, m, out_structure=theta.blocks)
             vals, vecs, info = groundstate_eigsolve_backend(
                 h2, theta, 1, "SR", precision=precision, ishermitian=True, krylovdim=krylovdim,
                 tol=eig_tol)

@@ -76,7 +76,8 @@ class TTNO:
 
 
 def norm(psi):
-    return B.matrix_view(psi.tensors[psi.center], 0).abs().pow(2).sum().sqrt().item()
+    """Norm of a canonical state: the norm of its centre tensor."""
+    return float(psi.tensors[psi.center].data.abs().pow(2).sum().sqrt().item())
 
 
 def normalize_(psi):
@@ -88,6 +89,9 @@ def normalize_(psi):
 
 def _move_center_edge_(psi, m, cache):
     """ttns.jl:181-203 (QR up- and down-moves)."""
+    if psi.tensors[psi.center].blocks is not None:
+        from . import sectors
+        return sectors.move_center_edge_(psi, m, cache)
     t = psi.topo
     n = psi.center
     A = psi.tensors[n]

@@ -107,3 +107,118 @@ def tfi_random_ttno(topo, Js, hs):
 
 def heisenberg_ttno(topo, J=1.0):
     return _upload(topo, heisenberg_host(topo, J))
+
+
+# ---------------------------------------------------------------------------
+# Spinful Hubbard model on the tree graph with U(1) x U(1) (N, 2 S_z) sectors.
+#
+# Fermions are mapped to d = 4 sites (|0>, |up>, |dn>, |up dn>) with the
+# Jordan-Wigner order = depth-first preorder of the tree (a parent precedes
+# its subtree, children in leg order), up before down on a site.  For a hop
+# between a parent i and its child j the string covers the subtrees of the
+# children of i that precede j, which the automaton carries in a parity
+# channel P = prod (-1)^n:
+#     c^dag_i c_j = (c^dag_i P_i) (prod_{k in earlier sibling subtrees} P_k) c_j
+# Channels: 0 identity, 1 done, 2 parity, 3.. open operators placed below
+# (c_up, c_dn, c^dag_up, c^dag_dn), completed at the parent.
+# ---------------------------------------------------------------------------
+
+_A_UP = np.zeros((4, 4), dtype=complex)
+_A_UP[0, 1] = 1.0
+_A_UP[2, 3] = 1.0
+_A_DN = np.zeros((4, 4), dtype=complex)
+_A_DN[0, 2] = 1.0
+_A_DN[1, 3] = 1.0
+FUP = np.diag([1.0, -1.0, 1.0, -1.0]).astype(complex)
+C_UP = _A_UP                       # annihilates the up fermion
+C_DN = FUP @ _A_DN                 # intra-site string (-1)^{n_up}
+CD_UP = C_UP.conj().T
+CD_DN = C_DN.conj().T
+PAR = np.diag([1.0, -1.0, -1.0, 1.0]).astype(complex)   # (-1)^(n_up + n_dn)
+N_UP = CD_UP @ C_UP
+N_DN = CD_DN @ C_DN
+ID4 = np.eye(4, dtype=complex)
+
+HUBBARD_OPEN = [C_UP, C_DN, CD_UP, CD_DN]
+# channel charges (N, 2 S_z) of the open operators placed below
+HUBBARD_OPEN_CHARGES = [(-1, -1), (-1, 1), (1, 1), (1, -1)]
+HUBBARD_PHYS_CHARGES = [(0, 0), (1, 1), (1, -1), (2, 0)]
+
+
+def hubbard_host(topo, t=1.0, U=4.0):
+    """Dense TTNO arrays of H = -t sum_{tree edges, s} (c^dag c + h.c.) + U sum n_up n_dn."""
+    w = 3 + len(HUBBARD_OPEN)
+    W = []
+    for n in range(nnodes(topo)):
+        K = nchildren(topo, n)
+        root = topo.parent[n] < 0
+        T = np.zeros([w] * K + [4, 4, 1 if root else w], dtype=complex)
+
+        def put(children, up, op):
+            if root:
+                if up != 1:
+                    return
+                up = 0
+            T[tuple(children) + (slice(None), slice(None), up)] += op
+
+        idle = [0] * K
+        put(idle, 0, ID4)
+        put([2] * K, 2, PAR)
+        put(idle, 1, U * (N_UP @ N_DN))
+        for k in range(K):
+            ch = list(idle)
+            ch[k] = 1
+            put(ch, 1, ID4)
+            for a, op in enumerate(HUBBARD_OPEN):
+                ch = [2] * k + [3 + a] + [0] * (K - k - 1)
+                if a < 2:      # child carries c_s: complete with -t c^dag_s P
+                    put(ch, 1, -t * (HUBBARD_OPEN[a + 2] @ PAR))
+                else:          # child carries c^dag_s: complete with -t P c_s
+                    put(ch, 1, -t * (PAR @ HUBBARD_OPEN[a - 2]))
+        for a, op in enumerate(HUBBARD_OPEN):
+            put(idle, 3 + a, op)
+        W.append(T)
+    return W
+
+
+def hubbard_channel_leg():
+    """Channel space of the Hubbard TTNO as (charge, degeneracy) in channel order."""
+    from .planning import SectorLeg
+    return SectorLeg([((0, 0), 3)] + [(q, 1) for q in HUBBARD_OPEN_CHARGES])
+
+
+def hubbard_phys_leg():
+    from .planning import SectorLeg
+    return SectorLeg([(q, 1) for q in HUBBARD_PHYS_CHARGES])
+
+
+def blockify(arr, legs, ncod):
+    """Split a dense array into the invariant blocks of ``legs`` (flat-leg charges).
+
+    Raises if a non-zero entry lies outside every invariant block."""
+    from .sectors import HostBlocks, invariant_blocks
+    starts = [np.concatenate([[0], np.cumsum(l.degs())]) for l in legs]
+    blocks = {}
+    covered = np.zeros(arr.shape, dtype=bool)
+    for key in invariant_blocks(legs):
+        sl = tuple(slice(int(starts[l][s]), int(starts[l][s + 1])) for l, s in enumerate(key))
+        blocks[key] = np.array(arr[sl])
+        covered[sl] = True
+    if np.abs(arr[~covered]).max(initial=0.0) > 0:
+        raise ValueError("tensor has entries outside the charge-conserving blocks")
+    return HostBlocks(legs, blocks, ncod)
+
+
+def hubbard_sector_ttno(topo, t=1.0, U=4.0):
+    """The Hubbard TTNO as U(1) x U(1) sector tensors on the device."""
+    from .planning import SectorLeg
+    W = hubbard_host(topo, t, U)
+    ch = hubbard_channel_leg()
+    ph = hubbard_phys_leg()
+    tens = []
+    for n, a in enumerate(W):
+        K = nchildren(topo, n)
+        up = SectorLeg([((0, 0), 1)]) if topo.parent[n] < 0 else ch
+        legs = [ch] * K + [ph, ph.dual(), up.dual()]
+        tens.append(blockify(a, legs, K + 1).to_device())
+    return TTNO(topo, tens, ishermitian=True)

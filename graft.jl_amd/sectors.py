@@ -52,10 +52,18 @@ class HostBlocks:
             blocks[b] = host[o:o + bs.sizes[i]].reshape(bs.block_dims[i], order="F").copy()
         return HostBlocks(bs.legs, blocks, t.ncod)
 
-    def to_device(self):
-        keys = sorted(self.blocks)
+    def to_device(self, complete=True):
+        """Upload.  ``complete``: store every charge-conserving block of the legs
+        (absent ones as zeros), so that maps and Krylov vectors built on this
+        tensor share one block structure."""
+        blocks = dict(self.blocks)
+        if complete:
+            for key in invariant_blocks(self.legs):
+                if key not in blocks:
+                    blocks[key] = np.zeros(self.dims_of(key), dtype=np.complex128)
+        keys = sorted(blocks)
         struct = BlockStructure(self.legs, keys)
-        return DTensor.from_blocks(struct, [self.blocks[k] for k in keys], self.ncod)
+        return DTensor.from_blocks(struct, [blocks[k] for k in keys], self.ncod)
 
     def dims_of(self, key):
         return tuple(self.legs[l].sectors[s][1] for l, s in enumerate(key))
@@ -220,3 +228,177 @@ def scale_bond(T, S, leg):
         shape[leg] = len(sv)
         out[k] = v * sv.reshape(shape)
     return HostBlocks(T.legs, out, T.ncod)
+
+
+# ---------------------------------------------------------------------------
+# Gauge moves and two-site split for sector TTNS (ttns.jl:181-203,
+# effective.jl:827-876), executed per charge sector on the host.
+# ---------------------------------------------------------------------------
+
+
+def move_center_edge_(psi, m, cache):
+    from .trees import childslot
+    t = psi.topo
+    n = psi.center
+    A = HostBlocks.from_device(psi.tensors[n])
+    if t.parent[n] == m:
+        Q, R = qr_left(A)
+        k = childslot(t, m, n)
+        P = HostBlocks.from_device(psi.tensors[m])
+        psi.tensors[n] = Q.to_device()
+        psi.tensors[m] = contract_leg(P, k, R, 1, 0).to_device()
+    else:
+        k = childslot(t, n, m)
+        N = len(A.legs)
+        perm = [j for j in range(N) if j != k] + [k]
+        Q, R = qr_left(A.permute(perm))
+        back = [0] * N
+        for pos, j in enumerate(perm):
+            back[j] = pos
+        psi.tensors[n] = Q.permute(back, A.ncod).to_device()
+        C = HostBlocks.from_device(psi.tensors[m])
+        psi.tensors[m] = contract_leg(C, len(C.legs) - 1, R, 1, 0).to_device()
+    psi.center = m
+    if cache is not None:
+        cache.invalidate_edge_(n, m)
+    return psi
+
+
+def two_site_host(A, Bm, k):
+    """Theta = A (parent leg) x Bm (child slot k) on the full two-site space.
+
+    The result carries every charge-conserving block of its legs
+    (two_site_space, effective.jl:808-843): blocks that the product leaves
+    empty are stored as zeros, because H_eff and the Krylov vectors reach them."""
+    na = len(A.legs)
+    out = {}
+    for ka, va in A.blocks.items():
+        for kb, vb in Bm.blocks.items():
+            if kb[k] != ka[-1]:
+                continue
+            key = ka[:-1] + kb[:k] + kb[k + 1:]
+            v = np.tensordot(va, vb, axes=([na - 1], [k]))
+            out[key] = out[key] + v if key in out else v
+    legs = A.legs[:-1] + Bm.legs[:k] + Bm.legs[k + 1:]
+    full = {}
+    for key in invariant_blocks(legs):
+        blk = out.pop(key, None)
+        full[key] = blk if blk is not None else np.zeros(
+            tuple(legs[l].sectors[s][1] for l, s in enumerate(key)), dtype=np.complex128)
+    if out:
+        raise ValueError("two-site product produced a block outside the charge-conserving space")
+    return HostBlocks(legs, full, len(legs))
+
+
+def two_site_tensor(psi, n, m):
+    from .trees import childslot
+    k = childslot(psi.topo, m, n)
+    A = HostBlocks.from_device(psi.tensors[n])
+    Bm = HostBlocks.from_device(psi.tensors[m])
+    return two_site_host(A, Bm, k).to_device()
+
+
+def split_two_site_(psi, theta, n, m, trunc_rank, center_on="n"):
+    from .trees import childslot
+    k = childslot(psi.topo, m, n)
+    pn = psi.tensors[n].ncod
+    Km = psi.tensors[m].ncod
+    T = HostBlocks.from_device(theta)
+    U, S, Vh, spectrum, disc, bond = svd_split(T, pn, trunc_rank)
+    if center_on == "n":
+        U = scale_bond(U, S, len(U.legs) - 1)
+    else:
+        Vh = scale_bond(Vh, S, 0)
+    # Vh legs: (bond, m legs except slot k ...); move the bond to slot k
+    N = len(Vh.legs)
+    p1 = [0 if j == k else 1 + (j if j < k else j - 1) for j in range(Km)]
+    Am = Vh.permute(p1 + [N - 1], Km)
+    U.ncod = pn
+    psi.tensors[n] = U.to_device()
+    psi.tensors[m] = Am.to_device()
+    psi.center = n if center_on == "n" else m
+    return psi, dict(discarded=disc, spectrum=spectrum, kept=int(sum(d for _, d in bond.sectors)))
+
+
+# ---------------------------------------------------------------------------
+# Random symmetric states with a Gaussian spread of bond sectors.
+# ---------------------------------------------------------------------------
+
+
+def _subtree_sizes(topo):
+    from .trees import postorder
+    size = [1] * len(topo.ids)
+    for n in postorder(topo):
+        for c in topo.children[n]:
+            size[n] += size[c]
+    return size
+
+
+def _hubbard_multiplicity(s, N, sz2):
+    from math import comb
+    if (N + sz2) % 2:
+        return 0
+    nu, nd = (N + sz2) // 2, (N - sz2) // 2
+    if nu < 0 or nd < 0 or nu > s or nd > s:
+        return 0
+    return comb(s, nu) * comb(s, nd)
+
+
+def hubbard_bond_leg(s, n, D, sigma, n_total, sz2_total=0):
+    """Bond sectors (N, 2 S_z) below a subtree of s of the n sites: a discretized
+    Gaussian around the mean filling, capped by both sides' state counts."""
+    mean = s * n_total / n
+    cand = []
+    for N in range(0, 2 * s + 1):
+        for sz2 in range(-2 * s, 2 * s + 1):
+            m_in = _hubbard_multiplicity(s, N, sz2)
+            m_out = _hubbard_multiplicity(n - s, n_total - N, sz2_total - sz2)
+            if m_in and m_out:
+                w = np.exp(-((N - mean) ** 2 + sz2 ** 2) / (2 * sigma ** 2))
+                cand.append(((N, sz2), w, min(m_in, m_out)))
+    tot = sum(w for _, w, _ in cand)
+    sectors = []
+    for q, w, cap in sorted(cand):
+        d = min(cap, int(round(D * w / tot)))
+        if d > 0:
+            sectors.append((q, d))
+    return SectorLeg(sectors)
+
+
+def random_hubbard_ttns(rng, topo, D, sigma=None, n_total=None, sz2_total=None):
+    """Random canonical U(1) x U(1) TTNS on the device (centre at the root), normalized."""
+    from .networks import TTNS
+    host = random_hubbard_ttns_host(rng, topo, D, sigma, n_total, sz2_total)
+    return TTNS(topo, [h.to_device() for h in host], topo.root, [True] * len(topo.ids)), host
+
+
+def random_hubbard_ttns_host(rng, topo, D, sigma=None, n_total=None, sz2_total=None):
+    """Host blocks of a random canonical U(1) x U(1) TTNS (centre at the root).
+
+    Defaults: half filling (N = number of sites) and the lowest |2 S_z|."""
+    from .operators import hubbard_phys_leg
+    from .trees import childslot, postorder
+    n = len(topo.ids)
+    n_total = n if n_total is None else n_total
+    sz2_total = n_total % 2 if sz2_total is None else sz2_total
+    sigma = max(1.0, np.sqrt(n) / 1.5) if sigma is None else sigma
+    size = _subtree_sizes(topo)
+    ph = hubbard_phys_leg()
+    bond = {c: hubbard_bond_leg(size[c], n, D, sigma, n_total, sz2_total)
+            for c in range(n) if topo.parent[c] >= 0}
+    host = []
+    for v in range(n):
+        up = SectorLeg([((n_total, sz2_total), 1)]) if topo.parent[v] < 0 else bond[v]
+        legs = [bond[c] for c in topo.children[v]] + [ph, up.dual()]
+        host.append(random_blocks(rng, legs, len(legs) - 1))
+    for v in postorder(topo):
+        if topo.parent[v] < 0:
+            continue
+        Q, R = qr_left(host[v])
+        host[v] = Q
+        p = topo.parent[v]
+        host[p] = contract_leg(host[p], childslot(topo, p, v), R, 1, 0)
+    r = topo.root
+    nrm = np.sqrt(sum(np.sum(np.abs(b) ** 2) for b in host[r].blocks.values()))
+    host[r] = HostBlocks(host[r].legs, {k: b / nrm for k, b in host[r].blocks.items()}, host[r].ncod)
+    return host

@@ -61,6 +61,7 @@ enum graft_precision {
 
 typedef struct graft_ctx graft_ctx;
 typedef struct graft_plan graft_plan;
+typedef struct graft_graph graft_graph;
 
 /*
  * One binary contraction, the reference's PairStep with 0-based indices:
@@ -111,6 +112,9 @@ int graft_ctx_set_stream(graft_ctx* ctx, void* hip_stream);
 int graft_ctx_set_workspace_cap(graft_ctx* ctx, uint64_t bytes);
 int graft_ctx_workspace_bytes(graft_ctx* ctx, uint64_t* bytes);
 int graft_ctx_release_workspace(graft_ctx* ctx);
+/* Grow the workspace arena to at least `bytes` now (plans executed inside a
+ * graph capture must find their workspace allocated). */
+int graft_ctx_reserve_workspace(graft_ctx* ctx, uint64_t bytes);
 int graft_synchronize(graft_ctx* ctx);
 const char* graft_last_error(void);
 const char* graft_version(void);
@@ -187,6 +191,29 @@ int graft_vec_multi_dotc(graft_ctx* ctx, int64_t n, int32_t k, const void* const
 /* y += sum_j c_j V_j (one pass over y) */
 int graft_vec_multi_axpy(graft_ctx* ctx, int64_t n, int32_t k, const void* const* V,
                          const double* c2k, void* y);
+
+/* Device-resident variants (results and coefficients stay in device memory,
+ * nothing synchronizes): the building blocks of a captured Lanczos step. */
+int graft_vec_multi_dotc_dev(graft_ctx* ctx, int64_t n, int32_t k, const void* const* V,
+                             const void* y, double* d_out2k);
+/* y += sign * sum_j c_j V_j with c read from device memory at run time */
+int graft_vec_multi_axpy_dev(graft_ctx* ctx, int64_t n, int32_t k, const void* const* V,
+                             const double* d_coef2k, double sign, void* y);
+int graft_vec_nrm2_dev(graft_ctx* ctx, int64_t n, const void* x, double* d_out);
+int graft_vec_scale_inv_dev(graft_ctx* ctx, int64_t n, const double* d_beta, void* x);
+/* O_j = sum_l coef[l + j*k] V_l for j < p (host coefficients, column-major
+ * k x p complex, p <= 32).  Outputs may alias inputs, so a Krylov basis is
+ * recombined in place at a thick restart.  Not capturable. */
+int graft_vec_block_combine(graft_ctx* ctx, int64_t n, int32_t k, const void* const* V,
+                            int32_t p, const double* coef, void* const* O);
+
+/* ---- HIP graphs of work issued on the context's stream ---------------- */
+/* Capture every launch issued on the context's stream between begin and end
+ * (plans must have their workspace already: run them once first). */
+int graft_graph_capture_begin(graft_ctx* ctx);
+int graft_graph_capture_end(graft_ctx* ctx, graft_graph** out);
+int graft_graph_launch(graft_ctx* ctx, graft_graph* graph);
+int graft_graph_destroy(graft_graph* graph);
 
 #ifdef __cplusplus
 }

@@ -771,3 +771,88 @@ def brute_force_orders(n_tensors):
                         yield (lt, rt)
 
     return list(trees(list(range(n_tensors))))
+
+
+# ---------------------------------------------------------------------------
+# Spinful fermions (d = 4 per site: |0>, |up>, |dn>, |up dn>), Jordan-Wigner
+# order = depth-first preorder of the tree (examples/hubbard_2x2_common.jl
+# uses the same explicit JW encoding, :80-121).  Independent dense
+# construction used to pin the Hubbard TTNO.
+# ---------------------------------------------------------------------------
+
+
+def preorder(t):
+    out = []
+
+    def rec(i):
+        out.append(i)
+        for c in t.children[i]:
+            rec(c)
+
+    rec(0)
+    return out
+
+
+def fermion_operators(t):
+    """Dense global c_{i,s} (s = 0 up, 1 dn) for every site, JW in preorder."""
+    n = t.n
+    a = np.zeros((2, 2))
+    a[0, 1] = 1.0                        # single-mode annihilator
+    z = np.diag([1.0, -1.0])
+    i2 = np.eye(2)
+    # each site = two modes (up = mode 0 = fast bit, dn = mode 1); global mode
+    # order: preorder of sites, up before dn
+    order = preorder(t)
+    mode_rank = {}
+    for r, site in enumerate(order):
+        mode_rank[(site, 0)] = 2 * r
+        mode_rank[(site, 1)] = 2 * r + 1
+    nm = 2 * n
+    ops = {}
+    for (site, s), rank in mode_rank.items():
+        mats = []
+        # kron position of mode (site, s): site-major in node index, up fastest
+        for k in range(n):
+            for ss in range(2):
+                rk = mode_rank[(k, ss)]
+                if (k, ss) == (site, s):
+                    mats.append(a)
+                elif rk < rank:
+                    mats.append(z)
+                else:
+                    mats.append(i2)
+        full = mats[-1]
+        for m in reversed(mats[:-1]):
+            full = np.kron(full, m)
+        ops[(site, s)] = full.astype(complex)
+    return ops
+
+
+def hubbard_dense(t, hop=1.0, U=4.0):
+    """-hop sum_{tree edges, s} (c^dag c + h.c.) + U sum n_up n_dn, dense."""
+    c = fermion_operators(t)
+    D = 4 ** t.n
+    H = np.zeros((D, D), dtype=complex)
+    for ch, p in tree_edges(t):
+        for s in range(2):
+            h = c[(p, s)].conj().T @ c[(ch, s)]
+            H += -hop * (h + h.conj().T)
+    for i in range(t.n):
+        nu = c[(i, 0)].conj().T @ c[(i, 0)]
+        nd = c[(i, 1)].conj().T @ c[(i, 1)]
+        H += U * (nu @ nd)
+    return H, c
+
+
+def number_operators(c, n):
+    N = sum(c[(i, s)].conj().T @ c[(i, s)] for i in range(n) for s in range(2))
+    Sz2 = sum(c[(i, 0)].conj().T @ c[(i, 0)] - c[(i, 1)].conj().T @ c[(i, 1)] for i in range(n))
+    return N, Sz2
+
+
+def sector_ground_energy(H, N, Sz2, n_tot, sz2_tot):
+    """Lowest eigenvalue of H in the (N, 2 S_z) sector."""
+    d = np.real(np.diag(N))
+    s = np.real(np.diag(Sz2))
+    idx = np.where((np.abs(d - n_tot) < 1e-9) & (np.abs(s - sz2_tot) < 1e-9))[0]
+    return np.linalg.eigvalsh(H[np.ix_(idx, idx)])[0]

@@ -1,0 +1,178 @@
+"""GPU checks of the device-resident Krylov driver (capturable vector kernels,
+in-place basis recombination, HIP-graph replays of the Lanczos expansion).
+
+The device driver must reproduce the per-call driver (same Krylov spaces and
+decisions) to 1e-12, dense linear algebra on the map's matrix to 1e-10, and a
+replayed graph must give bit-identical coefficients to the eager launches.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import heff_oracle as ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def g():
+    import torch
+    import graft_amd
+    from graft_amd import _native
+    torch.cuda.set_device(0)
+    _native.lib()
+    return graft_amd
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
+
+
+def _h2_map(D=8, seed=3):
+    from graft_amd import contractions as C, networks as N, operators as O, trees as Tr
+    topo = Tr.binary_topology(3)
+    rng = np.random.default_rng(seed)
+    psi = N.random_ttns(rng, topo, 2, D)
+    H = O.heisenberg_ttno(topo, 1.0)
+    cache = C.EnvCache(topo)
+    h2 = C.eff_h2(cache, psi, H, 1, 0)
+    return h2, C.two_site_tensor(psi, 1, 0)
+
+
+def _dense_matrix(A, like):
+    from graft_amd.tensors import DTensor
+    n = int(np.prod(like.dims))
+    M = np.zeros((n, n), dtype=complex)
+    e = np.zeros(n, dtype=complex)
+    for i in range(n):
+        e[:] = 0
+        e[i] = 1
+        M[:, i] = A(DTensor.from_numpy(e.reshape(like.dims, order="F"), like.ncod)).to_numpy().reshape(-1, order="F")
+    return M
+
+
+def test_device_vector_ops(g):
+    import torch
+    from graft_amd import _native as nat, krylov as K, tensors as T
+    from graft_amd.tensors import DTensor
+    rng = np.random.default_rng(5)
+    n, k = 70001, 7
+    _say("device vector ops", n, k)
+    xs = [ref.random_complex(rng, n) for _ in range(k + 1)]
+    ds = [DTensor.from_numpy(x) for x in xs]
+    L = nat.lib()
+    ctx = nat.default_context().handle
+    dv = torch.zeros(2 * k + 2, dtype=torch.float64, device="cuda")
+    V = nat.ptr_array([d.ptr for d in ds[:k]])
+    y = ctypes.c_void_p(ds[k].ptr)
+    nat.check(L.graft_vec_multi_dotc_dev(ctx, n, k, V, y, ctypes.c_void_p(dv.data_ptr())))
+    got = dv[: 2 * k].cpu().numpy().view(np.complex128)
+    want = np.array([np.vdot(x, xs[k]) for x in xs[:k]])
+    assert np.abs(got - want).max() < 1e-10 * np.abs(want).max()
+    # y -= V c with c read from device memory
+    nat.check(L.graft_vec_multi_axpy_dev(ctx, n, k, V, ctypes.c_void_p(dv.data_ptr()), -1.0, y))
+    ref_y = xs[k] - sum(c * x for c, x in zip(got, xs[:k]))
+    assert _rel(ds[k].to_numpy(), ref_y) < 1e-13
+    nat.check(L.graft_vec_nrm2_dev(ctx, n, y, ctypes.c_void_p(dv.data_ptr() + 8 * 2 * k)))
+    nrm = float(dv[2 * k].item())
+    assert abs(nrm - np.linalg.norm(ref_y)) < 1e-12 * nrm
+    nat.check(L.graft_vec_scale_inv_dev(ctx, n, ctypes.c_void_p(dv.data_ptr() + 8 * 2 * k), y))
+    assert _rel(ds[k].to_numpy(), ref_y / nrm) < 1e-13
+    # in-place block combine: outputs alias the first p + 1 inputs
+    basis = [DTensor.from_numpy(x) for x in xs[:k]]
+    host = [b.to_numpy() for b in basis]
+    coef = rng.standard_normal((k, 4)) + 1j * rng.standard_normal((k, 4))
+    lz = K.DeviceLanczos.__new__(K.DeviceLanczos)
+    lz.n, lz.ptrs = n, [b.ptr for b in basis]
+    lz.combine(k, coef, basis[:4])
+    for j in range(4):
+        assert _rel(basis[j].to_numpy(), sum(coef[l, j] * host[l] for l in range(k))) < 1e-13
+    for j in range(4, k):
+        assert np.array_equal(basis[j].to_numpy(), host[j])
+
+
+def test_graph_replays_match_eager_launches(g):
+    """A captured expansion replayed three times, with eager work and host
+    synchronizations between the replays, matches the eager launches bit for bit."""
+    import torch
+    from graft_amd import krylov as K
+    h2, x = _h2_map()
+    _say("graph replays", h2.xdims)
+    h2(x)                                    # the plan's workspace exists before capture
+    with K._krylov_stream():
+        eager = K.DeviceLanczos(h2, x, 8, use_graphs=False)
+        eager.start(x)
+        h_e, b_e = eager.expand(0, 6)
+        v_e = eager.V[6].to_numpy()
+        graph = K.DeviceLanczos(h2, x, 8, use_graphs=True)
+        try:
+            for r in range(3):
+                graph.start(x)               # eager work between replays
+                torch.cuda.current_stream().synchronize()
+                h_g, b_g = graph.expand(0, 6)
+                assert len(graph.graphs) == 1
+                assert b_g == b_e, r
+                for a, b in zip(h_g, h_e):
+                    assert np.array_equal(a, b), r
+                assert np.array_equal(graph.V[6].to_numpy(), v_e), r
+        finally:
+            graph.close()
+    # the expansion is a Lanczos process: orthonormal basis, tridiagonal projection
+    Vm = np.stack([v.to_numpy().reshape(-1) for v in eager.V[:7]], axis=1)
+    assert np.abs(Vm.conj().T @ Vm - np.eye(7)).max() < 1e-12
+    for j, h in enumerate(h_e):
+        assert np.abs(h[: max(0, j - 1)]).max(initial=0.0) < 1e-10
+
+
+def test_graph_launch_refused_after_workspace_moves(g):
+    from graft_amd import _native as nat, krylov as K
+    h2, x = _h2_map()
+    h2(x)
+    _say("graph workspace guard")
+    with K._krylov_stream():
+        lz = K.DeviceLanczos(h2, x, 4, use_graphs=True)
+        lz.start(x)
+        lz.expand(0, 2)
+        nat.default_context().release_workspace()
+        with pytest.raises(nat.GraftError, match="workspace moved"):
+            lz.graphs[(0, 2)].launch()
+        lz.close()
+
+
+def test_device_eigsolve_matches_host_driver_and_dense(g):
+    from graft_amd import krylov as K, tensors as T
+    h2, x = _h2_map()
+    _say("device eigsolve", h2.xdims)
+    M = _dense_matrix(h2, x)
+    assert np.abs(M - M.conj().T).max() < 1e-11 * np.abs(M).max()
+    e0 = np.linalg.eigvalsh(M)[0]
+    vd, xd, infod = K.eigsolve(h2, x, 1, "SR", krylovdim=20, tol=1e-12, device=True)
+    vh, xh, infoh = K.eigsolve(h2, x, 1, "SR", krylovdim=20, tol=1e-12, device=False)
+    assert abs(vd[0] - e0) < 1e-10 * abs(e0)
+    assert abs(vd[0] - vh[0]) < 1e-12 * abs(e0)
+    assert infod.numiter == infoh.numiter
+    ov = abs(T.inner(xd[0], xh[0]))
+    assert abs(ov - 1) < 1e-10
+    r = M @ xd[0].to_numpy().reshape(-1, order="F") - vd[0] * xd[0].to_numpy().reshape(-1, order="F")
+    assert np.linalg.norm(r) < 1e-9
+
+
+def test_device_exponentiate_matches_host_driver_and_dense(g):
+    from graft_amd import krylov as K
+    h2, x = _h2_map()
+    _say("device exponentiate", h2.xdims)
+    M = _dense_matrix(h2, x)
+    x0 = x.to_numpy().reshape(-1, order="F")
+    w, U = np.linalg.eigh((M + M.conj().T) / 2)
+    for dz in (-0.05j, -0.7j, -0.3):
+        want = U @ (np.exp(dz * w) * (U.conj().T @ x0))
+        yd, infod = K.exponentiate(h2, dz, x, krylovdim=30, tol=1e-12, device=True)
+        yh, infoh = K.exponentiate(h2, dz, x, krylovdim=30, tol=1e-12, device=False)
+        assert _rel(yd.to_numpy(), yh.to_numpy()) < 1e-12, dz
+        assert _rel(yd.to_numpy().reshape(-1, order="F"), want) < 1e-10, dz
+        assert infod.numiter == infoh.numiter

@@ -233,3 +233,48 @@ def test_exact_evolve_unitary_and_groundstate():
     assert abs(np.linalg.norm(w) - 1) < 1e-12
     back = ref.exact_evolve(H, w, 0.3j)
     assert np.linalg.norm(back - v) < 1e-12
+
+
+# --------------------------------------------------------------------------
+# spinful fermions (Hubbard, U(1) x U(1))
+# --------------------------------------------------------------------------
+
+
+def test_fermion_operators_anticommute():
+    t = ref.star_tree(2, 1)
+    _, c = ref.hubbard_dense(t)
+    I = np.eye(4 ** t.n)
+    for k1 in c:
+        for k2 in c:
+            assert np.allclose(c[k1] @ c[k2].conj().T + c[k2].conj().T @ c[k1], I * (k1 == k2))
+            assert np.allclose(c[k1] @ c[k2] + c[k2] @ c[k1], 0)
+
+
+@pytest.mark.parametrize("U", [0.0, 4.0, 9.0])
+def test_hubbard_dimer_closed_form(U):
+    t = ref.chain_tree(2)
+    H, c = ref.hubbard_dense(t, 1.0, U)
+    N, S = ref.number_operators(c, 2)
+    assert abs(ref.sector_ground_energy(H, N, S, 2, 0) - (U - np.sqrt(U * U + 16)) / 2) < 1e-12
+    assert np.allclose(H @ N, N @ H) and np.allclose(H @ S, S @ H)
+
+
+@pytest.mark.parametrize("topo_fn", [lambda: Tr.star_topology(3, 1), lambda: Tr.binary_topology(1),
+                                     lambda: Tr.mps_topology(3), lambda: Tr.star_topology(2, 2),
+                                     lambda: Tr.fork_topology(2, 1)])
+def test_hubbard_ttno_equals_fermionic_hamiltonian(topo_fn):
+    """The parity-channel automaton reproduces the Jordan–Wigner fermion Hamiltonian."""
+    topo = topo_fn()
+    t = _product_tree(topo)
+    Hd, _ = ref.hubbard_dense(t, 1.0, 4.0)
+    assert np.abs(ref.ttno_to_dense(t, O.hubbard_host(topo, 1.0, 4.0)) - Hd).max() < 1e-12
+
+
+def test_hubbard_ttno_blocks_conserve_charge():
+    topo = Tr.binary_topology(2)
+    ch, ph = O.hubbard_channel_leg(), O.hubbard_phys_leg()
+    for n, a in enumerate(O.hubbard_host(topo)):
+        K = len(topo.children[n])
+        from graft_amd.planning import SectorLeg
+        up = SectorLeg([((0, 0), 1)]) if topo.parent[n] < 0 else ch
+        O.blockify(a, [ch] * K + [ph, ph.dual(), up.dual()], K + 1)   # raises if not
