@@ -99,7 +99,7 @@ def cpu_baseline(D_cpu, depth=5, seed=7, budget_s=20.0):
     return dict(seconds_per_apply=sec, flops=8.0 * plan.flops, D=D_cpu, samples=len(times))
 
 
-def sweep_bench(D, precision, nsweeps=1, depth=5, seed=7, krylovdim=20):
+def sweep_bench(D, precision, nsweeps=1, depth=5, seed=7, krylovdim=20, phase_times=False):
     """dmrg2_ sweeps on the depth-5 binary-tree Heisenberg model with maxdim D."""
     import torch
     import __graft_entry__
@@ -115,16 +115,22 @@ def sweep_bench(D, precision, nsweeps=1, depth=5, seed=7, krylovdim=20):
         counts["fp64"] += info.numops
         counts["lowp"] += getattr(info, "lowp_ops", 0)
 
-    energies = []
+    energies, phases = [], []
     for s in range(nsweeps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        tm = {} if phase_times else None
         _, Es = GS.dmrg2_(psi, H, trunc=B.TruncationScheme(maxdim=D), nsweeps=1, krylovdim=krylovdim,
-                          precision=precision, callback=cb)
+                          precision=precision, callback=cb, timings=tm)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
         energies.append(Es[-1])
-    return dict(D=D, precision=precision, sweep_s=times, energies=energies, applies=counts)
+        if tm is not None:
+            tm["eigsolve_tflops_fp64_equiv"] = (tm.get("flops_fp64", 0) + tm.get("flops_lowp", 0)) \
+                / max(tm.get("eigsolve", 1e-9), 1e-9) / 1e12
+            phases.append(tm)
+    return dict(D=D, precision=precision, sweep_s=times, energies=energies, applies=counts,
+                phases=phases)
 
 
 def blas_threads():
@@ -150,6 +156,8 @@ def main():
     ap.add_argument("--sweep-D", type=int, default=32)
     ap.add_argument("--sweeps", type=int, default=2)
     ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--phase-times", action="store_true",
+                    help="sweep mode: synchronized wall time per dmrg2_ phase")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -161,7 +169,7 @@ def main():
     import torch
     torch.cuda.set_device(0)
     if args.mode == "sweep":
-        r = sweep_bench(args.sweep_D, args.precision, args.sweeps)
+        r = sweep_bench(args.sweep_D, args.precision, args.sweeps, phase_times=args.phase_times)
         print(json.dumps({"metric": "dmrg2 sweep seconds", "value": r["sweep_s"][-1], "unit": "s",
                           "higher_is_better": False, "n_gpus": 1, "config": {
                               "workload": f"dmrg2_ sweep, depth-5 binary tree Heisenberg, maxdim {r['D']}",

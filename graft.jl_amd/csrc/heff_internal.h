@@ -71,7 +71,10 @@ hipError_t launch_zgemm_lowp(int config, int precision, const void* A, const voi
 constexpr int kSkinnyConfig = kNumTileConfigs;
 constexpr int kSkinnyMaxK = 16;
 constexpr int kSkinnyMaxS = 64;
-hipError_t launch_zskinny(bool big_is_a, const void* A, const void* B, void* C,
+constexpr int kSkinnyAccMaxK = 64;   // accumulator variant: longer K, smaller side
+constexpr int kSkinnyAccMaxS = 16;
+// acc selects the accumulator variant (K > kSkinnyMaxK or S > kSkinnyMaxS).
+hipError_t launch_zskinny(bool big_is_a, bool acc, const void* A, const void* B, void* C,
                           const GemmItem* item, const GemmSeg* seg, const uint32_t* koff, int grid,
                           int conjA, int conjB, hipStream_t stream);
 int skinny_grid(int64_t J);

@@ -5,6 +5,8 @@
 // in include/graft_heff.h.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -496,11 +498,16 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
   }
   // Streaming-kernel eligibility: one dense block, short K, one small open side.
   ex.skinny = 0;
-  if (ex.items.size() == 1 && ex.segs.size() == 1 && ex.segs[0].K <= kSkinnyMaxK) {
+  if (ex.items.size() == 1 && ex.segs.size() == 1) {
     const GemmItem& it = ex.items[0];
-    if (it.N <= kSkinnyMaxS && it.N * ex.segs[0].K <= kSkinnyMaxK * kSkinnyMaxS && it.M > it.N)
+    const int64_t K = ex.segs[0].K;
+    auto fits = [&](int64_t S) {
+      return (K <= kSkinnyMaxK && S <= kSkinnyMaxS) ||
+             (K <= kSkinnyAccMaxK && S <= kSkinnyAccMaxS);
+    };
+    if (fits(it.N) && it.M > it.N)
       ex.skinny = 1;
-    else if (it.M <= kSkinnyMaxS && it.N > it.M)
+    else if (fits(it.M) && it.N > it.M)
       ex.skinny = 2;
   }
   return 0;
@@ -629,7 +636,9 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
   if (ex.config == kSkinnyConfig) {
     // HBM-bound: the streaming kernel always multiplies in FP64, whatever
     // precision the compute-bound steps use.
-    GRAFT_CHECK_HIP(launch_zskinny(ex.skinny == 1, A, B, C, ex.d_items, ex.d_segs, ex.d_koff,
+    const int64_t S = ex.skinny == 1 ? ex.items[0].N : ex.items[0].M;
+    const bool acc = ex.segs[0].K > kSkinnyMaxK || S > kSkinnyMaxS;
+    GRAFT_CHECK_HIP(launch_zskinny(ex.skinny == 1, acc, A, B, C, ex.d_items, ex.d_segs, ex.d_koff,
                                    ex.total_tiles, ex.conjA, ex.conjB, ctx->stream));
     return GRAFT_OK;
   }
@@ -1013,10 +1022,13 @@ int graft_plan_autotune(graft_ctx* ctx, graft_plan* plan, const void* const* inp
   int rc = execute_plan(ctx, p, inputs, output, precision);  // fills every slot once
   if (rc) return rc;
   if (repeats < 1) repeats = 1;
+  static const bool log = std::getenv("GRAFT_AUTOTUNE_LOG") != nullptr;
   hipEvent_t e0, e1;
   GRAFT_CHECK_HIP(hipEventCreate(&e0));
   GRAFT_CHECK_HIP(hipEventCreate(&e1));
+  int step_index = -1;
   for (auto& ex : p.steps) {
+    ++step_index;
     if (ex.pinned >= 0 && ex.tr_pinned >= 0) continue;
     int best = ex.config;
     bool best_tr = ex.tr;
@@ -1044,6 +1056,10 @@ int graft_plan_autotune(graft_ctx* ctx, graft_plan* plan, const void* const* inp
         GRAFT_CHECK_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
         GRAFT_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (log)
+          std::fprintf(stderr, "autotune step %d m=%lld n=%lld k=%lld blocks=%zu config %d tr %d: %.4f ms\n",
+                       step_index, (long long)ex.maxM, (long long)ex.maxN, (long long)ex.maxK, ex.items.size(), c,
+                       (int)ex.tr, ms / repeats);
         if (!rc && ms < best_ms) {
           best_ms = ms;
           best = c;

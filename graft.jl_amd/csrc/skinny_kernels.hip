@@ -1,6 +1,7 @@
-// Streaming kernel for the HBM-bound TTNO-absorb steps (small K, one small
+// Streaming kernels for the HBM-bound TTNO-absorb steps (small K, one small
 // open dimension): C[j, s] = sum_k big[j, k] * small[k, s] for every j of the
-// large operand.  The small operand (K x S <= 16 x 64 complex) is staged once
+// large operand.  The small operand (K x S <= 16 x 64 complex, or <= 64 x 16
+// in the accumulator variant) is staged once
 // per workgroup in LDS and read as a broadcast; each thread owns one j,
 // gathers its K inputs (consecutive threads read consecutive addresses when
 // the large operand's first open leg has unit stride) and writes its S
@@ -101,6 +102,76 @@ __global__ void __launch_bounds__(kSkinnyThreads)
   }
 }
 
+// Variant for longer K (<= kSkinnyAccMaxK) and a very small side
+// (S <= kSkinnyAccMaxS): each thread keeps its S outputs in registers and
+// streams its K inputs one at a time, so registers do not grow with K.
+template <bool BIG_IS_A>
+__global__ void __launch_bounds__(kSkinnyThreads)
+    zskinny_acc_f64_kernel(const double2* __restrict__ Abase, const double2* __restrict__ Bbase,
+                           double2* __restrict__ Cbase, const GemmItem* __restrict__ item,
+                           const GemmSeg* __restrict__ seg, const uint32_t* __restrict__ koff,
+                           int conjA, int conjB) {
+  __shared__ double2 sm[kSkinnyAccMaxK * kSkinnyAccMaxS];
+  __shared__ uint32_t cso[kSkinnyAccMaxS];
+  __shared__ uint32_t kbig[kSkinnyAccMaxK];
+  const int M = item->M, N = item->N, K = seg->K;
+  const int S = BIG_IS_A ? N : M;
+  const int64_t J = BIG_IS_A ? M : N;
+  const double2* A = Abase + seg->a_off;
+  const double2* B = Bbase + seg->b_off;
+  const uint32_t* kA = koff + seg->koff_idx;
+  const uint32_t* kB = kA + K;
+  double2* C = Cbase + item->c_off;
+  const double sgA = conjA ? -1.0 : 1.0, sgB = conjB ? -1.0 : 1.0;
+
+  for (int e = threadIdx.x; e < S * K; e += blockDim.x) {
+    const int s = e % S, k = e / S;
+    double2 v;
+    if (BIG_IS_A) {
+      v = B[sk_decode((uint32_t)s, item->nrank, item->ndims, seg->b_nstr) + kB[k]];
+      v.y *= sgB;
+    } else {
+      v = A[sk_decode((uint32_t)s, item->mrank, item->mdims, seg->a_mstr) + kA[k]];
+      v.y *= sgA;
+    }
+    sm[k * S + s] = v;
+  }
+  for (int s = threadIdx.x; s < S; s += blockDim.x)
+    cso[s] = BIG_IS_A ? sk_decode((uint32_t)s, item->nrank, item->ndims, item->c_nstr)
+                      : sk_decode((uint32_t)s, item->mrank, item->mdims, item->c_mstr);
+  for (int k = threadIdx.x; k < K; k += blockDim.x) kbig[k] = BIG_IS_A ? kA[k] : kB[k];
+  __syncthreads();
+
+  const double sgBig = BIG_IS_A ? sgA : sgB;
+  const double2* big = BIG_IS_A ? A : B;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < J;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t bo = BIG_IS_A ? sk_decode((uint32_t)j, item->mrank, item->mdims, seg->a_mstr)
+                                 : sk_decode((uint32_t)j, item->nrank, item->ndims, seg->b_nstr);
+    const uint32_t co = BIG_IS_A ? sk_decode((uint32_t)j, item->mrank, item->mdims, item->c_mstr)
+                                 : sk_decode((uint32_t)j, item->nrank, item->ndims, item->c_nstr);
+    double2 acc[kSkinnyAccMaxS];
+#pragma unroll
+    for (int s = 0; s < kSkinnyAccMaxS; ++s) acc[s] = double2{0.0, 0.0};
+#pragma unroll 4
+    for (int k = 0; k < K; ++k) {
+      double2 x = big[(uint64_t)bo + kbig[k]];
+      x.y *= sgBig;
+#pragma unroll
+      for (int s = 0; s < kSkinnyAccMaxS; ++s) {
+        if (s < S) {
+          const double2 w = sm[k * S + s];
+          acc[s].x += x.x * w.x - x.y * w.y;
+          acc[s].y += x.x * w.y + x.y * w.x;
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < kSkinnyAccMaxS; ++s)
+      if (s < S) C[(uint64_t)co + cso[s]] = acc[s];
+  }
+}
+
 int skinny_grid(int64_t J) {
   int64_t g = (J + kSkinnyThreads - 1) / kSkinnyThreads;
   if (g > 16384) g = 16384;
@@ -108,9 +179,20 @@ int skinny_grid(int64_t J) {
   return (int)g;
 }
 
-hipError_t launch_zskinny(bool big_is_a, const void* A, const void* B, void* C,
+hipError_t launch_zskinny(bool big_is_a, bool acc, const void* A, const void* B, void* C,
                           const GemmItem* item, const GemmSeg* seg, const uint32_t* koff, int grid,
                           int conjA, int conjB, hipStream_t stream) {
+  if (acc) {
+    if (big_is_a)
+      hipLaunchKernelGGL(zskinny_acc_f64_kernel<true>, dim3(grid), dim3(kSkinnyThreads), 0, stream,
+                         (const double2*)A, (const double2*)B, (double2*)C, item, seg, koff, conjA,
+                         conjB);
+    else
+      hipLaunchKernelGGL(zskinny_acc_f64_kernel<false>, dim3(grid), dim3(kSkinnyThreads), 0,
+                         stream, (const double2*)A, (const double2*)B, (double2*)C, item, seg, koff,
+                         conjA, conjB);
+    return hipGetLastError();
+  }
   if (big_is_a)
     hipLaunchKernelGGL(zskinny_f64_kernel<true>, dim3(grid), dim3(kSkinnyThreads), 0, stream,
                        (const double2*)A, (const double2*)B, (double2*)C, item, seg, koff, conjA,

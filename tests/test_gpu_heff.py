@@ -132,6 +132,33 @@ def test_streaming_kernel_matches_oracle(g, big_is_a):
     assert _rel(C.to_numpy(), want) < TOL
 
 
+@pytest.mark.parametrize("big_is_a", [True, False])
+def test_streaming_accumulator_kernel_matches_oracle(g, big_is_a):
+    """The accumulator variant: K = 50 (> 16), one open side = 10 (<= 16), the
+    W-absorb shape of a degree-3 node with TTNO bond 5 and d = 2."""
+    from graft_amd.contractions import NativePlan
+    from graft_amd.planning import ContractionPlan, PairStep
+    from graft_amd.tensors import DTensor
+    rng = np.random.default_rng(13)
+    big = ref.random_complex(rng, (33, 5, 5, 2, 41))       # j legs (33, 41), k legs (5, 5, 2)
+    small = ref.random_complex(rng, (5, 5, 5, 2, 2))       # k legs (5, 5, 2) at (0, 2, 4), s (5, 2)
+    if big_is_a:
+        A, Bm, oa, ca, ob, cb, conjA, conjB = big, small, (0, 4), (1, 2, 3), (1, 3), (0, 2, 4), False, True
+    else:
+        A, Bm, oa, ca, ob, cb, conjA, conjB = small, big, (1, 3), (0, 2, 4), (0, 4), (1, 2, 3), True, False
+    _say("streaming accumulator kernel", big_is_a)
+    nout = len(oa) + len(ob)
+    out = tuple(range(nout))
+    step = PairStep(0, 1, 2, oa, ca, ob, cb, conjA, conjB, (out, ()))
+    dA, dB = DTensor.from_numpy(A), DTensor.from_numpy(Bm)
+    np_ = NativePlan(ContractionPlan(3, 2, [step]), [dA, dB])
+    np_.set_tile_config(0, nat_skinny_config())
+    want = ref.contract_pair(A, oa, ca, conjA, Bm, cb, ob, conjB, out)
+    C = DTensor.empty(want.shape)
+    np_.execute([dA, dB], C)
+    assert _rel(C.to_numpy(), want) < TOL
+
+
 def nat_skinny_config():
     from graft_amd import _native as nat
     return nat.lib().graft_num_tile_configs() - 1

@@ -176,3 +176,31 @@ def test_device_exponentiate_matches_host_driver_and_dense(g):
         assert _rel(yd.to_numpy(), yh.to_numpy()) < 1e-12, dz
         assert _rel(yd.to_numpy().reshape(-1, order="F"), want) < 1e-10, dz
         assert infod.numiter == infoh.numiter
+
+
+@pytest.mark.parametrize("shape", [(600, 900), (1100, 700)])
+def test_gram_route_svd_matches_dense_svd(g, shape):
+    """The Gram-route truncated SVD against LAPACK: kept singular values and
+    the truncated reconstruction, on a matrix with a decaying spectrum."""
+    import torch
+    from graft_amd import backend as B
+    rows, cols = shape
+    _say("gram svd", shape)
+    rng = np.random.default_rng(rows)
+    U0, _ = np.linalg.qr(ref.random_complex(rng, (rows, min(shape))))
+    V0, _ = np.linalg.qr(ref.random_complex(rng, (cols, min(shape))))
+    sig = np.exp(-np.arange(min(shape)) / 25.0)
+    M = (U0 * sig) @ V0.conj().T
+    Md = torch.from_numpy(M).cuda()
+    ts = B.TruncationScheme(maxdim=64)
+    U, S, Vh, disc, spec = B.split_svd(Md, ts, method="gram")
+    s_ref = np.linalg.svd(M, compute_uv=False)
+    assert U.shape == (rows, 64) and Vh.shape == (64, cols)
+    assert np.abs(S.cpu().numpy() - s_ref[:64]).max() < 1e-12 * s_ref[0]
+    assert np.abs(spec[64:] - s_ref[64:]).max() < 1e-6 * s_ref[0]     # tail: absolute ~sqrt(n eps)
+    assert abs(disc - np.sqrt(np.sum(s_ref[64:] ** 2))) < 1e-10
+    low = (U * S) @ Vh
+    want = (U0[:, :64] * sig[:64]) @ V0[:, :64].conj().T
+    assert np.abs(low.cpu().numpy() - want).max() < 1e-11
+    Ud, Sd, Vhd, discd, _ = B.split_svd(Md, ts, method="direct")
+    assert np.abs(Sd.cpu().numpy() - S.cpu().numpy()).max() < 1e-12
