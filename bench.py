@@ -133,6 +133,82 @@ def sweep_bench(D, precision, nsweeps=1, depth=5, seed=7, krylovdim=20, phase_ti
                 phases=phases)
 
 
+def hubbard_tree():
+    """16-site binary tree (heap order: node i's parent is (i - 1) // 2)."""
+    import __graft_entry__
+    __graft_entry__.load_package()
+    from graft_amd import trees as Tr
+    return Tr.TreeTopology("0", [(str((i - 1) // 2), str(i)) for i in range(1, 16)])
+
+
+def hubbard_bench(D, nsweeps=2, precision="fp64", sigma=1.6, seed=5, dense=True, krylovdim=20):
+    """Spinful Hubbard (t=1, U=4, half filling) on the 16-site tree: dmrg2_ sweeps
+    with U(1) x U(1) sector-batched maps against the plain dense FP64 maps on the
+    dense embedding of the same initial state, plus one root-edge apply of each."""
+    import torch
+    topo = hubbard_tree()
+    from graft_amd import backend as B, contractions as C, groundstate as GS, networks as N
+    from graft_amd import operators as O, sectors as S
+    from graft_amd.tensors import DTensor
+    rng = np.random.default_rng(seed)
+    psi, host = S.random_hubbard_ttns(rng, topo, D, sigma=sigma)
+    Hs = O.hubbard_sector_ttno(topo, 1.0, 4.0)
+    out = {"D": D, "sigma": sigma, "bond_sectors": {}, "precision": precision}
+    for c in range(len(topo.ids)):
+        if topo.parent[c] >= 0:
+            leg = psi.tensors[c].blocks.legs[-1]
+            out["bond_sectors"][topo.ids[c]] = [len(leg.sectors), leg.dim, max(leg.degs())]
+    psi_d = None
+    if dense:
+        Hd = N.TTNO(topo, [DTensor.from_numpy(w, w.ndim - 2) for w in O.hubbard_host(topo, 1.0, 4.0)],
+                    ishermitian=True)
+        psi_d = N.TTNS(topo, [DTensor.from_numpy(h.dense(), len(h.legs) - 1) for h in host], psi.center)
+
+    def one_apply(state, H, blocks):
+        cache = C.EnvCache(topo)
+        x = C.two_site_tensor(state, 1, 0)
+        h2 = C.eff_h2(cache, state, H, 1, 0, out_structure=x.blocks if blocks else None)
+        y = h2(x)
+        torch.cuda.synchronize()
+        reps = 5
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            y = h2(x, out=y)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        info = h2.native.step_info()
+        return {"ms": ms, "flops": h2.flops, "tflops": h2.flops / ms / 1e9,
+                "theta_elements": int(x.numel), "max_blocks_per_step": max(s["blocks"] for s in info)}
+
+    out["apply_sector"] = one_apply(psi, Hs, True)
+    if dense:
+        out["apply_dense"] = one_apply(psi_d, Hd, False)
+
+    def sweeps(state, H, prec):
+        times, energies, tm_all = [], [], []
+        for _ in range(nsweeps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            tm = {}
+            _, Es = GS.dmrg2_(state, H, trunc=B.TruncationScheme(maxdim=D), nsweeps=1,
+                              krylovdim=krylovdim, precision=prec, timings=tm)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+            energies.append(Es[-1])
+            tm_all.append(tm)
+        return times, energies, tm_all
+
+    out["sector_sweep_s"], out["sector_energies"], out["sector_phases"] = sweeps(psi, Hs, precision)
+    if dense:
+        out["dense_sweep_s"], out["dense_energies"], out["dense_phases"] = sweeps(psi_d, Hd, "fp64")
+        out["speedup"] = [d / s for d, s in zip(out["dense_sweep_s"], out["sector_sweep_s"])]
+        out["energy_rel_diff"] = [abs(a - b) / abs(b) for a, b in zip(out["sector_energies"],
+                                                                      out["dense_energies"])]
+    return out
+
+
 def blas_threads():
     try:
         from threadpoolctl import threadpool_info
@@ -152,7 +228,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the heuristic tile configurations (default: autotune every step)")
-    ap.add_argument("--mode", choices=["apply", "sweep"], default="apply")
+    ap.add_argument("--mode", choices=["apply", "sweep", "hubbard"], default="apply")
+    ap.add_argument("--hub-D", type=int, default=64)
+    ap.add_argument("--no-dense", action="store_true", help="hubbard mode: sector form only")
     ap.add_argument("--sweep-D", type=int, default=32)
     ap.add_argument("--sweeps", type=int, default=2)
     ap.add_argument("--precision", default="fp64")
@@ -174,6 +252,15 @@ def main():
                           "higher_is_better": False, "n_gpus": 1, "config": {
                               "workload": f"dmrg2_ sweep, depth-5 binary tree Heisenberg, maxdim {r['D']}",
                               "precision": r["precision"]}, "detail": r}), flush=True)
+        return
+    if args.mode == "hubbard":
+        r = hubbard_bench(args.hub_D, args.sweeps, args.precision, dense=not args.no_dense)
+        print(json.dumps({"metric": "dmrg2 sweep seconds, sector-batched vs plain dense FP64",
+                          "value": r["sector_sweep_s"][-1], "unit": "s", "higher_is_better": False,
+                          "n_gpus": 1, "config": {
+                              "workload": f"dmrg2_ sweep, Hubbard t=1 U=4 half filling, 16-site tree, "
+                                          f"U(1)xU(1), maxdim {r['D']}", "precision": r["precision"]},
+                          "detail": r}), flush=True)
         return
     t_setup = time.perf_counter()
     g, topo, psi, H, cache, h2, x = build_workload(args.D)

@@ -353,8 +353,10 @@ class EffectiveMap:
 
     x is never modified; y is a fresh device tensor (executor.jl:122-124)."""
 
-    def __init__(self, native, statics, xproto, kind, precision="fp64"):
+    def __init__(self, native, statics, xproto, kind, precision="fp64", spec=None, cache=None):
         self.native = native
+        self.spec = spec          # the map's network (adjoints are built from it)
+        self.cache = cache        # plan cache the adjoint networks share
         self.statics = list(statics)
         self.xdims = xproto.dims
         self.ncod = xproto.ncod
@@ -395,7 +397,7 @@ class EffectiveMap:
     def with_precision(self, precision):
         """The same map with another product arithmetic (fp64, fp32, bf16x3)."""
         return EffectiveMap(self.native, self.statics, TensorProto(self.xdims, self.ncod, self.blocks),
-                            self.kind, precision)
+                            self.kind, precision, self.spec, self.cache)
 
 
 def with_workspace_map(effective, f):
@@ -463,7 +465,7 @@ def eff_h1(cache, psi, H, n, optimize=True, memory_weight=1.0, sector_aware=True
     x = psi.tensors[n]
     native = cache.get_plan("h1", spec, [x] + statics, optimize, memory_weight, sector_aware,
                             memory_cap_bytes, out_structure=x.blocks)
-    return EffectiveMap(native, statics, x, "h1", precision)
+    return EffectiveMap(native, statics, x, "h1", precision, spec, cache)
 
 
 def _h0_spec(cache, psi, H, n, m):
@@ -486,7 +488,7 @@ def eff_h0(cache, psi, H, n, m, optimize=True, memory_weight=1.0, sector_aware=T
     proto = TensorProto((En.dims[0], Em.dims[0]), 1, None)
     native = cache.get_plan("h0", spec, [proto] + statics, optimize, memory_weight, sector_aware,
                             memory_cap_bytes)
-    return EffectiveMap(native, statics, proto, "h0", precision)
+    return EffectiveMap(native, statics, proto, "h0", precision, spec, cache)
 
 
 def two_site_dims(psi, n, m):
@@ -639,4 +641,4 @@ def eff_h2(cache, psi, H, n, m, optimize=True, memory_weight=1.0, sector_aware=T
     proto = TensorProto(xdims, len(xdims), out_structure)
     native = cache.get_plan("h2", spec, [proto] + statics, optimize, memory_weight, sector_aware,
                             memory_cap_bytes, out_structure=out_structure)
-    return EffectiveMap(native, statics, proto, "h2", precision)
+    return EffectiveMap(native, statics, proto, "h2", precision, spec, cache)

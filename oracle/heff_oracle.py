@@ -856,3 +856,55 @@ def sector_ground_energy(H, N, Sz2, n_tot, sz2_tot):
     s = np.real(np.diag(Sz2))
     idx = np.where((np.abs(d - n_tot) < 1e-9) & (np.abs(s - sz2_tot) < 1e-9))[0]
     return np.linalg.eigvalsh(H[np.ix_(idx, idx)])[0]
+
+
+# ---------------------------------------------------------------------------
+# Gradients of the energy functional E = <psi|H|psi> / <psi|psi>
+# ---------------------------------------------------------------------------
+
+
+def identity_ttno(t, d=2):
+    """TTNO of the identity operator: channel dimension 1 on every bond."""
+    W = []
+    for n in range(t.n):
+        K = len(t.children[n])
+        T = np.zeros([1] * K + [d, d, 1], dtype=complex)
+        T[(0,) * K + (slice(None), slice(None), 0)] = np.eye(d)
+        W.append(T)
+    return W
+
+
+def expectation_gradient(t, psi, W, d=2):
+    """E, <psi|psi>, dE/d conj(A_n) for every site and d<psi|H|psi>/dW_n.
+
+    Site gradients: (H_n A_n - E N_n A_n) / <psi|psi> with H_n, N_n the
+    one-site maps of H and of the identity built from environments of the
+    given (not necessarily canonical) tensors.  TTNO derivatives: the closure
+    at n is linear in W_n, so entry idx is the closure with W_n replaced by the
+    unit tensor at idx (no derivative network is used)."""
+    Wi = identity_ttno(t, d)
+    eH = all_envs(t, psi, W)
+    eI = all_envs(t, psi, Wi)
+    num = env_closure(t, psi, W, eH, 0)
+    Z = env_closure(t, psi, Wi, eI, 0)
+    E = (num / Z).real
+    grads = [(h1_apply(t, W, eH, n, psi[n]) - E * h1_apply(t, Wi, eI, n, psi[n])) / Z
+             for n in range(t.n)]
+    Rs = []
+    for n in range(t.n):
+        R = np.zeros(W[n].shape, dtype=complex)
+        for idx in np.ndindex(*W[n].shape):
+            unit = np.zeros(W[n].shape, dtype=complex)
+            unit[idx] = 1.0
+            W2 = list(W)
+            W2[n] = unit
+            R[idx] = env_closure(t, psi, W2, eH, n)
+        Rs.append(R)
+    return E, Z, grads, Rs
+
+
+def dense_energy(t, psi, terms):
+    """<psi|H|psi> / <psi|psi> of the dense state vector (any gauge)."""
+    v = ttns_to_dense(t, psi, [True] * t.n)
+    H = dense_hamiltonian(t.n, terms)
+    return float((np.vdot(v, H @ v) / np.vdot(v, v)).real)
