@@ -113,6 +113,7 @@ def lib():
         "graft_vec_block_combine": [vp, i64, i32, pp, i32, dp, pp],
         "graft_graph_capture_begin": [vp],
         "graft_ctx_reserve_workspace": [vp, ctypes.c_uint64],
+        "graft_ctx_set_sector_fusion": [vp, i32],
         "graft_graph_capture_end": [vp, ctypes.POINTER(vp)],
         "graft_graph_launch": [vp, vp],
         "graft_graph_destroy": [vp],
@@ -214,6 +215,10 @@ class Context:
         v = ctypes.c_uint64()
         check(lib().graft_ctx_workspace_bytes(self.handle, ctypes.byref(v)))
         return v.value
+
+    def set_sector_fusion(self, mode):
+        """0 per-block GEMM items, 1 automatic fused per-charge matrices, 2 always fused."""
+        check(lib().graft_ctx_set_sector_fusion(self.handle, int(mode)))
 
     def reserve_workspace(self, nbytes):
         check(lib().graft_ctx_reserve_workspace(self.handle, int(nbytes)))

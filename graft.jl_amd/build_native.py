@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 SOURCES = ["heff_runtime.hip", "zgemm_kernels.hip", "skinny_kernels.hip", "zgemm_lowp_kernels.hip",
-           "vector_kernels.hip"]
+           "vector_kernels.hip", "sector_kernels.hip"]
 OUT = os.path.join(HERE, "libgraft_heff.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

@@ -112,6 +112,9 @@ class NativePlan:
             pass
 
 
+_ORDER_CACHE = {}    # pairwise orders of sector networks, shared by every EnvCache
+
+
 def _legs_of(t):
     return None if t.blocks is None else t.blocks.legs
 
@@ -198,7 +201,7 @@ class EnvCache:
             legs = [_legs_of(op) for op in operands]
         plan = plan_contraction(spec, dims, legs, optimize=optimize, memory_weight=memory_weight,
                                 sector_aware=sector_aware, memory_cap_bytes=memory_cap_bytes,
-                                fused_permutes=True)
+                                fused_permutes=True, order_cache=_ORDER_CACHE)
         out_dims = slot_dims(plan, dims)[plan.output_slot]
         p = NativePlan(plan, operands, out_structure, out_dims)
         if out_structure is None and legs is not None:

@@ -79,6 +79,25 @@ hipError_t launch_zskinny(bool big_is_a, bool acc, const void* A, const void* B,
                           int conjA, int conjB, hipStream_t stream);
 int skinny_grid(int64_t J);
 
+// Block <-> fused-matrix copies of the fused sector path (sector_kernels.hip).
+// A PermBlock maps one dense block (contiguous, first leg fastest) of a
+// block-sparse tensor onto a strided rectangle of a fused per-charge matrix.
+constexpr int kPermChunk = 256;   // elements per chunk of the chunk -> block table
+struct PermBlock {
+  int64_t base;                   // first element of the block in the blocked tensor
+  int64_t size;                   // elements
+  int64_t other;                  // fused-matrix offset of the block's first element, -1 = none
+  int32_t rank;
+  int32_t dims[GRAFT_MAX_RANK];
+  int64_t ostr[GRAFT_MAX_RANK];   // fused-matrix stride of every block leg
+};
+// to_fused: fused[other + ...] = blocked[e]; otherwise blocked[e] = fused[...]
+// (zero where the block has no fused rectangle).  e runs over [0, total).
+hipError_t launch_perm_copy(bool to_fused, const void* blocked, void* fused, const PermBlock* blocks,
+                            int nblocks, const int32_t* chunk_first, int64_t total,
+                            hipStream_t stream);
+hipError_t launch_fill_zero(void* x, int64_t n, hipStream_t stream);
+
 // Krylov vector kernels (vector_kernels.hip).
 hipError_t launch_dotc_partials(int64_t n, const void* x, const void* y, double* partials,
                                 int nblocks, hipStream_t stream);

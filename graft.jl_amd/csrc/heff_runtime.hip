@@ -7,8 +7,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <chrono>
 #include <cstring>
+#include <functional>
 #include <map>
+#include <unordered_map>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -50,11 +53,23 @@ struct Block {
   int64_t size = 1;
 };
 
+struct SecHash {
+  size_t operator()(const std::vector<int32_t>& v) const {
+    uint64_t h = 1469598103934665603ull;   // FNV-1a over the sector indices
+    for (int32_t x : v) {
+      h ^= (uint32_t)x;
+      h *= 1099511628211ull;
+    }
+    return (size_t)h;
+  }
+};
+using SecIndex = std::unordered_map<std::vector<int32_t>, int, SecHash>;
+
 struct TStruct {
   int rank = 0;
   std::vector<std::vector<int64_t>> secdims;  // per leg
   std::vector<Block> blocks;
-  std::map<std::vector<int32_t>, int> index;
+  SecIndex index;
   int64_t total = 0;  // elements spanned by the buffer
 };
 
@@ -159,6 +174,21 @@ struct StepExec {
   int tr_pinned = -1;         // -1 = automatic
   int skinny = 0;             // streaming kernel eligible: 1 = A is the large operand, 2 = B
   uint64_t partial_bytes = 0; // workspace for the split-K partial tiles
+  // Fused sector path: the GEMM items above run on per-charge dense matrices
+  // gathered from A and B and scattered into C.
+  bool fused = false;
+  std::vector<PermBlock> pa, pb, pc;
+  std::vector<int32_t> ca, cb, cc;       // chunk -> first candidate block
+  int64_t fa = 0, fb = 0, fc = 0;        // fused matrix elements of A, B, C
+  int64_t totA = 0, totB = 0, totC = 0;  // element ranges of the blocked tensors
+  bool fillA = false, fillB = false;     // fused A / B have entries no block covers
+  PermBlock* d_pa = nullptr;
+  PermBlock* d_pb = nullptr;
+  PermBlock* d_pc = nullptr;
+  int32_t* d_ca = nullptr;
+  int32_t* d_cb = nullptr;
+  int32_t* d_cc = nullptr;
+  uint64_t fused_bytes = 0;
 };
 
 struct graft_plan_impl {
@@ -167,7 +197,9 @@ struct graft_plan_impl {
   std::vector<StepExec> steps;
   std::vector<int64_t> ws_offset;  // per slot byte offset in workspace (-1 = not in workspace)
   uint64_t ws_slots = 0;           // bytes of the intermediate slots
-  uint64_t ws_bytes = 0;           // slots + split-K partial region
+  uint64_t ws_bytes = 0;           // slots + split-K partial region + fused matrices
+  uint64_t ws_partial = 0;         // byte offset of the split-K partial region
+  uint64_t ws_fused = 0;           // byte offset of the fused-matrix region
   double macs = 0.0;
   void* d_meta = nullptr;  // one allocation holding every step's items/segs/koff
   int device = 0;
@@ -191,6 +223,7 @@ struct graft_ctx {
   double* d_coef = nullptr;
   double* d_comb = nullptr;      // block-combine coefficients
   bool capturing = false;        // a graph capture is open on the stream
+  int sector_fuse = 1;           // 0 never, 1 automatic, 2 every multi-block step
   uint64_t ws_generation = 0;    // bumped whenever the workspace moves
 };
 
@@ -249,8 +282,282 @@ static int choose_config(int64_t M, int64_t N, int skinny) {
   return 2;
 }
 
+static bool canonical_blocks(const TStruct& T) {
+  for (const auto& b : T.blocks) {
+    int64_t s = 1;
+    for (size_t l = 0; l < b.dims.size(); ++l) {
+      if (b.dims[l] > 1 && b.strides[l] != s) return false;
+      s *= b.dims[l];
+    }
+  }
+  return true;
+}
+
+static std::vector<int32_t> chunk_table(const std::vector<PermBlock>& blocks, int64_t total) {
+  const int64_t nchunks = (total + kPermChunk - 1) / kPermChunk;
+  std::vector<int32_t> t(nchunks + 1, 0);
+  size_t b = 0;
+  for (int64_t c = 0; c <= nchunks; ++c) {
+    const int64_t e = std::min<int64_t>(c * kPermChunk, std::max<int64_t>(total - 1, 0));
+    while (b + 1 < blocks.size() && blocks[b + 1].base <= e) ++b;
+    t[c] = (int32_t)b;
+  }
+  // a chunk's candidates run up to the block that owns its last element
+  for (int64_t c = 0; c < nchunks; ++c) {
+    const int64_t last = std::min<int64_t>((c + 1) * kPermChunk - 1, total - 1);
+    size_t hb = t[c];
+    while (hb + 1 < blocks.size() && blocks[hb + 1].base <= last) ++hb;
+    t[c + 1] = std::max<int32_t>(t[c + 1], (int32_t)hb);
+  }
+  return t;
+}
+
+// Per-charge fusion of one step (TensorKit's matrix blocks by coupled sector):
+// union-find over A-open tuples r, contracted tuples c and B-open tuples s,
+// linked by the stored A blocks (r, c) and B blocks (c, s).  Every component
+// with both A and B blocks becomes one dense GEMM C_g = A_g B_g with rows
+// (r, open index), contraction (c, contracted index), columns (s, open index).
+static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruct& B,
+                       const TStruct& C, const std::vector<int>& pos_of_out, StepExec& ex,
+                       std::string& err) {
+  const int nOA = st.n_oindA, nCA = st.n_cindA, nOB = st.n_oindB;
+  using Key = std::vector<int32_t>;
+  std::map<Key, int> rid, cid, sid;
+  auto node = [](std::map<Key, int>& m, const Key& k) {
+    auto f = m.find(k);
+    if (f != m.end()) return f->second;
+    const int id = (int)m.size();
+    m.emplace(k, id);
+    return id;
+  };
+  std::vector<Key> aR(A.blocks.size()), aC(A.blocks.size()), bC(B.blocks.size()), bS(B.blocks.size());
+  for (size_t i = 0; i < A.blocks.size(); ++i) {
+    for (int p = 0; p < nOA; ++p) aR[i].push_back(A.blocks[i].sec[st.oindA[p]]);
+    for (int q = 0; q < nCA; ++q) aC[i].push_back(A.blocks[i].sec[st.cindA[q]]);
+    node(rid, aR[i]);
+    node(cid, aC[i]);
+  }
+  for (size_t i = 0; i < B.blocks.size(); ++i) {
+    for (int q = 0; q < nCA; ++q) bC[i].push_back(B.blocks[i].sec[st.cindB[q]]);
+    for (int p = 0; p < nOB; ++p) bS[i].push_back(B.blocks[i].sec[st.oindB[p]]);
+    node(cid, bC[i]);
+    node(sid, bS[i]);
+  }
+  const int nr = (int)rid.size(), nc = (int)cid.size(), ns = (int)sid.size();
+  std::vector<int> par(nr + nc + ns);
+  std::iota(par.begin(), par.end(), 0);
+  std::function<int(int)> find = [&](int x) { return par[x] == x ? x : (par[x] = find(par[x])); };
+  auto unite = [&](int x, int y) { par[find(x)] = find(y); };
+  for (size_t i = 0; i < A.blocks.size(); ++i) unite(rid[aR[i]], nr + cid[aC[i]]);
+  for (size_t i = 0; i < B.blocks.size(); ++i) unite(nr + cid[bC[i]], nr + nc + sid[bS[i]]);
+  std::vector<char> hasA(nr + nc + ns, 0), hasB(nr + nc + ns, 0);
+  for (size_t i = 0; i < A.blocks.size(); ++i) hasA[find(rid[aR[i]])] = 1;
+  for (size_t i = 0; i < B.blocks.size(); ++i) hasB[find(nr + cid[bC[i]])] = 1;
+  // sizes and strides of a tuple over a list of legs of a tensor
+  auto tuple_size = [](const TStruct& T, const int32_t* legs, const Key& k) {
+    int64_t n = 1;
+    for (size_t i = 0; i < k.size(); ++i) n *= T.secdims[legs[i]][k[i]];
+    return n;
+  };
+  // components in a fixed order (first appearance over sorted r)
+  std::map<int, int> comp;         // root -> component index
+  struct Comp {
+    std::vector<std::pair<Key, int64_t>> rows, ks, cols;  // tuple, offset
+    int64_t M = 0, K = 0, N = 0, aoff = 0, boff = 0, coff = 0;
+  };
+  std::vector<Comp> comps;
+  auto comp_of = [&](int root) {
+    auto f = comp.find(root);
+    if (f != comp.end()) return f->second;
+    const int g = (int)comps.size();
+    comp.emplace(root, g);
+    comps.emplace_back();
+    return g;
+  };
+  std::map<Key, std::pair<int, int64_t>> rpos, cpos, spos;  // tuple -> (component, offset)
+  for (auto& kv : rid) {
+    const int root = find(kv.second);
+    if (!hasA[root] || !hasB[root]) continue;
+    Comp& g = comps[comp_of(root)];
+    const int gi = comp.at(root);
+    rpos[kv.first] = {gi, g.M};
+    g.M += tuple_size(A, st.oindA, kv.first);
+  }
+  for (auto& kv : cid) {
+    const int root = find(nr + kv.second);
+    if (!hasA[root] || !hasB[root]) continue;
+    const int gi = comp_of(root);
+    Comp& g = comps[gi];
+    cpos[kv.first] = {gi, g.K};
+    g.K += tuple_size(A, st.cindA, kv.first);
+  }
+  for (auto& kv : sid) {
+    const int root = find(nr + nc + kv.second);
+    if (!hasA[root] || !hasB[root]) continue;
+    const int gi = comp_of(root);
+    Comp& g = comps[gi];
+    spos[kv.first] = {gi, g.N};
+    g.N += tuple_size(B, st.oindB, kv.first);
+  }
+  int64_t fa = 0, fb = 0, fc = 0;
+  for (auto& g : comps) {
+    g.aoff = fa;
+    g.boff = fb;
+    g.coff = fc;
+    fa += g.M * g.K;
+    fb += g.K * g.N;
+    fc += g.M * g.N;
+  }
+  if (comps.empty()) {
+    err = "contraction step has no matching blocks";
+    return 1;
+  }
+  if (fa >= (1ll << 32) || fb >= (1ll << 32) || fc >= (1ll << 32)) {
+    err = "fused sector matrices exceed 2^32 elements";
+    return 1;
+  }
+  // mixed-radix strides of the legs of a tuple (first leg fastest)
+  auto radix = [](const TStruct& T, const int32_t* legs, const Key& k) {
+    std::vector<int64_t> r(k.size());
+    int64_t s = 1;
+    for (size_t i = 0; i < k.size(); ++i) {
+      r[i] = s;
+      s *= T.secdims[legs[i]][k[i]];
+    }
+    return r;
+  };
+  auto sorted_blocks = [](const TStruct& T) {
+    std::vector<int> idx(T.blocks.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int x, int y) { return T.blocks[x].offset < T.blocks[y].offset; });
+    return idx;
+  };
+  auto make_pb = [](const Block& b) {
+    PermBlock pbk;
+    std::memset(&pbk, 0, sizeof(pbk));
+    pbk.base = b.offset;
+    pbk.size = b.size;
+    pbk.other = -1;
+    pbk.rank = (int32_t)b.dims.size();
+    for (size_t l = 0; l < b.dims.size(); ++l) pbk.dims[l] = (int32_t)b.dims[l];
+    return pbk;
+  };
+  ex.pa.clear();
+  ex.pb.clear();
+  ex.pc.clear();
+  int64_t coveredA = 0, coveredB = 0;
+  for (int i : sorted_blocks(A)) {
+    const Block& b = A.blocks[i];
+    PermBlock pbk = make_pb(b);
+    auto fr = rpos.find(aR[i]);
+    auto fk = cpos.find(aC[i]);
+    if (fr != rpos.end() && fk != cpos.end() && fr->second.first == fk->second.first) {
+      const Comp& g = comps[fr->second.first];
+      pbk.other = g.aoff + fr->second.second + g.M * fk->second.second;
+      const auto rr = radix(A, st.oindA, aR[i]);
+      const auto rk = radix(A, st.cindA, aC[i]);
+      for (int p = 0; p < nOA; ++p) pbk.ostr[st.oindA[p]] = rr[p];
+      for (int q = 0; q < nCA; ++q) pbk.ostr[st.cindA[q]] = g.M * rk[q];
+      coveredA += b.size;
+    }
+    ex.pa.push_back(pbk);
+  }
+  for (int i : sorted_blocks(B)) {
+    const Block& b = B.blocks[i];
+    PermBlock pbk = make_pb(b);
+    auto fk = cpos.find(bC[i]);
+    auto fs = spos.find(bS[i]);
+    if (fk != cpos.end() && fs != spos.end() && fk->second.first == fs->second.first) {
+      const Comp& g = comps[fk->second.first];
+      pbk.other = g.boff + fk->second.second + g.K * fs->second.second;
+      const auto rk = radix(B, st.cindB, bC[i]);
+      const auto rs = radix(B, st.oindB, bS[i]);
+      for (int q = 0; q < nCA; ++q) pbk.ostr[st.cindB[q]] = rk[q];
+      for (int p = 0; p < nOB; ++p) pbk.ostr[st.oindB[p]] = g.K * rs[p];
+      coveredB += b.size;
+    }
+    ex.pb.push_back(pbk);
+  }
+  for (int i : sorted_blocks(C)) {
+    const Block& b = C.blocks[i];
+    PermBlock pbk = make_pb(b);
+    Key r(nOA), sk(nOB);
+    for (int p = 0; p < nOA; ++p) r[p] = b.sec[pos_of_out[p]];
+    for (int p = 0; p < nOB; ++p) sk[p] = b.sec[pos_of_out[nOA + p]];
+    auto fr = rpos.find(r);
+    auto fs = spos.find(sk);
+    if (fr != rpos.end() && fs != spos.end() && fr->second.first == fs->second.first) {
+      const Comp& g = comps[fr->second.first];
+      pbk.other = g.coff + fr->second.second + g.M * fs->second.second;
+      const auto rr = radix(A, st.oindA, r);
+      const auto rs = radix(B, st.oindB, sk);
+      for (int q = 0; q < (int)b.dims.size(); ++q) {
+        const int p = st.out[q];
+        pbk.ostr[q] = p < nOA ? rr[p] : g.M * rs[p - nOA];
+      }
+    }
+    ex.pc.push_back(pbk);
+  }
+  ex.fa = fa;
+  ex.fb = fb;
+  ex.fc = fc;
+  ex.totA = A.total;
+  ex.totB = B.total;
+  ex.totC = C.total;
+  ex.fillA = coveredA != fa;
+  ex.fillB = coveredB != fb;
+  ex.ca = chunk_table(ex.pa, A.total);
+  ex.cb = chunk_table(ex.pb, B.total);
+  ex.cc = chunk_table(ex.pc, C.total);
+  // grouped dense GEMM over the components (column-major fused matrices)
+  ex.items.clear();
+  ex.segs.clear();
+  ex.koff.clear();
+  ex.maxM = ex.maxN = ex.maxK = 0;
+  for (auto& g : comps) {
+    if (g.M == 0 || g.N == 0 || g.K == 0) continue;
+    GemmItem it;
+    std::memset(&it, 0, sizeof(it));
+    it.c_off = g.coff;
+    it.M = (int32_t)g.M;
+    it.N = (int32_t)g.N;
+    it.mrank = 1;
+    it.nrank = 1;
+    it.mdims[0] = (int32_t)g.M;
+    it.ndims[0] = (int32_t)g.N;
+    it.c_mstr[0] = 1;
+    it.c_nstr[0] = (uint32_t)g.M;
+    it.seg_begin = (int)ex.segs.size();
+    it.seg_count = 1;
+    GemmSeg sg;
+    std::memset(&sg, 0, sizeof(sg));
+    sg.a_off = g.aoff;
+    sg.b_off = g.boff;
+    sg.K = (int32_t)g.K;
+    sg.a_mstr[0] = 1;
+    sg.b_nstr[0] = (uint32_t)g.K;
+    sg.koff_idx = (int32_t)ex.koff.size();
+    for (int64_t k = 0; k < g.K; ++k) ex.koff.push_back((uint32_t)(k * g.M));
+    for (int64_t k = 0; k < g.K; ++k) ex.koff.push_back((uint32_t)k);
+    sg.a_kfast = 0;
+    sg.b_kfast = 1;
+    ex.segs.push_back(sg);
+    ex.items.push_back(it);
+    ex.maxM = std::max(ex.maxM, g.M);
+    ex.maxN = std::max(ex.maxN, g.N);
+    ex.maxK = std::max(ex.maxK, g.K);
+  }
+  ex.fused = true;
+  ex.skinny = 0;
+  ex.tr_auto = ex.tr = true;      // output m-fastest, as the dense path decides for it
+  auto al = [](int64_t x) { return (uint64_t)((x * 16 + 255) / 256) * 256; };
+  ex.fused_bytes = al(fa) + al(fb) + al(fc);
+  return 0;
+}
+
 static int compile_step(const graft_pair_step& st, const TStruct& A, const TStruct& B,
-                        const TStruct* fixed_out, TStruct& C, StepExec& ex, std::string& err) {
+                        const TStruct* fixed_out, TStruct& C, StepExec& ex, std::string& err,
+                        int fuse_mode) {
   const int nOA = st.n_oindA, nCA = st.n_cindA, nOB = st.n_oindB, nCB = st.n_cindB;
   if (nCA != nCB || nOA + nCA != A.rank || nOB + nCB != B.rank) {
     err = "pair step leg counts do not match operand ranks";
@@ -286,13 +593,13 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
     C.secdims[q] = p < nOA ? A.secdims[st.oindA[p]] : B.secdims[st.oindB[p - nOA]];
   }
   // Enumerate contributions: group B blocks by contracted sector tuple.
-  std::map<std::vector<int32_t>, std::vector<int>> bByC;
+  std::unordered_map<std::vector<int32_t>, std::vector<int>, SecHash> bByC;
   for (int bi = 0; bi < (int)B.blocks.size(); ++bi) {
     std::vector<int32_t> key(nCB);
     for (int i = 0; i < nCB; ++i) key[i] = B.blocks[bi].sec[st.cindB[i]];
     bByC[key].push_back(bi);
   }
-  std::map<std::vector<int32_t>, std::vector<std::pair<int, int>>> contrib;
+  std::unordered_map<std::vector<int32_t>, std::vector<std::pair<int, int>>, SecHash> contrib;
   for (int ai = 0; ai < (int)A.blocks.size(); ++ai) {
     std::vector<int32_t> key(nCA);
     for (int i = 0; i < nCA; ++i) key[i] = A.blocks[ai].sec[st.cindA[i]];
@@ -326,7 +633,13 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
     int64_t off = 0;
     C.blocks.clear();
     C.index.clear();
-    for (auto& kv : contrib) {
+    std::vector<const std::vector<int32_t>*> keys;
+    keys.reserve(contrib.size());
+    for (auto& kv : contrib) keys.push_back(&kv.first);
+    std::sort(keys.begin(), keys.end(),
+              [](const std::vector<int32_t>* x, const std::vector<int32_t>* y) { return *x < *y; });
+    for (const auto* kp : keys) {
+      const auto& kv = *contrib.find(*kp);
       Block blk;
       blk.sec = kv.first;
       blk.dims.resize(nOut);
@@ -363,6 +676,28 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
   ex.segs.clear();
   ex.koff.clear();
   ex.macs = 0.0;
+  // Fused sector path for steps made of many small blocks (decided from the
+  // output blocks alone, before any per-block GEMM item is built).
+  ex.fused = false;
+  if (fuse_mode > 0 && C.blocks.size() >= 2 && canonical_blocks(A) && canonical_blocks(B) &&
+      canonical_blocks(C)) {
+    const double avg = (double)C.total / (double)C.blocks.size();
+    if (fuse_mode == 2 || (avg < 64.0 * 64.0 && C.blocks.size() >= 8)) {
+      double macs = 0.0;
+      for (auto& kv : contrib) {
+        const Block& cblk = C.blocks[C.index.at(kv.first)];
+        for (auto& pr : kv.second) {
+          const Block& ab = A.blocks[pr.first];
+          double K = 1.0;
+          for (int i = 0; i < nCA; ++i) K *= (double)ab.dims[st.cindA[i]];
+          macs += (double)cblk.size * K;
+        }
+      }
+      if (build_fused(st, A, B, C, pos_of_out, ex, err)) return 1;
+      ex.macs = macs;   // algorithmic (sector) work, not the padded fused work
+      return 0;
+    }
+  }
   // Build items (one per output block) and segments.
   for (int cb = 0; cb < (int)C.blocks.size(); ++cb) {
     const Block& cblk = C.blocks[cb];
@@ -549,9 +884,14 @@ static void assign_tiles(StepExec& ex) {
 }
 
 static void update_workspace_size(graft_plan_impl& p) {
-  uint64_t part = 0;
-  for (auto& ex : p.steps) part = std::max(part, ex.partial_bytes);
-  p.ws_bytes = ((p.ws_slots + 255) / 256) * 256 + part;
+  uint64_t part = 0, fused = 0;
+  for (auto& ex : p.steps) {
+    part = std::max(part, ex.partial_bytes);
+    fused = std::max(fused, ex.fused_bytes);
+  }
+  p.ws_partial = ((p.ws_slots + 255) / 256) * 256;
+  p.ws_fused = p.ws_partial + ((part + 255) / 256) * 256;
+  p.ws_bytes = p.ws_fused + fused;
 }
 
 static int upload_step_meta(graft_plan_impl& p) {
@@ -562,6 +902,8 @@ static int upload_step_meta(graft_plan_impl& p) {
     total += al(ex.items.size() * sizeof(GemmItem));
     total += al(std::max<size_t>(1, ex.segs.size()) * sizeof(GemmSeg));
     total += al(std::max<size_t>(1, ex.koff.size()) * sizeof(uint32_t));
+    for (auto* v : {&ex.pa, &ex.pb, &ex.pc}) total += al(v->size() * sizeof(PermBlock));
+    for (auto* v : {&ex.ca, &ex.cb, &ex.cc}) total += al(v->size() * sizeof(int32_t));
   }
   if (p.d_meta) {
     (void)hipFree(p.d_meta);
@@ -583,6 +925,22 @@ static int upload_step_meta(graft_plan_impl& p) {
     if (s3) std::memcpy(host.data() + off, ex.koff.data(), s3);
     ex.d_koff = (uint32_t*)((char*)p.d_meta + off);
     off += al(std::max<size_t>(1, ex.koff.size()) * sizeof(uint32_t));
+    PermBlock** dp[3] = {&ex.d_pa, &ex.d_pb, &ex.d_pc};
+    std::vector<PermBlock>* hp[3] = {&ex.pa, &ex.pb, &ex.pc};
+    for (int i = 0; i < 3; ++i) {
+      const size_t sz = hp[i]->size() * sizeof(PermBlock);
+      if (sz) std::memcpy(host.data() + off, hp[i]->data(), sz);
+      *dp[i] = sz ? (PermBlock*)((char*)p.d_meta + off) : nullptr;
+      off += al(sz);
+    }
+    int32_t** dc[3] = {&ex.d_ca, &ex.d_cb, &ex.d_cc};
+    std::vector<int32_t>* hc[3] = {&ex.ca, &ex.cb, &ex.cc};
+    for (int i = 0; i < 3; ++i) {
+      const size_t sz = hc[i]->size() * sizeof(int32_t);
+      if (sz) std::memcpy(host.data() + off, hc[i]->data(), sz);
+      *dc[i] = sz ? (int32_t*)((char*)p.d_meta + off) : nullptr;
+      off += al(sz);
+    }
   }
   GRAFT_CHECK_HIP(hipMemcpy(p.d_meta, host.data(), total, hipMemcpyHostToDevice));
   return GRAFT_OK;
@@ -642,7 +1000,25 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
                                    ex.total_tiles, ex.conjA, ex.conjB, ctx->stream));
     return GRAFT_OK;
   }
-  void* partial = ex.ksplit > 1 ? (char*)ctx->ws + ((p.ws_slots + 255) / 256) * 256 : nullptr;
+  void* partial = ex.ksplit > 1 ? (char*)ctx->ws + p.ws_partial : nullptr;
+  void* Cdst = C;
+  if (ex.fused) {
+    // gather both operands into the per-charge matrices, GEMM, scatter below
+    auto al = [](int64_t x) { return (size_t)((x * 16 + 255) / 256) * 256; };
+    char* fb = (char*)ctx->ws + p.ws_fused;
+    void* Af = fb;
+    void* Bf = fb + al(ex.fa);
+    void* Cf = fb + al(ex.fa) + al(ex.fb);
+    if (ex.fillA) GRAFT_CHECK_HIP(launch_fill_zero(Af, ex.fa, ctx->stream));
+    if (ex.fillB) GRAFT_CHECK_HIP(launch_fill_zero(Bf, ex.fb, ctx->stream));
+    GRAFT_CHECK_HIP(launch_perm_copy(true, A, Af, ex.d_pa, (int)ex.pa.size(), ex.d_ca, ex.totA,
+                                     ctx->stream));
+    GRAFT_CHECK_HIP(launch_perm_copy(true, B, Bf, ex.d_pb, (int)ex.pb.size(), ex.d_cb, ex.totB,
+                                     ctx->stream));
+    A = Af;
+    B = Bf;
+    C = Cf;
+  }
   hipError_t e = launch_zgemm(ex.config, precision, A, B, C, ex.d_items, (int)ex.items.size(),
                               ex.d_segs, ex.d_koff, ex.total_tiles, ex.conjA, ex.conjB,
                               ex.ksplit, partial, ex.tr, ctx->stream);
@@ -651,6 +1027,9 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     return GRAFT_E_UNSUPPORTED;
   }
   GRAFT_CHECK_HIP(e);
+  if (ex.fused)
+    GRAFT_CHECK_HIP(launch_perm_copy(false, Cdst, C, ex.d_pc, (int)ex.pc.size(), ex.d_cc, ex.totC,
+                                     ctx->stream));
   return GRAFT_OK;
 }
 
@@ -728,6 +1107,13 @@ int graft_ctx_set_workspace_cap(graft_ctx* ctx, uint64_t bytes) {
 int graft_ctx_workspace_bytes(graft_ctx* ctx, uint64_t* bytes) {
   GRAFT_REQUIRE(ctx != nullptr && bytes != nullptr, "null argument");
   *bytes = ctx->ws_size;
+  return GRAFT_OK;
+}
+
+int graft_ctx_set_sector_fusion(graft_ctx* ctx, int32_t mode) {
+  GRAFT_REQUIRE(ctx != nullptr, "null context");
+  GRAFT_REQUIRE(mode >= 0 && mode <= 2, "sector fusion mode must be 0, 1 or 2");
+  ctx->sector_fuse = mode;
   return GRAFT_OK;
 }
 
@@ -826,8 +1212,12 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
     have_out = true;
   }
   p.steps.resize(nsteps);
+  static const bool timing = std::getenv("GRAFT_PLAN_TIMING") != nullptr;
+  auto now = [] { return std::chrono::steady_clock::now(); };
+  auto t_start = now();
   for (int s = 0; s < nsteps; ++s) {
     const graft_pair_step& st = steps[s];
+    auto t_step = now();
     GRAFT_REQUIRE(st.a >= 0 && st.a < nslots && st.b >= 0 && st.b < nslots && st.dst >= 0 &&
                       st.dst < nslots,
                   "step slot out of range");
@@ -843,7 +1233,8 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
                   "step leg counts out of range");
     const TStruct* fixed = (st.dst == output_slot && have_out) ? &outfixed : nullptr;
     TStruct C;
-    if (compile_step(st, p.slots[st.a], p.slots[st.b], fixed, C, p.steps[s], err)) {
+    if (compile_step(st, p.slots[st.a], p.slots[st.b], fixed, C, p.steps[s], err,
+                     ctx->sector_fuse)) {
       set_error("step " + std::to_string(s) + ": " + err);
       return GRAFT_E_INVALID;
     }
@@ -858,6 +1249,11 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
     ex.config = choose_config(ex.maxM, ex.maxN, ex.skinny);
     assign_tiles(ex);
     p.macs += ex.macs;
+    if (timing)
+      std::fprintf(stderr, "plan step %d: %.2f ms, blocks A %zu B %zu C %zu, items %zu, fused %d\n", s,
+                   std::chrono::duration<double, std::milli>(now() - t_step).count(),
+                   p.slots[st.a].blocks.size(), p.slots[st.b].blocks.size(),
+                   p.slots[st.dst].blocks.size(), ex.items.size(), (int)ex.fused);
   }
   GRAFT_REQUIRE(defined[output_slot], "output slot never produced");
   for (int s = ninputs; s < nslots; ++s)
@@ -886,8 +1282,13 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
   }
   p.ws_slots = peak;
   update_workspace_size(p);
+  auto t_up = now();
   rc = upload_step_meta(p);
   if (rc) return rc;
+  if (timing)
+    std::fprintf(stderr, "plan create: %.2f ms total, %.2f ms upload\n",
+                 std::chrono::duration<double, std::milli>(now() - t_start).count(),
+                 std::chrono::duration<double, std::milli>(now() - t_up).count());
   *out = plan.release();
   return GRAFT_OK;
 }

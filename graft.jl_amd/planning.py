@@ -111,6 +111,7 @@ class ContractionPlan:
 class SectorLeg:
     def __init__(self, sectors):
         self.sectors = [(tuple(int(x) for x in q), int(d)) for q, d in sectors]
+        self._hash = hash(tuple(self.sectors))
 
     @property
     def dim(self):
@@ -126,10 +127,10 @@ class SectorLeg:
         return [d for _, d in self.sectors]
 
     def __eq__(self, other):
-        return isinstance(other, SectorLeg) and self.sectors == other.sectors
+        return isinstance(other, SectorLeg) and self._hash == other._hash and self.sectors == other.sectors
 
     def __hash__(self):
-        return hash(tuple(self.sectors))
+        return self._hash
 
     def __repr__(self):
         return f"SectorLeg({self.sectors})"
@@ -139,26 +140,51 @@ def _zero_like(q):
     return tuple(0 for _ in q)
 
 
+_CHARGE_DIST = {}
+_CODE_BASE = 1 << 21        # charge components stay far inside (-2^20, 2^20)
+
+
+def _leg_codes(leg):
+    """(charge code, degeneracy) per sector: code(q) = sum_i q_i B^i is linear
+    in q and zero only for q = 0, so charge sums become integer sums."""
+    c = getattr(leg, "_codes", None)
+    if c is None:
+        c = leg._codes = [(sum(int(x) * _CODE_BASE ** i for i, x in enumerate(q)), d)
+                          for q, d in leg.sectors]
+    return c
+
+
 def charge_distribution(legs):
-    """dict total-charge -> number of elements over all sector tuples."""
-    dist = {}
-    for leg in legs:
-        new = {}
-        base = dist if dist else {_zero_like(leg.sectors[0][0]): 1}
-        for c, w in base.items():
-            for q, d in leg.sectors:
-                s = tuple(a + b for a, b in zip(c, q))
-                new[s] = new.get(s, 0) + w * d
-        dist = new
-    return dist
+    """dict total-charge code -> number of elements over all sector tuples.
+
+    Charges are integer codes (``_leg_codes``; the zero charge is 0, negation
+    is integer negation).  Memoized on the leg tuple; a tuple extends the
+    distribution of its prefix, so the planner's overlapping sub-networks
+    share the work.  The returned dict is shared: callers must not modify it."""
+    key = tuple(legs)
+    hit = _CHARGE_DIST.get(key)
+    if hit is not None:
+        return hit
+    if not key:
+        return {}
+    base = charge_distribution(key[:-1]) if len(key) > 1 else {0: 1}
+    codes = _leg_codes(key[-1])
+    new = {}
+    get = new.get
+    for c, w in base.items():
+        for q, d in codes:
+            s = c + q
+            new[s] = get(s, 0) + w * d
+    if len(_CHARGE_DIST) > 200000:
+        _CHARGE_DIST.clear()
+    _CHARGE_DIST[key] = new
+    return new
 
 
 def stored_elements(legs):
     if not legs:
         return 1.0
-    dist = charge_distribution(legs)
-    z = _zero_like(legs[0].sectors[0][0])
-    return float(dist.get(z, 0))
+    return float(charge_distribution(legs).get(0, 0))
 
 
 def block_count(legs):
@@ -167,7 +193,7 @@ def block_count(legs):
         return 1
     rows = charge_distribution(legs[:-1])
     cols = charge_distribution([legs[-1]])
-    return sum(1 for c in rows if tuple(-x for x in c) in cols)
+    return sum(1 for c in rows if -c in cols)
 
 
 def conj_legs(legs, conj):
@@ -182,16 +208,15 @@ def sector_pair_profile(legsA, oA, cA, legsB, oB):
     mdist = charge_distribution([legsA[i] for i in oA]) if oA else None
     kdist = charge_distribution([legsA[i] for i in cA]) if cA else None
     ndist = charge_distribution([legsB[i] for i in oB]) if oB else None
-    z = _zero_like((legsA or legsB)[0].sectors[0][0])
-    mdist = mdist or {z: 1}
-    kdist = kdist or {z: 1}
-    ndist = ndist or {z: 1}
+    mdist = mdist or {0: 1}
+    kdist = kdist or {0: 1}
+    ndist = ndist or {0: 1}
     flops = 0.0
     out_el = 0.0
     largest = 0.0
     nblocks = 0
     for c, m in mdist.items():
-        neg = tuple(-x for x in c)
+        neg = -c
         k = kdist.get(neg, 0)
         n = ndist.get(neg, 0)
         if k and n:
@@ -519,7 +544,7 @@ def compile_plan(tree, spec, dims, legs=None, strategy="env_first", canonical=Fa
         live_d[s.dst] = st["out_dense"][i]
         live_s[s.dst] = st["out_sector"][i]
     sb = float(scalar_bytes)
-    return ContractionPlan(
+    plan = ContractionPlan(
         nslots=st["nextslot"] + 1, output_slot=output, steps=st["steps"], strategy=strategy,
         flops=st["flops"], peak_elements=st["peak"], sector_flops=st["sflops"],
         sector_peak_elements=st["speak"], sector_peak_block_elements=st["sbpeak"],
@@ -529,6 +554,8 @@ def compile_plan(tree, spec, dims, legs=None, strategy="env_first", canonical=Fa
         sector_known_temporary_peak_bytes=stmp_pk * sb,
         sector_known_permutation_peak_bytes=sperm_pk * sb,
         scalar_output=spec.nopen == 0)
+    plan.tree = tree
+    return plan
 
 
 def _score(plan, mw):
@@ -561,12 +588,16 @@ def _sector_nontrivial(legs_list):
 
 
 def plan_contraction(spec, dims, legs=None, optimize=True, memory_weight=1.0, sector_aware=True,
-                     scalar_bytes=16, memory_cap_bytes=None, fused_permutes=False):
+                     scalar_bytes=16, memory_cap_bytes=None, fused_permutes=False, order_cache=None):
     """plan_contraction — planner.jl:584-700.
 
     ``dims[i]`` are the flat leg dimensions of slot i; ``legs[i]`` (optional)
     its abelian SectorLeg list.  ``fused_permutes`` selects the device
-    executor's live-memory model (see ``compile_plan``)."""
+    executor's live-memory model (see ``compile_plan``).  ``order_cache``
+    (a dict) reuses the pairwise order chosen for a sector network of the same
+    spec and leg dimensions within a factor of two: sweeps meet a new sector
+    structure on almost every bond, and the sector search costs far more than
+    compiling a known order for the new structure."""
     if not (math.isfinite(memory_weight) and memory_weight >= 0):
         raise ValueError("memory_weight must be finite and nonnegative")
     cap = math.inf if memory_cap_bytes is None else float(memory_cap_bytes)
@@ -580,6 +611,17 @@ def plan_contraction(spec, dims, legs=None, optimize=True, memory_weight=1.0, se
                 raise ValueError(f"label {l} has incompatible dimensions {label_dims[l]} and {x}")
     structural = optimize and legs is not None and _sector_nontrivial(legs)
     slegs = legs if structural else None
+    okey = None
+    if order_cache is not None and structural:
+        okey = (spec.key(), tuple(tuple(int(round(math.log2(max(x, 1)))) for x in d) for d in dims),
+                memory_weight, sector_aware, cap, fused_permutes, scalar_bytes)
+        hit = order_cache.get(okey)
+        if hit is not None:
+            tree, strategy = hit
+            p = compile_plan(tree, spec, dims, slegs, strategy, canonical=strategy == "sector_exact",
+                             scalar_bytes=scalar_bytes, fused_permutes=fused_permutes)
+            if _within_cap(p, cap):
+                return p
     heuristic = compile_plan(_heuristic_tree(spec), spec, dims, slegs, "env_first",
                              scalar_bytes=scalar_bytes, fused_permutes=fused_permutes)
     if not optimize:
@@ -614,6 +656,10 @@ def plan_contraction(spec, dims, legs=None, optimize=True, memory_weight=1.0, se
     if best is None:
         raise ValueError(f"no contraction plan fits memory_cap_bytes={cap}; env-first requires at "
                          f"least {max(heuristic.live_peak_bytes, heuristic.sector_live_peak_bytes)} bytes")
+    if okey is not None:
+        if len(order_cache) > 4096:
+            order_cache.clear()
+        order_cache[okey] = (best.tree, best.strategy)
     return best
 
 

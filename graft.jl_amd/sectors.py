@@ -291,11 +291,25 @@ def two_site_host(A, Bm, k):
 
 
 def two_site_tensor(psi, n, m):
+    """Theta on the device: one planned pair contraction of A_n and A_m over
+    their bond, written into every charge-conserving block of Theta's legs
+    (blocks the product leaves empty come out as zeros)."""
+    from .contractions import NativePlan
+    from .planning import ContractionPlan, PairStep
+    from .tensors import BlockStructure
     from .trees import childslot
     k = childslot(psi.topo, m, n)
-    A = HostBlocks.from_device(psi.tensors[n])
-    Bm = HostBlocks.from_device(psi.tensors[m])
-    return two_site_host(A, Bm, k).to_device()
+    A, Bm = psi.tensors[n], psi.tensors[m]
+    pn = A.rank - 1
+    bopen = tuple(j for j in range(Bm.rank) if j != k)
+    legs = A.blocks.legs[:-1] + [Bm.blocks.legs[j] for j in bopen]
+    out = BlockStructure.invariant(legs)
+    step = PairStep(0, 1, 2, tuple(range(pn)), (pn,), bopen, (k,), False, False,
+                    (tuple(range(len(legs))), ()))
+    plan = NativePlan(ContractionPlan(3, 2, [step]), [A, Bm], out)
+    theta = DTensor.empty(out.dims, len(legs), out)
+    plan.execute([A, Bm], theta)
+    return theta
 
 
 def split_two_site_(psi, theta, n, m, trunc_rank, center_on="n"):

@@ -11,10 +11,29 @@ from __future__ import annotations
 
 import ctypes
 
+import math
+
 import numpy as np
 import torch
 
 from . import _native as nat
+
+
+class _StructKey:
+    """Hashable identity of a block structure with the hash computed once."""
+
+    __slots__ = ("value", "_hash")
+
+    def __init__(self, value):
+        self.value = value
+        self._hash = hash(value)
+
+    def __hash__(self):
+        return self._hash
+
+    def __eq__(self, other):
+        return isinstance(other, _StructKey) and (self is other or (
+            self._hash == other._hash and self.value == other.value))
 
 
 class BlockStructure:
@@ -25,9 +44,11 @@ class BlockStructure:
         blocks = [tuple(int(x) for x in b) for b in blocks]
         self.blocks = blocks
         self.index = {b: i for i, b in enumerate(blocks)}
-        self.block_dims = [tuple(self.legs[l].sectors[s][1] for l, s in enumerate(b)) for b in blocks]
-        sizes = [int(np.prod(d)) if d else 1 for d in self.block_dims]
-        self.offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64) if sizes else np.zeros(0, np.int64)
+        degs = [[d for _, d in l.sectors] for l in self.legs]
+        self.block_dims = [tuple(degs[l][s] for l, s in enumerate(b)) for b in blocks]
+        sizes = [math.prod(d) for d in self.block_dims]
+        self.offsets = (np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64) if sizes
+                        else np.zeros(0, np.int64))
         self.sizes = sizes
         self.total = int(sum(sizes))
 
@@ -37,25 +58,36 @@ class BlockStructure:
         legs = list(legs)
         if not legs:
             return BlockStructure(legs, [()])
-        zero = tuple(0 for _ in legs[0].sectors[0][0])
-        ranges = [range(len(l.sectors)) for l in legs]
-        blocks = []
-        # depth-first enumeration with charge pruning would be faster; ranks are small
-        import itertools
-        for tup in itertools.product(*ranges):
-            tot = zero
-            for l, s in enumerate(tup):
-                tot = tuple(a + b for a, b in zip(tot, legs[l].sectors[s][0]))
-            if tot == zero:
-                blocks.append(tup)
-        return BlockStructure(legs, blocks)
+        from .planning import _leg_codes
+        n = len(legs)
+        codes = [[q for q, _ in _leg_codes(l)] for l in legs]
+        # reach[l]: charge codes the legs l.. can produce; a partial tuple over
+        # legs < l survives only if its charge can still be cancelled.
+        reach = [None] * (n + 1)
+        reach[n] = {0}
+        for l in range(n - 1, -1, -1):
+            reach[l] = {q + r for q in set(codes[l]) for r in reach[l + 1]}
+        partial = [((), 0)]
+        for l in range(n):
+            nxt = []
+            rl = reach[l + 1]
+            cl = list(enumerate(codes[l]))
+            for tup, tot in partial:
+                for s, q in cl:
+                    if -(tot + q) in rl:
+                        nxt.append((tup + (s,), tot + q))
+            partial = nxt
+        return BlockStructure(legs, sorted(tup for tup, _ in partial))
 
     @property
     def dims(self):
         return tuple(l.dim for l in self.legs)
 
     def key(self):
-        return (tuple(self.legs), tuple(self.blocks))
+        k = getattr(self, "_key", None)
+        if k is None:
+            k = self._key = _StructKey((tuple(self.legs), tuple(self.blocks)))
+        return k
 
     def permuted(self, perm):
         legs = [self.legs[p] for p in perm]
@@ -64,9 +96,13 @@ class BlockStructure:
         return BlockStructure(legs, [blocks[i] for i in order])
 
     def desc(self):
-        return nat.DescKeepalive(self.dims, ([l.degs() for l in self.legs],
-                                             np.asarray(self.blocks, dtype=np.int32).reshape(len(self.blocks), -1),
-                                             self.offsets, None))
+        d = getattr(self, "_desc", None)
+        if d is None:
+            d = self._desc = nat.DescKeepalive(
+                self.dims, ([l.degs() for l in self.legs],
+                            np.asarray(self.blocks, dtype=np.int32).reshape(len(self.blocks), -1),
+                            self.offsets, None))
+        return d
 
 
 class TensorProto:

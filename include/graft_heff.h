@@ -115,6 +115,12 @@ int graft_ctx_release_workspace(graft_ctx* ctx);
 /* Grow the workspace arena to at least `bytes` now (plans executed inside a
  * graph capture must find their workspace allocated). */
 int graft_ctx_reserve_workspace(graft_ctx* ctx, uint64_t bytes);
+/* How plans created afterwards run block-sparse steps: 0 one GEMM item per
+ * output block; 1 (default) per-charge fused matrices (gather, one grouped
+ * dense GEMM over the charge components, scatter) for steps made of many
+ * small blocks; 2 fused for every multi-block step.  TensorKit multiplies
+ * per coupled sector the same way (BlockSparse mul!, as in Backend.jl:166-222). */
+int graft_ctx_set_sector_fusion(graft_ctx* ctx, int32_t mode);
 int graft_synchronize(graft_ctx* ctx);
 const char* graft_last_error(void);
 const char* graft_version(void);

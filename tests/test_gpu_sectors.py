@@ -111,3 +111,44 @@ def test_sector_and_dense_forms_agree_at_scale(g):
     vs, _, _ = K.eigsolve(hs, theta, 1, "SR", krylovdim=20, tol=1e-12)
     vd, _, _ = K.eigsolve(hd, theta_d, 1, "SR", krylovdim=20, tol=1e-12)
     assert abs(vs[0] - vd[0]) < 1e-10 * abs(vd[0])
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_sector_fusion_modes_agree(g, mode):
+    """Per-block GEMM items (mode 0) and fused per-charge matrices (mode 2)
+    give the oracle's envs and maps; at D=48 on the 16-site tree both modes
+    agree with the automatic choice."""
+    from graft_amd import _native as nat, contractions as C, operators as O, sectors as S, trees as Tr
+    ctx = nat.default_context()
+    try:
+        ctx.set_sector_fusion(mode)
+        topo = Tr.star_topology(3, 1)
+        t = _tree(topo)
+        _say("sector fusion mode", mode, "star(3,1)")
+        rng = np.random.default_rng(3)
+        psi, host = S.random_hubbard_ttns(rng, topo, 12)
+        H = O.hubbard_sector_ttno(topo, 1.0, 4.0)
+        Wd = O.hubbard_host(topo, 1.0, 4.0)
+        dense = _dense_state(psi)
+        envs = ref.all_envs(t, dense, Wd)
+        cache = C.EnvCache(topo)
+        for (u, v), E in envs.items():
+            assert _rel(C.env_(cache, psi, H, u, v).to_numpy(), E) < TOL, (u, v)
+        theta = C.two_site_tensor(psi, 1, 0)
+        h2 = C.eff_h2(cache, psi, H, 1, 0, out_structure=theta.blocks)
+        want = ref.h2_apply(t, Wd, envs, 1, 0, ref.two_site_tensor(t, dense, 1, 0))
+        assert _rel(h2(theta).to_numpy(), want) < TOL
+        # 16-site tree: this mode against the automatic one
+        edges = [(str((i - 1) // 2), str(i)) for i in range(1, 16)]
+        topo = Tr.TreeTopology("0", edges)
+        _say("sector fusion mode", mode, "16-site D=48")
+        rng = np.random.default_rng(8)
+        psi, _ = S.random_hubbard_ttns(rng, topo, 48, sigma=1.6)
+        Hs = O.hubbard_sector_ttno(topo, 1.0, 4.0)
+        theta = C.two_site_tensor(psi, 1, 0)
+        y_mode = C.eff_h2(C.EnvCache(topo), psi, Hs, 1, 0, out_structure=theta.blocks)(theta).to_numpy()
+        ctx.set_sector_fusion(1)
+        y_auto = C.eff_h2(C.EnvCache(topo), psi, Hs, 1, 0, out_structure=theta.blocks)(theta).to_numpy()
+        assert _rel(y_mode, y_auto) < 1e-13
+    finally:
+        ctx.set_sector_fusion(1)
