@@ -209,6 +209,76 @@ def hubbard_bench(D, nsweeps=2, precision="fp64", sigma=1.6, seed=5, dense=True,
     return out
 
 
+def precision_bench(D, nsteps, dt=0.05, seed=3, arms=3, armlen=10, precisions=("fp32", "bf16x3")):
+    """TDVP1 (order 2) of the transverse-field Ising model (J=1, h=0.9) on a
+    star tree, one run per product precision from the same initial state:
+    per-step energy drift, fidelity against the FP64 run, and the error of
+    every reduced-precision Krylov exponential against the FP64 exponential of
+    the same map and vector."""
+    import torch
+    import __graft_entry__
+    __graft_entry__.load_package()
+    from graft_amd import evolution as EV, gradients as G, krylov as K, networks as N
+    from graft_amd import operators as O, tensors as T, trees as Tr
+    topo = Tr.star_topology(arms, armlen)
+    rng = np.random.default_rng(seed)
+    psi0 = N.random_ttns(rng, topo, 2, D, center=topo.root)
+    H = O.tfi_ttno(topo, 1.0, 0.9)
+    dz = -1j * dt
+    runs = {}
+    kry_err = {}
+    check_s = [0.0]
+    orig = EV.evolution_exponentiate_backend
+
+    def checked(effective, z, x, **kw):
+        y, info = K.exponentiate(effective, z, x, **kw)
+        if effective.precision != "fp64":
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            y64, _ = K.exponentiate(effective.with_precision("fp64"), z, x, **kw)
+            e = T.norm(T.axpby_(1.0, y, -1.0, y64.clone())) / max(T.norm(y64), 1e-300)
+            kry_err.setdefault(effective.precision, []).append(e)
+            check_s[0] += time.perf_counter() - t0
+        return y, info
+
+    EV.evolution_exponentiate_backend = checked
+    try:
+        for prec in ("fp64",) + tuple(precisions):
+            psi = psi0.copy()
+            ev = EV.TDVP1(order=2, precision=prec)
+            states, energies, times = [], [G.energy(psi, H)], []
+            for s in range(nsteps):
+                torch.cuda.synchronize()
+                check_s[0] = 0.0
+                t0 = time.perf_counter()
+                EV.step_(ev, psi, H, dz)
+                torch.cuda.synchronize()
+                times.append(time.perf_counter() - t0 - check_s[0])   # without the FP64 checks
+                energies.append(G.energy(psi, H))
+                states.append(psi.copy())
+            runs[prec] = dict(states=states, energies=energies, times=times)
+    finally:
+        EV.evolution_exponentiate_backend = orig
+    out = {"D": D, "steps": nsteps, "dt": dt, "sites": len(topo.ids), "results": {}}
+    ref_states = runs["fp64"]["states"]
+    for prec, r in runs.items():
+        E0 = r["energies"][0]
+        res = {"step_s_mean": float(np.mean(r["times"])),
+               "energy_drift_max": float(max(abs(e - E0) for e in r["energies"])),
+               "energy_final": r["energies"][-1]}
+        if prec != "fp64":
+            fid = [abs(G.overlap(a, b)) for a, b in zip(r["states"], ref_states)]
+            res["infidelity_vs_fp64_max"] = float(max(1 - f for f in fid))
+            res["infidelity_vs_fp64_final"] = float(1 - fid[-1])
+            res["energy_diff_vs_fp64_max"] = float(max(abs(a - b) for a, b in
+                                                       zip(r["energies"], runs["fp64"]["energies"])))
+            errs = kry_err.get(prec, [])
+            res["krylov_rel_error_max"] = float(max(errs)) if errs else None
+            res["krylov_rel_error_median"] = float(np.median(errs)) if errs else None
+        out["results"][prec] = res
+    return out
+
+
 def blas_threads():
     try:
         from threadpoolctl import threadpool_info
@@ -228,7 +298,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the heuristic tile configurations (default: autotune every step)")
-    ap.add_argument("--mode", choices=["apply", "sweep", "hubbard"], default="apply")
+    ap.add_argument("--mode", choices=["apply", "sweep", "hubbard", "precision"], default="apply")
+    ap.add_argument("--prec-D", type=int, default=64)
+    ap.add_argument("--prec-steps", type=int, default=10)
     ap.add_argument("--hub-D", type=int, default=64)
     ap.add_argument("--no-dense", action="store_true", help="hubbard mode: sector form only")
     ap.add_argument("--sweep-D", type=int, default=32)
@@ -252,6 +324,15 @@ def main():
                           "higher_is_better": False, "n_gpus": 1, "config": {
                               "workload": f"dmrg2_ sweep, depth-5 binary tree Heisenberg, maxdim {r['D']}",
                               "precision": r["precision"]}, "detail": r}), flush=True)
+        return
+    if args.mode == "precision":
+        r = precision_bench(args.prec_D, args.prec_steps)
+        print(json.dumps({"metric": "TDVP1 reduced-precision products: drift, fidelity, Krylov error",
+                          "value": r["results"].get("bf16x3", {}).get("infidelity_vs_fp64_max"),
+                          "unit": "1 - |<psi_lowp|psi_fp64>|", "higher_is_better": False, "n_gpus": 1,
+                          "config": {"workload": f"TDVP1 order 2, TFI J=1 h=0.9, star 3x10 "
+                                                 f"({r['sites']} sites), D={r['D']}, dt={r['dt']}"},
+                          "detail": r}), flush=True)
         return
     if args.mode == "hubbard":
         r = hubbard_bench(args.hub_D, args.sweeps, args.precision, dense=not args.no_dense)

@@ -152,6 +152,22 @@ def energy_and_gradient(psi, H, d=2):
     return EnergyGradient(E, Z, site, ttno)
 
 
+def overlap(bra, ket, d=2):
+    """<bra|ket> of two TTNS on the same tree (any gauge): the closure of the
+    identity TTNO with ``bra`` on the conjugated side."""
+    topo = ket.topo
+    cache = EnvCache(topo)
+    I = identity_ttno(topo, d)
+    r = topo.root
+    for w in neighbors(topo, r):
+        env_(cache, ket, I, w, r, bra)
+    spec, ops = _build_env_spec(cache, ket, I, bra, r, None)
+    plan = cache.get_plan("overlap", spec, ops)
+    out = DTensor.empty((), 0, plan.out_structure)
+    plan.execute(ops, out)
+    return complex(out.data.cpu().numpy()[0])
+
+
 def energy(psi, H, d=2):
     """E = <psi|H|psi>/<psi|psi> in any gauge (closures of H and the identity)."""
     topo = psi.topo

@@ -32,6 +32,22 @@ from . import _native as nat
 from . import tensors as T
 
 
+# A Lanczos step whose residual norm is below BREAKDOWN times the largest
+# projected-matrix entry has found an invariant subspace: the next vector
+# would be rounding noise divided by rounding noise.
+BREAKDOWN = 1e-13
+
+# Smallest Krylov tolerance a map with reduced-precision products can meet:
+# its products carry relative errors near these (FP32: 2^-24 per operation,
+# split BF16x3: the dropped lo*lo term and FP32 accumulation), so residuals
+# and exponential error estimates stall there.
+LOWP_TOL = {"fp32": 1e-6, "bf16x3": 1e-4}
+
+
+def _floor_tol(A, tol):
+    return max(tol, LOWP_TOL.get(getattr(A, "precision", "fp64"), 0.0))
+
+
 @dataclass
 class ConvergenceInfo:
     converged: int
@@ -223,7 +239,7 @@ def _device_capable(A, x0, m):
 
 
 def _eigsolve_device(A, x0, howmany, krylovdim, tol, maxiter):
-    m = max(krylovdim, howmany + 2)
+    m = min(max(krylovdim, howmany + 2), x0.numel)
     with _krylov_stream() as (prev, s):
         L = DeviceLanczos(A, x0, m)
         try:
@@ -233,12 +249,12 @@ def _eigsolve_device(A, x0, howmany, krylovdim, tol, maxiter):
             for it in range(1, maxiter + 1):
                 j0 = k
                 hs, betas = L.expand(j0, m)
-                invariant = False
+                invariant = m == x0.numel      # the basis spans the whole space
                 k = m
                 for r, j in enumerate(range(j0, m)):
                     H[: j + 1, j] = hs[r]
                     H[j + 1, j] = betas[r]
-                    if betas[r] < 1e-14 * max(1.0, abs(hs[r][-1])):
+                    if betas[r] < BREAKDOWN * max(1.0, np.abs(H[: j + 2, : j + 1]).max()):
                         k, invariant = j + 1, True
                         break
                 beta = float(H[k, k - 1].real)
@@ -280,7 +296,10 @@ def eigsolve(A, x0, howmany=1, which="SR", ishermitian=True, krylovdim=20, tol=1
     the device-resident driver (``device=False`` forces the per-call path)."""
     if which != "SR" or not ishermitian:
         raise NotImplementedError("eigsolve supports the hermitian :SR problem used by dmrg2!")
-    m = max(krylovdim, howmany + 2)
+    tol = _floor_tol(A, tol)
+    m = min(max(krylovdim, howmany + 2), x0.numel)
+    if m < howmany:
+        raise ValueError("eigsolve: more eigenpairs requested than the space has dimensions")
     if device is None:
         device = _device_capable(A, x0, m)
     if device:
@@ -291,7 +310,6 @@ def eigsolve(A, x0, howmany=1, which="SR", ishermitian=True, krylovdim=20, tol=1
     v = x0.clone()
     T.scale_(v, 1.0 / nrm)
     V = [v]
-    m = max(krylovdim, howmany + 2)
     H = np.zeros((m + 1, m + 1), dtype=np.complex128)
     k = 0            # number of vectors whose A-image is in H
     numops = 0
@@ -307,7 +325,7 @@ def eigsolve(A, x0, howmany=1, which="SR", ishermitian=True, krylovdim=20, tol=1
             beta = T.norm(w)
             H[k + 1, k] = beta
             k += 1
-            if beta < 1e-14 * max(1.0, abs(h[-1])):
+            if beta < BREAKDOWN * max(1.0, np.abs(H[: k + 1, :k]).max()):
                 del w
                 break
             T.scale_(w, 1.0 / beta)
@@ -319,7 +337,8 @@ def eigsolve(A, x0, howmany=1, which="SR", ishermitian=True, krylovdim=20, tol=1
         res = np.abs(beta * last[:howmany])
         resid = float(res.max()) if len(res) else 0.0
         converged = bool(np.all(res <= tol * np.maximum(1.0, np.abs(theta[:howmany]))))
-        invariant = len(V) == k          # breakdown: no residual vector was appended
+        # breakdown (no residual vector appended) or a basis of the whole space
+        invariant = len(V) == k or k == x0.numel
         if converged or invariant or it == maxiter:
             vecs = [T.lincomb(V[:k], U[:, j], V[0]) for j in range(howmany)]
             nconv = int(np.sum(res <= tol * np.maximum(1.0, np.abs(theta[:howmany])))) if not invariant else howmany
@@ -354,6 +373,7 @@ def _exponentiate_device(A, dz, x, krylovdim, tol, eager, maxiter):
     iteration loop picks.  The propagated vector is combined into V_0 in place."""
     total = complex(dz)
     m = krylovdim
+    whole = m == x.numel           # the Krylov space can reach the whole space: exact there
     with _krylov_stream() as (prev, s):
         L = DeviceLanczos(A, x, m)
         try:
@@ -370,7 +390,8 @@ def _exponentiate_device(A, dz, x, krylovdim, tol, eager, maxiter):
                         H[: j + 1, j] = hs[r]
                         beta = betas[r]
                         kk = j + 1
-                        breakdown = beta < 1e-14
+                        scale = max(1.0, np.abs(H[:kk, :kk]).max(), np.abs(beta))
+                        breakdown = beta < BREAKDOWN * scale or (whole and kk == m)
                         E = _expm_hermitian_small(H[:kk, :kk], frac * total)
                         err = nrm * abs(beta * E[kk - 1, 0])
                         if breakdown or (eager and err <= tol * frac) or kk == m:
@@ -393,7 +414,9 @@ def _exponentiate_device(A, dz, x, krylovdim, tol, eager, maxiter):
                 else:
                     frac /= 2.0
                     if numiter > maxiter:
-                        raise RuntimeError("exponentiate: step size underflow")
+                        raise RuntimeError(f"exponentiate: step size underflow (dim {x.numel}, "
+                                           f"krylovdim {m}, last error {err:.3e}, beta {beta:.3e}, "
+                                           f"fraction {frac:.3e})")
             y = _handoff(L.V[0].clone(), prev, s)
             return y, ConvergenceInfo(1, worst, numiter, L.numops)
         finally:
@@ -406,12 +429,17 @@ def exponentiate(A, dz, x, ishermitian=True, krylovdim=30, tol=1e-12, eager=True
 
     Returns (y, info).  The local error estimate of a step of size tau is
     ||x|| * |beta_k * [exp(tau T_k) e_1]_k|; a step is accepted when the
-    estimate is <= tol * |tau| / |dz|, else tau is halved."""
+    estimate is <= tol * |tau| / |dz|, else tau is halved.  A map with
+    reduced-precision products (FP32, split BF16x3) cannot meet a tolerance
+    below its products' accuracy: ``tol`` is raised to LOWP_TOL for it."""
     if not ishermitian:
         raise NotImplementedError("exponentiate supports hermitian maps (Lanczos)")
     total = complex(dz)
     if total == 0:
         return x.clone(), ConvergenceInfo(1, 0.0, 0, 0)
+    tol = _floor_tol(A, tol)
+    krylovdim = min(krylovdim, x.numel)
+    whole = krylovdim == x.numel
     if device is None:
         device = _device_capable(A, x, krylovdim)
     if device:
@@ -441,7 +469,8 @@ def exponentiate(A, dz, x, ishermitian=True, krylovdim=30, tol=1e-12, eager=True
             H[: len(V), k] = h
             beta = T.norm(w)
             k += 1
-            breakdown = beta < 1e-14
+            breakdown = (beta < BREAKDOWN * max(1.0, np.abs(H[:k, :k]).max())
+                         or (whole and k == krylovdim))
             Hk = H[:k, :k]
             tau = frac * total
             E = _expm_hermitian_small(Hk, tau)
@@ -464,8 +493,6 @@ def exponentiate(A, dz, x, ishermitian=True, krylovdim=30, tol=1e-12, eager=True
                 raise RuntimeError("exponentiate: step size underflow")
     return y, ConvergenceInfo(1, worst, numiter, numops)
 
-
-LOWP_TOL = {"fp32": 1e-6, "bf16x3": 1e-4}
 
 
 def eigsolve_mixed(A, x0, lowp="bf16x3", howmany=1, which="SR", ishermitian=True, krylovdim=20,
