@@ -29,7 +29,7 @@ MI355X_FP64_MFMA_TFLOPS = 78.6
 MI355X_HBM_GBPS = 8000.0
 # HBM bytes per launch of the dominant step, from the two counter runs
 # (FETCH_SIZE and WRITE_SIZE, scaled by a 1 GiB copy) summarized in this file.
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "zgemm_heff_apply_D256.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "zgemm_heff_apply_D256.txt")
 
 
 def profiled_traffic(step, D):

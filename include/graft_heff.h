@@ -113,7 +113,8 @@ int graft_ctx_set_workspace_cap(graft_ctx* ctx, uint64_t bytes);
 int graft_ctx_workspace_bytes(graft_ctx* ctx, uint64_t* bytes);
 int graft_ctx_release_workspace(graft_ctx* ctx);
 /* Grow the workspace arena to at least `bytes` now (plans executed inside a
- * graph capture must find their workspace allocated). */
+ * graph capture must find their workspace allocated).  The reference sizes
+ * its PlanWorkspace up front the same way (types.jl:539-554). */
 int graft_ctx_reserve_workspace(graft_ctx* ctx, uint64_t bytes);
 /* How plans created afterwards run block-sparse steps: 0 one GEMM item per
  * output block; 1 (default) per-charge fused matrices (gather, one grouped
@@ -199,7 +200,11 @@ int graft_vec_multi_axpy(graft_ctx* ctx, int64_t n, int32_t k, const void* const
                          const double* c2k, void* y);
 
 /* Device-resident variants (results and coefficients stay in device memory,
- * nothing synchronizes): the building blocks of a captured Lanczos step. */
+ * nothing synchronizes): the building blocks of a captured Lanczos step.
+ * They replace the same KrylovKit vector operations as above
+ * (lib/GraftGroundState/src/GroundState/GroundState.jl:25-26,
+ * lib/GraftEvolution/src/Evolution/Evolution.jl:134-135); the reference has no
+ * device-scalar form: KrylovKit reads every inner product on the host. */
 int graft_vec_multi_dotc_dev(graft_ctx* ctx, int64_t n, int32_t k, const void* const* V,
                              const void* y, double* d_out2k);
 /* y += sign * sum_j c_j V_j with c read from device memory at run time */
@@ -214,8 +219,11 @@ int graft_vec_block_combine(graft_ctx* ctx, int64_t n, int32_t k, const void* co
                             int32_t p, const double* coef, void* const* O);
 
 /* ---- HIP graphs of work issued on the context's stream ---------------- */
-/* Capture every launch issued on the context's stream between begin and end
- * (plans must have their workspace already: run them once first). */
+/* No reference counterpart (the reference has no device): whole Lanczos
+ * iterations are captured once and replayed at every restart.  Capture every
+ * launch issued on the context's stream between begin and end (plans must
+ * have their workspace already: graft_ctx_reserve_workspace or one run).
+ * A graph refuses to launch once the context's workspace has moved. */
 int graft_graph_capture_begin(graft_ctx* ctx);
 int graft_graph_capture_end(graft_ctx* ctx, graft_graph** out);
 int graft_graph_launch(graft_ctx* ctx, graft_graph* graph);
