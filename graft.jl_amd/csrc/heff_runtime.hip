@@ -45,16 +45,59 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 // Tensor structure: legs with sector sizes, blocks with sector indices,
 // offsets, dims and strides (elements).  Dense = one sector per leg, one block.
 // ---------------------------------------------------------------------------
+// Fixed-capacity vector of at most GRAFT_MAX_RANK entries: block sector
+// tuples, dims and strides without a heap allocation per block.
+template <typename T>
+struct SmallVec {
+  T v[GRAFT_MAX_RANK];
+  int32_t n = 0;
+  SmallVec() = default;
+  explicit SmallVec(size_t k) : n((int32_t)k) {
+    for (int32_t i = 0; i < n; ++i) v[i] = T();
+  }
+  size_t size() const { return (size_t)n; }
+  bool empty() const { return n == 0; }
+  void resize(size_t k) {
+    for (int32_t i = n; i < (int32_t)k; ++i) v[i] = T();
+    n = (int32_t)k;
+  }
+  void assign(size_t k, T x) {
+    n = (int32_t)k;
+    for (int32_t i = 0; i < n; ++i) v[i] = x;
+  }
+  void push_back(T x) { v[n++] = x; }
+  T& operator[](size_t i) { return v[i]; }
+  const T& operator[](size_t i) const { return v[i]; }
+  T* begin() { return v; }
+  T* end() { return v + n; }
+  const T* begin() const { return v; }
+  const T* end() const { return v + n; }
+  bool operator==(const SmallVec& o) const {
+    if (n != o.n) return false;
+    for (int32_t i = 0; i < n; ++i)
+      if (v[i] != o.v[i]) return false;
+    return true;
+  }
+  bool operator!=(const SmallVec& o) const { return !(*this == o); }
+  bool operator<(const SmallVec& o) const {
+    const int32_t m = n < o.n ? n : o.n;
+    for (int32_t i = 0; i < m; ++i)
+      if (v[i] != o.v[i]) return v[i] < o.v[i];
+    return n < o.n;
+  }
+};
+using SecKey = SmallVec<int32_t>;
+
 struct Block {
-  std::vector<int32_t> sec;
-  std::vector<int64_t> dims;
-  std::vector<int64_t> strides;
+  SecKey sec;
+  SmallVec<int64_t> dims;
+  SmallVec<int64_t> strides;
   int64_t offset = 0;
   int64_t size = 1;
 };
 
 struct SecHash {
-  size_t operator()(const std::vector<int32_t>& v) const {
+  size_t operator()(const SecKey& v) const {
     uint64_t h = 1469598103934665603ull;   // FNV-1a over the sector indices
     for (int32_t x : v) {
       h ^= (uint32_t)x;
@@ -63,7 +106,7 @@ struct SecHash {
     return (size_t)h;
   }
 };
-using SecIndex = std::unordered_map<std::vector<int32_t>, int, SecHash>;
+using SecIndex = std::unordered_map<SecKey, int, SecHash>;
 
 struct TStruct {
   int rank = 0;
@@ -321,51 +364,69 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
                        const TStruct& C, const std::vector<int>& pos_of_out, StepExec& ex,
                        std::string& err) {
   const int nOA = st.n_oindA, nCA = st.n_cindA, nOB = st.n_oindB;
-  using Key = std::vector<int32_t>;
-  std::map<Key, int> rid, cid, sid;
-  auto node = [](std::map<Key, int>& m, const Key& k) {
-    auto f = m.find(k);
-    if (f != m.end()) return f->second;
-    const int id = (int)m.size();
-    m.emplace(k, id);
-    return id;
+  using Key = SecKey;
+  // Tuple ids in order of first appearance over the blocks (deterministic).
+  struct Ids {
+    std::unordered_map<Key, int, SecHash> id;
+    std::vector<Key> keys;
+    int get(const Key& k) {
+      auto f = id.find(k);
+      if (f != id.end()) return f->second;
+      const int v = (int)keys.size();
+      id.emplace(k, v);
+      keys.push_back(k);
+      return v;
+    }
+    int find(const Key& k) const {
+      auto f = id.find(k);
+      return f == id.end() ? -1 : f->second;
+    }
   };
-  std::vector<Key> aR(A.blocks.size()), aC(A.blocks.size()), bC(B.blocks.size()), bS(B.blocks.size());
+  Ids R, Cc, S;
+  std::vector<int> aR(A.blocks.size()), aC(A.blocks.size()), bC(B.blocks.size()), bS(B.blocks.size());
+  Key key;
   for (size_t i = 0; i < A.blocks.size(); ++i) {
-    for (int p = 0; p < nOA; ++p) aR[i].push_back(A.blocks[i].sec[st.oindA[p]]);
-    for (int q = 0; q < nCA; ++q) aC[i].push_back(A.blocks[i].sec[st.cindA[q]]);
-    node(rid, aR[i]);
-    node(cid, aC[i]);
+    key.assign(nOA, 0);
+    for (int p = 0; p < nOA; ++p) key[p] = A.blocks[i].sec[st.oindA[p]];
+    aR[i] = R.get(key);
+    key.assign(nCA, 0);
+    for (int q = 0; q < nCA; ++q) key[q] = A.blocks[i].sec[st.cindA[q]];
+    aC[i] = Cc.get(key);
   }
   for (size_t i = 0; i < B.blocks.size(); ++i) {
-    for (int q = 0; q < nCA; ++q) bC[i].push_back(B.blocks[i].sec[st.cindB[q]]);
-    for (int p = 0; p < nOB; ++p) bS[i].push_back(B.blocks[i].sec[st.oindB[p]]);
-    node(cid, bC[i]);
-    node(sid, bS[i]);
+    key.assign(nCA, 0);
+    for (int q = 0; q < nCA; ++q) key[q] = B.blocks[i].sec[st.cindB[q]];
+    bC[i] = Cc.get(key);
+    key.assign(nOB, 0);
+    for (int p = 0; p < nOB; ++p) key[p] = B.blocks[i].sec[st.oindB[p]];
+    bS[i] = S.get(key);
   }
-  const int nr = (int)rid.size(), nc = (int)cid.size(), ns = (int)sid.size();
+  const int nr = (int)R.keys.size(), nc = (int)Cc.keys.size(), ns = (int)S.keys.size();
   std::vector<int> par(nr + nc + ns);
   std::iota(par.begin(), par.end(), 0);
-  std::function<int(int)> find = [&](int x) { return par[x] == x ? x : (par[x] = find(par[x])); };
+  auto find = [&](int x) {
+    while (par[x] != x) {
+      par[x] = par[par[x]];
+      x = par[x];
+    }
+    return x;
+  };
   auto unite = [&](int x, int y) { par[find(x)] = find(y); };
-  for (size_t i = 0; i < A.blocks.size(); ++i) unite(rid[aR[i]], nr + cid[aC[i]]);
-  for (size_t i = 0; i < B.blocks.size(); ++i) unite(nr + cid[bC[i]], nr + nc + sid[bS[i]]);
+  for (size_t i = 0; i < A.blocks.size(); ++i) unite(aR[i], nr + aC[i]);
+  for (size_t i = 0; i < B.blocks.size(); ++i) unite(nr + bC[i], nr + nc + bS[i]);
   std::vector<char> hasA(nr + nc + ns, 0), hasB(nr + nc + ns, 0);
-  for (size_t i = 0; i < A.blocks.size(); ++i) hasA[find(rid[aR[i]])] = 1;
-  for (size_t i = 0; i < B.blocks.size(); ++i) hasB[find(nr + cid[bC[i]])] = 1;
-  // sizes and strides of a tuple over a list of legs of a tensor
+  for (size_t i = 0; i < A.blocks.size(); ++i) hasA[find(aR[i])] = 1;
+  for (size_t i = 0; i < B.blocks.size(); ++i) hasB[find(nr + bC[i])] = 1;
   auto tuple_size = [](const TStruct& T, const int32_t* legs, const Key& k) {
     int64_t n = 1;
     for (size_t i = 0; i < k.size(); ++i) n *= T.secdims[legs[i]][k[i]];
     return n;
   };
-  // components in a fixed order (first appearance over sorted r)
-  std::map<int, int> comp;         // root -> component index
   struct Comp {
-    std::vector<std::pair<Key, int64_t>> rows, ks, cols;  // tuple, offset
     int64_t M = 0, K = 0, N = 0, aoff = 0, boff = 0, coff = 0;
   };
   std::vector<Comp> comps;
+  std::unordered_map<int, int> comp;   // union-find root -> component index
   auto comp_of = [&](int root) {
     auto f = comp.find(root);
     if (f != comp.end()) return f->second;
@@ -374,30 +435,32 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     comps.emplace_back();
     return g;
   };
-  std::map<Key, std::pair<int, int64_t>> rpos, cpos, spos;  // tuple -> (component, offset)
-  for (auto& kv : rid) {
-    const int root = find(kv.second);
-    if (!hasA[root] || !hasB[root]) continue;
-    Comp& g = comps[comp_of(root)];
-    const int gi = comp.at(root);
-    rpos[kv.first] = {gi, g.M};
-    g.M += tuple_size(A, st.oindA, kv.first);
-  }
-  for (auto& kv : cid) {
-    const int root = find(nr + kv.second);
+  // (component, offset) of every tuple id; component -1 = no GEMM
+  std::vector<int> rg(nr, -1), cg(nc, -1), sg(ns, -1);
+  std::vector<int64_t> ro(nr, 0), co(nc, 0), so(ns, 0);
+  for (int r = 0; r < nr; ++r) {
+    const int root = find(r);
     if (!hasA[root] || !hasB[root]) continue;
     const int gi = comp_of(root);
-    Comp& g = comps[gi];
-    cpos[kv.first] = {gi, g.K};
-    g.K += tuple_size(A, st.cindA, kv.first);
+    rg[r] = gi;
+    ro[r] = comps[gi].M;
+    comps[gi].M += tuple_size(A, st.oindA, R.keys[r]);
   }
-  for (auto& kv : sid) {
-    const int root = find(nr + nc + kv.second);
+  for (int c = 0; c < nc; ++c) {
+    const int root = find(nr + c);
     if (!hasA[root] || !hasB[root]) continue;
     const int gi = comp_of(root);
-    Comp& g = comps[gi];
-    spos[kv.first] = {gi, g.N};
-    g.N += tuple_size(B, st.oindB, kv.first);
+    cg[c] = gi;
+    co[c] = comps[gi].K;
+    comps[gi].K += tuple_size(A, st.cindA, Cc.keys[c]);
+  }
+  for (int q = 0; q < ns; ++q) {
+    const int root = find(nr + nc + q);
+    if (!hasA[root] || !hasB[root]) continue;
+    const int gi = comp_of(root);
+    sg[q] = gi;
+    so[q] = comps[gi].N;
+    comps[gi].N += tuple_size(B, st.oindB, S.keys[q]);
   }
   int64_t fa = 0, fb = 0, fc = 0;
   for (auto& g : comps) {
@@ -417,14 +480,12 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     return 1;
   }
   // mixed-radix strides of the legs of a tuple (first leg fastest)
-  auto radix = [](const TStruct& T, const int32_t* legs, const Key& k) {
-    std::vector<int64_t> r(k.size());
+  auto radix = [](const TStruct& T, const int32_t* legs, const Key& k, int64_t* r) {
     int64_t s = 1;
     for (size_t i = 0; i < k.size(); ++i) {
       r[i] = s;
       s *= T.secdims[legs[i]][k[i]];
     }
-    return r;
   };
   auto sorted_blocks = [](const TStruct& T) {
     std::vector<int> idx(T.blocks.size());
@@ -445,17 +506,20 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
   ex.pa.clear();
   ex.pb.clear();
   ex.pc.clear();
+  ex.pa.reserve(A.blocks.size());
+  ex.pb.reserve(B.blocks.size());
+  ex.pc.reserve(C.blocks.size());
   int64_t coveredA = 0, coveredB = 0;
+  int64_t rr[GRAFT_MAX_RANK], rk[GRAFT_MAX_RANK], rs[GRAFT_MAX_RANK];
   for (int i : sorted_blocks(A)) {
     const Block& b = A.blocks[i];
     PermBlock pbk = make_pb(b);
-    auto fr = rpos.find(aR[i]);
-    auto fk = cpos.find(aC[i]);
-    if (fr != rpos.end() && fk != cpos.end() && fr->second.first == fk->second.first) {
-      const Comp& g = comps[fr->second.first];
-      pbk.other = g.aoff + fr->second.second + g.M * fk->second.second;
-      const auto rr = radix(A, st.oindA, aR[i]);
-      const auto rk = radix(A, st.cindA, aC[i]);
+    const int g1 = rg[aR[i]], g2 = cg[aC[i]];
+    if (g1 >= 0 && g1 == g2) {
+      const Comp& g = comps[g1];
+      pbk.other = g.aoff + ro[aR[i]] + g.M * co[aC[i]];
+      radix(A, st.oindA, R.keys[aR[i]], rr);
+      radix(A, st.cindA, Cc.keys[aC[i]], rk);
       for (int p = 0; p < nOA; ++p) pbk.ostr[st.oindA[p]] = rr[p];
       for (int q = 0; q < nCA; ++q) pbk.ostr[st.cindA[q]] = g.M * rk[q];
       coveredA += b.size;
@@ -465,35 +529,33 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
   for (int i : sorted_blocks(B)) {
     const Block& b = B.blocks[i];
     PermBlock pbk = make_pb(b);
-    auto fk = cpos.find(bC[i]);
-    auto fs = spos.find(bS[i]);
-    if (fk != cpos.end() && fs != spos.end() && fk->second.first == fs->second.first) {
-      const Comp& g = comps[fk->second.first];
-      pbk.other = g.boff + fk->second.second + g.K * fs->second.second;
-      const auto rk = radix(B, st.cindB, bC[i]);
-      const auto rs = radix(B, st.oindB, bS[i]);
+    const int g1 = cg[bC[i]], g2 = sg[bS[i]];
+    if (g1 >= 0 && g1 == g2) {
+      const Comp& g = comps[g1];
+      pbk.other = g.boff + co[bC[i]] + g.K * so[bS[i]];
+      radix(B, st.cindB, Cc.keys[bC[i]], rk);
+      radix(B, st.oindB, S.keys[bS[i]], rs);
       for (int q = 0; q < nCA; ++q) pbk.ostr[st.cindB[q]] = rk[q];
       for (int p = 0; p < nOB; ++p) pbk.ostr[st.oindB[p]] = g.K * rs[p];
       coveredB += b.size;
     }
     ex.pb.push_back(pbk);
   }
+  Key rkey(nOA), skey(nOB);
   for (int i : sorted_blocks(C)) {
     const Block& b = C.blocks[i];
     PermBlock pbk = make_pb(b);
-    Key r(nOA), sk(nOB);
-    for (int p = 0; p < nOA; ++p) r[p] = b.sec[pos_of_out[p]];
-    for (int p = 0; p < nOB; ++p) sk[p] = b.sec[pos_of_out[nOA + p]];
-    auto fr = rpos.find(r);
-    auto fs = spos.find(sk);
-    if (fr != rpos.end() && fs != spos.end() && fr->second.first == fs->second.first) {
-      const Comp& g = comps[fr->second.first];
-      pbk.other = g.coff + fr->second.second + g.M * fs->second.second;
-      const auto rr = radix(A, st.oindA, r);
-      const auto rs = radix(B, st.oindB, sk);
-      for (int q = 0; q < (int)b.dims.size(); ++q) {
-        const int p = st.out[q];
-        pbk.ostr[q] = p < nOA ? rr[p] : g.M * rs[p - nOA];
+    for (int p = 0; p < nOA; ++p) rkey[p] = b.sec[pos_of_out[p]];
+    for (int p = 0; p < nOB; ++p) skey[p] = b.sec[pos_of_out[nOA + p]];
+    const int r = R.find(rkey), q = S.find(skey);
+    if (r >= 0 && q >= 0 && rg[r] >= 0 && rg[r] == sg[q]) {
+      const Comp& g = comps[rg[r]];
+      pbk.other = g.coff + ro[r] + g.M * so[q];
+      radix(A, st.oindA, rkey, rr);
+      radix(B, st.oindB, skey, rs);
+      for (int qq = 0; qq < (int)b.dims.size(); ++qq) {
+        const int p = st.out[qq];
+        pbk.ostr[qq] = p < nOA ? rr[p] : g.M * rs[p - nOA];
       }
     }
     ex.pc.push_back(pbk);
@@ -592,26 +654,34 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
     pos_of_out[p] = q;
     C.secdims[q] = p < nOA ? A.secdims[st.oindA[p]] : B.secdims[st.oindB[p - nOA]];
   }
+  static const bool timing = std::getenv("GRAFT_PLAN_TIMING") != nullptr;
+  auto clk = [] { return std::chrono::steady_clock::now(); };
+  auto ms_since = [](std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+  };
+  auto t0 = clk();
   // Enumerate contributions: group B blocks by contracted sector tuple.
-  std::unordered_map<std::vector<int32_t>, std::vector<int>, SecHash> bByC;
+  std::unordered_map<SecKey, std::vector<int>, SecHash> bByC;
   for (int bi = 0; bi < (int)B.blocks.size(); ++bi) {
-    std::vector<int32_t> key(nCB);
+    SecKey key(nCB);
     for (int i = 0; i < nCB; ++i) key[i] = B.blocks[bi].sec[st.cindB[i]];
     bByC[key].push_back(bi);
   }
-  std::unordered_map<std::vector<int32_t>, std::vector<std::pair<int, int>>, SecHash> contrib;
+  std::unordered_map<SecKey, std::vector<std::pair<int, int>>, SecHash> contrib;
   for (int ai = 0; ai < (int)A.blocks.size(); ++ai) {
-    std::vector<int32_t> key(nCA);
+    SecKey key(nCA);
     for (int i = 0; i < nCA; ++i) key[i] = A.blocks[ai].sec[st.cindA[i]];
     auto f = bByC.find(key);
     if (f == bByC.end()) continue;
     for (int bi : f->second) {
-      std::vector<int32_t> csec(nOut);
+      SecKey csec(nOut);
       for (int p = 0; p < nOA; ++p) csec[pos_of_out[p]] = A.blocks[ai].sec[st.oindA[p]];
       for (int p = 0; p < nOB; ++p) csec[pos_of_out[nOA + p]] = B.blocks[bi].sec[st.oindB[p]];
       contrib[csec].push_back({ai, bi});
     }
   }
+  const double t_contrib = timing ? ms_since(t0) : 0.0;
+  auto t1 = clk();
   // Output blocks.
   if (fixed_out != nullptr) {
     if (fixed_out->rank != nOut) {
@@ -633,11 +703,13 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
     int64_t off = 0;
     C.blocks.clear();
     C.index.clear();
-    std::vector<const std::vector<int32_t>*> keys;
+    std::vector<const SecKey*> keys;
     keys.reserve(contrib.size());
+    C.blocks.reserve(contrib.size());
+    C.index.reserve(contrib.size());
     for (auto& kv : contrib) keys.push_back(&kv.first);
     std::sort(keys.begin(), keys.end(),
-              [](const std::vector<int32_t>* x, const std::vector<int32_t>* y) { return *x < *y; });
+              [](const SecKey* x, const SecKey* y) { return *x < *y; });
     for (const auto* kp : keys) {
       const auto& kv = *contrib.find(*kp);
       Block blk;
@@ -653,8 +725,8 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
       blk.size = s;
       blk.offset = off;
       off += s;
-      C.index[blk.sec] = (int)C.blocks.size();
-      C.blocks.push_back(blk);
+      C.index.emplace(blk.sec, (int)C.blocks.size());
+      C.blocks.push_back(std::move(blk));
     }
     C.total = off;
     if (C.blocks.empty()) {
@@ -662,6 +734,7 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
       return 1;
     }
   }
+  const double t_cblocks = timing ? ms_since(t1) : 0.0;
   if ((uint64_t)A.total >= (1ull << 32) || (uint64_t)B.total >= (1ull << 32) ||
       (uint64_t)C.total >= (1ull << 32)) {
     err = "tensor exceeds 2^32 elements (32-bit tile offsets)";
@@ -693,7 +766,11 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
           macs += (double)cblk.size * K;
         }
       }
+      auto t2 = clk();
       if (build_fused(st, A, B, C, pos_of_out, ex, err)) return 1;
+      if (timing)
+        std::fprintf(stderr, "  contrib %.2f ms (%zu keys), C blocks %.2f ms, fused build %.2f ms\n",
+                     t_contrib, contrib.size(), t_cblocks, ms_since(t2));
       ex.macs = macs;   // algorithmic (sector) work, not the padded fused work
       return 0;
     }
