@@ -327,6 +327,7 @@ static int choose_config(int64_t M, int64_t N, int skinny) {
 
 static bool canonical_blocks(const TStruct& T) {
   for (const auto& b : T.blocks) {
+    if (b.size >= (1ll << 31)) return false;   // the block copies index blocks in 32 bits
     int64_t s = 1;
     for (size_t l = 0; l < b.dims.size(); ++l) {
       if (b.dims[l] > 1 && b.strides[l] != s) return false;
@@ -503,6 +504,17 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     for (size_t l = 0; l < b.dims.size(); ++l) pbk.dims[l] = (int32_t)b.dims[l];
     return pbk;
   };
+  // Fused-matrix orientation per operand: the leg that is fastest inside the
+  // blocks is made fastest in the fused matrix too, so the copies write runs
+  // of consecutive elements instead of one element per stride.
+  auto is_in = [](int leg, const int32_t* list, int n) {
+    for (int i = 0; i < n; ++i)
+      if (list[i] == leg) return true;
+    return false;
+  };
+  const bool a_kf = is_in(0, st.cindA, nCA);         // A_g stored (m, k) at m*K + k
+  const bool b_kf = is_in(0, st.cindB, nCA);         // B_g stored (k, n) at k + K*n
+  const bool c_mf = st.out[0] < nOA;                 // C_g stored (m, n) at m + M*n
   ex.pa.clear();
   ex.pb.clear();
   ex.pc.clear();
@@ -517,11 +529,12 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     const int g1 = rg[aR[i]], g2 = cg[aC[i]];
     if (g1 >= 0 && g1 == g2) {
       const Comp& g = comps[g1];
-      pbk.other = g.aoff + ro[aR[i]] + g.M * co[aC[i]];
+      const int64_t sm = a_kf ? g.K : 1, sk = a_kf ? 1 : g.M;
+      pbk.other = g.aoff + ro[aR[i]] * sm + co[aC[i]] * sk;
       radix(A, st.oindA, R.keys[aR[i]], rr);
       radix(A, st.cindA, Cc.keys[aC[i]], rk);
-      for (int p = 0; p < nOA; ++p) pbk.ostr[st.oindA[p]] = rr[p];
-      for (int q = 0; q < nCA; ++q) pbk.ostr[st.cindA[q]] = g.M * rk[q];
+      for (int p = 0; p < nOA; ++p) pbk.ostr[st.oindA[p]] = rr[p] * sm;
+      for (int q = 0; q < nCA; ++q) pbk.ostr[st.cindA[q]] = rk[q] * sk;
       coveredA += b.size;
     }
     ex.pa.push_back(pbk);
@@ -532,11 +545,12 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     const int g1 = cg[bC[i]], g2 = sg[bS[i]];
     if (g1 >= 0 && g1 == g2) {
       const Comp& g = comps[g1];
-      pbk.other = g.boff + co[bC[i]] + g.K * so[bS[i]];
+      const int64_t sk = b_kf ? 1 : g.N, sn = b_kf ? g.K : 1;
+      pbk.other = g.boff + co[bC[i]] * sk + so[bS[i]] * sn;
       radix(B, st.cindB, Cc.keys[bC[i]], rk);
       radix(B, st.oindB, S.keys[bS[i]], rs);
-      for (int q = 0; q < nCA; ++q) pbk.ostr[st.cindB[q]] = rk[q];
-      for (int p = 0; p < nOB; ++p) pbk.ostr[st.oindB[p]] = g.K * rs[p];
+      for (int q = 0; q < nCA; ++q) pbk.ostr[st.cindB[q]] = rk[q] * sk;
+      for (int p = 0; p < nOB; ++p) pbk.ostr[st.oindB[p]] = rs[p] * sn;
       coveredB += b.size;
     }
     ex.pb.push_back(pbk);
@@ -550,12 +564,13 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     const int r = R.find(rkey), q = S.find(skey);
     if (r >= 0 && q >= 0 && rg[r] >= 0 && rg[r] == sg[q]) {
       const Comp& g = comps[rg[r]];
-      pbk.other = g.coff + ro[r] + g.M * so[q];
+      const int64_t sm = c_mf ? 1 : g.N, sn = c_mf ? g.M : 1;
+      pbk.other = g.coff + ro[r] * sm + so[q] * sn;
       radix(A, st.oindA, rkey, rr);
       radix(B, st.oindB, skey, rs);
       for (int qq = 0; qq < (int)b.dims.size(); ++qq) {
         const int p = st.out[qq];
-        pbk.ostr[qq] = p < nOA ? rr[p] : g.M * rs[p - nOA];
+        pbk.ostr[qq] = p < nOA ? rr[p] * sm : rs[p - nOA] * sn;
       }
     }
     ex.pc.push_back(pbk);
@@ -587,8 +602,8 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     it.nrank = 1;
     it.mdims[0] = (int32_t)g.M;
     it.ndims[0] = (int32_t)g.N;
-    it.c_mstr[0] = 1;
-    it.c_nstr[0] = (uint32_t)g.M;
+    it.c_mstr[0] = c_mf ? 1u : (uint32_t)g.N;
+    it.c_nstr[0] = c_mf ? (uint32_t)g.M : 1u;
     it.seg_begin = (int)ex.segs.size();
     it.seg_count = 1;
     GemmSeg sg;
@@ -596,13 +611,13 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     sg.a_off = g.aoff;
     sg.b_off = g.boff;
     sg.K = (int32_t)g.K;
-    sg.a_mstr[0] = 1;
-    sg.b_nstr[0] = (uint32_t)g.K;
+    sg.a_mstr[0] = a_kf ? (uint32_t)g.K : 1u;
+    sg.b_nstr[0] = b_kf ? (uint32_t)g.K : 1u;
     sg.koff_idx = (int32_t)ex.koff.size();
-    for (int64_t k = 0; k < g.K; ++k) ex.koff.push_back((uint32_t)(k * g.M));
-    for (int64_t k = 0; k < g.K; ++k) ex.koff.push_back((uint32_t)k);
-    sg.a_kfast = 0;
-    sg.b_kfast = 1;
+    for (int64_t k = 0; k < g.K; ++k) ex.koff.push_back((uint32_t)(a_kf ? k : k * g.M));
+    for (int64_t k = 0; k < g.K; ++k) ex.koff.push_back((uint32_t)(b_kf ? k : k * g.N));
+    sg.a_kfast = a_kf ? 1 : 0;
+    sg.b_kfast = b_kf ? 1 : 0;
     ex.segs.push_back(sg);
     ex.items.push_back(it);
     ex.maxM = std::max(ex.maxM, g.M);
@@ -610,8 +625,22 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     ex.maxK = std::max(ex.maxK, g.K);
   }
   ex.fused = true;
+  ex.tr_auto = ex.tr = c_mf;      // transposed MFMA issue when the output is m-fastest
+  // Streaming kernel when every component has a short K and one small side
+  // on the same operand (the TTNO-absorb steps of a sector map).
   ex.skinny = 0;
-  ex.tr_auto = ex.tr = true;      // output m-fastest, as the dense path decides for it
+  {
+    auto fits = [](int64_t K, int64_t S) {
+      return (K <= kSkinnyMaxK && S <= kSkinnyMaxS) || (K <= kSkinnyAccMaxK && S <= kSkinnyAccMaxS);
+    };
+    bool all1 = !ex.items.empty(), all2 = !ex.items.empty();
+    for (size_t i = 0; i < ex.items.size(); ++i) {
+      const int64_t K = ex.segs[ex.items[i].seg_begin].K, M = ex.items[i].M, N = ex.items[i].N;
+      all1 = all1 && fits(K, N) && M > N;
+      all2 = all2 && fits(K, M) && N > M;
+    }
+    ex.skinny = all1 ? 1 : (all2 ? 2 : 0);
+  }
   auto al = [](int64_t x) { return (uint64_t)((x * 16 + 255) / 256) * 256; };
   ex.fused_bytes = al(fa) + al(fb) + al(fc);
   return 0;
@@ -1068,19 +1097,10 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     set_error("null operand pointer");
     return GRAFT_E_INVALID;
   }
-  if (ex.config == kSkinnyConfig) {
-    // HBM-bound: the streaming kernel always multiplies in FP64, whatever
-    // precision the compute-bound steps use.
-    const int64_t S = ex.skinny == 1 ? ex.items[0].N : ex.items[0].M;
-    const bool acc = ex.segs[0].K > kSkinnyMaxK || S > kSkinnyMaxS;
-    GRAFT_CHECK_HIP(launch_zskinny(ex.skinny == 1, acc, A, B, C, ex.d_items, ex.d_segs, ex.d_koff,
-                                   ex.total_tiles, ex.conjA, ex.conjB, ctx->stream));
-    return GRAFT_OK;
-  }
   void* partial = ex.ksplit > 1 ? (char*)ctx->ws + p.ws_partial : nullptr;
   void* Cdst = C;
   if (ex.fused) {
-    // gather both operands into the per-charge matrices, GEMM, scatter below
+    // gather both operands into the per-charge matrices, multiply, scatter below
     auto al = [](int64_t x) { return (size_t)((x * 16 + 255) / 256) * 256; };
     char* fb = (char*)ctx->ws + p.ws_fused;
     void* Af = fb;
@@ -1096,14 +1116,30 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     B = Bf;
     C = Cf;
   }
-  hipError_t e = launch_zgemm(ex.config, precision, A, B, C, ex.d_items, (int)ex.items.size(),
-                              ex.d_segs, ex.d_koff, ex.total_tiles, ex.conjA, ex.conjB,
-                              ex.ksplit, partial, ex.tr, ctx->stream);
-  if (e == hipErrorNotSupported) {
-    set_error("precision not supported by this build");
-    return GRAFT_E_UNSUPPORTED;
+  if (ex.config == kSkinnyConfig) {
+    // HBM-bound: the streaming kernel always multiplies in FP64, whatever
+    // precision the compute-bound steps use.  One launch per item (a fused
+    // step has one item per charge component).
+    for (size_t i = 0; i < ex.items.size(); ++i) {
+      const GemmItem& it = ex.items[i];
+      const GemmSeg& sg = ex.segs[it.seg_begin];
+      const int64_t S = ex.skinny == 1 ? it.N : it.M;
+      const bool acc = sg.K > kSkinnyMaxK || S > kSkinnyMaxS;
+      const int grid = ex.items.size() == 1 ? ex.total_tiles : skinny_grid(ex.skinny == 1 ? it.M : it.N);
+      GRAFT_CHECK_HIP(launch_zskinny(ex.skinny == 1, acc, A, B, C, ex.d_items + i,
+                                     ex.d_segs + it.seg_begin, ex.d_koff, grid, ex.conjA, ex.conjB,
+                                     ctx->stream));
+    }
+  } else {
+    hipError_t e = launch_zgemm(ex.config, precision, A, B, C, ex.d_items, (int)ex.items.size(),
+                                ex.d_segs, ex.d_koff, ex.total_tiles, ex.conjA, ex.conjB,
+                                ex.ksplit, partial, ex.tr, ctx->stream);
+    if (e == hipErrorNotSupported) {
+      set_error("precision not supported by this build");
+      return GRAFT_E_UNSUPPORTED;
+    }
+    GRAFT_CHECK_HIP(e);
   }
-  GRAFT_CHECK_HIP(e);
   if (ex.fused)
     GRAFT_CHECK_HIP(launch_perm_copy(false, Cdst, C, ex.d_pc, (int)ex.pc.size(), ex.d_cc, ex.totC,
                                      ctx->stream));

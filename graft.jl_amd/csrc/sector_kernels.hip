@@ -35,12 +35,14 @@ __global__ void __launch_bounds__(kPermThreads)
       if (!to_fused) dst[e] = double2{0.0, 0.0};
       continue;
     }
+    // blocks hold < 2^31 elements: 32-bit index arithmetic
     int64_t o = b.other;
+    uint32_t loc = (uint32_t)local;
     for (int l = 0; l < b.rank; ++l) {
-      const int64_t d = b.dims[l];
-      const int64_t q = local / d;
-      o += (local - q * d) * b.ostr[l];
-      local = q;
+      const uint32_t d = (uint32_t)b.dims[l];
+      const uint32_t q = loc / d;
+      o += (int64_t)(loc - q * d) * b.ostr[l];
+      loc = q;
     }
     if (to_fused) dst[o] = src[e];
     else dst[e] = src[o];
