@@ -279,6 +279,32 @@ def precision_bench(D, nsteps, dt=0.05, seed=3, arms=3, armlen=10, precisions=("
     return out
 
 
+def hubbard_apply_bench(D, reps=20, sigma=1.6, seed=5):
+    """One root-edge U(1) x U(1) eff_h2 on the 16-site Hubbard tree: per-step
+    and whole-apply timing of the sector-batched map (the profiled workload)."""
+    import torch
+    topo = hubbard_tree()
+    from graft_amd import contractions as C, operators as O, sectors as S
+    rng = np.random.default_rng(seed)
+    psi, _ = S.random_hubbard_ttns(rng, topo, D, sigma=sigma)
+    H = O.hubbard_sector_ttno(topo, 1.0, 4.0)
+    cache = C.EnvCache(topo)
+    x = C.two_site_tensor(psi, 1, 0)
+    h2 = C.eff_h2(cache, psi, H, 1, 0, out_structure=x.blocks)
+    y = h2(x)
+    step_ms = h2.native.execute_timed(h2.operands(x), y)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y = h2(x, out=y)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"D": D, "ms_per_apply": ms, "flops": h2.flops, "tflops": h2.flops / ms / 1e9,
+            "step_ms": step_ms, "steps": h2.native.step_info(), "theta_elements": int(x.numel)}
+
+
 def blas_threads():
     try:
         from threadpoolctl import threadpool_info
@@ -298,7 +324,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the heuristic tile configurations (default: autotune every step)")
-    ap.add_argument("--mode", choices=["apply", "sweep", "hubbard", "precision"], default="apply")
+    ap.add_argument("--mode", choices=["apply", "sweep", "hubbard", "hubbard-apply", "precision"],
+                    default="apply")
     ap.add_argument("--prec-D", type=int, default=64)
     ap.add_argument("--prec-steps", type=int, default=10)
     ap.add_argument("--hub-D", type=int, default=64)
@@ -324,6 +351,13 @@ def main():
                           "higher_is_better": False, "n_gpus": 1, "config": {
                               "workload": f"dmrg2_ sweep, depth-5 binary tree Heisenberg, maxdim {r['D']}",
                               "precision": r["precision"]}, "detail": r}), flush=True)
+        return
+    if args.mode == "hubbard-apply":
+        r = hubbard_apply_bench(args.hub_D)
+        print(json.dumps({"metric": "sector-batched eff_h2 ms per apply", "value": r["ms_per_apply"],
+                          "unit": "ms", "higher_is_better": False, "n_gpus": 1,
+                          "config": {"workload": f"eff_h2 root edge, Hubbard 16-site tree, U(1)xU(1), "
+                                                 f"D={r['D']}"}, "detail": r}), flush=True)
         return
     if args.mode == "precision":
         r = precision_bench(args.prec_D, args.prec_steps)

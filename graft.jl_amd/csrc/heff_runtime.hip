@@ -480,10 +480,23 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     err = "fused sector matrices exceed 2^32 elements";
     return 1;
   }
-  // mixed-radix strides of the legs of a tuple (first leg fastest)
-  auto radix = [](const TStruct& T, const int32_t* legs, const Key& k, int64_t* r) {
+  // Mixed-radix strides of the legs of a tuple, the legs taken in `order`
+  // (positions into `legs`).  Rows, contracted indices and columns follow the
+  // memory order of the operand that owns them, so a block's consecutive
+  // elements tend to land on consecutive fused-matrix entries.
+  auto order_of = [](const int32_t* legs, int n) {
+    std::vector<int> o(n);
+    std::iota(o.begin(), o.end(), 0);
+    std::sort(o.begin(), o.end(), [&](int x, int y) { return legs[x] < legs[y]; });
+    return o;
+  };
+  const std::vector<int> rorder = order_of(st.oindA, nOA);
+  const std::vector<int> sorder = order_of(st.oindB, nOB);
+  const std::vector<int> korder = A.total >= B.total ? order_of(st.cindA, nCA) : order_of(st.cindB, nCA);
+  auto radix_o = [](const TStruct& T, const int32_t* legs, const Key& k, const std::vector<int>& ord,
+                    int64_t* r) {
     int64_t s = 1;
-    for (size_t i = 0; i < k.size(); ++i) {
+    for (int i : ord) {
       r[i] = s;
       s *= T.secdims[legs[i]][k[i]];
     }
@@ -504,17 +517,81 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     for (size_t l = 0; l < b.dims.size(); ++l) pbk.dims[l] = (int32_t)b.dims[l];
     return pbk;
   };
-  // Fused-matrix orientation per operand: the leg that is fastest inside the
-  // blocks is made fastest in the fused matrix too, so the copies write runs
-  // of consecutive elements instead of one element per stride.
-  auto is_in = [](int leg, const int32_t* list, int n) {
-    for (int i = 0; i < n; ++i)
-      if (list[i] == leg) return true;
-    return false;
+  // Fused-matrix orientation per operand, chosen to minimize the number of
+  // separate runs the block copies write or read: a run continues while the
+  // next block leg (in memory order, unit legs skipped) steps the fused offset
+  // by the run length so far.
+  auto runs = [](const Block& b, const int64_t* ostr) {
+    int64_t run = 1;
+    for (size_t l = 0; l < b.dims.size(); ++l) {
+      if (b.dims[l] == 1) continue;
+      if (ostr[l] != run) break;
+      run *= b.dims[l];
+    }
+    return (double)b.size / (double)run;
   };
-  const bool a_kf = is_in(0, st.cindA, nCA);         // A_g stored (m, k) at m*K + k
-  const bool b_kf = is_in(0, st.cindB, nCA);         // B_g stored (k, n) at k + K*n
-  const bool c_mf = st.out[0] < nOA;                 // C_g stored (m, n) at m + M*n
+  int64_t rr[GRAFT_MAX_RANK], rk[GRAFT_MAX_RANK], rs[GRAFT_MAX_RANK];
+  int64_t o1[GRAFT_MAX_RANK], o2[GRAFT_MAX_RANK];
+  // A: (m, k) at m*K + k (a_kf) or m + M*k
+  auto a_strides = [&](int i, bool kf, int64_t* o) {
+    const Comp& g = comps[rg[aR[i]]];
+    radix_o(A, st.oindA, R.keys[aR[i]], rorder, rr);
+    radix_o(A, st.cindA, Cc.keys[aC[i]], korder, rk);
+    for (int p = 0; p < nOA; ++p) o[st.oindA[p]] = rr[p] * (kf ? g.K : 1);
+    for (int q = 0; q < nCA; ++q) o[st.cindA[q]] = rk[q] * (kf ? 1 : g.M);
+  };
+  // B: (k, n) at k + K*n (b_kf) or n + N*k
+  auto b_strides = [&](int i, bool kf, int64_t* o) {
+    const Comp& g = comps[cg[bC[i]]];
+    radix_o(B, st.cindB, Cc.keys[bC[i]], korder, rk);
+    radix_o(B, st.oindB, S.keys[bS[i]], sorder, rs);
+    for (int q = 0; q < nCA; ++q) o[st.cindB[q]] = rk[q] * (kf ? 1 : g.N);
+    for (int p = 0; p < nOB; ++p) o[st.oindB[p]] = rs[p] * (kf ? g.K : 1);
+  };
+  // C: (m, n) at m + M*n (c_mf) or n + N*m
+  Key rkey(nOA), skey(nOB);
+  auto c_lookup = [&](const Block& b, int& r, int& q) {
+    for (int p = 0; p < nOA; ++p) rkey[p] = b.sec[pos_of_out[p]];
+    for (int p = 0; p < nOB; ++p) skey[p] = b.sec[pos_of_out[nOA + p]];
+    r = R.find(rkey);
+    q = S.find(skey);
+    return r >= 0 && q >= 0 && rg[r] >= 0 && rg[r] == sg[q];
+  };
+  auto c_strides = [&](const Block& b, int r, int q, bool mf, int64_t* o) {
+    const Comp& g = comps[rg[r]];
+    radix_o(A, st.oindA, rkey, rorder, rr);
+    radix_o(B, st.oindB, skey, sorder, rs);
+    for (int qq = 0; qq < (int)b.dims.size(); ++qq) {
+      const int p = st.out[qq];
+      o[qq] = p < nOA ? rr[p] * (mf ? 1 : g.N) : rs[p - nOA] * (mf ? g.M : 1);
+    }
+  };
+  double ra1 = 0, ra2 = 0, rb1 = 0, rb2 = 0, rc1 = 0, rc2 = 0;
+  for (size_t i = 0; i < A.blocks.size(); ++i) {
+    if (rg[aR[i]] < 0 || rg[aR[i]] != cg[aC[i]]) continue;
+    a_strides((int)i, false, o1);
+    a_strides((int)i, true, o2);
+    ra1 += runs(A.blocks[i], o1);
+    ra2 += runs(A.blocks[i], o2);
+  }
+  for (size_t i = 0; i < B.blocks.size(); ++i) {
+    if (cg[bC[i]] < 0 || cg[bC[i]] != sg[bS[i]]) continue;
+    b_strides((int)i, false, o1);
+    b_strides((int)i, true, o2);
+    rb1 += runs(B.blocks[i], o1);
+    rb2 += runs(B.blocks[i], o2);
+  }
+  for (size_t i = 0; i < C.blocks.size(); ++i) {
+    int r, q;
+    if (!c_lookup(C.blocks[i], r, q)) continue;
+    c_strides(C.blocks[i], r, q, true, o1);
+    c_strides(C.blocks[i], r, q, false, o2);
+    rc1 += runs(C.blocks[i], o1);
+    rc2 += runs(C.blocks[i], o2);
+  }
+  const bool a_kf = ra2 < ra1;
+  const bool b_kf = rb2 < rb1;
+  const bool c_mf = rc1 <= rc2;
   ex.pa.clear();
   ex.pb.clear();
   ex.pc.clear();
@@ -522,19 +599,15 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
   ex.pb.reserve(B.blocks.size());
   ex.pc.reserve(C.blocks.size());
   int64_t coveredA = 0, coveredB = 0;
-  int64_t rr[GRAFT_MAX_RANK], rk[GRAFT_MAX_RANK], rs[GRAFT_MAX_RANK];
   for (int i : sorted_blocks(A)) {
     const Block& b = A.blocks[i];
     PermBlock pbk = make_pb(b);
     const int g1 = rg[aR[i]], g2 = cg[aC[i]];
     if (g1 >= 0 && g1 == g2) {
       const Comp& g = comps[g1];
-      const int64_t sm = a_kf ? g.K : 1, sk = a_kf ? 1 : g.M;
-      pbk.other = g.aoff + ro[aR[i]] * sm + co[aC[i]] * sk;
-      radix(A, st.oindA, R.keys[aR[i]], rr);
-      radix(A, st.cindA, Cc.keys[aC[i]], rk);
-      for (int p = 0; p < nOA; ++p) pbk.ostr[st.oindA[p]] = rr[p] * sm;
-      for (int q = 0; q < nCA; ++q) pbk.ostr[st.cindA[q]] = rk[q] * sk;
+      pbk.other = g.aoff + ro[aR[i]] * (a_kf ? g.K : 1) + co[aC[i]] * (a_kf ? 1 : g.M);
+      a_strides(i, a_kf, o1);
+      for (size_t l = 0; l < b.dims.size(); ++l) pbk.ostr[l] = o1[l];
       coveredA += b.size;
     }
     ex.pa.push_back(pbk);
@@ -545,33 +618,22 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     const int g1 = cg[bC[i]], g2 = sg[bS[i]];
     if (g1 >= 0 && g1 == g2) {
       const Comp& g = comps[g1];
-      const int64_t sk = b_kf ? 1 : g.N, sn = b_kf ? g.K : 1;
-      pbk.other = g.boff + co[bC[i]] * sk + so[bS[i]] * sn;
-      radix(B, st.cindB, Cc.keys[bC[i]], rk);
-      radix(B, st.oindB, S.keys[bS[i]], rs);
-      for (int q = 0; q < nCA; ++q) pbk.ostr[st.cindB[q]] = rk[q] * sk;
-      for (int p = 0; p < nOB; ++p) pbk.ostr[st.oindB[p]] = rs[p] * sn;
+      pbk.other = g.boff + co[bC[i]] * (b_kf ? 1 : g.N) + so[bS[i]] * (b_kf ? g.K : 1);
+      b_strides(i, b_kf, o1);
+      for (size_t l = 0; l < b.dims.size(); ++l) pbk.ostr[l] = o1[l];
       coveredB += b.size;
     }
     ex.pb.push_back(pbk);
   }
-  Key rkey(nOA), skey(nOB);
   for (int i : sorted_blocks(C)) {
     const Block& b = C.blocks[i];
     PermBlock pbk = make_pb(b);
-    for (int p = 0; p < nOA; ++p) rkey[p] = b.sec[pos_of_out[p]];
-    for (int p = 0; p < nOB; ++p) skey[p] = b.sec[pos_of_out[nOA + p]];
-    const int r = R.find(rkey), q = S.find(skey);
-    if (r >= 0 && q >= 0 && rg[r] >= 0 && rg[r] == sg[q]) {
+    int r, q;
+    if (c_lookup(b, r, q)) {
       const Comp& g = comps[rg[r]];
-      const int64_t sm = c_mf ? 1 : g.N, sn = c_mf ? g.M : 1;
-      pbk.other = g.coff + ro[r] * sm + so[q] * sn;
-      radix(A, st.oindA, rkey, rr);
-      radix(B, st.oindB, skey, rs);
-      for (int qq = 0; qq < (int)b.dims.size(); ++qq) {
-        const int p = st.out[qq];
-        pbk.ostr[qq] = p < nOA ? rr[p] * sm : rs[p - nOA] * sn;
-      }
+      pbk.other = g.coff + ro[r] * (c_mf ? 1 : g.N) + so[q] * (c_mf ? g.M : 1);
+      c_strides(b, r, q, c_mf, o1);
+      for (size_t l = 0; l < b.dims.size(); ++l) pbk.ostr[l] = o1[l];
     }
     ex.pc.push_back(pbk);
   }

@@ -11,9 +11,13 @@ from __future__ import annotations
 
 from dataclasses import dataclass, field
 
+import numpy as np
+import torch
+
 from . import backend as B
 from . import krylov
-from .contractions import EnvCache, eff_h0, eff_h1
+from . import tensors as T
+from .contractions import EnvCache, eff_h0, eff_h1, eff_h2, two_site_tensor
 from .networks import move_center_, update_tensor_
 from .trees import childslot, nnodes, path_between, postorder
 
@@ -32,6 +36,132 @@ class TDVP1:
     eager: bool = True
     precision: str = "fp64"
     cache: object = None
+
+
+@dataclass
+class PredictorCBE:
+    """PredictorCBE(; max_add=32, spawn_threshold=1e-10, neigs=2, krylovdim=4) —
+    cbe.jl:115-149: projected two-site energy-variation selector.  Directions are
+    the leading left singular vectors of the double-complement core
+    M = (1 - Q Q^H) . H_eff(Theta) . (1 - P_partner); at most min(room, max_add,
+    neigs) are spawned, each only if |dz| * sigma / ||Theta|| > spawn_threshold.
+    The selection here is an exact eigendecomposition of the core's Gram matrix
+    on its smaller side (the reference keeps the same exact path for sector maps
+    and a 4-dimensional Golub-Kahan solve for dense ones; ``krylovdim`` is kept
+    for the interface)."""
+    max_add: int = 32
+    spawn_threshold: float = 1e-10
+    neigs: int = 2
+    krylovdim: int = 4
+
+    def __post_init__(self):
+        if self.max_add < 0:
+            raise ValueError("PredictorCBE max_add must be nonnegative")
+        if self.spawn_threshold < 0:
+            raise ValueError("PredictorCBE spawn_threshold must be nonnegative")
+        if self.neigs < 1:
+            raise ValueError("PredictorCBE neigs must be positive")
+        if self.krylovdim < self.neigs:
+            raise ValueError("PredictorCBE krylovdim must be at least neigs")
+
+
+@dataclass
+class TDVP1_CBE:
+    """TDVP1_CBE (cbe.jl:240-261): TDVP1 whose link splits spawn new bond
+    directions (controlled bond expansion) before the backward link evolution.
+    ``enabled=False`` is TDVP1 exactly."""
+    cbe: PredictorCBE = field(default_factory=PredictorCBE)
+    order: int = 2
+    trunc: B.TruncationScheme = field(default_factory=lambda: B.TruncationScheme(maxdim=100))
+    enabled: bool = True
+    krylovdim: int = 30
+    tol: float = 1e-12
+    eager: bool = True
+    precision: str = "fp64"
+    cache: object = None
+    last_cbe_info: dict = None
+
+
+def _cbe_directions(ev, psi, H, n, m, active_is_n, Q, dz, room):
+    """cbe.jl:332-352, 590-619: new orthonormal directions for the active site,
+    orthogonal to its current range Q, or None."""
+    strat = ev.cbe
+    budget = min(room, strat.max_add, strat.neigs)
+    info = {"selected": 0, "budget": budget}
+    ev.last_cbe_info = info
+    if budget <= 0 or dz == 0:
+        return None
+    theta = two_site_tensor(psi, n, m)
+    Y = eff_h2(ev.cache, psi, H, n, m)(theta)
+    pn = psi.tensors[n].ncod
+    k = childslot(psi.topo, m, n)
+    Am = psi.tensors[m]
+    Am_k = B.permute(Am, tuple(j for j in range(Am.rank) if j != k) + (k,))
+    Ym = B.matrix_view(Y, pn)                         # (n legs) x (m legs but k)
+    if active_is_n:
+        Z = Ym.clone()
+        Bp = B.matrix_view(Am_k, Am_k.rank - 1)       # partner (m legs but k) x bond
+    else:
+        Z = Ym.T.contiguous()
+        An = psi.tensors[n]
+        Bp = B.matrix_view(An, An.rank - 1)           # partner (n legs) x bond
+    # double complement: Theta = M_a . Bp^T, so the partner's range acts on the right
+    Z = Z - (Z @ Bp.conj()) @ Bp.T
+    Z = Z - Q @ (Q.conj().T @ Z)
+    norm_theta = T.norm(theta)
+    cutoff = strat.spawn_threshold * max(norm_theta, np.finfo(float).eps) / abs(complex(dz))
+    rows, cols = Z.shape
+    if cols <= rows:
+        lam, V = torch.linalg.eigh(Z.conj().T @ Z)
+        lam, V = torch.flip(lam, (0,)), torch.flip(V, (1,))
+        sig = torch.sqrt(lam.clamp(min=0.0))
+        sel = [i for i in range(min(budget, len(lam))) if float(sig[i]) > cutoff]
+        if not sel:
+            return None
+        E = (Z @ V[:, sel]) / sig[sel]
+    else:
+        lam, U = torch.linalg.eigh(Z @ Z.conj().T)
+        lam, U = torch.flip(lam, (0,)), torch.flip(U, (1,))
+        sig = torch.sqrt(lam.clamp(min=0.0))
+        sel = [i for i in range(min(budget, len(lam))) if float(sig[i]) > cutoff]
+        if not sel:
+            return None
+        E = U[:, sel]
+    # exact orthogonality to the kept range, then orthonormal columns
+    E = E - Q @ (Q.conj().T @ E)
+    E, _ = torch.linalg.qr(E, mode="reduced")
+    info["selected"] = len(sel)
+    info["sigma_max"] = float(sig[0])
+    return E
+
+
+def _cbe_split(ev, psi, H, u, v, dz):
+    """_cbe_projected_split + _cbe_state_preserving_splice (cbe.jl:575-584, 743-760):
+    A_u = U . R with U = [Q, E] and R = U^H A_u = [R_qr; 0]; the state is unchanged.
+    Returns (U as the site tensor in A_u's leg order, R :: new bond <- old bond)."""
+    t = psi.topo
+    A = psi.tensors[u]
+    if A.blocks is not None:
+        raise NotImplementedError("controlled bond expansion is built for dense states only")
+    up = t.parent[u] == v
+    if up:
+        k = A.rank - 1
+        At = A
+    else:
+        k = childslot(t, u, v)
+        At = B.permute(A, tuple(j for j in range(A.rank) if j != k) + (k,))
+    Ma = B.matrix_view(At, At.rank - 1)
+    Q, _ = torch.linalg.qr(Ma, mode="reduced")
+    room = max(0, ev.trunc.maxdim - Q.shape[1])
+    n, m = (u, v) if up else (v, u)
+    E = _cbe_directions(ev, psi, H, n, m, up, Q, dz, room)
+    U = Q if E is None else torch.cat([Q, E], dim=1)
+    R = U.conj().T @ Ma
+    Ut = B.from_matrix(U, At.dims[:-1] + (U.shape[1],), At.rank - 1)
+    if not up:
+        Ut = B.permute(Ut, B._restore_perm(A.rank, k), A.ncod)
+    Rt = B.from_matrix(R, (R.shape[0], R.shape[1]), 1)
+    return Ut, Rt
 
 
 def step_(ev, psi, H, dz):
@@ -90,8 +220,12 @@ def _evolve_link_and_move_(ev, psi, H, u, v, dz, herm):
     t = psi.topo
     cache = ev.cache
     assert psi.center == u
+    cbe = isinstance(ev, TDVP1_CBE) and ev.enabled
     if t.parent[u] == v:
-        Q, C = B.left_orth(psi.tensors[u])            # C :: V_new <- V_e
+        if cbe:
+            Q, C = _cbe_split(ev, psi, H, u, v, dz)    # expanded: C :: V_new <- V_e
+        else:
+            Q, C = B.left_orth(psi.tensors[u])        # C :: V_new <- V_e
         psi.tensors[u] = Q
         cache.invalidate_node_(u)
         k0 = eff_h0(cache, psi, H, u, v, precision=ev.precision)
@@ -100,7 +234,10 @@ def _evolve_link_and_move_(ev, psi, H, u, v, dz, herm):
         psi.tensors[v] = B.absorb_on_leg(psi.tensors[v], C, childslot(t, v, u))
     else:
         k = childslot(t, u, v)
-        Q, Cd = B.orth_factor_leg(psi.tensors[u], k)   # Cd :: Y <- dual(V_e)
+        if cbe:
+            Q, Cd = _cbe_split(ev, psi, H, u, v, dz)
+        else:
+            Q, Cd = B.orth_factor_leg(psi.tensors[u], k)   # Cd :: Y <- dual(V_e)
         psi.tensors[u] = Q
         C = B.transpose2(Cd)                           # :: V_e <- Y
         cache.invalidate_node_(u)
