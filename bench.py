@@ -209,9 +209,11 @@ def hubbard_bench(D, nsweeps=2, precision="fp64", sigma=1.6, seed=5, dense=True,
     return out
 
 
-def precision_bench(D, nsteps, dt=0.05, seed=3, arms=3, armlen=10, precisions=("fp32", "bf16x3")):
-    """TDVP1 (order 2) of the transverse-field Ising model (J=1, h=0.9) on a
-    star tree, one run per product precision from the same initial state:
+def precision_bench(D, nsteps, dt=0.05, seed=3, arms=3, armlen=10, precisions=("fp32", "bf16x3"),
+                    cbe=True):
+    """TDVP1_CBE (order 2, PredictorCBE(max_add=32), maxdim D) of the
+    transverse-field Ising model (J=1, h=0.9) on a star tree (TDVP1 with
+    ``cbe=False``), one run per product precision from the same initial state:
     per-step energy drift, fidelity against the FP64 run, and the error of
     every reduced-precision Krylov exponential against the FP64 exponential of
     the same map and vector."""
@@ -245,7 +247,12 @@ def precision_bench(D, nsteps, dt=0.05, seed=3, arms=3, armlen=10, precisions=("
     try:
         for prec in ("fp64",) + tuple(precisions):
             psi = psi0.copy()
-            ev = EV.TDVP1(order=2, precision=prec)
+            if cbe:
+                from graft_amd import backend as B
+                ev = EV.TDVP1_CBE(order=2, precision=prec, trunc=B.TruncationScheme(maxdim=D),
+                                  cbe=EV.PredictorCBE(max_add=32))
+            else:
+                ev = EV.TDVP1(order=2, precision=prec)
             states, energies, times = [], [G.energy(psi, H)], []
             for s in range(nsteps):
                 torch.cuda.synchronize()
@@ -259,7 +266,8 @@ def precision_bench(D, nsteps, dt=0.05, seed=3, arms=3, armlen=10, precisions=("
             runs[prec] = dict(states=states, energies=energies, times=times)
     finally:
         EV.evolution_exponentiate_backend = orig
-    out = {"D": D, "steps": nsteps, "dt": dt, "sites": len(topo.ids), "results": {}}
+    out = {"D": D, "steps": nsteps, "dt": dt, "sites": len(topo.ids), "evolver":
+           "TDVP1_CBE(PredictorCBE(max_add=32))" if cbe else "TDVP1", "results": {}}
     ref_states = runs["fp64"]["states"]
     for prec, r in runs.items():
         E0 = r["energies"][0]
@@ -275,6 +283,7 @@ def precision_bench(D, nsteps, dt=0.05, seed=3, arms=3, armlen=10, precisions=("
             errs = kry_err.get(prec, [])
             res["krylov_rel_error_max"] = float(max(errs)) if errs else None
             res["krylov_rel_error_median"] = float(np.median(errs)) if errs else None
+        res["bond_dims_final"] = max(r["states"][-1].bond_dims())
         out["results"][prec] = res
     return out
 
@@ -328,6 +337,7 @@ def main():
                     default="apply")
     ap.add_argument("--prec-D", type=int, default=64)
     ap.add_argument("--prec-steps", type=int, default=10)
+    ap.add_argument("--no-cbe", action="store_true", help="precision mode: TDVP1 without expansion")
     ap.add_argument("--hub-D", type=int, default=64)
     ap.add_argument("--no-dense", action="store_true", help="hubbard mode: sector form only")
     ap.add_argument("--sweep-D", type=int, default=32)
@@ -360,11 +370,11 @@ def main():
                                                  f"D={r['D']}"}, "detail": r}), flush=True)
         return
     if args.mode == "precision":
-        r = precision_bench(args.prec_D, args.prec_steps)
+        r = precision_bench(args.prec_D, args.prec_steps, cbe=not args.no_cbe)
         print(json.dumps({"metric": "TDVP1 reduced-precision products: drift, fidelity, Krylov error",
                           "value": r["results"].get("bf16x3", {}).get("infidelity_vs_fp64_max"),
                           "unit": "1 - |<psi_lowp|psi_fp64>|", "higher_is_better": False, "n_gpus": 1,
-                          "config": {"workload": f"TDVP1 order 2, TFI J=1 h=0.9, star 3x10 "
+                          "config": {"workload": f"{r['evolver']} order 2, TFI J=1 h=0.9, star 3x10 "
                                                  f"({r['sites']} sites), D={r['D']}, dt={r['dt']}"},
                           "detail": r}), flush=True)
         return
