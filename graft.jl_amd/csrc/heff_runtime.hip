@@ -231,6 +231,9 @@ struct StepExec {
   int32_t* d_ca = nullptr;
   int32_t* d_cb = nullptr;
   int32_t* d_cc = nullptr;
+  uint32_t* d_ma = nullptr;              // per-element fused offsets (built once per plan)
+  uint32_t* d_mb = nullptr;
+  uint32_t* d_mc = nullptr;
   uint64_t fused_bytes = 0;
 };
 
@@ -245,6 +248,7 @@ struct graft_plan_impl {
   uint64_t ws_fused = 0;           // byte offset of the fused-matrix region
   double macs = 0.0;
   void* d_meta = nullptr;  // one allocation holding every step's items/segs/koff
+  void* d_maps = nullptr;  // element -> fused-offset maps of every fused step
   int device = 0;
 };
 
@@ -476,7 +480,7 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     err = "contraction step has no matching blocks";
     return 1;
   }
-  if (fa >= (1ll << 32) || fb >= (1ll << 32) || fc >= (1ll << 32)) {
+  if (fa >= (1ll << 32) - 2 || fb >= (1ll << 32) - 2 || fc >= (1ll << 32) - 2) {
     err = "fused sector matrices exceed 2^32 elements";
     return 1;
   }
@@ -1114,6 +1118,33 @@ static int upload_step_meta(graft_plan_impl& p) {
   return GRAFT_OK;
 }
 
+// Element -> fused-offset maps for every fused step, built on the device once
+// per plan from the block tables (4 bytes per element of each operand).
+static int build_maps(graft_ctx* ctx, graft_plan_impl& p) {
+  size_t total = 0;
+  auto al = [](size_t n) { return (n * sizeof(uint32_t) + 255) & ~size_t(255); };
+  for (auto& ex : p.steps)
+    if (ex.fused) total += al(ex.totA) + al(ex.totB) + al(ex.totC);
+  if (total == 0) return GRAFT_OK;
+  GRAFT_CHECK_HIP(hipMalloc(&p.d_maps, total));
+  char* base = (char*)p.d_maps;
+  for (auto& ex : p.steps) {
+    if (!ex.fused) continue;
+    ex.d_ma = (uint32_t*)base;
+    base += al(ex.totA);
+    ex.d_mb = (uint32_t*)base;
+    base += al(ex.totB);
+    ex.d_mc = (uint32_t*)base;
+    base += al(ex.totC);
+    GRAFT_CHECK_HIP(launch_perm_map(ex.d_pa, (int)ex.pa.size(), ex.d_ca, ex.totA, ex.d_ma, ctx->stream));
+    GRAFT_CHECK_HIP(launch_perm_map(ex.d_pb, (int)ex.pb.size(), ex.d_cb, ex.totB, ex.d_mb, ctx->stream));
+    GRAFT_CHECK_HIP(launch_perm_map(ex.d_pc, (int)ex.pc.size(), ex.d_cc, ex.totC, ex.d_mc, ctx->stream));
+  }
+  // the maps read the block tables, which a later autotune may re-upload
+  GRAFT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return GRAFT_OK;
+}
+
 static int ensure_workspace(graft_ctx* ctx, uint64_t bytes) {
   if (bytes <= ctx->ws_size) return GRAFT_OK;
   if (ctx->capturing) {
@@ -1170,10 +1201,8 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     void* Cf = fb + al(ex.fa) + al(ex.fb);
     if (ex.fillA) GRAFT_CHECK_HIP(launch_fill_zero(Af, ex.fa, ctx->stream));
     if (ex.fillB) GRAFT_CHECK_HIP(launch_fill_zero(Bf, ex.fb, ctx->stream));
-    GRAFT_CHECK_HIP(launch_perm_copy(true, A, Af, ex.d_pa, (int)ex.pa.size(), ex.d_ca, ex.totA,
-                                     ctx->stream));
-    GRAFT_CHECK_HIP(launch_perm_copy(true, B, Bf, ex.d_pb, (int)ex.pb.size(), ex.d_cb, ex.totB,
-                                     ctx->stream));
+    GRAFT_CHECK_HIP(launch_map_copy(true, A, Af, ex.d_ma, ex.totA, ctx->stream));
+    GRAFT_CHECK_HIP(launch_map_copy(true, B, Bf, ex.d_mb, ex.totB, ctx->stream));
     A = Af;
     B = Bf;
     C = Cf;
@@ -1202,9 +1231,7 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     }
     GRAFT_CHECK_HIP(e);
   }
-  if (ex.fused)
-    GRAFT_CHECK_HIP(launch_perm_copy(false, Cdst, C, ex.d_pc, (int)ex.pc.size(), ex.d_cc, ex.totC,
-                                     ctx->stream));
+  if (ex.fused) GRAFT_CHECK_HIP(launch_map_copy(false, Cdst, C, ex.d_mc, ex.totC, ctx->stream));
   return GRAFT_OK;
 }
 
@@ -1460,6 +1487,8 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
   auto t_up = now();
   rc = upload_step_meta(p);
   if (rc) return rc;
+  rc = build_maps(ctx, p);
+  if (rc) return rc;
   if (timing)
     std::fprintf(stderr, "plan create: %.2f ms total, %.2f ms upload\n",
                  std::chrono::duration<double, std::milli>(now() - t_start).count(),
@@ -1470,9 +1499,10 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
 
 int graft_plan_destroy(graft_plan* plan) {
   if (!plan) return GRAFT_OK;
-  if (plan->impl.d_meta) {
+  if (plan->impl.d_meta || plan->impl.d_maps) {
     (void)hipSetDevice(plan->impl.device);
-    (void)hipFree(plan->impl.d_meta);
+    if (plan->impl.d_meta) (void)hipFree(plan->impl.d_meta);
+    if (plan->impl.d_maps) (void)hipFree(plan->impl.d_maps);
   }
   delete plan;
   return GRAFT_OK;

@@ -4,38 +4,42 @@
 // tensors as (1) gather of every operand block into the dense matrix of its
 // charge component (rows: open-leg sector tuples, columns: contracted sector
 // tuples), (2) one grouped dense GEMM over the components, (3) scatter of the
-// result matrices into the output blocks.  Both copies walk the blocked
-// tensor's elements in storage order (one thread per element, coalesced on
-// the blocked side); a chunk table narrows the binary search for the block
-// that owns an element to the blocks that start inside its chunk.
+// result matrices into the output blocks.  Once per plan, a map kernel walks
+// each blocked tensor's elements in storage order (a chunk table narrows the
+// binary search for the block that owns an element) and records the element's
+// fused-matrix offset; every apply then copies through that map with one
+// coalesced 4-byte read per element and no search.
 #include "heff_internal.h"
 
 namespace graft {
 
 static constexpr int kPermThreads = 256;
 
+// Run once per plan: for every element
+// of the blocked tensor its fused-matrix offset (kMapZero for a block without
+// a fused rectangle, kMapGap between blocks).
 __global__ void __launch_bounds__(kPermThreads)
-    perm_copy_kernel(int to_fused, const double2* __restrict__ src, double2* __restrict__ dst,
-                     const PermBlock* __restrict__ blocks, int nblocks,
-                     const int32_t* __restrict__ chunk_first, int64_t total) {
+    perm_map_kernel(const PermBlock* __restrict__ blocks, const int32_t* __restrict__ chunk_first,
+                    int64_t total, uint32_t* __restrict__ map) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
     const int64_t c = e / kPermChunk;
     int lo = chunk_first[c], hi = chunk_first[c + 1];
-    // last block with base <= e in [lo, hi]
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (blocks[mid].base <= e) lo = mid;
       else hi = mid - 1;
     }
     const PermBlock& b = blocks[lo];
-    int64_t local = e - b.base;
-    if (local < 0 || local >= b.size) continue;     // a gap between blocks
-    if (b.other < 0) {
-      if (!to_fused) dst[e] = double2{0.0, 0.0};
+    const int64_t local = e - b.base;   // blocks hold < 2^31 elements: 32-bit indexing below
+    if (local < 0 || local >= b.size) {
+      map[e] = kMapGap;
       continue;
     }
-    // blocks hold < 2^31 elements: 32-bit index arithmetic
+    if (b.other < 0) {
+      map[e] = kMapZero;
+      continue;
+    }
     int64_t o = b.other;
     uint32_t loc = (uint32_t)local;
     for (int l = 0; l < b.rank; ++l) {
@@ -44,8 +48,23 @@ __global__ void __launch_bounds__(kPermThreads)
       o += (int64_t)(loc - q * d) * b.ostr[l];
       loc = q;
     }
-    if (to_fused) dst[o] = src[e];
-    else dst[e] = src[o];
+    map[e] = (uint32_t)o;
+  }
+}
+
+// The per-apply copy: one coalesced map read per element, no block search.
+__global__ void __launch_bounds__(kPermThreads)
+    map_copy_kernel(int to_fused, const double2* __restrict__ src, double2* __restrict__ dst,
+                    const uint32_t* __restrict__ map, int64_t total) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const uint32_t o = map[e];
+    if (o == kMapGap) continue;
+    if (to_fused) {
+      if (o != kMapZero) dst[o] = src[e];
+    } else {
+      dst[e] = (o == kMapZero) ? double2{0.0, 0.0} : src[o];
+    }
   }
 }
 
@@ -62,14 +81,21 @@ static int perm_grid(int64_t n) {
   return (int)g;
 }
 
-hipError_t launch_perm_copy(bool to_fused, const void* blocked, void* fused, const PermBlock* blocks,
-                            int nblocks, const int32_t* chunk_first, int64_t total,
-                            hipStream_t stream) {
+hipError_t launch_perm_map(const PermBlock* blocks, int nblocks, const int32_t* chunk_first,
+                           int64_t total, uint32_t* map, hipStream_t stream) {
   if (total <= 0 || nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(perm_map_kernel, dim3(perm_grid(total)), dim3(kPermThreads), 0, stream, blocks,
+                     chunk_first, total, map);
+  return hipGetLastError();
+}
+
+hipError_t launch_map_copy(bool to_fused, const void* blocked, void* fused, const uint32_t* map,
+                           int64_t total, hipStream_t stream) {
+  if (total <= 0) return hipSuccess;
   const double2* src = to_fused ? (const double2*)blocked : (const double2*)fused;
   double2* dst = to_fused ? (double2*)fused : (double2*)blocked;
-  hipLaunchKernelGGL(perm_copy_kernel, dim3(perm_grid(total)), dim3(kPermThreads), 0, stream,
-                     to_fused ? 1 : 0, src, dst, blocks, nblocks, chunk_first, total);
+  hipLaunchKernelGGL(map_copy_kernel, dim3(perm_grid(total)), dim3(kPermThreads), 0, stream,
+                     to_fused ? 1 : 0, src, dst, map, total);
   return hipGetLastError();
 }
 

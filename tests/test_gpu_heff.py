@@ -1,7 +1,7 @@
 """GPU parity: the HIP path (through the C ABI) against the numpy oracle.
 
 FP64 results must agree to 1e-12 relative (the engine's plain form; the
-north-star bound is 1e-10).  Every case prints its name and shape first so a
+bound the workload asks for is 1e-10).  Every case prints its name and shape first so a
 failing run names its case.
 """
 import numpy as np
