@@ -343,6 +343,9 @@ def main():
     ap.add_argument("--sweep-D", type=int, default=32)
     ap.add_argument("--sweeps", type=int, default=2)
     ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--apply-precision", default="fp64", choices=["fp64", "fp32", "bf16x3"],
+                    help="apply mode: product arithmetic of the map (reduced precision is not the "
+                         "metric's FP64 form and is reported separately)")
     ap.add_argument("--phase-times", action="store_true",
                     help="sweep mode: synchronized wall time per dmrg2_ phase")
     ap.add_argument("--calib-copy", action="store_true",
@@ -389,6 +392,10 @@ def main():
         return
     t_setup = time.perf_counter()
     g, topo, psi, H, cache, h2, x = build_workload(args.D)
+    if args.apply_precision != "fp64":
+        h2 = h2.with_precision(args.apply_precision)
+    from graft_amd import _native as nat
+    prec_code = nat.PRECISIONS[args.apply_precision]
     native = h2.native
     if not args.no_autotune:
         t_at = time.perf_counter()
@@ -422,7 +429,7 @@ def main():
     reps = max(1, min(3, args.steps))
     step_ms = np.zeros(len(native.plan.steps))
     for _ in range(reps):
-        step_ms += np.array(native.execute_timed(ops, y))
+        step_ms += np.array(native.execute_timed(ops, y, prec_code))
     step_ms /= reps
     info = native.step_info()
     dom = int(np.argmax([s["macs"] for s in info]))
@@ -463,7 +470,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128 (fp64)",
+        "dtype": "c128 (fp64)" if args.apply_precision == "fp64" else
+                 f"c128 storage, {args.apply_precision} products",
         "data": "synthetic",
         "config": {"workload": f"eff_h2 apply, root edge of a depth-5 binary tree (63 sites), "
                                f"Heisenberg TTNO w=5, D={args.D}, complex FP64",
