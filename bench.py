@@ -310,8 +310,22 @@ def hubbard_apply_bench(D, reps=20, sigma=1.6, seed=5):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
+    # two Lanczos cycles of 20 applies: device driver (expansions replayed as
+    # HIP graphs) against the per-call driver, wall time per apply
+    from graft_amd import krylov as K
+    lanczos = {}
+    for device in (True, False, True, False):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        vals, _, info = K.eigsolve(h2, x, 1, "SR", krylovdim=20, tol=1e-14, maxiter=2, device=device)
+        torch.cuda.synchronize()
+        lanczos["device_graph" if device else "per_call"] = {
+            "ms_per_apply": (time.perf_counter() - t) * 1e3 / info.numops, "applies": info.numops,
+            "ritz_value": float(vals[0])}
     return {"D": D, "ms_per_apply": ms, "flops": h2.flops, "tflops": h2.flops / ms / 1e9,
-            "step_ms": step_ms, "steps": h2.native.step_info(), "theta_elements": int(x.numel)}
+            "step_ms": step_ms, "steps": h2.native.step_info(), "theta_elements": int(x.numel),
+            "blocks": len(x.blocks.blocks) if hasattr(x.blocks, "blocks") else None,
+            "lanczos": lanczos}
 
 
 def blas_threads():

@@ -67,8 +67,11 @@ class NativePlan:
             ks = ctypes.c_int32()
             nat.check(nat.lib().graft_plan_step_launch(self.handle, i, ctypes.byref(cfg),
                                                        ctypes.byref(orient), ctypes.byref(ks)))
+            abc = (ctypes.c_int64 * 3)()
+            nat.check(nat.lib().graft_plan_step_elements(self.handle, i, abc))
             out.append(dict(m=mkn[0], k=mkn[1], n=mkn[2], blocks=nb.value, macs=macs.value,
-                            config=cfg.value, orientation=orient.value, ksplit=ks.value))
+                            config=cfg.value, orientation=orient.value, ksplit=ks.value,
+                            elements=list(abc)))
         return out
 
     def execute(self, operands, out, precision=nat.PREC_FP64):

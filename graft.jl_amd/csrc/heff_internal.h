@@ -79,6 +79,25 @@ hipError_t launch_zskinny(bool big_is_a, bool acc, const void* A, const void* B,
                           int conjA, int conjB, hipStream_t stream);
 int skinny_grid(int64_t J);
 
+// Per-block streaming kernel for multi-block (sector) steps whose small
+// operand has few open elements per output block: it reads the blocks in
+// place (no gather into fused matrices) through the same item/segment/koff
+// description.  Virtual tile configuration kBlockStreamConfig.
+constexpr int kBlockStreamConfig = kNumTileConfigs + 1;
+constexpr int kBlockStreamS = 16;       // small-side outputs per pass (registers per lane)
+constexpr int kBlockStreamRows = 64;    // big-side elements per work entry (one per lane)
+constexpr int kBlockStreamPanel = 16;   // most output blocks one work entry walks
+// One wave's work: output blocks order[begin .. begin + count) (they share
+// the big operand's open sector tuple, so the rows they read coincide) at
+// big-side rows j0 .. j0 + 63.
+struct StreamWork {
+  int32_t begin, count, j0, pad;
+};
+hipError_t launch_zstream_blocks(bool big_is_a, const void* A, const void* B, void* C,
+                                 const GemmItem* items, const GemmSeg* segs, const uint32_t* koff,
+                                 const int32_t* order, const StreamWork* work, int nwork, int conjA,
+                                 int conjB, hipStream_t stream);
+
 // Block <-> fused-matrix copies of the fused sector path (sector_kernels.hip).
 // A PermBlock maps one dense block (contiguous, first leg fastest) of a
 // block-sparse tensor onto a strided rectangle of a fused per-charge matrix.

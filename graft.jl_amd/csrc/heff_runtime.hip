@@ -216,6 +216,14 @@ struct StepExec {
   bool tr_auto = false;       // orientation from the output's unit-stride leg
   int tr_pinned = -1;         // -1 = automatic
   int skinny = 0;             // streaming kernel eligible: 1 = A is the large operand, 2 = B
+  // Per-block streaming (multi-block steps): 1 = A is the large operand, 2 = B.
+  int bstream = 0;
+  bool bstream_pref = false;          // the default choice for this step
+  std::vector<int32_t> sorder;        // output blocks grouped by the large operand's open tuple
+  std::vector<int32_t> spanel;        // group starts in sorder, then sorder.size()
+  std::vector<StreamWork> swork;      // one entry per wave
+  int32_t* d_sorder = nullptr;
+  StreamWork* d_swork = nullptr;
   uint64_t partial_bytes = 0; // workspace for the split-K partial tiles
   // Fused sector path: the GEMM items above run on per-charge dense matrices
   // gathered from A and B and scattered into C.
@@ -270,7 +278,7 @@ struct graft_ctx {
   double* d_coef = nullptr;
   double* d_comb = nullptr;      // block-combine coefficients
   bool capturing = false;        // a graph capture is open on the stream
-  int sector_fuse = 1;           // 0 never, 1 automatic, 2 every multi-block step
+  int sector_fuse = 1;  // 0 per-block GEMM, 1 automatic, 2 fused / 3 per-block streaming for every multi-block step
   uint64_t ws_generation = 0;    // bumped whenever the workspace moves
 };
 
@@ -287,31 +295,29 @@ static int set_device(graft_ctx* ctx) {
   return GRAFT_OK;
 }
 
-// Merge adjacent legs of an index group when they are contiguous in every
-// tensor addressing that group.  dims/strides: [tensor][leg].
-static void merge_group(std::vector<int64_t>& dims, std::vector<std::vector<int64_t>*> strides) {
-  std::vector<int64_t> nd;
-  std::vector<std::vector<int64_t>> ns(strides.size());
-  for (size_t l = 0; l < dims.size(); ++l) {
-    if (dims[l] == 1) continue;
-    bool merge = !nd.empty();
+// Merge adjacent legs of one index group (m, n or k) when they are
+// contiguous in every tensor addressing that group: legs l-1, l merge when
+// every stride set (C's and each contribution's) steps by the dimension.
+static void merge_legs(SmallVec<int64_t>& dims, SmallVec<int64_t>& cs, SmallVec<int64_t>* ss,
+                       size_t nss) {
+  int32_t w = 0;
+  for (int32_t l = 0; l < dims.n; ++l) {
+    const int64_t d = dims[l];
+    if (d == 1) continue;
+    bool merge = w > 0 && cs[l] == cs[w - 1] * dims[w - 1];
+    for (size_t t = 0; merge && t < nss; ++t) merge = ss[t][l] == ss[t][w - 1] * dims[w - 1];
     if (merge) {
-      for (size_t t = 0; t < strides.size(); ++t) {
-        if ((*strides[t])[l] != ns[t].back() * nd.back()) {
-          merge = false;
-          break;
-        }
-      }
-    }
-    if (merge) {
-      nd.back() *= dims[l];
+      dims[w - 1] *= d;
     } else {
-      nd.push_back(dims[l]);
-      for (size_t t = 0; t < strides.size(); ++t) ns[t].push_back((*strides[t])[l]);
+      dims[w] = d;
+      cs[w] = cs[l];
+      for (size_t t = 0; t < nss; ++t) ss[t][w] = ss[t][l];
+      ++w;
     }
   }
-  dims = nd;
-  for (size_t t = 0; t < strides.size(); ++t) *strides[t] = ns[t];
+  dims.n = w;
+  cs.n = w;
+  for (size_t t = 0; t < nss; ++t) ss[t].n = w;
 }
 
 static int choose_config(int64_t M, int64_t N, int skinny) {
@@ -755,29 +761,90 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
   };
   auto t0 = clk();
-  // Enumerate contributions: group B blocks by contracted sector tuple.
-  std::unordered_map<SecKey, std::vector<int>, SecHash> bByC;
-  for (int bi = 0; bi < (int)B.blocks.size(); ++bi) {
-    SecKey key(nCB);
-    for (int i = 0; i < nCB; ++i) key[i] = B.blocks[bi].sec[st.cindB[i]];
-    bByC[key].push_back(bi);
-  }
-  std::unordered_map<SecKey, std::vector<std::pair<int, int>>, SecHash> contrib;
-  for (int ai = 0; ai < (int)A.blocks.size(); ++ai) {
-    SecKey key(nCA);
-    for (int i = 0; i < nCA; ++i) key[i] = A.blocks[ai].sec[st.cindA[i]];
-    auto f = bByC.find(key);
-    if (f == bByC.end()) continue;
-    for (int bi : f->second) {
-      SecKey csec(nOut);
-      for (int p = 0; p < nOA; ++p) csec[pos_of_out[p]] = A.blocks[ai].sec[st.oindA[p]];
-      for (int p = 0; p < nOB; ++p) csec[pos_of_out[nOA + p]] = B.blocks[bi].sec[st.oindB[p]];
-      contrib[csec].push_back({ai, bi});
+  // Enumerate contributions with integer ids of the sector tuples: r (A's
+  // open legs), c (contracted legs), s (B's open legs).  A pair (A block,
+  // B block) with the same c feeds output block (r, s).  The pairs of each
+  // output block keep enumeration order (A blocks ascending, then B blocks
+  // ascending): that is the block's summation order.
+  struct TupleIds {
+    std::unordered_map<SecKey, int32_t, SecHash> id;
+    std::vector<SecKey> keys;
+    int32_t get(const SecKey& k) {
+      auto f = id.find(k);
+      if (f != id.end()) return f->second;
+      const int32_t v = (int32_t)keys.size();
+      id.emplace(k, v);
+      keys.push_back(k);
+      return v;
     }
+    int32_t find(const SecKey& k) const {
+      auto f = id.find(k);
+      return f == id.end() ? -1 : f->second;
+    }
+  };
+  TupleIds Rt, Ct, St;
+  const int nAb = (int)A.blocks.size(), nBb = (int)B.blocks.size();
+  std::vector<int32_t> aR(nAb, -1), aC(nAb), bS(nBb, -1);
+  SecKey key;
+  for (int ai = 0; ai < nAb; ++ai) {
+    key.assign(nCA, 0);
+    for (int i = 0; i < nCA; ++i) key[i] = A.blocks[ai].sec[st.cindA[i]];
+    aC[ai] = Ct.get(key);
+  }
+  std::vector<std::vector<int32_t>> bByC(Ct.keys.size());
+  for (int bi = 0; bi < nBb; ++bi) {
+    key.assign(nCB, 0);
+    for (int i = 0; i < nCB; ++i) key[i] = B.blocks[bi].sec[st.cindB[i]];
+    const int32_t c = Ct.find(key);
+    if (c < 0) continue;
+    key.assign(nOB, 0);
+    for (int p = 0; p < nOB; ++p) key[p] = B.blocks[bi].sec[st.oindB[p]];
+    bS[bi] = St.get(key);
+    bByC[c].push_back(bi);
+  }
+  for (int ai = 0; ai < nAb; ++ai) {
+    if (bByC[aC[ai]].empty()) continue;
+    key.assign(nOA, 0);
+    for (int p = 0; p < nOA; ++p) key[p] = A.blocks[ai].sec[st.oindA[p]];
+    aR[ai] = Rt.get(key);
+  }
+  const uint64_t ns64 = std::max<uint64_t>(1, St.keys.size());
+  std::unordered_map<uint64_t, int32_t> kid;   // r * ns + s -> key index
+  std::vector<uint64_t> kval;                  // key index -> r * ns + s
+  std::vector<int32_t> pa, pb, pk;             // pairs in enumeration order
+  kid.reserve(2 * (size_t)nAb + 16);
+  for (int ai = 0; ai < nAb; ++ai) {
+    if (aR[ai] < 0) continue;
+    for (int bi : bByC[aC[ai]]) {
+      const uint64_t k = (uint64_t)aR[ai] * ns64 + (uint64_t)bS[bi];
+      auto f = kid.find(k);
+      int32_t id;
+      if (f == kid.end()) {
+        id = (int32_t)kval.size();
+        kid.emplace(k, id);
+        kval.push_back(k);
+      } else {
+        id = f->second;
+      }
+      pa.push_back(ai);
+      pb.push_back(bi);
+      pk.push_back(id);
+    }
+  }
+  const size_t nkeys = kval.size();
+  std::vector<SecKey> ksec(nkeys);   // output sector tuple of every key
+  for (size_t i = 0; i < nkeys; ++i) {
+    const SecKey& rk = Rt.keys[kval[i] / ns64];
+    const SecKey& sk = St.keys[kval[i] % ns64];
+    SecKey& cs = ksec[i];
+    cs.assign(nOut, 0);
+    for (int p = 0; p < nOA; ++p) cs[pos_of_out[p]] = rk[p];
+    for (int p = 0; p < nOB; ++p) cs[pos_of_out[nOA + p]] = sk[p];
   }
   const double t_contrib = timing ? ms_since(t0) : 0.0;
   auto t1 = clk();
   // Output blocks.
+  std::vector<int32_t> kblk(nkeys);   // key -> output block
   if (fixed_out != nullptr) {
     if (fixed_out->rank != nOut) {
       err = "output descriptor rank does not match the plan";
@@ -788,38 +855,41 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
         err = "output descriptor legs do not match the plan";
         return 1;
       }
-    for (auto& kv : contrib)
-      if (!fixed_out->index.count(kv.first)) {
+    C = *fixed_out;
+    for (size_t i = 0; i < nkeys; ++i) {
+      auto f = C.index.find(ksec[i]);
+      if (f == C.index.end()) {
         err = "contraction produces a block outside the output structure";
         return 1;
       }
-    C = *fixed_out;
+      kblk[i] = f->second;
+    }
   } else {
+    std::vector<int32_t> order(nkeys);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(),
+              [&](int32_t x, int32_t y) { return ksec[x] < ksec[y]; });
     int64_t off = 0;
     C.blocks.clear();
     C.index.clear();
-    std::vector<const SecKey*> keys;
-    keys.reserve(contrib.size());
-    C.blocks.reserve(contrib.size());
-    C.index.reserve(contrib.size());
-    for (auto& kv : contrib) keys.push_back(&kv.first);
-    std::sort(keys.begin(), keys.end(),
-              [](const SecKey* x, const SecKey* y) { return *x < *y; });
-    for (const auto* kp : keys) {
-      const auto& kv = *contrib.find(*kp);
+    C.blocks.reserve(nkeys);
+    C.index.reserve(nkeys);
+    for (size_t j = 0; j < nkeys; ++j) {
+      const SecKey& cs = ksec[order[j]];
       Block blk;
-      blk.sec = kv.first;
+      blk.sec = cs;
       blk.dims.resize(nOut);
       blk.strides.resize(nOut);
-      int64_t s = 1;
+      int64_t sz = 1;
       for (int q = 0; q < nOut; ++q) {
-        blk.dims[q] = C.secdims[q][kv.first[q]];
-        blk.strides[q] = s;
-        s *= blk.dims[q];
+        blk.dims[q] = C.secdims[q][cs[q]];
+        blk.strides[q] = sz;
+        sz *= blk.dims[q];
       }
-      blk.size = s;
+      blk.size = sz;
       blk.offset = off;
-      off += s;
+      off += sz;
+      kblk[order[j]] = (int32_t)j;
       C.index.emplace(blk.sec, (int)C.blocks.size());
       C.blocks.push_back(std::move(blk));
     }
@@ -828,6 +898,15 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
       err = "contraction step has no matching blocks";
       return 1;
     }
+  }
+  // pairs grouped per output block, enumeration order kept inside a block
+  std::vector<int64_t> pstart(C.blocks.size() + 1, 0);
+  for (int32_t k : pk) pstart[kblk[k] + 1]++;
+  for (size_t b = 0; b < C.blocks.size(); ++b) pstart[b + 1] += pstart[b];
+  std::vector<std::pair<int, int>> pairs(pk.size());
+  {
+    std::vector<int64_t> fill(pstart.begin(), pstart.end() - 1);
+    for (size_t i = 0; i < pk.size(); ++i) pairs[fill[kblk[pk[i]]]++] = {pa[i], pb[i]};
   }
   const double t_cblocks = timing ? ms_since(t1) : 0.0;
   if ((uint64_t)A.total >= (1ull << 32) || (uint64_t)B.total >= (1ull << 32) ||
@@ -844,76 +923,105 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
   ex.segs.clear();
   ex.koff.clear();
   ex.macs = 0.0;
+  // Per-block streaming: in every output block the smaller operand
+  // contributes at most kBlockStreamS open elements (TTNO absorbs), so the
+  // step is HBM-bound and runs best where the blocks lie.
+  int bigside = 0;
+  int64_t maxS = 0;
+  if (C.blocks.size() >= 2) {
+    bigside = A.total >= B.total ? 1 : 2;
+    for (const auto& b : C.blocks) {
+      int64_t sz = 1;
+      if (bigside == 1)
+        for (int p = 0; p < nOB; ++p) sz *= b.dims[pos_of_out[nOA + p]];
+      else
+        for (int p = 0; p < nOA; ++p) sz *= b.dims[pos_of_out[p]];
+      maxS = std::max(maxS, sz);
+    }
+  }
+  const bool stream_first =
+      bigside != 0 && (fuse_mode == 3 || (fuse_mode == 1 && maxS <= kBlockStreamS));
   // Fused sector path for steps made of many small blocks (decided from the
   // output blocks alone, before any per-block GEMM item is built).
   ex.fused = false;
-  if (fuse_mode > 0 && C.blocks.size() >= 2 && canonical_blocks(A) && canonical_blocks(B) &&
-      canonical_blocks(C)) {
+  ex.bstream = 0;
+  ex.bstream_pref = false;
+  ex.sorder.clear();
+  ex.spanel.clear();
+  if (!stream_first && (fuse_mode == 1 || fuse_mode == 2) && C.blocks.size() >= 2 &&
+      canonical_blocks(A) && canonical_blocks(B) && canonical_blocks(C)) {
     const double avg = (double)C.total / (double)C.blocks.size();
     if (fuse_mode == 2 || (avg < 64.0 * 64.0 && C.blocks.size() >= 8)) {
       double macs = 0.0;
-      for (auto& kv : contrib) {
-        const Block& cblk = C.blocks[C.index.at(kv.first)];
-        for (auto& pr : kv.second) {
-          const Block& ab = A.blocks[pr.first];
+      for (size_t cb = 0; cb < C.blocks.size(); ++cb) {
+        for (int64_t i = pstart[cb]; i < pstart[cb + 1]; ++i) {
+          const Block& ab = A.blocks[pairs[i].first];
           double K = 1.0;
-          for (int i = 0; i < nCA; ++i) K *= (double)ab.dims[st.cindA[i]];
-          macs += (double)cblk.size * K;
+          for (int q = 0; q < nCA; ++q) K *= (double)ab.dims[st.cindA[q]];
+          macs += (double)C.blocks[cb].size * K;
         }
       }
       auto t2 = clk();
       if (build_fused(st, A, B, C, pos_of_out, ex, err)) return 1;
       if (timing)
         std::fprintf(stderr, "  contrib %.2f ms (%zu keys), C blocks %.2f ms, fused build %.2f ms\n",
-                     t_contrib, contrib.size(), t_cblocks, ms_since(t2));
+                     t_contrib, nkeys, t_cblocks, ms_since(t2));
       ex.macs = macs;   // algorithmic (sector) work, not the padded fused work
       return 0;
     }
   }
-  // Build items (one per output block) and segments.
+  // Build items (one per output block) and segments.  Merged index groups
+  // use fixed-capacity records: no heap allocation per block or pair.
+  size_t koff_total = 0;
+  for (const auto& pr : pairs) {
+    const Block& ab = A.blocks[pr.first];
+    int64_t K = 1;
+    for (int i = 0; i < nCA; ++i) K *= ab.dims[st.cindA[i]];
+    koff_total += 2 * (size_t)K;
+  }
+  ex.items.reserve(C.blocks.size());
+  ex.segs.reserve(pairs.size());
+  ex.koff.reserve(koff_total);
+  std::vector<SmallVec<int64_t>> am, bn;
   for (int cb = 0; cb < (int)C.blocks.size(); ++cb) {
     const Block& cblk = C.blocks[cb];
     GemmItem it;
     std::memset(&it, 0, sizeof(it));
     it.c_off = cblk.offset;
     it.seg_begin = (int)ex.segs.size();
-    auto f = contrib.find(cblk.sec);
-    const std::vector<std::pair<int, int>> empty;
-    const auto& pairs = (f == contrib.end()) ? empty : f->second;
+    const std::pair<int, int>* prs = pairs.data() + pstart[cb];
+    const size_t np = (size_t)(pstart[cb + 1] - pstart[cb]);
     // m/n dims are fixed by the output block; strides per contribution.
-    std::vector<int64_t> mdims0, ndims0;
-    std::vector<int64_t> cm0, cn0;
+    SmallVec<int64_t> md, nd, cm0, cn0;
     for (int p = 0; p < nOA; ++p) {
       const int q = pos_of_out[p];
-      mdims0.push_back(cblk.dims[q]);
+      md.push_back(cblk.dims[q]);
       cm0.push_back(cblk.strides[q]);
     }
     for (int p = 0; p < nOB; ++p) {
       const int q = pos_of_out[nOA + p];
-      ndims0.push_back(cblk.dims[q]);
+      nd.push_back(cblk.dims[q]);
       cn0.push_back(cblk.strides[q]);
     }
     int64_t M = 1, N = 1;
-    for (auto d : mdims0) M *= d;
-    for (auto d : ndims0) N *= d;
-    // Merge decisions must hold for every contribution; compute a common
-    // merged layout by merging over (C, every A block, every B block).
-    std::vector<std::vector<int64_t>> am(pairs.size()), bn(pairs.size());
-    for (size_t s = 0; s < pairs.size(); ++s) {
-      const Block& ab = A.blocks[pairs[s].first];
-      const Block& bb = B.blocks[pairs[s].second];
-      for (int p = 0; p < nOA; ++p) am[s].push_back(ab.strides[st.oindA[p]]);
-      for (int p = 0; p < nOB; ++p) bn[s].push_back(bb.strides[st.oindB[p]]);
+    for (auto d : md) M *= d;
+    for (auto d : nd) N *= d;
+    // Merge decisions must hold for every contribution: one common merged
+    // layout over C, every A block and every B block of this output block.
+    if (am.size() < np) {
+      am.resize(np);
+      bn.resize(np);
     }
-    std::vector<int64_t> md = mdims0, nd = ndims0;
-    {
-      std::vector<std::vector<int64_t>*> grp{&cm0};
-      for (auto& v : am) grp.push_back(&v);
-      merge_group(md, grp);
-      std::vector<std::vector<int64_t>*> grpn{&cn0};
-      for (auto& v : bn) grpn.push_back(&v);
-      merge_group(nd, grpn);
+    for (size_t sgi = 0; sgi < np; ++sgi) {
+      const Block& ab = A.blocks[prs[sgi].first];
+      const Block& bb = B.blocks[prs[sgi].second];
+      am[sgi].n = 0;
+      bn[sgi].n = 0;
+      for (int p = 0; p < nOA; ++p) am[sgi].push_back(ab.strides[st.oindA[p]]);
+      for (int p = 0; p < nOB; ++p) bn[sgi].push_back(bb.strides[st.oindB[p]]);
     }
+    merge_legs(md, cm0, am.data(), np);
+    merge_legs(nd, cn0, bn.data(), np);
     if ((int)md.size() > GL || (int)nd.size() > GL) {
       err = "too many unmerged legs in one GEMM index group";
       return 1;
@@ -936,24 +1044,23 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
     }
     ex.maxM = std::max(ex.maxM, M);
     ex.maxN = std::max(ex.maxN, N);
-    for (size_t s = 0; s < pairs.size(); ++s) {
-      const Block& ab = A.blocks[pairs[s].first];
-      const Block& bb = B.blocks[pairs[s].second];
+    for (size_t sgi = 0; sgi < np; ++sgi) {
+      const Block& ab = A.blocks[prs[sgi].first];
+      const Block& bb = B.blocks[prs[sgi].second];
       GemmSeg sg;
       std::memset(&sg, 0, sizeof(sg));
       sg.a_off = ab.offset;
       sg.b_off = bb.offset;
-      for (int l = 0; l < it.mrank; ++l) sg.a_mstr[l] = (uint32_t)am[s][l];
-      for (int l = 0; l < it.nrank; ++l) sg.b_nstr[l] = (uint32_t)bn[s][l];
+      for (int l = 0; l < it.mrank; ++l) sg.a_mstr[l] = (uint32_t)am[sgi][l];
+      for (int l = 0; l < it.nrank; ++l) sg.b_nstr[l] = (uint32_t)bn[sgi][l];
       // contracted legs
-      std::vector<int64_t> kd, ka, kb;
+      SmallVec<int64_t> kd, ka, kb;
       for (int i = 0; i < nCA; ++i) {
         kd.push_back(ab.dims[st.cindA[i]]);
         ka.push_back(ab.strides[st.cindA[i]]);
         kb.push_back(bb.strides[st.cindB[i]]);
       }
-      std::vector<std::vector<int64_t>*> grpk{&ka, &kb};
-      merge_group(kd, grpk);
+      merge_legs(kd, ka, &kb, 1);
       int64_t K = 1;
       for (auto d : kd) K *= d;
       if (K >= (1ll << 31)) {
@@ -976,8 +1083,8 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
       }
       // unit-stride leg decides the load direction
       int64_t minm = INT64_MAX, mink = INT64_MAX, minn = INT64_MAX, minkb = INT64_MAX;
-      for (int l = 0; l < it.mrank; ++l) minm = std::min(minm, am[s][l]);
-      for (int l = 0; l < it.nrank; ++l) minn = std::min(minn, bn[s][l]);
+      for (int l = 0; l < it.mrank; ++l) minm = std::min(minm, am[sgi][l]);
+      for (int l = 0; l < it.nrank; ++l) minn = std::min(minn, bn[sgi][l]);
       for (size_t l = 0; l < kd.size(); ++l) {
         mink = std::min(mink, ka[l]);
         minkb = std::min(minkb, kb[l]);
@@ -990,6 +1097,47 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
     }
     it.seg_count = (int)ex.segs.size() - it.seg_begin;
     ex.items.push_back(it);
+  }
+  // Output blocks grouped by the large operand's open sector tuple (the
+  // blocks of one group read the same rows of the large operand).
+  if (bigside != 0) {
+    std::unordered_map<SecKey, int, SecHash> pid;
+    std::vector<std::vector<int32_t>> groups;
+    SecKey key;
+    for (int cb = 0; cb < (int)C.blocks.size(); ++cb) {
+      const Block& b = C.blocks[cb];
+      if (bigside == 1) {
+        key.assign(nOA, 0);
+        for (int p = 0; p < nOA; ++p) key[p] = b.sec[pos_of_out[p]];
+      } else {
+        key.assign(nOB, 0);
+        for (int p = 0; p < nOB; ++p) key[p] = b.sec[pos_of_out[nOA + p]];
+      }
+      auto f = pid.find(key);
+      int id;
+      if (f == pid.end()) {
+        id = (int)groups.size();
+        pid.emplace(key, id);
+        groups.emplace_back();
+      } else {
+        id = f->second;
+      }
+      groups[id].push_back(cb);
+    }
+    for (const auto& grp : groups) {
+      const int J0 = bigside == 1 ? ex.items[grp[0]].M : ex.items[grp[0]].N;
+      for (int cb : grp) {
+        if ((bigside == 1 ? ex.items[cb].M : ex.items[cb].N) != J0) {
+          err = "output blocks of one open sector tuple differ in extent";
+          return 1;
+        }
+      }
+      ex.spanel.push_back((int32_t)ex.sorder.size());
+      ex.sorder.insert(ex.sorder.end(), grp.begin(), grp.end());
+    }
+    ex.spanel.push_back((int32_t)ex.sorder.size());
+    ex.bstream = bigside;
+    ex.bstream_pref = stream_first;
   }
   // Orientation: decided by the largest output block's unit-stride leg.
   {
@@ -1023,7 +1171,28 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
 // Target number of workgroups for one launch (two per CU on 256 CUs).
 static constexpr int kTargetBlocks = 512;
 
+static int default_config(const StepExec& ex) {
+  if (ex.bstream_pref) return kBlockStreamConfig;
+  return choose_config(ex.maxM, ex.maxN, ex.skinny);
+}
+
 static void assign_tiles(StepExec& ex) {
+  if (ex.config == kBlockStreamConfig) {
+    // rows outer, groups inner: neighbouring waves read the same rows
+    ex.swork.clear();
+    for (size_t g = 0; g + 1 < ex.spanel.size(); ++g) {
+      const int b = ex.spanel[g], e = ex.spanel[g + 1];
+      const GemmItem& it0 = ex.items[ex.sorder[b]];
+      const int J = ex.bstream == 1 ? it0.M : it0.N;
+      for (int j0 = 0; j0 < J; j0 += kBlockStreamRows)
+        for (int c0 = b; c0 < e; c0 += kBlockStreamPanel)
+          ex.swork.push_back({c0, std::min(kBlockStreamPanel, e - c0), j0, 0});
+    }
+    ex.total_tiles = (int)ex.swork.size();
+    ex.ksplit = 1;
+    ex.partial_bytes = 0;
+    return;
+  }
   if (ex.config == kSkinnyConfig) {
     const GemmItem& it = ex.items[0];
     ex.total_tiles = skinny_grid(ex.skinny == 1 ? it.M : it.N);
@@ -1076,6 +1245,7 @@ static int upload_step_meta(graft_plan_impl& p) {
     total += al(std::max<size_t>(1, ex.koff.size()) * sizeof(uint32_t));
     for (auto* v : {&ex.pa, &ex.pb, &ex.pc}) total += al(v->size() * sizeof(PermBlock));
     for (auto* v : {&ex.ca, &ex.cb, &ex.cc}) total += al(v->size() * sizeof(int32_t));
+    total += al(ex.sorder.size() * sizeof(int32_t)) + al(ex.swork.size() * sizeof(StreamWork));
   }
   if (p.d_meta) {
     (void)hipFree(p.d_meta);
@@ -1113,6 +1283,14 @@ static int upload_step_meta(graft_plan_impl& p) {
       *dc[i] = sz ? (int32_t*)((char*)p.d_meta + off) : nullptr;
       off += al(sz);
     }
+    const size_t so = ex.sorder.size() * sizeof(int32_t);
+    if (so) std::memcpy(host.data() + off, ex.sorder.data(), so);
+    ex.d_sorder = so ? (int32_t*)((char*)p.d_meta + off) : nullptr;
+    off += al(so);
+    const size_t sw = ex.swork.size() * sizeof(StreamWork);
+    if (sw) std::memcpy(host.data() + off, ex.swork.data(), sw);
+    ex.d_swork = sw ? (StreamWork*)((char*)p.d_meta + off) : nullptr;
+    off += al(sw);
   }
   GRAFT_CHECK_HIP(hipMemcpy(p.d_meta, host.data(), total, hipMemcpyHostToDevice));
   return GRAFT_OK;
@@ -1207,7 +1385,12 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     B = Bf;
     C = Cf;
   }
-  if (ex.config == kSkinnyConfig) {
+  if (ex.config == kBlockStreamConfig) {
+    // HBM-bound, FP64 whatever the precision (as the streaming kernel below)
+    GRAFT_CHECK_HIP(launch_zstream_blocks(ex.bstream == 1, A, B, C, ex.d_items, ex.d_segs,
+                                          ex.d_koff, ex.d_sorder, ex.d_swork,
+                                          (int)ex.swork.size(), ex.conjA, ex.conjB, ctx->stream));
+  } else if (ex.config == kSkinnyConfig) {
     // HBM-bound: the streaming kernel always multiplies in FP64, whatever
     // precision the compute-bound steps use.  One launch per item (a fused
     // step has one item per charge component).
@@ -1314,7 +1497,7 @@ int graft_ctx_workspace_bytes(graft_ctx* ctx, uint64_t* bytes) {
 
 int graft_ctx_set_sector_fusion(graft_ctx* ctx, int32_t mode) {
   GRAFT_REQUIRE(ctx != nullptr, "null context");
-  GRAFT_REQUIRE(mode >= 0 && mode <= 2, "sector fusion mode must be 0, 1 or 2");
+  GRAFT_REQUIRE(mode >= 0 && mode <= 3, "sector fusion mode must be 0, 1, 2 or 3");
   ctx->sector_fuse = mode;
   return GRAFT_OK;
 }
@@ -1448,8 +1631,12 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
     defined[st.a] = (st.a < ninputs);  // inputs stay valid; intermediates are consumed
     defined[st.b] = (st.b < ninputs);
     StepExec& ex = p.steps[s];
-    ex.config = choose_config(ex.maxM, ex.maxN, ex.skinny);
+    ex.config = default_config(ex);
     assign_tiles(ex);
+    if (ex.swork.size() >= (1u << 26)) {
+      set_error("step " + std::to_string(s) + ": too many streaming work entries");
+      return GRAFT_E_UNSUPPORTED;
+    }
     p.macs += ex.macs;
     if (timing)
       std::fprintf(stderr, "plan step %d: %.2f ms, blocks A %zu B %zu C %zu, items %zu, fused %d\n", s,
@@ -1544,6 +1731,16 @@ int graft_plan_step_launch(graft_plan* plan, int32_t step, int32_t* config, int3
   return GRAFT_OK;
 }
 
+int graft_plan_step_elements(graft_plan* plan, int32_t step, int64_t* abc3) {
+  GRAFT_REQUIRE(plan != nullptr && abc3 != nullptr, "null argument");
+  GRAFT_REQUIRE(step >= 0 && step < (int32_t)plan->impl.steps.size(), "step out of range");
+  const StepExec& ex = plan->impl.steps[step];
+  abc3[0] = plan->impl.slots[ex.a].total;
+  abc3[1] = plan->impl.slots[ex.b].total;
+  abc3[2] = plan->impl.slots[ex.dst].total;
+  return GRAFT_OK;
+}
+
 int graft_plan_output_blocks(graft_plan* plan, int32_t* rank, int64_t* nblocks,
                              int32_t* block_sectors, int64_t* block_offsets) {
   GRAFT_REQUIRE(plan != nullptr && rank != nullptr && nblocks != nullptr, "null argument");
@@ -1562,19 +1759,23 @@ int graft_plan_output_blocks(graft_plan* plan, int32_t* rank, int64_t* nblocks,
   return GRAFT_OK;
 }
 
-int graft_num_tile_configs(void) { return kNumTileConfigs + 1; }
+int graft_num_tile_configs(void) { return kNumTileConfigs + 2; }
 
 int graft_plan_set_tile_config(graft_plan* plan, int32_t step, int32_t config) {
   GRAFT_REQUIRE(plan != nullptr, "null plan");
   GRAFT_REQUIRE(step >= 0 && step < (int32_t)plan->impl.steps.size(), "step out of range");
-  GRAFT_REQUIRE(config >= -1 && config <= kNumTileConfigs, "tile configuration out of range");
+  GRAFT_REQUIRE(config >= -1 && config <= kBlockStreamConfig, "tile configuration out of range");
   StepExec& ex = plan->impl.steps[step];
   if (config == kSkinnyConfig && ex.skinny == 0) {
     set_error("the streaming kernel does not apply to this step");
     return GRAFT_E_UNSUPPORTED;
   }
+  if (config == kBlockStreamConfig && ex.bstream == 0) {
+    set_error("the per-block streaming kernel does not apply to this step");
+    return GRAFT_E_UNSUPPORTED;
+  }
   ex.pinned = config;
-  ex.config = config >= 0 ? config : choose_config(ex.maxM, ex.maxN, ex.skinny);
+  ex.config = config >= 0 ? config : default_config(ex);
   assign_tiles(ex);
   update_workspace_size(plan->impl);
   return upload_step_meta(plan->impl);
@@ -1639,9 +1840,10 @@ int graft_plan_autotune(graft_ctx* ctx, graft_plan* plan, const void* const* inp
     int best = ex.config;
     bool best_tr = ex.tr;
     float best_ms = 1e30f;
-    for (int c = 0; c <= kNumTileConfigs && !rc; ++c) {
+    for (int c = 0; c <= kBlockStreamConfig && !rc; ++c) {
       if (ex.pinned >= 0 && c != ex.pinned) continue;
       if (c == kSkinnyConfig && ex.skinny == 0) continue;
+      if (c == kBlockStreamConfig && ex.bstream == 0) continue;
       ex.config = c;
       assign_tiles(ex);
       update_workspace_size(p);
@@ -1651,7 +1853,7 @@ int graft_plan_autotune(graft_ctx* ctx, graft_plan* plan, const void* const* inp
       if (rc) break;
       for (int o = 0; o < 2 && !rc; ++o) {
         if (ex.tr_pinned >= 0 && o != ex.tr_pinned) continue;
-        if (c == kSkinnyConfig && o == 1) continue;
+        if ((c == kSkinnyConfig || c == kBlockStreamConfig) && o == 1) continue;
         ex.tr = (o == 1);
         // warm-up, then time
         rc = run_step(ctx, p, ex, inputs, output, precision);

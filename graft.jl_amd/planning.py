@@ -8,8 +8,13 @@ a memory-greedy tree and, for abelian sector spaces, an exact sector DP over
 a Pareto frontier.  A candidate is admissible only within the env-first
 plan's peak and live-byte floors and an optional hard byte cap; the score is
 ``flops + memory_weight * live_peak_bytes`` (sector_flops / sector bytes when
-the sector model applies).  Indices are 0-based; labels follow ncon (> 0
-contracted, < 0 open, output ordered -1, -2, ...).
+the sector model applies).  Sector plans for the device executor are ranked
+by a roofline time model instead (``device_time``: per step the larger of its
+FLOPs over an effective MFMA rate and its stored operand + result bytes over
+an effective HBM rate), because a block-sparse step with a short contraction
+is HBM-bound and the FLOP-optimal order can materialize a far larger
+intermediate.  Indices are 0-based; labels follow ncon (> 0 contracted, < 0
+open, output ordered -1, -2, ...).
 """
 from __future__ import annotations
 
@@ -20,6 +25,18 @@ from itertools import product as iproduct
 import numpy as np
 
 EXACT_TENSOR_LIMIT = 10   # planner.jl:232 (_EXACT_TENSOR_LIMIT)
+
+# Roofline time model of the device executor (MI355X, complex FP64): the
+# rates sector steps reach in practice, not the peaks (8 TB/s, 78.6 TFLOP/s).
+DEVICE_FLOPS = 40e12      # real FLOP/s of a block GEMM step
+DEVICE_BYTES = 3e12       # B/s of a streaming step (operands + result once)
+
+
+def step_device_time(sector_flops, elements, scalar_bytes=16):
+    """Modelled seconds of one pair step: ``sector_flops`` is 2 m k n summed
+    over blocks (real FLOPs = 4 x that for complex scalars); ``elements`` the
+    stored elements of A, B and C."""
+    return max(4.0 * sector_flops / DEVICE_FLOPS, scalar_bytes * elements / DEVICE_BYTES)
 
 
 # ---------------------------------------------------------------------------
@@ -93,6 +110,7 @@ class ContractionPlan:
     sector_known_permutation_peak_bytes: float = math.nan
     scalar_output: bool = False
     candidate_failures: tuple = ()
+    device_time: float = math.nan       # step_device_time summed over the steps
 
     @property
     def ninputs(self):
@@ -360,9 +378,10 @@ def _sector_dp_trees(spec, dims, legs):
     frontiers = {}
     for i in range(n):
         L = conj_legs(legs[i], spec.conjs[i])
+        el = stored_elements(L)
         frontiers[1 << i] = [dict(tree=i, labels=list(spec.labels[i]), dims=list(dims[i]), legs=L,
                                   flops=0.0, peak=_prod(dims[i]), sflops=0.0,
-                                  speak=stored_elements(L))]
+                                  speak=el, el=el, dev=0.0)]
     for mask in range(1, full + 1):
         if bin(mask).count("1") < 2:
             continue
@@ -386,7 +405,8 @@ def _sector_dp_trees(spec, dims, legs):
                                   flops=a["flops"] + b["flops"] + m["flops"],
                                   peak=max(a["peak"], b["peak"], m["peak_elements"]),
                                   sflops=a["sflops"] + b["sflops"] + sfl,
-                                  speak=max(a["speak"], b["speak"], oel))
+                                  speak=max(a["speak"], b["speak"], oel), el=oel,
+                                  dev=a["dev"] + b["dev"] + step_device_time(sfl, a["el"] + b["el"] + oel))
                         _insert_pareto(front, st)
             left = (left - 1) & mask
         if front:
@@ -398,7 +418,8 @@ def _insert_pareto(front, cand):
     key = (tuple(cand["labels"]), tuple(cand["dims"]), tuple(cand["legs"]))
 
     def no_worse(a, b):
-        return a["peak"] <= b["peak"] and a["speak"] <= b["speak"] and a["sflops"] <= b["sflops"]
+        return (a["peak"] <= b["peak"] and a["speak"] <= b["speak"] and a["sflops"] <= b["sflops"]
+                and a["dev"] <= b["dev"])
 
     same = []
     for i, cur in enumerate(front):
@@ -442,7 +463,7 @@ def compile_plan(tree, spec, dims, legs=None, strategy="env_first", canonical=Fa
     n = len(dims)
     structural = legs is not None
     st = dict(nextslot=n - 1, steps=[], flops=0.0, peak=max(_prod(d) for d in dims),
-              sflops=0.0, speak=0.0, sbpeak=0.0, out_dense=[], out_sector=[],
+              sflops=0.0, speak=0.0, sbpeak=0.0, dev=0.0, out_dense=[], out_sector=[],
               tmp_dense=[], perm_dense=[], tmp_sector=[], perm_sector=[])
     slot_legs = [conj_legs(legs[i], spec.conjs[i]) for i in range(n)] if structural else None
     if structural:
@@ -466,14 +487,19 @@ def compile_plan(tree, spec, dims, legs=None, strategy="env_first", canonical=Fa
         dst = st["nextslot"]
         if root:
             out = _output_partition(m["labels"], spec)
+        elif fused_permutes:
+            # Device layout (for sector trees too; their canonical order only
+            # keys the search): the larger operand's open legs first, so the
+            # new intermediate keeps that operand's unit-stride leg and both
+            # the GEMM / streaming stores and the next step's loads stay
+            # coalesced.
+            na, nb = len(m["oindA"]), len(m["oindB"])
+            if _prod(db[i] for i in m["oindB"]) > _prod(da[i] for i in m["oindA"]):
+                out = (tuple(range(na, na + nb)) + tuple(range(na)), ())
+            else:
+                out = (tuple(range(na + nb)), ())
         elif canonical:
             out = (tuple(_canonical_order(m["labels"])), ())
-        elif fused_permutes and _prod(db[i] for i in m["oindB"]) > _prod(da[i] for i in m["oindA"]):
-            # Device layout: the larger operand's open legs first, so the new
-            # intermediate keeps that operand's unit-stride leg and both the
-            # GEMM stores and the next step's loads stay coalesced.
-            na, nb = len(m["oindA"]), len(m["oindB"])
-            out = (tuple(range(na, na + nb)) + tuple(range(na)), ())
         else:
             out = (tuple(range(len(m["labels"]))), ())
         flat = list(out[0]) + list(out[1])
@@ -518,7 +544,9 @@ def compile_plan(tree, spec, dims, legs=None, strategy="env_first", canonical=Fa
             st["sflops"] += sfl
             st["speak"] = max(st["speak"], sector_out)
             st["sbpeak"] = max(st["sbpeak"], largest)
+            st["dev"] += step_device_time(sfl, sca + scb + sector_out, scalar_bytes)
         else:
+            st["dev"] += step_device_time(2 * m["flops"], dna + dnb + dense_out, scalar_bytes)
             st["sflops"] += 2 * m["flops"]
             st["speak"] = max(st["speak"], m["peak_elements"])
             st["sbpeak"] = max(st["sbpeak"], m["peak_elements"])
@@ -555,6 +583,7 @@ def compile_plan(tree, spec, dims, legs=None, strategy="env_first", canonical=Fa
         sector_known_permutation_peak_bytes=sperm_pk * sb,
         scalar_output=spec.nopen == 0)
     plan.tree = tree
+    plan.device_time = st["dev"]
     return plan
 
 
@@ -564,6 +593,12 @@ def _score(plan, mw):
 
 def _sector_score(plan, mw):
     return plan.sector_flops + mw * plan.sector_live_peak_bytes
+
+
+def _device_score(plan, mw):
+    """Modelled time in sector-FLOP units (DEVICE_FLOPS / 4 per second), so
+    ``memory_weight`` keeps its meaning of one sector FLOP per live byte."""
+    return plan.device_time * DEVICE_FLOPS / 4.0 + mw * plan.sector_live_peak_bytes
 
 
 def _within_floors(c, h):
@@ -593,7 +628,9 @@ def plan_contraction(spec, dims, legs=None, optimize=True, memory_weight=1.0, se
 
     ``dims[i]`` are the flat leg dimensions of slot i; ``legs[i]`` (optional)
     its abelian SectorLeg list.  ``fused_permutes`` selects the device
-    executor's live-memory model (see ``compile_plan``).  ``order_cache``
+    executor's live-memory model and layouts (see ``compile_plan``) and, for
+    sector networks, ranks the admissible candidates by the device time model
+    (``step_device_time``) plus ``memory_weight`` times live bytes.  ``order_cache``
     (a dict) reuses the pairwise order chosen for a sector network of the same
     spec and leg dimensions within a factor of two: sweeps meet a new sector
     structure on almost every bond, and the sector search costs far more than
@@ -650,7 +687,10 @@ def plan_contraction(spec, dims, legs=None, optimize=True, memory_weight=1.0, se
     for c in candidates:
         if not _within_floors(c, heuristic) or not _within_cap(c, cap):
             continue
-        s = _sector_score(c, memory_weight) if use_sector else _score(c, memory_weight)
+        if use_sector:
+            s = (_device_score if fused_permutes else _sector_score)(c, memory_weight)
+        else:
+            s = _score(c, memory_weight)
         if s < best_score:
             best, best_score = c, s
     if best is None:

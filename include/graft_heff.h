@@ -117,10 +117,13 @@ int graft_ctx_release_workspace(graft_ctx* ctx);
  * its PlanWorkspace up front the same way (types.jl:539-554). */
 int graft_ctx_reserve_workspace(graft_ctx* ctx, uint64_t bytes);
 /* How plans created afterwards run block-sparse steps: 0 one GEMM item per
- * output block; 1 (default) per-charge fused matrices (gather, one grouped
+ * output block; 1 (default) automatic: per-block streaming (blocks read in
+ * place, FP64) for steps whose smaller operand adds at most 16 open elements
+ * to each output block, else per-charge fused matrices (gather, one grouped
  * dense GEMM over the charge components, scatter) for steps made of many
- * small blocks; 2 fused for every multi-block step.  TensorKit multiplies
- * per coupled sector the same way (BlockSparse mul!, as in Backend.jl:166-222). */
+ * small blocks; 2 fused for every multi-block step; 3 per-block streaming for
+ * every multi-block step.  TensorKit multiplies per coupled sector the same
+ * way (BlockSparse mul!, as in Backend.jl:166-222). */
 int graft_ctx_set_sector_fusion(graft_ctx* ctx, int32_t mode);
 int graft_synchronize(graft_ctx* ctx);
 const char* graft_last_error(void);
@@ -148,16 +151,24 @@ int graft_plan_info(graft_plan* plan, uint64_t* workspace_bytes, double* macs,
 /* Per-step GEMM profile: m, k, n of the largest block, block count and MACs. */
 int graft_plan_step_info(graft_plan* plan, int32_t step, int64_t* mkn3, int64_t* nblocks,
                          double* macs, int32_t* tile_config);
-/* Launch shape chosen for one step: tile configuration (the last index is
- * the streaming kernel for skinny steps), MFMA orientation, split-K factor. */
+/* Launch shape chosen for one step: tile configuration (numbered as in
+ * graft_plan_set_tile_config), MFMA orientation, split-K factor. */
 int graft_plan_step_launch(graft_plan* plan, int32_t step, int32_t* config, int32_t* orientation,
                            int32_t* ksplit);
+/* Stored complex elements of one step's operands and result (A, B, C): the
+ * step's algorithmic HBM bytes are 16 * (A + B + C).  No counterpart in the
+ * reference (its pair_cost, Backend.jl:328-368, counts FLOPs only). */
+int graft_plan_step_elements(graft_plan* plan, int32_t step, int64_t* abc3);
 /* Block list of the plan's result (sector tensors): call with NULL arrays to
  * get rank and nblocks, then with arrays of nblocks*rank sector indices and
  * nblocks element offsets (packed, in this order). */
 int graft_plan_output_blocks(graft_plan* plan, int32_t* rank, int64_t* nblocks,
                              int32_t* block_sectors, int64_t* block_offsets);
-/* Pin the tile configuration of one step (-1 = heuristic). */
+/* Pin the tile configuration of one step (-1 = heuristic).  Configurations
+ * 0 .. n-3 are MFMA tile shapes, n-2 the streaming kernel of single-block
+ * steps, n-1 the per-block streaming kernel of multi-block steps
+ * (n = graft_num_tile_configs()); a kernel that does not apply to the step
+ * returns GRAFT_E_UNSUPPORTED. */
 int graft_plan_set_tile_config(graft_plan* plan, int32_t step, int32_t config);
 int graft_num_tile_configs(void);
 /* Pin the MFMA orientation of one step: 0 = rows on lanes 16-group,

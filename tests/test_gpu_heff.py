@@ -161,7 +161,7 @@ def test_streaming_accumulator_kernel_matches_oracle(g, big_is_a):
 
 def nat_skinny_config():
     from graft_amd import _native as nat
-    return nat.lib().graft_num_tile_configs() - 1
+    return nat.lib().graft_num_tile_configs() - 2
 
 
 # --------------------------------------------------------------------------
@@ -271,7 +271,8 @@ def test_reduced_precision_pair_every_config(g, prec):
             np_.execute([dA, dB], out, nat.PRECISIONS[prec])
             err = _rel(out.to_numpy(), want)
             assert err < PREC_TOL[prec], (cfg, orient, err)
-            assert err > 1e-13 or cfg == nat.lib().graft_num_tile_configs() - 1  # really reduced
+            # really reduced (the two streaming kernels always run FP64)
+            assert err > 1e-13 or cfg >= nat.lib().graft_num_tile_configs() - 2
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])

@@ -113,11 +113,13 @@ def test_sector_and_dense_forms_agree_at_scale(g):
     assert abs(vs[0] - vd[0]) < 1e-10 * abs(vd[0])
 
 
-@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("mode", [0, 2, 3])
 def test_sector_fusion_modes_agree(g, mode):
-    """Per-block GEMM items (mode 0) and fused per-charge matrices (mode 2)
-    give the oracle's envs and maps; at D=48 on the 16-site tree both modes
-    agree with the automatic choice."""
+    """Per-block GEMM items (mode 0), fused per-charge matrices (mode 2) and
+    per-block streaming for every multi-block step (mode 3: both operand
+    sides, conjugated bras, more than 16 small-side elements per block) give
+    the oracle's envs and maps; at D=48 on the 16-site tree every mode agrees
+    with the automatic choice."""
     from graft_amd import _native as nat, contractions as C, operators as O, sectors as S, trees as Tr
     ctx = nat.default_context()
     try:
