@@ -70,32 +70,55 @@ __global__ void __launch_bounds__(kVecThreads)
   }
 }
 
-template <int KMAX>
+// Multi-dot <V_j, y> in passes of CH vectors over y: the accumulators keep
+// compile-time indices (registers, no scratch); the wave sums go through
+// LDS in a fixed order.  partials layout: [block][j][re, im].
+template <int CH>
 __global__ void __launch_bounds__(kVecThreads)
     multi_dotc_partials_kernel(int64_t n, int k, PtrList V, const double2* __restrict__ y,
                                double* __restrict__ partials) {
-  __shared__ double sh[2 * (kVecThreads / 64)];
-  double re[KMAX], im[KMAX];
-#pragma unroll
-  for (int j = 0; j < KMAX; ++j) { re[j] = 0.0; im[j] = 0.0; }
+  constexpr int kWaves = kVecThreads / 64;
+  __shared__ double sh[kWaves][2 * CH];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const double2 b = y[i];
+  for (int j0 = 0; j0 < k; j0 += CH) {
+    const int kc = min(CH, k - j0);
+    double re[CH], im[CH];
 #pragma unroll
-    for (int j = 0; j < KMAX; ++j) {
-      if (j < k) {
-        const double2 a = V.p[j][i];
-        re[j] += a.x * b.x + a.y * b.y;
-        im[j] += a.x * b.y - a.y * b.x;
+    for (int j = 0; j < CH; ++j) {
+      re[j] = 0.0;
+      im[j] = 0.0;
+    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+      const double2 b = y[i];
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        if (j < kc) {
+          const double2 a = V.p[j0 + j][i];
+          re[j] += a.x * b.x + a.y * b.y;   // conj(a) * b
+          im[j] += a.x * b.y - a.y * b.x;
+        }
       }
     }
-  }
-  for (int j = 0; j < k; ++j) {
-    double r = re[j], m = im[j];
-    block_sum2(r, m, sh);
-    if (threadIdx.x == 0) {
-      partials[(size_t)blockIdx.x * 2 * k + 2 * j] = r;
-      partials[(size_t)blockIdx.x * 2 * k + 2 * j + 1] = m;
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      if (j < kc) {
+        double r = re[j], m = im[j];
+        for (int off = 32; off > 0; off >>= 1) {
+          r += __shfl_down(r, off, 64);
+          m += __shfl_down(m, off, 64);
+        }
+        if (lane == 0) {
+          sh[wave][2 * j] = r;
+          sh[wave][2 * j + 1] = m;
+        }
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 2 * kc; t += blockDim.x) {
+      double acc = 0.0;
+      for (int w = 0; w < kWaves; ++w) acc += sh[w][t];
+      partials[(size_t)blockIdx.x * 2 * k + 2 * j0 + t] = acc;
     }
     __syncthreads();
   }
@@ -159,16 +182,12 @@ hipError_t launch_reduce_partials(const double* partials, int nblocks, int k, do
 
 hipError_t launch_multi_dotc_partials(int64_t n, int k, const PtrList& V, const void* y,
                                       double* partials, int nblocks, hipStream_t stream) {
-  if (k <= 8) {
+  if (k <= 8)
     hipLaunchKernelGGL(multi_dotc_partials_kernel<8>, dim3(nblocks), dim3(kVecThreads), 0, stream,
                        n, k, V, (const double2*)y, partials);
-  } else if (k <= 32) {
-    hipLaunchKernelGGL(multi_dotc_partials_kernel<32>, dim3(nblocks), dim3(kVecThreads), 0,
-                       stream, n, k, V, (const double2*)y, partials);
-  } else {
-    hipLaunchKernelGGL(multi_dotc_partials_kernel<kMaxMultiVec>, dim3(nblocks), dim3(kVecThreads),
-                       0, stream, n, k, V, (const double2*)y, partials);
-  }
+  else
+    hipLaunchKernelGGL(multi_dotc_partials_kernel<16>, dim3(nblocks), dim3(kVecThreads), 0, stream,
+                       n, k, V, (const double2*)y, partials);
   return hipGetLastError();
 }
 

@@ -56,12 +56,14 @@ def _dense_matrix(A, like):
     return M
 
 
-def test_device_vector_ops(g):
+@pytest.mark.parametrize("k", [7, 21])
+def test_device_vector_ops(g, k):
+    """k = 21 takes two passes of the multi-dot kernel (16 vectors per pass)."""
     import torch
     from graft_amd import _native as nat, krylov as K, tensors as T
     from graft_amd.tensors import DTensor
     rng = np.random.default_rng(5)
-    n, k = 70001, 7
+    n = 70001
     _say("device vector ops", n, k)
     xs = [ref.random_complex(rng, n) for _ in range(k + 1)]
     ds = [DTensor.from_numpy(x) for x in xs]
