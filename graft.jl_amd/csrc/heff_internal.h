@@ -85,11 +85,13 @@ int skinny_grid(int64_t J);
 // description.  Virtual tile configuration kBlockStreamConfig.
 constexpr int kBlockStreamConfig = kNumTileConfigs + 1;
 constexpr int kBlockStreamS = 16;       // small-side outputs per pass (registers per lane)
-constexpr int kBlockStreamRows = 64;    // big-side elements per work entry (one per lane)
+constexpr int kBlockStreamLaneRows = 1; // big-side elements per lane, one after another
+constexpr int kBlockStreamRows = 64 * kBlockStreamLaneRows;   // big-side elements per work entry
 constexpr int kBlockStreamPanel = 16;   // most output blocks one work entry walks
+constexpr int kBlockStreamWaves = 65536; // work entries above which one entry walks several blocks
 // One wave's work: output blocks order[begin .. begin + count) (they share
 // the big operand's open sector tuple, so the rows they read coincide) at
-// big-side rows j0 .. j0 + 63.
+// big-side rows j0 .. j0 + kBlockStreamRows - 1.
 struct StreamWork {
   int32_t begin, count, j0, pad;
 };

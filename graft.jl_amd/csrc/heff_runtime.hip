@@ -1178,15 +1178,26 @@ static int default_config(const StepExec& ex) {
 
 static void assign_tiles(StepExec& ex) {
   if (ex.config == kBlockStreamConfig) {
-    // rows outer, groups inner: neighbouring waves read the same rows
+    // One output block per entry while that keeps the wave count moderate
+    // (small steps are latency-bound: they need every wave they can get);
+    // beyond it an entry walks up to kBlockStreamPanel blocks of one group.
+    // Rows outer, blocks inner: neighbouring waves read the same rows.
+    int64_t single = 0;
+    for (size_t g = 0; g + 1 < ex.spanel.size(); ++g) {
+      const GemmItem& it0 = ex.items[ex.sorder[ex.spanel[g]]];
+      const int64_t J = ex.bstream == 1 ? it0.M : it0.N;
+      single += ((J + kBlockStreamRows - 1) / kBlockStreamRows) * (ex.spanel[g + 1] - ex.spanel[g]);
+    }
+    const int panel = (int)std::min<int64_t>(
+        kBlockStreamPanel, std::max<int64_t>(1, (single + kBlockStreamWaves - 1) / kBlockStreamWaves));
     ex.swork.clear();
     for (size_t g = 0; g + 1 < ex.spanel.size(); ++g) {
       const int b = ex.spanel[g], e = ex.spanel[g + 1];
       const GemmItem& it0 = ex.items[ex.sorder[b]];
       const int J = ex.bstream == 1 ? it0.M : it0.N;
       for (int j0 = 0; j0 < J; j0 += kBlockStreamRows)
-        for (int c0 = b; c0 < e; c0 += kBlockStreamPanel)
-          ex.swork.push_back({c0, std::min(kBlockStreamPanel, e - c0), j0, 0});
+        for (int c0 = b; c0 < e; c0 += panel)
+          ex.swork.push_back({c0, std::min(panel, e - c0), j0, 0});
     }
     ex.total_tiles = (int)ex.swork.size();
     ex.ksplit = 1;

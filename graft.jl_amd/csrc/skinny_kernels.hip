@@ -173,8 +173,8 @@ __global__ void __launch_bounds__(kSkinnyThreads)
 }
 
 // Per-block streaming for multi-block (sector) steps.  One wave per work
-// entry, one big-side element j per lane, kBlockStreamS small-side outputs
-// per pass held in registers.  Every address of the small operand, the koff
+// entry, kBlockStreamLaneRows big-side elements j per lane (one after
+// another), kBlockStreamS small-side outputs per pass held in registers.  Every address of the small operand, the koff
 // tables and the item/segment records is the same across the wave (scalar
 // loads); the big operand is read where its blocks lie, so no block is
 // gathered into or scattered out of a fused matrix.  The output blocks of one
@@ -195,62 +195,65 @@ __global__ void __launch_bounds__(kSkinnyThreads)
   const StreamWork wk = work[w];
   const double sgBig = (BIG_IS_A ? conjA : conjB) ? -1.0 : 1.0;
   const double sgSml = (BIG_IS_A ? conjB : conjA) ? -1.0 : 1.0;
-  const int j = wk.j0 + lane;
   for (int t = 0; t < wk.count; ++t) {
     const GemmItem* it = items + order[wk.begin + t];
     const int S = BIG_IS_A ? it->N : it->M;
     const int J = BIG_IS_A ? it->M : it->N;
-    const bool live = j < J;
     const int brank = BIG_IS_A ? it->mrank : it->nrank;
     const int srank = BIG_IS_A ? it->nrank : it->mrank;
     const int32_t* bdims = BIG_IS_A ? it->mdims : it->ndims;
     const int32_t* sdims = BIG_IS_A ? it->ndims : it->mdims;
     const uint32_t* cbstr = BIG_IS_A ? it->c_mstr : it->c_nstr;
     const uint32_t* csstr = BIG_IS_A ? it->c_nstr : it->c_mstr;
-    const uint32_t co = live ? sk_decode((uint32_t)j, brank, bdims, cbstr) : 0u;
     double2* Cb = C + it->c_off;
     for (int s0 = 0; s0 < S; s0 += kBlockStreamS) {
       const int sn = min(kBlockStreamS, S - s0);
-      double2 acc[kBlockStreamS];
+      for (int r = 0; r < kBlockStreamLaneRows; ++r) {
+        if (wk.j0 + 64 * r >= J) break;
+        const int j = wk.j0 + 64 * r + lane;
+        const bool live = j < J;
+        double2 acc[kBlockStreamS];
 #pragma unroll
-      for (int s = 0; s < kBlockStreamS; ++s) acc[s] = double2{0.0, 0.0};
-      for (int g = 0; g < it->seg_count; ++g) {
-        const GemmSeg* sg = segs + it->seg_begin + g;
-        const int K = sg->K;
-        const uint32_t* kA = koff + sg->koff_idx;
-        const uint32_t* kbig = BIG_IS_A ? kA : kA + K;
-        const uint32_t* ksml = BIG_IS_A ? kA + K : kA;
-        const double2* big = BIG_IS_A ? A + sg->a_off : B + sg->b_off;
-        const double2* sml = BIG_IS_A ? B + sg->b_off : A + sg->a_off;
-        const uint32_t* bstr = BIG_IS_A ? sg->a_mstr : sg->b_nstr;
-        const uint32_t* sstr = BIG_IS_A ? sg->b_nstr : sg->a_mstr;
-        uint32_t so[kBlockStreamS];
+        for (int s = 0; s < kBlockStreamS; ++s) acc[s] = double2{0.0, 0.0};
+        for (int g = 0; g < it->seg_count; ++g) {
+          const GemmSeg* sg = segs + it->seg_begin + g;
+          const int K = sg->K;
+          const uint32_t* kA = koff + sg->koff_idx;
+          const uint32_t* kbig = BIG_IS_A ? kA : kA + K;
+          const uint32_t* ksml = BIG_IS_A ? kA + K : kA;
+          const double2* big = BIG_IS_A ? A + sg->a_off : B + sg->b_off;
+          const double2* sml = BIG_IS_A ? B + sg->b_off : A + sg->a_off;
+          const uint32_t* bstr = BIG_IS_A ? sg->a_mstr : sg->b_nstr;
+          const uint32_t* sstr = BIG_IS_A ? sg->b_nstr : sg->a_mstr;
+          uint32_t so[kBlockStreamS];
 #pragma unroll
-        for (int s = 0; s < kBlockStreamS; ++s)
-          so[s] = s < sn ? sk_decode((uint32_t)(s0 + s), srank, sdims, sstr) : 0u;
-        if (live) {
-          const uint32_t bo = sk_decode((uint32_t)j, brank, bdims, bstr);
+          for (int s = 0; s < kBlockStreamS; ++s)
+            so[s] = s < sn ? sk_decode((uint32_t)(s0 + s), srank, sdims, sstr) : 0u;
+          if (live) {
+            const uint32_t bo = sk_decode((uint32_t)j, brank, bdims, bstr);
 #pragma unroll 4
-          for (int k = 0; k < K; ++k) {
-            double2 x = big[(uint64_t)bo + kbig[k]];
-            x.y *= sgBig;
-            const uint32_t kk = ksml[k];
+            for (int k = 0; k < K; ++k) {
+              double2 x = big[(uint64_t)bo + kbig[k]];
+              x.y *= sgBig;
+              const uint32_t kk = ksml[k];
 #pragma unroll
-            for (int s = 0; s < kBlockStreamS; ++s) {
-              if (s < sn) {
-                double2 v = sml[(uint64_t)so[s] + kk];
-                v.y *= sgSml;
-                acc[s].x += x.x * v.x - x.y * v.y;
-                acc[s].y += x.x * v.y + x.y * v.x;
+              for (int s = 0; s < kBlockStreamS; ++s) {
+                if (s < sn) {
+                  double2 v = sml[(uint64_t)so[s] + kk];
+                  v.y *= sgSml;
+                  acc[s].x += x.x * v.x - x.y * v.y;
+                  acc[s].y += x.x * v.y + x.y * v.x;
+                }
               }
             }
           }
         }
-      }
-      if (live) {
+        if (live) {
+          const uint32_t co = sk_decode((uint32_t)j, brank, bdims, cbstr);
 #pragma unroll
-        for (int s = 0; s < kBlockStreamS; ++s)
-          if (s < sn) Cb[(uint64_t)co + sk_decode((uint32_t)(s0 + s), srank, sdims, csstr)] = acc[s];
+          for (int s = 0; s < kBlockStreamS; ++s)
+            if (s < sn) Cb[(uint64_t)co + sk_decode((uint32_t)(s0 + s), srank, sdims, csstr)] = acc[s];
+        }
       }
     }
   }
