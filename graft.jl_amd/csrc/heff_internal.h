@@ -112,14 +112,18 @@ struct PermBlock {
   int32_t dims[GRAFT_MAX_RANK];
   int64_t ostr[GRAFT_MAX_RANK];   // fused-matrix stride of every block leg
 };
-hipError_t launch_fill_zero(void* x, int64_t n, hipStream_t stream);
 // Per-element offset maps, built once per plan, and the copies that use them.
+// Blocked tensors of the fused path hold fewer than kMapZero elements.
 constexpr uint32_t kMapGap = 0xFFFFFFFFu;    // element between blocks: untouched
 constexpr uint32_t kMapZero = 0xFFFFFFFEu;   // block without a fused rectangle: zero
 hipError_t launch_perm_map(const PermBlock* blocks, int nblocks, const int32_t* chunk_first,
                            int64_t total, uint32_t* map, hipStream_t stream);
-hipError_t launch_map_copy(bool to_fused, const void* blocked, void* fused, const uint32_t* map,
-                           int64_t total, hipStream_t stream);
+// inv[0 .. ninv) = kMapZero, then inv[map[e]] = e for every mapped element.
+hipError_t launch_invert_map(const uint32_t* map, int64_t total, uint32_t* inv, int64_t ninv,
+                             hipStream_t stream);
+// dst[i] = src[map[i]] for i < total (kMapZero: zero, kMapGap: untouched).
+hipError_t launch_map_gather(const void* src, void* dst, const uint32_t* map, int64_t total,
+                             hipStream_t stream);
 
 // Krylov vector kernels (vector_kernels.hip).
 hipError_t launch_dotc_partials(int64_t n, const void* x, const void* y, double* partials,

@@ -232,7 +232,6 @@ struct StepExec {
   std::vector<int32_t> ca, cb, cc;       // chunk -> first candidate block
   int64_t fa = 0, fb = 0, fc = 0;        // fused matrix elements of A, B, C
   int64_t totA = 0, totB = 0, totC = 0;  // element ranges of the blocked tensors
-  bool fillA = false, fillB = false;     // fused A / B have entries no block covers
   PermBlock* d_pa = nullptr;
   PermBlock* d_pb = nullptr;
   PermBlock* d_pc = nullptr;
@@ -608,7 +607,6 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
   ex.pa.reserve(A.blocks.size());
   ex.pb.reserve(B.blocks.size());
   ex.pc.reserve(C.blocks.size());
-  int64_t coveredA = 0, coveredB = 0;
   for (int i : sorted_blocks(A)) {
     const Block& b = A.blocks[i];
     PermBlock pbk = make_pb(b);
@@ -618,7 +616,6 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
       pbk.other = g.aoff + ro[aR[i]] * (a_kf ? g.K : 1) + co[aC[i]] * (a_kf ? 1 : g.M);
       a_strides(i, a_kf, o1);
       for (size_t l = 0; l < b.dims.size(); ++l) pbk.ostr[l] = o1[l];
-      coveredA += b.size;
     }
     ex.pa.push_back(pbk);
   }
@@ -631,7 +628,6 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
       pbk.other = g.boff + co[bC[i]] * (b_kf ? 1 : g.N) + so[bS[i]] * (b_kf ? g.K : 1);
       b_strides(i, b_kf, o1);
       for (size_t l = 0; l < b.dims.size(); ++l) pbk.ostr[l] = o1[l];
-      coveredB += b.size;
     }
     ex.pb.push_back(pbk);
   }
@@ -653,8 +649,6 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
   ex.totA = A.total;
   ex.totB = B.total;
   ex.totC = C.total;
-  ex.fillA = coveredA != fa;
-  ex.fillB = coveredB != fb;
   ex.ca = chunk_table(ex.pa, A.total);
   ex.cb = chunk_table(ex.pb, B.total);
   ex.cc = chunk_table(ex.pc, C.total);
@@ -909,8 +903,8 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
     for (size_t i = 0; i < pk.size(); ++i) pairs[fill[kblk[pk[i]]]++] = {pa[i], pb[i]};
   }
   const double t_cblocks = timing ? ms_since(t1) : 0.0;
-  if ((uint64_t)A.total >= (1ull << 32) || (uint64_t)B.total >= (1ull << 32) ||
-      (uint64_t)C.total >= (1ull << 32)) {
+  if ((uint64_t)A.total >= (1ull << 32) - 2 || (uint64_t)B.total >= (1ull << 32) - 2 ||
+      (uint64_t)C.total >= (1ull << 32) - 2) {
     err = "tensor exceeds 2^32 elements (32-bit tile offsets)";
     return 1;
   }
@@ -1307,30 +1301,49 @@ static int upload_step_meta(graft_plan_impl& p) {
   return GRAFT_OK;
 }
 
-// Element -> fused-offset maps for every fused step, built on the device once
-// per plan from the block tables (4 bytes per element of each operand).
+// Offset maps of every fused step, built on the device once per plan from
+// the block tables (4 bytes per element): for the result, blocked element ->
+// fused offset; for the operands the inverse, fused offset -> blocked
+// element (kMapZero where no block covers the entry), so that both copy
+// directions are gathers with coalesced writes.
 static int build_maps(graft_ctx* ctx, graft_plan_impl& p) {
   size_t total = 0;
-  auto al = [](size_t n) { return (n * sizeof(uint32_t) + 255) & ~size_t(255); };
+  int64_t tmp_n = 0;
+  auto al = [](int64_t n) { return ((size_t)n * sizeof(uint32_t) + 255) & ~size_t(255); };
   for (auto& ex : p.steps)
-    if (ex.fused) total += al(ex.totA) + al(ex.totB) + al(ex.totC);
+    if (ex.fused) {
+      total += al(ex.fa) + al(ex.fb) + al(ex.totC);
+      tmp_n = std::max(tmp_n, std::max(ex.totA, ex.totB));
+    }
   if (total == 0) return GRAFT_OK;
   GRAFT_CHECK_HIP(hipMalloc(&p.d_maps, total));
+  uint32_t* tmp = nullptr;
+  GRAFT_CHECK_HIP(hipMalloc(&tmp, al(tmp_n)));
   char* base = (char*)p.d_maps;
+  hipError_t e = hipSuccess;
   for (auto& ex : p.steps) {
-    if (!ex.fused) continue;
+    if (!ex.fused || e != hipSuccess) continue;
     ex.d_ma = (uint32_t*)base;
-    base += al(ex.totA);
+    base += al(ex.fa);
     ex.d_mb = (uint32_t*)base;
-    base += al(ex.totB);
+    base += al(ex.fb);
     ex.d_mc = (uint32_t*)base;
     base += al(ex.totC);
-    GRAFT_CHECK_HIP(launch_perm_map(ex.d_pa, (int)ex.pa.size(), ex.d_ca, ex.totA, ex.d_ma, ctx->stream));
-    GRAFT_CHECK_HIP(launch_perm_map(ex.d_pb, (int)ex.pb.size(), ex.d_cb, ex.totB, ex.d_mb, ctx->stream));
-    GRAFT_CHECK_HIP(launch_perm_map(ex.d_pc, (int)ex.pc.size(), ex.d_cc, ex.totC, ex.d_mc, ctx->stream));
+    // (an operand without blocks has no forward map: its inverse is all zero)
+    const int64_t nA = ex.pa.empty() ? 0 : ex.totA, nB = ex.pb.empty() ? 0 : ex.totB;
+    e = launch_perm_map(ex.d_pa, (int)ex.pa.size(), ex.d_ca, nA, tmp, ctx->stream);
+    if (e == hipSuccess) e = launch_invert_map(tmp, nA, ex.d_ma, ex.fa, ctx->stream);
+    if (e == hipSuccess)
+      e = launch_perm_map(ex.d_pb, (int)ex.pb.size(), ex.d_cb, nB, tmp, ctx->stream);
+    if (e == hipSuccess) e = launch_invert_map(tmp, nB, ex.d_mb, ex.fb, ctx->stream);
+    if (e == hipSuccess)
+      e = launch_perm_map(ex.d_pc, (int)ex.pc.size(), ex.d_cc, ex.totC, ex.d_mc, ctx->stream);
   }
   // the maps read the block tables, which a later autotune may re-upload
-  GRAFT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  const hipError_t es = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(tmp);
+  GRAFT_CHECK_HIP(e);
+  GRAFT_CHECK_HIP(es);
   return GRAFT_OK;
 }
 
@@ -1388,10 +1401,8 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     void* Af = fb;
     void* Bf = fb + al(ex.fa);
     void* Cf = fb + al(ex.fa) + al(ex.fb);
-    if (ex.fillA) GRAFT_CHECK_HIP(launch_fill_zero(Af, ex.fa, ctx->stream));
-    if (ex.fillB) GRAFT_CHECK_HIP(launch_fill_zero(Bf, ex.fb, ctx->stream));
-    GRAFT_CHECK_HIP(launch_map_copy(true, A, Af, ex.d_ma, ex.totA, ctx->stream));
-    GRAFT_CHECK_HIP(launch_map_copy(true, B, Bf, ex.d_mb, ex.totB, ctx->stream));
+    GRAFT_CHECK_HIP(launch_map_gather(A, Af, ex.d_ma, ex.fa, ctx->stream));
+    GRAFT_CHECK_HIP(launch_map_gather(B, Bf, ex.d_mb, ex.fb, ctx->stream));
     A = Af;
     B = Bf;
     C = Cf;
@@ -1425,7 +1436,7 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     }
     GRAFT_CHECK_HIP(e);
   }
-  if (ex.fused) GRAFT_CHECK_HIP(launch_map_copy(false, Cdst, C, ex.d_mc, ex.totC, ctx->stream));
+  if (ex.fused) GRAFT_CHECK_HIP(launch_map_gather(C, Cdst, ex.d_mc, ex.totC, ctx->stream));
   return GRAFT_OK;
 }
 

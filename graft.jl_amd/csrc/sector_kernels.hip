@@ -7,8 +7,11 @@
 // result matrices into the output blocks.  Once per plan, a map kernel walks
 // each blocked tensor's elements in storage order (a chunk table narrows the
 // binary search for the block that owns an element) and records the element's
-// fused-matrix offset; every apply then copies through that map with one
-// coalesced 4-byte read per element and no search.
+// fused-matrix offset; for the operands the map is then inverted (fused
+// offset -> blocked element, zero where no block covers the entry).  Every
+// apply copies through these maps as a gather: one coalesced 4-byte map read
+// and one coalesced write per destination element, the reads following the
+// blocks' runs.
 #include "heff_internal.h"
 
 namespace graft {
@@ -52,26 +55,34 @@ __global__ void __launch_bounds__(kPermThreads)
   }
 }
 
-// The per-apply copy: one coalesced map read per element, no block search.
+// Inverse of a forward map (run once per plan): inv[map[e]] = e for every
+// element with a fused offset; inv starts as kMapZero everywhere.
 __global__ void __launch_bounds__(kPermThreads)
-    map_copy_kernel(int to_fused, const double2* __restrict__ src, double2* __restrict__ dst,
-                    const uint32_t* __restrict__ map, int64_t total) {
+    invert_map_kernel(const uint32_t* __restrict__ map, int64_t total, uint32_t* __restrict__ inv) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
     const uint32_t o = map[e];
-    if (o == kMapGap) continue;
-    if (to_fused) {
-      if (o != kMapZero) dst[o] = src[e];
-    } else {
-      dst[e] = (o == kMapZero) ? double2{0.0, 0.0} : src[o];
-    }
+    if (o < kMapZero) inv[o] = (uint32_t)e;
   }
 }
 
-__global__ void __launch_bounds__(kPermThreads) fill_zero_kernel(double2* __restrict__ x, int64_t n) {
+__global__ void __launch_bounds__(kPermThreads)
+    fill_u32_kernel(uint32_t* __restrict__ x, int64_t n, uint32_t v) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    x[i] = double2{0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = v;
+}
+
+// The per-apply copy: dst[i] = src[map[i]] (zero for kMapZero, untouched for
+// kMapGap).  Coalesced map read and write, no block search.
+__global__ void __launch_bounds__(kPermThreads)
+    map_gather_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                      const uint32_t* __restrict__ map, int64_t total) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    const uint32_t o = map[i];
+    if (o == kMapGap) continue;
+    dst[i] = (o == kMapZero) ? double2{0.0, 0.0} : src[o];
+  }
 }
 
 static int perm_grid(int64_t n) {
@@ -89,20 +100,22 @@ hipError_t launch_perm_map(const PermBlock* blocks, int nblocks, const int32_t* 
   return hipGetLastError();
 }
 
-hipError_t launch_map_copy(bool to_fused, const void* blocked, void* fused, const uint32_t* map,
-                           int64_t total, hipStream_t stream) {
-  if (total <= 0) return hipSuccess;
-  const double2* src = to_fused ? (const double2*)blocked : (const double2*)fused;
-  double2* dst = to_fused ? (double2*)fused : (double2*)blocked;
-  hipLaunchKernelGGL(map_copy_kernel, dim3(perm_grid(total)), dim3(kPermThreads), 0, stream,
-                     to_fused ? 1 : 0, src, dst, map, total);
+hipError_t launch_invert_map(const uint32_t* map, int64_t total, uint32_t* inv, int64_t ninv,
+                             hipStream_t stream) {
+  if (ninv > 0)
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(perm_grid(ninv)), dim3(kPermThreads), 0, stream, inv,
+                       ninv, kMapZero);
+  if (total > 0)
+    hipLaunchKernelGGL(invert_map_kernel, dim3(perm_grid(total)), dim3(kPermThreads), 0, stream,
+                       map, total, inv);
   return hipGetLastError();
 }
 
-hipError_t launch_fill_zero(void* x, int64_t n, hipStream_t stream) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(fill_zero_kernel, dim3(perm_grid(n)), dim3(kPermThreads), 0, stream,
-                     (double2*)x, n);
+hipError_t launch_map_gather(const void* src, void* dst, const uint32_t* map, int64_t total,
+                             hipStream_t stream) {
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(map_gather_kernel, dim3(perm_grid(total)), dim3(kPermThreads), 0, stream,
+                     (const double2*)src, (double2*)dst, map, total);
   return hipGetLastError();
 }
 
