@@ -95,3 +95,72 @@ def test_optimal_tree_matches_brute_force():
     opt = P._optimal_tree(spec, dims)
     assert _tree_flops(opt, spec, dims)[0] == best
     assert plan.flops <= P.plan_contraction(spec, dims, optimize=False).flops
+
+
+def _connected(tree, spec):
+    """Every merge of the tree contracts at least one label (the sector search's rule)."""
+    def labels(t):
+        if isinstance(t, int):
+            return set(l for l in spec.labels[t] if l > 0), True
+        la, ca = labels(t[0])
+        lb, cb = labels(t[1])
+        return la ^ lb, ca and cb and bool(la & lb)
+    return labels(tree)[1]
+
+
+def test_device_time_model():
+    """The device executor's roofline time model: a single GEMM against its
+    closed form, and on a four-tensor U(1) chain the device plan is the
+    admissible tree of least modelled time over every binary tree (brute
+    force), while the reference objective (fused_permutes=False) is unchanged."""
+    m, k, n = 300, 20, 7
+    spec = P.ContractionSpec([[-1, 1], [1, -2]], [False] * 2, 2, (1, 1), 0)
+    plan = P.plan_contraction(spec, [[m, k], [k, n]], fused_permutes=True)
+    want = max(8.0 * m * k * n / P.DEVICE_FLOPS, 16.0 * (m * k + k * n + m * n) / P.DEVICE_BYTES)
+    assert math.isclose(plan.device_time, want, rel_tol=1e-12)
+    # the same GEMM bandwidth-bound and compute-bound
+    assert P.step_device_time(1.0, 1e6) == 16e6 / P.DEVICE_BYTES
+    assert P.step_device_time(1e12, 1.0) == 4e12 / P.DEVICE_FLOPS
+
+    L = lambda *s: P.SectorLeg([((q,), d) for q, d in s])
+    # one U(1) block GEMM by hand: charge q of o1 pairs a at -q and o2 at -q
+    for scale in (1, 40):
+        o1 = L((0, 4 * scale), (1, 3 * scale), (-1, 2 * scale))
+        a = L((0, 5 * scale), (1, 1 * scale), (-1, 6 * scale))
+        o2 = L((0, 3), (1, 2), (-1, 1))
+        deg = lambda leg, q: dict((c[0], d) for c, d in leg.sectors)[q]
+        sfl = sum(2.0 * deg(o1, q) * deg(a, -q) * deg(o2, -q) for q in (0, 1, -1))
+        els = sum(deg(o1, q) * deg(a, -q) + deg(a, -q) * deg(o2, -q) + deg(o1, q) * deg(o2, -q)
+                  for q in (0, 1, -1))
+        spec = P.ContractionSpec([[-1, 1], [1, -2]], [False] * 2, 2, (1, 1), 0)
+        plan = P.plan_contraction(spec, [[o1.dim, a.dim], [a.dim, o2.dim]],
+                                  [[o1, a], [a.dual(), o2]], fused_permutes=True)
+        assert plan.sector_flops == sfl
+        want = max(4.0 * sfl / P.DEVICE_FLOPS, 16.0 * els / P.DEVICE_BYTES)
+        assert math.isclose(plan.device_time, want, rel_tol=1e-12), scale
+
+    a = L((0, 3), (1, 5), (-1, 4))
+    b = L((0, 6), (1, 2), (-1, 2), (2, 1))
+    c = L((0, 2), (1, 1), (-1, 1))
+    o1 = L((0, 7), (1, 3), (-1, 3))
+    o2 = L((0, 2), (1, 1), (-1, 1))
+    legs = [[o1, a], [a.dual(), b], [b.dual(), c], [c.dual(), o2]]
+    spec = P.ContractionSpec([[-1, 1], [1, 2], [2, 3], [3, -2]], [False] * 4, 2, (1, 1), 0)
+    dims = [[l.dim for l in ls] for ls in legs]
+    heur = P.compile_plan(P._heuristic_tree(spec), spec, dims, legs, "env_first", fused_permutes=True)
+    heur_ref = P.compile_plan(P._heuristic_tree(spec), spec, dims, legs, "env_first")
+    times, flops = [], []
+    for t in _all_trees(list(range(4))):
+        if not _connected(t, spec):
+            continue
+        p = P.compile_plan(t, spec, dims, legs, "brute", fused_permutes=True)
+        if P._within_floors(p, heur):
+            times.append(p.device_time)
+        q = P.compile_plan(t, spec, dims, legs, "brute")
+        if P._within_floors(q, heur_ref):
+            flops.append(q.sector_flops)
+    assert max(times) > 1.2 * min(times)        # the trees do differ
+    chosen = P.plan_contraction(spec, dims, legs, memory_weight=0, fused_permutes=True)
+    assert math.isclose(chosen.device_time, min(times), rel_tol=1e-12)
+    ref = P.plan_contraction(spec, dims, legs, memory_weight=0)
+    assert ref.sector_flops == min(flops)
