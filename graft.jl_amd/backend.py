@@ -1,8 +1,9 @@
 """L0 Backend on the device — lib/GraftFoundation/src/Backend/Backend.jl.
 
 ``contract_pair`` runs one binary contraction in libgraft_heff.so (the same
-grouped strided MFMA GEMM as the effective maps).  The gauge factorizations
-(``left_orth``, ``orth_factor_leg``, ``split_svd``) call rocSOLVER through
+grouped strided MFMA GEMM as the effective maps); ``matmul`` puts the tall
+matrix products of the gauge and expansion steps on it.  The factorizations
+themselves (QR, Hermitian eigensolves, small SVDs) call rocSOLVER through
 torch.linalg on the device; they are outside the H_eff hot path.
 """
 from __future__ import annotations
@@ -57,6 +58,36 @@ def contract_pair(A, oindA, cindA, conjA, B, cindB, oindB, conjB, out):
     return C
 
 
+def _matrix_operand(M):
+    """(DTensor over the memory of the 2-D torch matrix M, leg of its row index,
+    leg of its column index); no copy when M or M^T is contiguous."""
+    if M.is_conj():
+        M = M.resolve_conj()
+    if M.is_contiguous():
+        return DTensor((M.shape[1], M.shape[0]), M.reshape(-1), 2), 1, 0
+    if M.T.is_contiguous():
+        return DTensor((M.shape[0], M.shape[1]), M.T.reshape(-1), 2), 0, 1
+    M = M.contiguous()
+    return DTensor((M.shape[1], M.shape[0]), M.reshape(-1), 2), 1, 0
+
+
+def matmul(X, Y, op_x="N", op_y="N"):
+    """op(X) @ op(Y) for complex128 device matrices on the library's GEMM
+    (split-K for long contractions: the Gram and projection products of the
+    gauge and expansion steps have K of 10^5 and a few output tiles).  op is
+    "N", "T" (transpose), "C" (conjugate) or "H" (conjugate transpose).
+    Returns a row-major torch matrix."""
+    A, ra, ca = _matrix_operand(X)
+    B, rb, cb = _matrix_operand(Y)
+    i_leg, k_a = (ca, ra) if op_x in ("T", "H") else (ra, ca)
+    k_b, j_leg = (cb, rb) if op_y in ("T", "H") else (rb, cb)
+    if A.dims[k_a] != B.dims[k_b]:
+        raise ValueError(f"matmul: inner dimensions {A.dims[k_a]} and {B.dims[k_b]} differ")
+    C = contract_pair(A, (i_leg,), (k_a,), op_x in ("C", "H"), B, (k_b,), (j_leg,),
+                      op_y in ("C", "H"), ((1, 0), ()))
+    return C.data.view(A.dims[i_leg], B.dims[j_leg])
+
+
 def _others(N, k):
     return tuple(j for j in range(N) if j != k)
 
@@ -107,10 +138,64 @@ def permute(A, perm, ncod=None):
     return DTensor([A.dims[p] for p in perm], data, A.ncod if ncod is None else ncod)
 
 
+TALL_QR_MIN_COLS = 64     # Cholesky route from this many columns ...
+TALL_QR_ASPECT = 8        # ... and at least this many rows per column
+
+
+def _householder_qr(M):
+    return torch.linalg.qr(M, mode="reduced")
+
+
+def tall_qr(M):
+    """Thin QR of a tall matrix by shifted Cholesky QR in three passes
+    (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa, "Shifted Cholesky QR
+    for computing the QR factorization of ill-conditioned matrices", SIAM J.
+    Sci. Comput. 42 (2020) A477): G = M^H M + s I with s = 11 (m n + n (n+1))
+    eps ||M||_F^2, R1 = chol(G), Q1 = M R1^-1, then two plain Cholesky QR passes
+    on Q1.  The Gram and triangular-factor products run on the library GEMM
+    (split-K over the 10^5 rows), where rocSOLVER's Householder panels are
+    sequences of matrix-vector products.  Falls back to Householder when a
+    Cholesky factorization breaks down (numerically rank-deficient M) or the
+    last Gram matrix is not within 1e-6 of the identity.  R has a positive
+    real diagonal (Householder's may carry phases: the gauge differs, the
+    factorization does not)."""
+    m, n = M.shape
+    eye = torch.eye(n, dtype=M.dtype, device=M.device)
+    G = matmul(M, M, "H")
+    G = (G + G.conj().T) / 2
+    shift = 11.0 * (m * n + n * (n + 1)) * np.finfo(np.float64).eps * float(torch.diagonal(G).real.sum())
+    R, info = torch.linalg.cholesky_ex(G + shift * eye, upper=True)
+    if int(info) != 0:
+        return _householder_qr(M)
+    Q = matmul(M, torch.linalg.solve_triangular(R, eye, upper=True))
+    Rtot = R
+    dev = math.inf
+    for _ in range(2):
+        G = matmul(Q, Q, "H")
+        G = (G + G.conj().T) / 2
+        dev = float((G - eye).abs().max())
+        R, info = torch.linalg.cholesky_ex(G, upper=True)
+        if int(info) != 0:
+            return _householder_qr(M)
+        Q = matmul(Q, torch.linalg.solve_triangular(R, eye, upper=True))
+        Rtot = R @ Rtot
+    if not dev <= 1e-6:                  # (also catches NaN from a near-singular pass)
+        return _householder_qr(M)
+    return Q, Rtot
+
+
+def thin_qr(M):
+    """Thin QR: shifted Cholesky QR for tall matrices with many columns, else Householder."""
+    rows, cols = M.shape
+    if cols >= TALL_QR_MIN_COLS and rows >= TALL_QR_ASPECT * cols:
+        return tall_qr(M)
+    return _householder_qr(M)
+
+
 def left_orth(A):
     """QR across (codomain | last leg): A = Q . C, Q isometric — MatrixAlgebraKit left_orth."""
     M = matrix_view(A, A.rank - 1)
-    Q, R = torch.linalg.qr(M, mode="reduced")
+    Q, R = thin_qr(M)
     Qt = from_matrix(Q, A.dims[:-1] + (Q.shape[1],), A.rank - 1)
     Ct = from_matrix(R, (R.shape[0], R.shape[1]), 1)
     return Qt, Ct
@@ -186,7 +271,7 @@ def split_svd(M, ts=NO_TRUNCATION, method=None):
         disc = float(np.sqrt(np.sum(s[r:] ** 2)))
         return U[:, :r], S[:r], Vh[:r, :], disc, s
     wide = rows <= cols
-    G = M @ M.conj().T if wide else M.conj().T @ M
+    G = matmul(M, M, "N", "H") if wide else matmul(M, M, "H", "N")
     lam, Q = torch.linalg.eigh(G)
     lam = torch.flip(lam, (0,)).clamp(min=0.0)
     Q = torch.flip(Q, (1,))
@@ -196,11 +281,11 @@ def split_svd(M, ts=NO_TRUNCATION, method=None):
         return split_svd(M, ts, "direct")
     Qr = Q[:, :r]
     if wide:
-        Ub, S, Vh = torch.linalg.svd(Qr.conj().T @ M, full_matrices=False)
-        U = Qr @ Ub
+        Ub, S, Vh = torch.linalg.svd(matmul(Qr, M, "H"), full_matrices=False)
+        U = matmul(Qr, Ub)
     else:
-        U, S, Vb = torch.linalg.svd(M @ Qr, full_matrices=False)
-        Vh = Vb @ Qr.conj().T
+        U, S, Vb = torch.linalg.svd(matmul(M, Qr), full_matrices=False)
+        Vh = matmul(Vb, Qr, "N", "H")
     s = s.copy()
     s[:r] = S.detach().cpu().numpy()
     disc = float(np.sqrt(np.sum(s[r:] ** 2)))

@@ -106,21 +106,21 @@ def _cbe_directions(ev, psi, H, n, m, active_is_n, Q, dz, room):
         An = psi.tensors[n]
         Bp = B.matrix_view(An, An.rank - 1)           # partner (n legs) x bond
     # double complement: Theta = M_a . Bp^T, so the partner's range acts on the right
-    Z = Z - (Z @ Bp.conj()) @ Bp.T
-    Z = Z - Q @ (Q.conj().T @ Z)
+    Z = Z - B.matmul(B.matmul(Z, Bp, "N", "C"), Bp, "N", "T")
+    Z = Z - B.matmul(Q, B.matmul(Q, Z, "H"))
     norm_theta = T.norm(theta)
     cutoff = strat.spawn_threshold * max(norm_theta, np.finfo(float).eps) / abs(complex(dz))
     rows, cols = Z.shape
     if cols <= rows:
-        lam, V = torch.linalg.eigh(Z.conj().T @ Z)
+        lam, V = torch.linalg.eigh(B.matmul(Z, Z, "H"))
         lam, V = torch.flip(lam, (0,)), torch.flip(V, (1,))
         sig = torch.sqrt(lam.clamp(min=0.0))
         sel = [i for i in range(min(budget, len(lam))) if float(sig[i]) > cutoff]
         if not sel:
             return None
-        E = (Z @ V[:, sel]) / sig[sel]
+        E = B.matmul(Z, V[:, sel]) / sig[sel]
     else:
-        lam, U = torch.linalg.eigh(Z @ Z.conj().T)
+        lam, U = torch.linalg.eigh(B.matmul(Z, Z, "N", "H"))
         lam, U = torch.flip(lam, (0,)), torch.flip(U, (1,))
         sig = torch.sqrt(lam.clamp(min=0.0))
         sel = [i for i in range(min(budget, len(lam))) if float(sig[i]) > cutoff]
@@ -128,7 +128,7 @@ def _cbe_directions(ev, psi, H, n, m, active_is_n, Q, dz, room):
             return None
         E = U[:, sel]
     # exact orthogonality to the kept range, then orthonormal columns
-    E = E - Q @ (Q.conj().T @ E)
+    E = E - B.matmul(Q, B.matmul(Q, E, "H"))
     E, _ = torch.linalg.qr(E, mode="reduced")
     info["selected"] = len(sel)
     info["sigma_max"] = float(sig[0])
@@ -151,12 +151,12 @@ def _cbe_split(ev, psi, H, u, v, dz):
         k = childslot(t, u, v)
         At = B.permute(A, tuple(j for j in range(A.rank) if j != k) + (k,))
     Ma = B.matrix_view(At, At.rank - 1)
-    Q, _ = torch.linalg.qr(Ma, mode="reduced")
+    Q, _ = B.thin_qr(Ma)
     room = max(0, ev.trunc.maxdim - Q.shape[1])
     n, m = (u, v) if up else (v, u)
     E = _cbe_directions(ev, psi, H, n, m, up, Q, dz, room)
     U = Q if E is None else torch.cat([Q, E], dim=1)
-    R = U.conj().T @ Ma
+    R = B.matmul(U, Ma, "H")
     Ut = B.from_matrix(U, At.dims[:-1] + (U.shape[1],), At.rank - 1)
     if not up:
         Ut = B.permute(Ut, B._restore_perm(A.rank, k), A.ncod)

@@ -206,3 +206,56 @@ def test_gram_route_svd_matches_dense_svd(g, shape):
     assert np.abs(low.cpu().numpy() - want).max() < 1e-11
     Ud, Sd, Vhd, discd, _ = B.split_svd(Md, ts, method="direct")
     assert np.abs(Sd.cpu().numpy() - S.cpu().numpy()).max() < 1e-12
+
+
+@pytest.mark.parametrize("shape", [(7, 5, 9), (300, 40000, 48)])
+def test_library_matmul_matches_torch(g, shape):
+    """backend.matmul (the library GEMM, split-K for the long contraction of
+    the second shape) against torch for every op pair, on contiguous and
+    transposed-view operands."""
+    import torch
+    from graft_amd import backend as B
+    m, k, n = shape
+    _say("library matmul", shape)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(m)
+    rnd = lambda r, c: torch.randn(r, c, dtype=torch.complex128, device="cuda", generator=gen)
+    ops = {"N": lambda M: M, "T": lambda M: M.T, "C": lambda M: M.conj(), "H": lambda M: M.conj().T}
+    for ox in "NTCH":
+        for oy in "NTCH":
+            X = rnd(m, k) if ox in "NC" else rnd(k, m)
+            Y = rnd(k, n) if oy in "NC" else rnd(n, k)
+            for view in (False, True):
+                Xv = X.T.contiguous().T if view else X        # same values, transposed storage
+                got = B.matmul(Xv, Y, ox, oy)
+                want = ops[ox](X) @ ops[oy](Y)
+                err = float((got - want).abs().max() / want.abs().max())
+                assert got.shape == (m, n) and err < 1e-13, (ox, oy, view, err)
+
+
+@pytest.mark.parametrize("case", ["random", "cond1e12", "rank_deficient"])
+def test_tall_qr(g, case):
+    """Shifted Cholesky QR (three passes) on tall matrices: orthonormal Q and
+    Q R = M to 1e-12 relative for a random matrix and one with singular values
+    down to 1e-12; a numerically rank-deficient matrix takes the Householder
+    fallback and satisfies the same bounds."""
+    import torch
+    from graft_amd import backend as B
+    m, n = 40000, 96
+    _say("tall qr", case)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(17)
+    rnd = lambda r, c: torch.randn(r, c, dtype=torch.complex128, device="cuda", generator=gen)
+    M = rnd(m, n)
+    if case != "random":
+        U, _ = torch.linalg.qr(rnd(m, n))
+        V, _ = torch.linalg.qr(rnd(n, n))
+        sig = torch.logspace(0, -12, n, dtype=torch.float64, device="cuda")
+        if case == "rank_deficient":
+            sig[n // 2:] = 0.0
+        M = (U * sig) @ V.conj().T
+    Q, R = B.tall_qr(M)
+    eye = torch.eye(n, dtype=torch.complex128, device="cuda")
+    assert float((Q.conj().T @ Q - eye).abs().max()) < 1e-12 * n
+    assert float((Q @ R - M).abs().max()) < 1e-12 * float(M.abs().max())
+    assert float(torch.tril(R, -1).abs().max()) == 0.0
