@@ -153,8 +153,9 @@ def _native_output_structure(native, plan, legs, conjs):
 class EnvCache:
     """Directed-edge environment cache with explicit invalidation events."""
 
-    def __init__(self, topo):
+    def __init__(self, topo, tune_orders=False):
         self.topo = topo
+        self.tune_orders = tune_orders   # measured pairwise order for heavy dense maps
         self.envs = {}
         self.node_generations = [0] * nnodes(topo)
         self.plans = {}
@@ -206,7 +207,12 @@ class EnvCache:
                                 sector_aware=sector_aware, memory_cap_bytes=memory_cap_bytes,
                                 fused_permutes=True, order_cache=_ORDER_CACHE)
         out_dims = slot_dims(plan, dims)[plan.output_slot]
-        p = NativePlan(plan, operands, out_structure, out_dims)
+        p = None
+        if (self.tune_orders and legs is None and optimize and memory_cap_bytes is None
+                and kind in _TUNED_KINDS):
+            p = _measured_order(spec, dims, plan, operands, out_dims)
+        if p is None:
+            p = NativePlan(plan, operands, out_structure, out_dims)
         if out_structure is None and legs is not None:
             p.out_structure = _native_output_structure(p, plan, legs, spec.conjs)
         self.plans[key] = p
@@ -233,6 +239,66 @@ class EnvCache:
                 c = HostBlocks(legs, blocks, len(legs)).to_device()
             self.rootcaps[key] = c
         return c
+
+
+_TUNED_KINDS = ("h1", "h2", "h0")
+_TUNED_ORDERS = {}           # (spec, log2 leg dims) -> measured fastest tree
+TUNE_MIN_SECONDS = 1e-3      # maps modelled below this keep the planner's order
+
+
+def _measured_order(spec, dims, plan, operands, out_dims):
+    """Pairwise order of a dense effective map chosen by measurement.
+
+    The planner ranks dense candidates by FLOPs and live bytes; on the device a
+    short-contraction order can win when the FLOP-optimal one streams a large
+    intermediate through HBM (the depth-1 edges of the D=32 binary tree: the
+    two environments of a degree-3 node open two TTNO channels at once, a
+    w^2 x |Theta| intermediate).  The exact search's frontier trees (trivial
+    sectors) within the planner's memory floor are compiled and each is timed
+    on two applies of zero input; the fastest is kept and remembered for the
+    same spec and leg dimensions within a factor of two.  Maps modelled below
+    TUNE_MIN_SECONDS keep the planner's order.  Returns a NativePlan or None."""
+    from .planning import (SectorLeg, _heuristic_tree, _sector_dp_trees, _within_floors,
+                           compile_plan)
+    key = (spec.key(), tuple(tuple(int(round(math.log2(max(x, 1)))) for x in d) for d in dims))
+    tree = _TUNED_ORDERS.get(key)
+    if tree is not None:
+        if tree == plan.tree:
+            return None
+        p = compile_plan(tree, spec, dims, None, "measured", fused_permutes=True)
+        return NativePlan(p, operands, None, out_dims)
+    if not plan.device_time >= TUNE_MIN_SECONDS:
+        return None
+    legs = [[SectorLeg([((0,), d)]) for d in ds] for ds in dims]
+    heur = compile_plan(_heuristic_tree(spec), spec, dims, None, "env_first", fused_permutes=True)
+    cands = [plan]
+    for t in _sector_dp_trees(spec, dims, legs):
+        if t != plan.tree:
+            c = compile_plan(t, spec, dims, None, "measured", fused_permutes=True)
+            if _within_floors(c, heur):
+                cands.append(c)
+    if len(cands) == 1:
+        _TUNED_ORDERS[key] = plan.tree
+        return None
+    ins = [DTensor.zeros(op.dims, op.ncod) if not isinstance(op, DTensor) else op for op in operands]
+    best, best_ms, best_native = None, math.inf, None
+    for c in cands:
+        native = NativePlan(c, operands, None, out_dims)
+        y = DTensor.empty(out_dims, len(out_dims))
+        native.execute(ins, y)                      # workspace, first-touch
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(2):
+            native.execute(ins, y)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        if ms < best_ms:
+            best, best_ms, best_native = c, ms, native
+    if len(_TUNED_ORDERS) > 4096:
+        _TUNED_ORDERS.clear()
+    _TUNED_ORDERS[key] = best.tree
+    return best_native
 
 
 def _stateleg(t, hp, u, w):

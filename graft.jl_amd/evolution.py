@@ -167,7 +167,7 @@ def _cbe_split(ev, psi, H, u, v, dz):
 def step_(ev, psi, H, dz):
     """step! — tdvp.jl:171-195."""
     if ev.cache is None or ev.cache.topo != psi.topo:
-        ev.cache = EnvCache(psi.topo)
+        ev.cache = EnvCache(psi.topo, tune_orders=True)
     if ev.order == 1:
         _tdvp1_sweep_(ev, psi, H, dz, rev=False)
     elif ev.order == 2:

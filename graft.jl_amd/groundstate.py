@@ -58,7 +58,7 @@ def dmrg2_(psi, H, trunc=B.NO_TRUNCATION, nsweeps=10, tol=1e-10, krylovdim=20, e
     if not H.ishermitian:
         raise ValueError("dmrg2_: DMRG requires ishermitian(H) == true")
     t = psi.topo
-    cache = EnvCache(t)
+    cache = EnvCache(t, tune_orders=True)
     energies = []
     bonds = [n for n in postorder(t) if t.parent[n] >= 0]
     schedule = [(n, "m") for n in bonds] + [(n, "n") for n in reversed(bonds)]

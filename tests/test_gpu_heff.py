@@ -362,3 +362,26 @@ def test_tdvp1_full_rank_equals_exact_evolution(g):
     v = ref.ttns_to_dense(t, [a.to_numpy() for a in psi.tensors], [True] * t.n)
     assert abs(1 - abs(np.vdot(v, vex))) < 1e-10
     assert abs(np.linalg.norm(v) - 1) < 1e-10
+
+
+def test_measured_order_selection(g, monkeypatch):
+    """With tune_orders the map may run another pairwise order (timed on the
+    device); its result equals the planner's order to 1e-12, and the chosen
+    tree is remembered for the next cache."""
+    from graft_amd import contractions as C, networks as N, operators as O, trees as Tr
+    monkeypatch.setattr(C, "TUNE_MIN_SECONDS", 0.0)
+    monkeypatch.setattr(C, "_TUNED_ORDERS", {})
+    topo = Tr.binary_topology(3)
+    _say("measured order selection, binary(3) D=8 edge (3,1)")
+    psi = N.random_ttns(np.random.default_rng(4), topo, 2, 8)
+    H = O.heisenberg_ttno(topo, 1.0)
+    N.move_center_(psi, 1)
+    x = C.two_site_tensor(psi, 3, 1)
+    want = C.eff_h2(C.EnvCache(topo), psi, H, 3, 1)(x).to_numpy()
+    tuned = C.eff_h2(C.EnvCache(topo, tune_orders=True), psi, H, 3, 1)
+    assert len(C._TUNED_ORDERS) == 1
+    assert _rel(tuned(x).to_numpy(), want) < 1e-12
+    again = C.eff_h2(C.EnvCache(topo, tune_orders=True), psi, H, 3, 1)
+    assert len(C._TUNED_ORDERS) == 1
+    assert again.native.plan.tree == tuned.native.plan.tree
+    assert _rel(again(x).to_numpy(), want) < 1e-12
