@@ -110,6 +110,7 @@ def lib():
         "graft_vec_multi_dotc_dev": [vp, i64, i32, pp, vp, vp],
         "graft_vec_multi_axpy_dev": [vp, i64, i32, pp, vp, ctypes.c_double, vp],
         "graft_vec_nrm2_dev": [vp, i64, vp, vp],
+        "graft_vec_orthonormalize_dev": [vp, i64, i32, pp, vp, ctypes.c_double, vp, vp, vp],
         "graft_vec_scale_inv_dev": [vp, i64, vp, vp],
         "graft_vec_block_combine": [vp, i64, i32, pp, i32, dp, pp],
         "graft_graph_capture_begin": [vp],

@@ -126,8 +126,9 @@ hipError_t launch_map_gather(const void* src, void* dst, const uint32_t* map, in
                              hipStream_t stream);
 
 // Krylov vector kernels (vector_kernels.hip).
+// gate: optional device flag; gate[0] == 0 skips the pass (zero partials, no traffic).
 hipError_t launch_dotc_partials(int64_t n, const void* x, const void* y, double* partials,
-                                int nblocks, hipStream_t stream);
+                                int nblocks, hipStream_t stream, const double* gate = nullptr);
 hipError_t launch_reduce_partials(const double* partials, int nblocks, int k, double* out,
                                   hipStream_t stream);
 constexpr int kMaxMultiVec = 64;
@@ -136,11 +137,16 @@ struct PtrList {
   const double2* p[kMaxMultiVec];
 };
 hipError_t launch_multi_dotc_partials(int64_t n, int k, const PtrList& V, const void* y,
-                                      double* partials, int nblocks, hipStream_t stream);
+                                      double* partials, int nblocks, hipStream_t stream,
+                                      const double* gate = nullptr);
 hipError_t launch_axpby(int64_t n, double ar, double ai, const void* x, double br, double bi,
                         void* y, hipStream_t stream);
 hipError_t launch_multi_axpy(int64_t n, int k, const PtrList& V, const double* c, double sign,
-                             void* y, hipStream_t stream);
+                             void* y, hipStream_t stream, const double* gate = nullptr);
+hipError_t launch_reorth_gate(const double* c, int k, const double* dot2, double eta, double* beta,
+                              double* gate, hipStream_t stream);
+hipError_t launch_reorth_finish(const double* dot2, const double* gate, double* beta,
+                                hipStream_t stream);
 hipError_t launch_sqrt_real(const double* dot2, double* out, hipStream_t stream);
 constexpr int kMaxCombine = 32;  // outputs of one block combine
 hipError_t launch_block_combine(int64_t n, int k, const PtrList& V, int p, const PtrList& O,

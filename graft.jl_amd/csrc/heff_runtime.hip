@@ -2040,6 +2040,33 @@ int graft_vec_multi_axpy_dev(graft_ctx* ctx, int64_t n, int32_t k, const void* c
   return GRAFT_OK;
 }
 
+int graft_vec_orthonormalize_dev(graft_ctx* ctx, int64_t n, int32_t k, const void* const* V,
+                                 void* w, double eta, double* d_c1, double* d_c2, double* d_beta2) {
+  GRAFT_REQUIRE(ctx && V && w && d_c1 && d_c2 && d_beta2 && n >= 0, "bad argument");
+  GRAFT_REQUIRE(k >= 1 && k <= kMaxMultiVec, "vector count out of range");
+  GRAFT_REQUIRE(eta > 0.0 && eta < 1.0, "eta must lie in (0, 1)");
+  const PtrList P = make_ptrs(k, V);
+  double* gate = d_beta2 + 1;
+  hipStream_t s = ctx->stream;
+  // first pass
+  GRAFT_CHECK_HIP(launch_multi_dotc_partials(n, k, P, w, ctx->d_partials, kReduceBlocks, s));
+  GRAFT_CHECK_HIP(launch_reduce_partials(ctx->d_partials, kReduceBlocks, k, d_c1, s));
+  GRAFT_CHECK_HIP(launch_multi_axpy(n, k, P, d_c1, -1.0, w, s));
+  // ||w_1|| and the refinement decision, on the device
+  GRAFT_CHECK_HIP(launch_dotc_partials(n, w, w, ctx->d_partials, kReduceBlocks, s));
+  GRAFT_CHECK_HIP(launch_reduce_partials(ctx->d_partials, kReduceBlocks, 1, ctx->d_result, s));
+  GRAFT_CHECK_HIP(launch_reorth_gate(d_c1, k, ctx->d_result, eta, d_beta2, gate, s));
+  // second pass (skipped inside the kernels when the gate is off)
+  GRAFT_CHECK_HIP(launch_multi_dotc_partials(n, k, P, w, ctx->d_partials, kReduceBlocks, s, gate));
+  GRAFT_CHECK_HIP(launch_reduce_partials(ctx->d_partials, kReduceBlocks, k, d_c2, s));
+  GRAFT_CHECK_HIP(launch_multi_axpy(n, k, P, d_c2, -1.0, w, s, gate));
+  GRAFT_CHECK_HIP(launch_dotc_partials(n, w, w, ctx->d_partials, kReduceBlocks, s, gate));
+  GRAFT_CHECK_HIP(launch_reduce_partials(ctx->d_partials, kReduceBlocks, 1, ctx->d_result, s));
+  GRAFT_CHECK_HIP(launch_reorth_finish(ctx->d_result, gate, d_beta2, s));
+  GRAFT_CHECK_HIP(launch_scale_inv(n, d_beta2, w, s));
+  return GRAFT_OK;
+}
+
 int graft_vec_nrm2_dev(graft_ctx* ctx, int64_t n, const void* x, double* d_out) {
   GRAFT_REQUIRE(ctx && x && d_out && n >= 0, "bad argument");
   GRAFT_CHECK_HIP(launch_dotc_partials(n, x, x, ctx->d_partials, kReduceBlocks, ctx->stream));

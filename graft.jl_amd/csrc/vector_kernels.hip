@@ -32,10 +32,18 @@ __device__ __forceinline__ void block_sum2(double& re, double& im, double* sh) {
   }
 }
 
+// A non-null gate with gate[0] == 0 turns the pass off: zero partials, no
+// vector traffic (a capturable graph keeps the launch and skips the work).
+__device__ __forceinline__ bool gated_off(const double* gate) { return gate != nullptr && gate[0] == 0.0; }
+
 __global__ void __launch_bounds__(kVecThreads)
     dotc_partials_kernel(int64_t n, const double2* __restrict__ x, const double2* __restrict__ y,
-                         double* __restrict__ partials) {
+                         double* __restrict__ partials, const double* __restrict__ gate) {
   __shared__ double sh[2 * (kVecThreads / 64)];
+  if (gated_off(gate)) {
+    if (threadIdx.x < 2) partials[2 * blockIdx.x + threadIdx.x] = 0.0;
+    return;
+  }
   double re = 0.0, im = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -76,9 +84,13 @@ __global__ void __launch_bounds__(kVecThreads)
 template <int CH>
 __global__ void __launch_bounds__(kVecThreads)
     multi_dotc_partials_kernel(int64_t n, int k, PtrList V, const double2* __restrict__ y,
-                               double* __restrict__ partials) {
+                               double* __restrict__ partials, const double* __restrict__ gate) {
   constexpr int kWaves = kVecThreads / 64;
   __shared__ double sh[kWaves][2 * CH];
+  if (gated_off(gate)) {
+    for (int t = threadIdx.x; t < 2 * k; t += blockDim.x) partials[(size_t)blockIdx.x * 2 * k + t] = 0.0;
+    return;
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int j0 = 0; j0 < k; j0 += CH) {
@@ -142,7 +154,8 @@ __global__ void __launch_bounds__(kVecThreads)
 template <int KMAX>
 __global__ void __launch_bounds__(kVecThreads)
     multi_axpy_kernel(int64_t n, int k, PtrList V, const double* __restrict__ c, double sign,
-                      double2* __restrict__ y) {
+                      double2* __restrict__ y, const double* __restrict__ gate) {
+  if (gated_off(gate)) return;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     double2 acc = y[i];
@@ -167,9 +180,9 @@ static int grid_for(int64_t n) {
 }
 
 hipError_t launch_dotc_partials(int64_t n, const void* x, const void* y, double* partials,
-                                int nblocks, hipStream_t stream) {
+                                int nblocks, hipStream_t stream, const double* gate) {
   hipLaunchKernelGGL(dotc_partials_kernel, dim3(nblocks), dim3(kVecThreads), 0, stream, n,
-                     (const double2*)x, (const double2*)y, partials);
+                     (const double2*)x, (const double2*)y, partials, gate);
   return hipGetLastError();
 }
 
@@ -181,13 +194,14 @@ hipError_t launch_reduce_partials(const double* partials, int nblocks, int k, do
 }
 
 hipError_t launch_multi_dotc_partials(int64_t n, int k, const PtrList& V, const void* y,
-                                      double* partials, int nblocks, hipStream_t stream) {
+                                      double* partials, int nblocks, hipStream_t stream,
+                                      const double* gate) {
   if (k <= 8)
     hipLaunchKernelGGL(multi_dotc_partials_kernel<8>, dim3(nblocks), dim3(kVecThreads), 0, stream,
-                       n, k, V, (const double2*)y, partials);
+                       n, k, V, (const double2*)y, partials, gate);
   else
     hipLaunchKernelGGL(multi_dotc_partials_kernel<16>, dim3(nblocks), dim3(kVecThreads), 0, stream,
-                       n, k, V, (const double2*)y, partials);
+                       n, k, V, (const double2*)y, partials, gate);
   return hipGetLastError();
 }
 
@@ -199,16 +213,16 @@ hipError_t launch_axpby(int64_t n, double ar, double ai, const void* x, double b
 }
 
 hipError_t launch_multi_axpy(int64_t n, int k, const PtrList& V, const double* c, double sign,
-                             void* y, hipStream_t stream) {
+                             void* y, hipStream_t stream, const double* gate) {
   if (k <= 8) {
     hipLaunchKernelGGL(multi_axpy_kernel<8>, dim3(grid_for(n)), dim3(kVecThreads), 0, stream, n, k,
-                       V, c, sign, (double2*)y);
+                       V, c, sign, (double2*)y, gate);
   } else if (k <= 32) {
     hipLaunchKernelGGL(multi_axpy_kernel<32>, dim3(grid_for(n)), dim3(kVecThreads), 0, stream, n,
-                       k, V, c, sign, (double2*)y);
+                       k, V, c, sign, (double2*)y, gate);
   } else {
     hipLaunchKernelGGL(multi_axpy_kernel<kMaxMultiVec>, dim3(grid_for(n)), dim3(kVecThreads), 0,
-                       stream, n, k, V, c, sign, (double2*)y);
+                       stream, n, k, V, c, sign, (double2*)y, gate);
   }
   return hipGetLastError();
 }
@@ -228,6 +242,39 @@ __global__ void __launch_bounds__(kVecThreads)
     v.y *= s;
     x[i] = v;
   }
+}
+
+// After the first Gram-Schmidt pass: beta = ||w_1|| and gate = 1 when the
+// pass removed more than 1 - eta of the norm (||w_1||^2 < eta^2 ||w_0||^2
+// with ||w_0||^2 = ||w_1||^2 + sum |c_j|^2 for an orthonormal basis).
+__global__ void reorth_gate_kernel(const double* __restrict__ c, int k,
+                                   const double* __restrict__ dot2, double eta,
+                                   double* __restrict__ beta, double* __restrict__ gate) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s += c[2 * j] * c[2 * j] + c[2 * j + 1] * c[2 * j + 1];
+    const double n2 = fmax(dot2[0], 0.0);
+    beta[0] = sqrt(n2);
+    gate[0] = (n2 < eta * eta * (n2 + s)) ? 1.0 : 0.0;
+  }
+}
+
+// After the (possibly skipped) second pass: beta = ||w_2|| when it ran.
+__global__ void reorth_finish_kernel(const double* __restrict__ dot2,
+                                     const double* __restrict__ gate, double* __restrict__ beta) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && gate[0] != 0.0) beta[0] = sqrt(fmax(dot2[0], 0.0));
+}
+
+hipError_t launch_reorth_gate(const double* c, int k, const double* dot2, double eta, double* beta,
+                              double* gate, hipStream_t stream) {
+  hipLaunchKernelGGL(reorth_gate_kernel, dim3(1), dim3(64), 0, stream, c, k, dot2, eta, beta, gate);
+  return hipGetLastError();
+}
+
+hipError_t launch_reorth_finish(const double* dot2, const double* gate, double* beta,
+                                hipStream_t stream) {
+  hipLaunchKernelGGL(reorth_finish_kernel, dim3(1), dim3(64), 0, stream, dot2, gate, beta);
+  return hipGetLastError();
 }
 
 hipError_t launch_sqrt_real(const double* dot2, double* out, hipStream_t stream) {

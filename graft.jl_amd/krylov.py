@@ -56,13 +56,22 @@ class ConvergenceInfo:
     numops: int
 
 
+REORTH_ETA = 1.0 / np.sqrt(2.0)   # KrylovKit's ModifiedGramSchmidtIR default
+
+
 def _orthogonalize(V, w):
-    """w <- w - V V^H w twice (CGS2); returns the accumulated coefficients."""
+    """w <- w - V V^H w (classical Gram-Schmidt), once more when the pass
+    removed more than 1 - REORTH_ETA of the norm: the refinement rule of
+    KrylovKit's ModifiedGramSchmidtIR (||w_1|| < eta ||w_0||, with ||w_0||^2 =
+    ||w_1||^2 + sum |h|^2 for an orthonormal V).  Returns the coefficients."""
     h = T.multi_inner(V, w)
     T.multi_axpy_(V, -h, w)
-    h2 = T.multi_inner(V, w)
-    T.multi_axpy_(V, -h2, w)
-    return h + h2
+    n1 = T.norm(w)
+    if n1 * n1 < REORTH_ETA ** 2 * (n1 * n1 + float(np.sum(np.abs(h) ** 2))):
+        h2 = T.multi_inner(V, w)
+        T.multi_axpy_(V, -h2, w)
+        h = h + h2
+    return h
 
 
 # ---------------------------------------------------------------------------
@@ -143,8 +152,10 @@ def _capture(issue):
 class DeviceLanczos:
     """Fixed Krylov basis V_0..V_m of one map, with device-resident coefficients.
 
-    Iteration j: V_{j+1} = A V_j; two CGS passes against V_0..V_j whose
-    coefficients land in device memory; beta_j = ||V_{j+1}||; V_{j+1} /= beta_j.
+    Iteration j: V_{j+1} = A V_j; a CGS pass against V_0..V_j and a second one
+    when the first cancelled most of the norm (``_orthogonalize``'s rule, decided
+    on the device), coefficients in device memory; beta_j = ||V_{j+1}||;
+    V_{j+1} /= beta_j.
     ``expand(j0, j1)`` queues iterations j0..j1-1 (replaying a captured graph
     when enabled) and returns their coefficients after one host read."""
 
@@ -156,7 +167,7 @@ class DeviceLanczos:
         self.V = [T.DTensor.empty(like.dims, like.ncod, like.blocks) for _ in range(m + 1)]
         self.n = like.numel
         self.ptrs = [v.ptr for v in self.V]
-        self.stride = 4 * (m + 1) + 2            # [pass-1 coeffs | pass-2 coeffs | beta, pad]
+        self.stride = 4 * (m + 1) + 2            # [pass-1 coeffs | pass-2 coeffs | beta, 2nd pass]
         self.dcoef = torch.zeros(m * self.stride, dtype=torch.float64, device="cuda")
         self.hcoef = torch.empty(m * self.stride, dtype=torch.float64, pin_memory=True)
         self.use_graphs = USE_GRAPHS if use_graphs is None else use_graphs
@@ -175,12 +186,9 @@ class DeviceLanczos:
             self.A(self.V[j], out=self.V[j + 1])
             Vp = nat.ptr_array(self.ptrs[: j + 1])
             w = ctypes.c_void_p(self.ptrs[j + 1])
-            nat.check(L.graft_vec_multi_dotc_dev(ctx, n, j + 1, Vp, w, ctypes.c_void_p(c1)))
-            nat.check(L.graft_vec_multi_axpy_dev(ctx, n, j + 1, Vp, ctypes.c_void_p(c1), -1.0, w))
-            nat.check(L.graft_vec_multi_dotc_dev(ctx, n, j + 1, Vp, w, ctypes.c_void_p(c2)))
-            nat.check(L.graft_vec_multi_axpy_dev(ctx, n, j + 1, Vp, ctypes.c_void_p(c2), -1.0, w))
-            nat.check(L.graft_vec_nrm2_dev(ctx, n, w, ctypes.c_void_p(cb)))
-            nat.check(L.graft_vec_scale_inv_dev(ctx, n, ctypes.c_void_p(cb), w))
+            nat.check(L.graft_vec_orthonormalize_dev(ctx, n, j + 1, Vp, w, REORTH_ETA,
+                                                     ctypes.c_void_p(c1), ctypes.c_void_p(c2),
+                                                     ctypes.c_void_p(cb)))
 
     def expand(self, j0, j1):
         """Run iterations j0..j1-1; returns (h[j] = CGS coefficients over V_0..V_j, beta[j])."""

@@ -223,6 +223,16 @@ int graft_vec_multi_axpy_dev(graft_ctx* ctx, int64_t n, int32_t k, const void* c
                              const double* d_coef2k, double sign, void* y);
 int graft_vec_nrm2_dev(graft_ctx* ctx, int64_t n, const void* x, double* d_out);
 int graft_vec_scale_inv_dev(graft_ctx* ctx, int64_t n, const double* d_beta, void* x);
+/* One Krylov orthonormalization of w against the orthonormal V_0..V_{k-1},
+ * all on the device and capturable: a classical Gram-Schmidt pass (d_c1,
+ * 2k doubles), a second pass (d_c2, zero when skipped) only when the first
+ * removed more than 1 - eta of the norm -- the refinement rule of KrylovKit's
+ * ModifiedGramSchmidtIR (eta = 1/sqrt(2)), the orthogonalizer behind the
+ * eigsolve / exponentiate hooks (GroundState.jl:25-26, Evolution.jl:134-135)
+ * -- then w /= ||w||.  d_beta2[0] = ||w|| before the scaling, d_beta2[1] = 1
+ * when the second pass ran, else 0. */
+int graft_vec_orthonormalize_dev(graft_ctx* ctx, int64_t n, int32_t k, const void* const* V,
+                                 void* w, double eta, double* d_c1, double* d_c2, double* d_beta2);
 /* O_j = sum_l coef[l + j*k] V_l for j < p (host coefficients, column-major
  * k x p complex, p <= 32).  Outputs may alias inputs, so a Krylov basis is
  * recombined in place at a thick restart.  Not capturable. */

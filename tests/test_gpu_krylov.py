@@ -259,3 +259,37 @@ def test_tall_qr(g, case):
     assert float((Q.conj().T @ Q - eye).abs().max()) < 1e-12 * n
     assert float((Q @ R - M).abs().max()) < 1e-12 * float(M.abs().max())
     assert float(torch.tril(R, -1).abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("near_span", [False, True])
+def test_device_orthonormalize(g, near_span):
+    """graft_vec_orthonormalize_dev: a generic vector takes one Gram-Schmidt
+    pass (second-pass flag 0, zero second coefficients); a vector within 1e-8
+    of span(V) takes the second pass (flag 1).  Either way the result is a unit
+    vector orthogonal to V to 1e-12 and w_0 = V (c1 + c2) + beta w."""
+    import torch
+    from graft_amd import _native as nat, krylov as K
+    from graft_amd.tensors import DTensor
+    rng = np.random.default_rng(23)
+    n, k = 50001, 9
+    _say("device orthonormalize", near_span)
+    Q, _ = np.linalg.qr(ref.random_complex(rng, (n, k)))
+    c = ref.random_complex(rng, k)
+    w0 = Q @ c + (1e-8 if near_span else 1.0) * ref.random_complex(rng, n)
+    V = [DTensor.from_numpy(np.ascontiguousarray(Q[:, j])) for j in range(k)]
+    w = DTensor.from_numpy(w0)
+    out = torch.zeros(4 * k + 2, dtype=torch.float64, device="cuda")
+    p = out.data_ptr()
+    nat.check(nat.lib().graft_vec_orthonormalize_dev(
+        nat.default_context().handle, n, k, nat.ptr_array([v.ptr for v in V]), ctypes.c_void_p(w.ptr),
+        K.REORTH_ETA, ctypes.c_void_p(p), ctypes.c_void_p(p + 16 * k), ctypes.c_void_p(p + 32 * k)))
+    o = out.cpu().numpy()
+    c1, c2 = o[: 2 * k].view(np.complex128), o[2 * k: 4 * k].view(np.complex128)
+    beta, flag = o[4 * k], o[4 * k + 1]
+    assert flag == (1.0 if near_span else 0.0)
+    if not near_span:
+        assert np.all(c2 == 0)
+    wn = w.to_numpy()
+    assert abs(np.linalg.norm(wn) - 1) < 1e-13
+    assert np.abs(Q.conj().T @ wn).max() < 1e-12
+    assert np.linalg.norm(Q @ (c1 + c2) + beta * wn - w0) < 1e-13 * np.linalg.norm(w0)
