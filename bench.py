@@ -99,38 +99,107 @@ def cpu_baseline(D_cpu, depth=5, seed=7, budget_s=20.0):
     return dict(seconds_per_apply=sec, flops=8.0 * plan.flops, D=D_cpu, samples=len(times))
 
 
-def sweep_bench(D, precision, nsweeps=1, depth=5, seed=7, krylovdim=20, phase_times=False):
-    """dmrg2_ sweeps on the depth-5 binary-tree Heisenberg model with maxdim D."""
+def cpu_edge_rate(n, m, D, depth=5, seed=7, reps=3):
+    """The oracle runs the FLOP-optimal pairwise plan (explicit permute + numpy/OpenBLAS
+    zgemm per step) of the eff_h2 map on edge (n, m) of the depth-``depth`` binary
+    Heisenberg tree with every bond at D and random environments; returns
+    (GFLOP/s on the host cores, seconds per apply, FLOP per apply)."""
+    from oracle import heff_oracle as ref
+    from graft_amd import operators as O, planning as P, trees as Tr
+    topo = Tr.binary_topology(depth)
+    t = ref.Tree(list(topo.ids), list(topo.parent), [list(c) for c in topo.children])
+    rng = np.random.default_rng(seed)
+    W = O.heisenberg_host(topo, 1.0)
+    envs = {(c, n): ref.random_complex(rng, (D, 5, D)) for c in t.children[n]}
+    envs.update({(c, m): ref.random_complex(rng, (D, 5, D)) for c in t.children[m] if c != n})
+    if t.parent[m] >= 0:
+        envs[(t.parent[m], m)] = ref.random_complex(rng, (D, 5, D))
+    shape = tuple([D] * len(t.children[n]) + [2] + [D] * (len(t.children[m]) - 1) + [2]
+                  + [D if t.parent[m] >= 0 else 1])
+    theta = ref.random_complex(rng, shape)
+    labels, statics = ref.h2_network(t, W, envs, n, m, shape)
+    spec = P.ContractionSpec(labels, [False] * len(labels), len(shape), (len(shape), 0), 0)
+    plan = P.plan_contraction(spec, [list(shape)] + [list(x.shape) for x in statics])
+    steps = [(q.a, q.b, q.dst, q.oindA, q.cindA, q.oindB, q.cindB, q.conjA, q.conjB,
+              tuple(q.out[0]) + tuple(q.out[1])) for q in plan.steps]
+    ops = [theta] + statics
+    ref.execute_pair_steps(steps, ops, plan.nslots, plan.output_slot)      # warm-up
+    times = []
+    for _ in range(reps):
+        a = time.perf_counter()
+        ref.execute_pair_steps(steps, ops, plan.nslots, plan.output_slot)
+        times.append(time.perf_counter() - a)
+    sec = float(np.median(times))
+    return 8.0 * plan.flops / sec / 1e9, sec, 8.0 * plan.flops
+
+
+def sweep_bench(D, precision, nsweeps=1, depth=5, seed=7, krylovdim=20, phase_times=False,
+                cpu_estimate=False):
+    """dmrg2_ sweeps on the depth-5 binary-tree Heisenberg model with maxdim D.
+
+    ``cpu_estimate``: the FLOP-optimal plan's FLOPs of every apply the sweep ran,
+    divided by the oracle's measured rate on the heaviest edge (a lower bound
+    for the host's sweep time: it leaves out environments, SVDs and the lower
+    rate of the smaller applies)."""
     import torch
     import __graft_entry__
     __graft_entry__.load_package()
     from graft_amd import backend as B, groundstate as GS, networks as N, operators as O, trees as Tr
+    from graft_amd import planning as P
     topo = Tr.binary_topology(depth)
     rng = np.random.default_rng(seed)
     psi = N.random_ttns(rng, topo, 2, D)
     H = O.heisenberg_ttno(topo, 1.0)
     times, counts = [], {"fp64": 0, "lowp": 0}
+    solves = []                        # (spec, operand dims, applies) of every local problem
+    orig_backend = GS.groundstate_eigsolve_backend
+
+    def counted(effective, x, howmany, which, precision="fp64", **kw):
+        r = orig_backend(effective, x, howmany, which, precision, **kw)
+        solves.append((effective.spec, [list(x.dims)] + [list(st.dims) for st in effective.statics],
+                       r[2].numops + getattr(r[2], "lowp_ops", 0)))
+        return r
 
     def cb(sweep, n, m, E, info):
         counts["fp64"] += info.numops
         counts["lowp"] += getattr(info, "lowp_ops", 0)
 
     energies, phases = [], []
-    for s in range(nsweeps):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        tm = {} if phase_times else None
-        _, Es = GS.dmrg2_(psi, H, trunc=B.TruncationScheme(maxdim=D), nsweeps=1, krylovdim=krylovdim,
-                          precision=precision, callback=cb, timings=tm)
-        torch.cuda.synchronize()
-        times.append(time.perf_counter() - t0)
-        energies.append(Es[-1])
-        if tm is not None:
-            tm["eigsolve_tflops_fp64_equiv"] = (tm.get("flops_fp64", 0) + tm.get("flops_lowp", 0)) \
-                / max(tm.get("eigsolve", 1e-9), 1e-9) / 1e12
-            phases.append(tm)
-    return dict(D=D, precision=precision, sweep_s=times, energies=energies, applies=counts,
-                phases=phases)
+    if cpu_estimate:
+        GS.groundstate_eigsolve_backend = counted
+    try:
+        for s in range(nsweeps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            tm = {} if phase_times else None
+            _, Es = GS.dmrg2_(psi, H, trunc=B.TruncationScheme(maxdim=D), nsweeps=1,
+                              krylovdim=krylovdim, precision=precision, callback=cb, timings=tm)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+            energies.append(Es[-1])
+            if tm is not None:
+                tm["eigsolve_tflops_fp64_equiv"] = (tm.get("flops_fp64", 0) + tm.get("flops_lowp", 0)) \
+                    / max(tm.get("eigsolve", 1e-9), 1e-9) / 1e12
+                phases.append(tm)
+    finally:
+        GS.groundstate_eigsolve_backend = orig_backend
+    out = dict(D=D, precision=precision, sweep_s=times, energies=energies, applies=counts,
+               phases=phases)
+    if cpu_estimate:
+        flops, cache = 0.0, {}
+        for spec, dims, ops in solves:        # after the timed sweeps: planning is host work
+            key = (spec.key(), tuple(tuple(d) for d in dims))
+            if key not in cache:
+                cache[key] = 8.0 * P.plan_contraction(spec, dims).flops
+            flops += cache[key] * ops
+        rate, sec, f = cpu_edge_rate(3, 1, D, depth, seed)
+        out["cpu_estimate"] = {
+            "seconds_per_sweep_lower_bound": flops / nsweeps / (rate * 1e9),
+            "apply_flops_per_sweep": flops / nsweeps, "cpu_gflops": rate,
+            "threads": blas_threads(),
+            "sample": f"oracle eff_h2 on edge (3, 1) at D={D}: {sec:.3f} s per apply "
+                      f"({f / 1e9:.1f} GFLOP), median of 3"}
+    return out
 
 
 def hubbard_tree():
@@ -373,7 +442,8 @@ def main():
     import torch
     torch.cuda.set_device(0)
     if args.mode == "sweep":
-        r = sweep_bench(args.sweep_D, args.precision, args.sweeps, phase_times=args.phase_times)
+        r = sweep_bench(args.sweep_D, args.precision, args.sweeps, phase_times=args.phase_times,
+                        cpu_estimate=not args.no_cpu)
         print(json.dumps({"metric": "dmrg2 sweep seconds", "value": r["sweep_s"][-1], "unit": "s",
                           "higher_is_better": False, "n_gpus": 1, "config": {
                               "workload": f"dmrg2_ sweep, depth-5 binary tree Heisenberg, maxdim {r['D']}",
