@@ -566,7 +566,8 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     q = S.find(skey);
     return r >= 0 && q >= 0 && rg[r] >= 0 && rg[r] == sg[q];
   };
-  auto c_strides = [&](const Block& b, int r, int q, bool mf, int64_t* o) {
+  // (reads the tuple keys c_lookup just filled for this block)
+  auto c_strides = [&](const Block& b, int r, bool mf, int64_t* o) {
     const Comp& g = comps[rg[r]];
     radix_o(A, st.oindA, rkey, rorder, rr);
     radix_o(B, st.oindB, skey, sorder, rs);
@@ -593,8 +594,8 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
   for (size_t i = 0; i < C.blocks.size(); ++i) {
     int r, q;
     if (!c_lookup(C.blocks[i], r, q)) continue;
-    c_strides(C.blocks[i], r, q, true, o1);
-    c_strides(C.blocks[i], r, q, false, o2);
+    c_strides(C.blocks[i], r, true, o1);
+    c_strides(C.blocks[i], r, false, o2);
     rc1 += runs(C.blocks[i], o1);
     rc2 += runs(C.blocks[i], o2);
   }
@@ -638,7 +639,7 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     if (c_lookup(b, r, q)) {
       const Comp& g = comps[rg[r]];
       pbk.other = g.coff + ro[r] * (c_mf ? 1 : g.N) + so[q] * (c_mf ? g.M : 1);
-      c_strides(b, r, q, c_mf, o1);
+      c_strides(b, r, c_mf, o1);
       for (size_t l = 0; l < b.dims.size(); ++l) pbk.ostr[l] = o1[l];
     }
     ex.pc.push_back(pbk);
