@@ -166,6 +166,9 @@ def _cbe_split(ev, psi, H, u, v, dz):
 
 def step_(ev, psi, H, dz):
     """step! — tdvp.jl:171-195."""
+    if any(t.blocks is not None for t in psi.tensors):
+        raise NotImplementedError("TDVP on U(1) x U(1) block-sparse states is not built "
+                                  "(dmrg2_ is): the link splits use dense QR")
     if ev.cache is None or ev.cache.topo != psi.topo:
         ev.cache = EnvCache(psi.topo, tune_orders=True)
     if ev.order == 1:
