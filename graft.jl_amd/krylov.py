@@ -279,8 +279,9 @@ def _eigsolve_device(A, x0, howmany, krylovdim, tol, maxiter):
                     vecs = [_handoff(v, prev, s) for v in vecs]
                     nconv = howmany if invariant else int(np.sum(ok))
                     return list(theta[:howmany]), vecs, ConvergenceInfo(nconv, resid, it, L.numops)
-                # thick restart in place: V_i <- V U[:, i] (i < p), V_p <- V_k
-                p = max(howmany, min(k - 1, m // 2))
+                # thick restart in place: V_i <- V U[:, i] (i < p), V_p <- V_k;
+                # p follows KrylovKit's Lanczos eigsolve: (3 m + 2 converged) / 5
+                p = max(howmany, min(k - 1, (3 * m + 2 * int(np.sum(ok))) // 5))
                 coef = np.zeros((k + 1, p + 1), dtype=np.complex128)
                 coef[:k, :p] = U[:, :p]
                 coef[k, p] = 1.0
@@ -352,8 +353,10 @@ def eigsolve(A, x0, howmany=1, which="SR", ishermitian=True, krylovdim=20, tol=1
             nconv = int(np.sum(res <= tol * np.maximum(1.0, np.abs(theta[:howmany])))) if not invariant else howmany
             return (list(theta[:howmany]), vecs,
                     ConvergenceInfo(nconv, resid, it, numops))
-        # thick restart: keep p lowest Ritz vectors plus the residual direction
-        p = max(howmany, min(k - 1, m // 2))
+        # thick restart: keep p lowest Ritz vectors plus the residual direction,
+        # p = (3 m + 2 converged) / 5 as in KrylovKit's Lanczos eigsolve
+        nok = int(np.sum(res <= tol * np.maximum(1.0, np.abs(theta[:howmany]))))
+        p = max(howmany, min(k - 1, (3 * m + 2 * nok) // 5))
         Y = [T.lincomb(V[:k], U[:, j], V[0]) for j in range(p)]
         r = V[k]
         V = Y + [r]
