@@ -172,29 +172,60 @@ def _leg_codes(leg):
     return c
 
 
+_CHARGE_ARR = {}             # leg tuple -> (sorted codes, counts) as int64 arrays
+_EXACT_INT64 = 1 << 62
+
+
+def _leg_arrays(leg):
+    a = getattr(leg, "_code_arrays", None)
+    if a is None:
+        codes = _leg_codes(leg)
+        a = leg._code_arrays = (np.array([c for c, _ in codes], dtype=np.int64),
+                                np.array([d for _, d in codes], dtype=np.int64))
+    return a
+
+
 def charge_distribution(legs):
     """dict total-charge code -> number of elements over all sector tuples.
 
     Charges are integer codes (``_leg_codes``; the zero charge is 0, negation
     is integer negation).  Memoized on the leg tuple; a tuple extends the
     distribution of its prefix, so the planner's overlapping sub-networks
-    share the work.  The returned dict is shared: callers must not modify it."""
+    share the work.  Each extension is an outer sum of codes and outer
+    product of counts aggregated by code in numpy (exact int64; Python
+    integers when a count could pass 2^62).  The returned dict is shared:
+    callers must not modify it."""
     key = tuple(legs)
     hit = _CHARGE_DIST.get(key)
     if hit is not None:
         return hit
     if not key:
         return {}
-    base = charge_distribution(key[:-1]) if len(key) > 1 else {0: 1}
-    codes = _leg_codes(key[-1])
-    new = {}
-    get = new.get
-    for c, w in base.items():
-        for q, d in codes:
-            s = c + q
-            new[s] = get(s, 0) + w * d
     if len(_CHARGE_DIST) > 200000:
         _CHARGE_DIST.clear()
+        _CHARGE_ARR.clear()
+    if len(key) > 1:
+        base = charge_distribution(key[:-1])
+        bc, bw = _CHARGE_ARR.get(key[:-1], (None, None))
+    else:
+        base = {0: 1}
+        bc, bw = np.zeros(1, dtype=np.int64), np.ones(1, dtype=np.int64)
+    lc, lw = _leg_arrays(key[-1])
+    if bc is not None and int(bw.max()) * int(lw.max()) < _EXACT_INT64:
+        c = (bc[:, None] + lc[None, :]).ravel()
+        w = (bw[:, None] * lw[None, :]).ravel()
+        uc, inv = np.unique(c, return_inverse=True)
+        uw = np.zeros(len(uc), dtype=np.int64)
+        np.add.at(uw, inv, w)
+        _CHARGE_ARR[key] = (uc, uw)
+        new = dict(zip(uc.tolist(), uw.tolist()))
+    else:
+        new = {}
+        get = new.get
+        for c0, w0 in base.items():
+            for q, d in _leg_codes(key[-1]):
+                t = c0 + q
+                new[t] = get(t, 0) + w0 * d
     _CHARGE_DIST[key] = new
     return new
 
