@@ -412,7 +412,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--D", type=int, default=256)
-    ap.add_argument("--cpu-D", type=int, default=96)
+    ap.add_argument("--cpu-D", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the heuristic tile configurations (default: autotune every step)")
