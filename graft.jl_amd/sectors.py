@@ -162,10 +162,16 @@ def svd_split(theta, nrow, trunc_rank):
     largest ``trunc_rank(spectrum)`` kept.  Returns (U, S dict, Vh, spectrum,
     discarded norm, bond SectorLeg) with U's last leg and Vh's first leg the
     new bond (charges of the row part's total for U)."""
-    rowsets, colsets = {}, {}
+    from .planning import _leg_codes
+    codes = [[c for c, _ in _leg_codes(leg)] for leg in theta.legs[:nrow]]
+    rowsets, colsets, charge_of = {}, {}, {}
     for key in theta.blocks:
         r, c = key[:nrow], key[nrow:]
-        q = tuple(np.sum([theta.legs[l].sectors[s][0] for l, s in enumerate(r)], axis=0))
+        code = sum(codes[l][x] for l, x in enumerate(r))       # integer charge code of the row part
+        q = charge_of.get(code)
+        if q is None:
+            q = charge_of[code] = tuple(np.sum([theta.legs[l].sectors[x][0] for l, x in enumerate(r)],
+                                               axis=0))
         rowsets.setdefault(q, set()).add(r)
         colsets.setdefault(q, set()).add(c)
     pieces = []
