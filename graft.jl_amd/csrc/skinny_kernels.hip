@@ -13,15 +13,19 @@ namespace graft {
 
 static constexpr int kSkinnyThreads = 256;
 
+// Offset of flat index idx over legs (dims, strides), first leg fastest.
+// Callers keep idx below the product of dims, so the last leg needs no
+// division (a merged single-leg group needs none at all).
 __device__ __forceinline__ uint32_t sk_decode(uint32_t idx, int rank, const int32_t* dims,
                                               const uint32_t* strides) {
   uint32_t off = 0;
-  for (int l = 0; l < rank; ++l) {
+  for (int l = 0; l + 1 < rank; ++l) {
     const uint32_t d = (uint32_t)dims[l];
     const uint32_t q = idx / d;
     off += (idx - q * d) * strides[l];
     idx = q;
   }
+  if (rank > 0) off += idx * strides[rank - 1];
   return off;
 }
 
