@@ -19,7 +19,7 @@ from . import krylov
 from . import tensors as T
 from .contractions import EnvCache, eff_h0, eff_h1, eff_h2, two_site_tensor
 from .networks import move_center_, update_tensor_
-from .trees import childslot, nnodes, path_between, postorder
+from .trees import childslot, path_between, postorder
 
 
 def evolution_exponentiate_backend(effective, dz, x, **kw):

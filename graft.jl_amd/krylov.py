@@ -23,7 +23,6 @@ from __future__ import annotations
 
 import contextlib
 import ctypes
-import math
 from dataclasses import dataclass
 
 import numpy as np

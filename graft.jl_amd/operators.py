@@ -16,7 +16,6 @@ Leg order (ttno.jl:1-9): children channels..., P_out, P_in, parent channel.
 """
 from __future__ import annotations
 
-import itertools
 
 import numpy as np
 

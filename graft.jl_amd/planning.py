@@ -19,8 +19,7 @@ open, output ordered -1, -2, ...).
 from __future__ import annotations
 
 import math
-from dataclasses import dataclass, field
-from itertools import product as iproduct
+from dataclasses import dataclass
 
 import numpy as np
 

@@ -16,10 +16,8 @@ sum(children) + phys - parent = 0, so bond charges count the subtree below.
 """
 from __future__ import annotations
 
-import itertools
 
 import numpy as np
-import torch
 
 from .planning import SectorLeg
 from .tensors import BlockStructure, DTensor

@@ -2,7 +2,6 @@
 import itertools
 import math
 
-import numpy as np
 import pytest
 
 from graft_amd import planning as P

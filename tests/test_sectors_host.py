@@ -1,10 +1,8 @@
 """Host-side sector machinery (CPU): block QR/SVD, random symmetric states, charge sectors."""
 import numpy as np
-import pytest
 
 from oracle import heff_oracle as ref
 
-from graft_amd import operators as O
 from graft_amd import sectors as S
 from graft_amd import trees as Tr
 from graft_amd.planning import SectorLeg
