@@ -483,6 +483,8 @@ def main():
     ap.add_argument("--no-cbe", action="store_true", help="precision mode: TDVP1 without expansion")
     ap.add_argument("--prec-lockstep", action="store_true",
                     help="precision mode: advance every precision in lockstep (long runs)")
+    ap.add_argument("--prec-lowp", default="fp32,bf16x3",
+                    help="precision mode: reduced precisions run beside FP64 (comma list, may be empty)")
     ap.add_argument("--hub-D", type=int, default=64)
     ap.add_argument("--no-dense", action="store_true", help="hubbard mode: sector form only")
     ap.add_argument("--sweep-D", type=int, default=32)
@@ -519,10 +521,11 @@ def main():
                                                  f"D={r['D']}"}, "detail": r}), flush=True)
         return
     if args.mode == "precision":
+        lowp = tuple(p for p in args.prec_lowp.split(",") if p)
         if args.prec_lockstep:
-            r = precision_lockstep(args.prec_D, args.prec_steps, cbe=not args.no_cbe)
+            r = precision_lockstep(args.prec_D, args.prec_steps, precisions=lowp, cbe=not args.no_cbe)
         else:
-            r = precision_bench(args.prec_D, args.prec_steps, cbe=not args.no_cbe)
+            r = precision_bench(args.prec_D, args.prec_steps, precisions=lowp, cbe=not args.no_cbe)
         print(json.dumps({"metric": "TDVP1 reduced-precision products: drift, fidelity, Krylov error",
                           "value": r["results"].get("bf16x3", {}).get("infidelity_vs_fp64_max"),
                           "unit": "1 - |<psi_lowp|psi_fp64>|", "higher_is_better": False, "n_gpus": 1,
