@@ -11,7 +11,6 @@ from __future__ import annotations
 
 import ctypes
 
-import math
 
 import numpy as np
 import torch
@@ -39,18 +38,35 @@ class _StructKey:
 class BlockStructure:
     """Abelian block layout: flat legs (SectorLeg), block sector tuples, offsets."""
 
-    def __init__(self, legs, blocks):
+    def __init__(self, legs, blocks, _int_tuples=False):
         self.legs = list(legs)
-        blocks = [tuple(int(x) for x in b) for b in blocks]
+        if not _int_tuples:
+            blocks = [tuple(int(x) for x in b) for b in blocks]
         self.blocks = blocks
         self.index = {b: i for i, b in enumerate(blocks)}
-        degs = [[d for _, d in l.sectors] for l in self.legs]
-        self.block_dims = [tuple(degs[l][s] for l, s in enumerate(b)) for b in blocks]
-        sizes = [math.prod(d) for d in self.block_dims]
-        self.offsets = (np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64) if sizes
+        n = len(self.legs)
+        if blocks and n:
+            arr = np.asarray(blocks, dtype=np.int64).reshape(len(blocks), n)
+            dims = np.empty_like(arr)
+            for l, leg in enumerate(self.legs):
+                dims[:, l] = np.asarray([d for _, d in leg.sectors], dtype=np.int64)[arr[:, l]]
+            sizes = dims.prod(axis=1)
+        else:
+            dims = np.zeros((len(blocks), 0), dtype=np.int64)
+            sizes = np.ones(len(blocks), dtype=np.int64)
+        self._dims = dims
+        self._block_dims = None
+        self.offsets = (np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64) if len(sizes)
                         else np.zeros(0, np.int64))
-        self.sizes = sizes
-        self.total = int(sum(sizes))
+        self.sizes = sizes.tolist()
+        self.total = int(sizes.sum())
+
+    @property
+    def block_dims(self):
+        """Per block, its extent on every leg (tuples, built on first use)."""
+        if self._block_dims is None:
+            self._block_dims = [tuple(r) for r in self._dims.tolist()]
+        return self._block_dims
 
     @staticmethod
     def invariant(legs):
@@ -77,7 +93,7 @@ class BlockStructure:
                     if -(tot + q) in rl:
                         nxt.append((tup + (s,), tot + q))
             partial = nxt
-        return BlockStructure(legs, sorted(tup for tup, _ in partial))
+        return BlockStructure(legs, sorted(tup for tup, _ in partial), _int_tuples=True)
 
     @property
     def dims(self):
@@ -93,7 +109,7 @@ class BlockStructure:
         legs = [self.legs[p] for p in perm]
         blocks = [tuple(b[p] for p in perm) for b in self.blocks]
         order = sorted(range(len(blocks)), key=lambda i: blocks[i])
-        return BlockStructure(legs, [blocks[i] for i in order])
+        return BlockStructure(legs, [blocks[i] for i in order], _int_tuples=True)
 
     def desc(self):
         d = getattr(self, "_desc", None)
