@@ -557,9 +557,17 @@ def eff_h0(cache, psi, H, n, m, optimize=True, memory_weight=1.0, sector_aware=T
     if psi.center not in (n, m):
         raise ValueError("eff_h0: orthogonality center must lie on the active edge")
     En, Em = statics
-    proto = TensorProto((En.dims[0], Em.dims[0]), 1, None)
+    blocks = None
+    if En.blocks is not None:
+        # the link tensor's legs close each environment's ket leg (leg 0); the
+        # output legs are the environments' bra legs (leg 2), the same spaces
+        xl = [En.blocks.legs[0].dual(), Em.blocks.legs[0].dual()]
+        if En.blocks.legs[2] != xl[0] or Em.blocks.legs[2] != xl[1]:
+            raise ValueError("eff_h0: the environments' ket and bra legs are not one bond space")
+        blocks = BlockStructure.invariant(xl)
+    proto = TensorProto((En.dims[0], Em.dims[0]), 1, blocks)
     native = cache.get_plan("h0", spec, [proto] + statics, optimize, memory_weight, sector_aware,
-                            memory_cap_bytes)
+                            memory_cap_bytes, out_structure=blocks)
     return EffectiveMap(native, statics, proto, "h0", precision, spec, cache)
 
 

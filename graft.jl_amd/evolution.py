@@ -6,6 +6,9 @@ edge is backward-evolved with the zero-site (link) map exactly when the
 centre crosses it child -> parent (parent -> child on the reverse sweep).
 Krylov exponentials go through ``evolution_exponentiate_backend``
 (Evolution.jl:134-148), here the device Lanczos of ``krylov.exponentiate``.
+States may be dense or U(1) x U(1) block-sparse; for the latter the link
+splits are per-sector block QRs and the link map carries the bond's block
+structure.  Controlled bond expansion (TDVP1_CBE) is built for dense states.
 """
 from __future__ import annotations
 
@@ -16,6 +19,7 @@ import torch
 
 from . import backend as B
 from . import krylov
+from . import sectors as S
 from . import tensors as T
 from .contractions import EnvCache, eff_h0, eff_h1, eff_h2, two_site_tensor
 from .networks import move_center_, update_tensor_
@@ -166,9 +170,9 @@ def _cbe_split(ev, psi, H, u, v, dz):
 
 def step_(ev, psi, H, dz):
     """step! — tdvp.jl:171-195."""
-    if any(t.blocks is not None for t in psi.tensors):
-        raise NotImplementedError("TDVP on U(1) x U(1) block-sparse states is not built "
-                                  "(dmrg2_ is): the link splits use dense QR")
+    if (any(t.blocks is not None for t in psi.tensors) and isinstance(ev, TDVP1_CBE)
+            and ev.enabled):
+        raise NotImplementedError("controlled bond expansion is built for dense states only")
     if ev.cache is None or ev.cache.topo != psi.topo:
         ev.cache = EnvCache(psi.topo, tune_orders=True)
     if ev.order == 1:
@@ -220,6 +224,8 @@ def _tdvp1_sweep_(ev, psi, H, dz, rev, terminal_dz=None, skip_initial_site=False
 
 def _evolve_link_and_move_(ev, psi, H, u, v, dz, herm):
     """tdvp.jl:257-288 with the QR splits of tdvp.jl:291-301."""
+    if psi.tensors[u].blocks is not None:
+        return _evolve_sector_link_(ev, psi, H, u, v, dz, herm)
     t = psi.topo
     cache = ev.cache
     assert psi.center == u
@@ -249,6 +255,37 @@ def _evolve_link_and_move_(ev, psi, H, u, v, dz, herm):
         C, _ = evolution_exponentiate_backend(k0, -dz, C, ishermitian=herm, krylovdim=ev.krylovdim,
                                               tol=ev.tol, eager=ev.eager)
         psi.tensors[v] = B.mul_domain(psi.tensors[v], C)
+    psi.center = v
+    cache.invalidate_node_(v)
+    return psi
+
+
+def _evolve_sector_link_(ev, psi, H, u, v, dz, herm):
+    """_evolve_link_and_move_ for U(1) x U(1) block-sparse states (tdvp.jl:257-301):
+    the link splits are per-sector block QRs, the link tensor carries the block
+    structure of its bond spaces and the backward step is the sector zero-site map."""
+    t = psi.topo
+    cache = ev.cache
+    assert psi.center == u
+    kw = dict(ishermitian=herm, krylovdim=ev.krylovdim, tol=ev.tol, eager=ev.eager)
+    if t.parent[u] == v:
+        Q, R = S.link_split(psi.tensors[u])             # R :: V_new <- V_e
+        psi.tensors[u] = Q
+        cache.invalidate_node_(u)
+        k0 = eff_h0(cache, psi, H, u, v, precision=ev.precision)
+        C = S.to_structure(R, k0.blocks, 1)
+        C, _ = evolution_exponentiate_backend(k0, -dz, C, **kw)
+        psi.tensors[v] = S.absorb_link(psi.tensors[v], childslot(t, v, u), C, 1, 0)
+    else:
+        Q, R = S.link_split(psi.tensors[u], childslot(t, u, v))   # R :: Y <- dual(V_e)
+        psi.tensors[u] = Q
+        cache.invalidate_node_(u)
+        psi.center = u
+        k0 = eff_h0(cache, psi, H, v, u, precision=ev.precision)
+        C = S.to_structure(R.permute((1, 0), 1), k0.blocks, 1)    # :: V_e <- Y
+        C, _ = evolution_exponentiate_backend(k0, -dz, C, **kw)
+        P = psi.tensors[v]
+        psi.tensors[v] = S.absorb_link(P, P.rank - 1, C, 0, 1)
     psi.center = v
     cache.invalidate_node_(v)
     return psi

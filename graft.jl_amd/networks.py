@@ -83,7 +83,7 @@ def norm(psi):
 def normalize_(psi):
     A = psi.tensors[psi.center]
     nrm = float(np.sqrt(float((A.data.abs() ** 2).sum().item())))
-    psi.tensors[psi.center] = DTensor(A.dims, A.data / nrm, A.ncod)
+    psi.tensors[psi.center] = DTensor(A.dims, A.data / nrm, A.ncod, A.blocks)
     return psi
 
 

@@ -268,6 +268,45 @@ def move_center_edge_(psi, m, cache):
     return psi
 
 
+def link_split(A, k=None):
+    """Block QR of a device sector tensor that isolates leg k (the last leg when
+    k is None): A = Q . R per sector of that leg, Q isometric from the other
+    legs onto a new bond in slot k, R :: new <- old (flat legs: the new bond's
+    dual, leg k) on the host — the QR link splits of tdvp.jl:291-301 and
+    Backend.jl:597-609."""
+    T = HostBlocks.from_device(A)
+    N = len(T.legs)
+    if k is None or k == N - 1:
+        Q, R = qr_left(T)
+        return Q.to_device(), R
+    perm = [j for j in range(N) if j != k] + [k]
+    Q, R = qr_left(T.permute(perm))
+    back = [0] * N
+    for pos, j in enumerate(perm):
+        back[j] = pos
+    return Q.permute(back, T.ncod).to_device(), R
+
+
+def to_structure(T, struct, ncod):
+    """Upload host blocks into a given block structure whose legs are T's legs;
+    blocks of the structure that T lacks are stored as zeros."""
+    if list(T.legs) != list(struct.legs):
+        raise ValueError("to_structure: the tensor's legs differ from the structure's")
+    keys = set(struct.blocks)
+    for key, blk in T.blocks.items():
+        if key not in keys and np.any(blk):
+            raise ValueError(f"to_structure: block {key} lies outside the structure")
+    data = [T.blocks[b] if b in T.blocks else np.zeros(T.dims_of(b), dtype=np.complex128)
+            for b in struct.blocks]
+    return DTensor.from_blocks(struct, data, ncod)
+
+
+def absorb_link(P, k, C, ck, cnew):
+    """Leg k of the device sector tensor P contracted with leg ck of the 2-leg
+    device tensor C, whose leg cnew takes its place."""
+    return contract_leg(HostBlocks.from_device(P), k, HostBlocks.from_device(C), ck, cnew).to_device()
+
+
 def two_site_host(A, Bm, k):
     """Theta = A (parent leg) x Bm (child slot k) on the full two-site space.
 

@@ -83,6 +83,68 @@ def test_sector_dmrg2_matches_sector_exact_diagonalization(g):
     assert abs(Es[-1] - E0) < 1e-10, (Es, E0)
 
 
+def _full_vector(t, psi):
+    return ref.ttns_to_dense(t, _dense_state(psi), list(psi.hasphys))
+
+
+def test_sector_tdvp1_is_exact_at_full_bonds(g):
+    """TDVP1 on block-sparse states (tdvp.jl:171-301, link splits by per-sector
+    QR).  On star(3,1) every leaf bond holds the leaf's whole local space, so a
+    leaf step and the backward link step after it cancel and the root step is
+    the full propagator: TDVP1 of order 1 and 2 equals exp(dz H) on the
+    half-filling sector to the Krylov tolerance.  H is the dense matrix of the
+    same TTNO (ttno_to_dense), the states their dense embeddings."""
+    from graft_amd import evolution as EV, operators as O, sectors as S, trees as Tr
+    topo = Tr.star_topology(3, 1)
+    t = _tree(topo)
+    _say("sector TDVP1 star(3,1) hubbard, full bonds, vs exact propagation")
+    rng = np.random.default_rng(11)
+    psi, _ = S.random_hubbard_ttns(rng, topo, 4)
+    H = O.hubbard_sector_ttno(topo, 1.0, 4.0)
+    Hd = ref.ttno_to_dense(t, O.hubbard_host(topo, 1.0, 4.0))
+    v0 = _full_vector(t, psi)
+    vex = ref.exact_evolve(Hd, v0, -0.15j)
+    assert np.linalg.norm(vex - v0) > 1e-2             # the evolution moves the state
+    for order in (1, 2):
+        p = psi.copy()
+        ev = EV.TDVP1(order=order)
+        for _ in range(3):
+            EV.step_(ev, p, H, -0.05j)
+        assert all(a.blocks is not None for a in p.tensors)
+        err = np.linalg.norm(_full_vector(t, p) - vex)
+        assert err < 1e-9, (order, err)
+
+
+def test_sector_tdvp1_matches_dense_tdvp1_and_conserves(g):
+    """Truncated bonds (7-node binary tree, D=16 of 64 subtree states): every
+    per-sector link QR is full rank, so the dense TDVP1 on the embedded tensors
+    and the sector TDVP1 give one physical state; both conserve the norm and
+    the energy, each sub-step being the exponential of a hermitian map."""
+    from graft_amd import contractions as C, evolution as EV, networks as N, operators as O
+    from graft_amd import sectors as S, trees as Tr
+    from graft_amd.tensors import DTensor
+    topo = Tr.binary_topology(2)
+    t = _tree(topo)
+    _say("sector TDVP1 vs dense TDVP1, binary(2) hubbard, D=16")
+    rng = np.random.default_rng(12)
+    psi, _ = S.random_hubbard_ttns(rng, topo, 16, sigma=1.6)
+    assert max(psi.bond_dims()) == 16
+    Hs = O.hubbard_sector_ttno(topo, 1.0, 4.0)
+    Wd = O.hubbard_host(topo, 1.0, 4.0)
+    Hd = N.TTNO(topo, [DTensor.from_numpy(w, w.ndim - 2) for w in Wd], ishermitian=True)
+    psi_d = N.TTNS(topo, [DTensor.from_numpy(a, a.ndim - 1) for a in _dense_state(psi)], psi.center)
+    E0 = C.expect(psi, Hs, C.EnvCache(topo)).real
+    evs, evd = EV.TDVP1(order=2), EV.TDVP1(order=2)
+    for _ in range(3):
+        EV.step_(evs, psi, Hs, -0.05j)
+        EV.step_(evd, psi_d, Hd, -0.05j)
+    vs, vd = _full_vector(t, psi), _full_vector(t, psi_d)
+    assert np.linalg.norm(vs - vd) < 1e-9, np.linalg.norm(vs - vd)
+    assert abs(np.linalg.norm(vs) - 1) < 1e-10
+    E1 = C.expect(psi, Hs, C.EnvCache(topo)).real
+    assert abs(E1 - E0) < 1e-9 * abs(E0), (E0, E1)
+
+
 def test_sector_and_dense_forms_agree_at_scale(g):
     """16-site tree, Hubbard U=4 at half filling: the grouped sector apply and the
     plain dense FP64 apply on the embedded tensors agree, and so do their
