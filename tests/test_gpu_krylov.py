@@ -233,12 +233,14 @@ def test_library_matmul_matches_torch(g, shape):
                 assert got.shape == (m, n) and err < 1e-13, (ox, oy, view, err)
 
 
-@pytest.mark.parametrize("case", ["random", "cond1e12", "rank_deficient"])
+@pytest.mark.parametrize("case", ["random", "cond1e12", "rank_deficient", "empty_blocks"])
 def test_tall_qr(g, case):
     """Shifted Cholesky QR (three passes) on tall matrices: orthonormal Q and
     Q R = M to 1e-12 relative for a random matrix and one with singular values
-    down to 1e-12; a numerically rank-deficient matrix takes the Householder
-    fallback and satisfies the same bounds."""
+    down to 1e-12; a numerically rank-deficient matrix, and one whose rank
+    deficiency comes from exactly empty row blocks (the dense form of a tensor
+    with charge sectors of fewer rows than columns), take the Householder
+    fallback and satisfy the same bounds."""
     import torch
     from graft_amd import backend as B
     m, n = 40000, 96
@@ -254,6 +256,12 @@ def test_tall_qr(g, case):
         if case == "rank_deficient":
             sig[n // 2:] = 0.0
         M = (U * sig) @ V.conj().T
+    if case == "empty_blocks":
+        M = torch.zeros(m, n, dtype=torch.complex128, device="cuda")
+        rows, cols = m // 12, n // 12
+        for b in range(12):
+            r = cols // 2 if b % 3 == 0 else rows           # every third block: fewer rows than columns
+            M[b * rows:b * rows + r, b * cols:(b + 1) * cols] = rnd(r, cols)
     Q, R = B.tall_qr(M)
     eye = torch.eye(n, dtype=torch.complex128, device="cuda")
     assert float((Q.conj().T @ Q - eye).abs().max()) < 1e-12 * n
