@@ -16,6 +16,7 @@ sum(children) + phys - parent = 0, so bond charges count the subtree below.
 """
 from __future__ import annotations
 
+from collections import OrderedDict
 
 import numpy as np
 
@@ -244,24 +245,15 @@ def move_center_edge_(psi, m, cache):
     from .trees import childslot
     t = psi.topo
     n = psi.center
-    A = HostBlocks.from_device(psi.tensors[n])
     if t.parent[n] == m:
-        Q, R = qr_left(A)
-        k = childslot(t, m, n)
-        P = HostBlocks.from_device(psi.tensors[m])
-        psi.tensors[n] = Q.to_device()
-        psi.tensors[m] = contract_leg(P, k, R, 1, 0).to_device()
+        Q, R = link_split(psi.tensors[n])                   # R :: new <- old
+        psi.tensors[n] = Q
+        psi.tensors[m] = absorb_link(psi.tensors[m], childslot(t, m, n), R.to_device(), 1, 0)
     else:
-        k = childslot(t, n, m)
-        N = len(A.legs)
-        perm = [j for j in range(N) if j != k] + [k]
-        Q, R = qr_left(A.permute(perm))
-        back = [0] * N
-        for pos, j in enumerate(perm):
-            back[j] = pos
-        psi.tensors[n] = Q.permute(back, A.ncod).to_device()
-        C = HostBlocks.from_device(psi.tensors[m])
-        psi.tensors[m] = contract_leg(C, len(C.legs) - 1, R, 1, 0).to_device()
+        Q, R = link_split(psi.tensors[n], childslot(t, n, m))
+        psi.tensors[n] = Q
+        P = psi.tensors[m]
+        psi.tensors[m] = absorb_link(P, P.rank - 1, R.to_device(), 1, 0)
     psi.center = m
     if cache is not None:
         cache.invalidate_edge_(n, m)
@@ -301,10 +293,42 @@ def to_structure(T, struct, ncod):
     return DTensor.from_blocks(struct, data, ncod)
 
 
+_ABSORB_PLANS = OrderedDict()
+ABSORB_PLAN_CACHE = 256
+
+
 def absorb_link(P, k, C, ck, cnew):
     """Leg k of the device sector tensor P contracted with leg ck of the 2-leg
-    device tensor C, whose leg cnew takes its place."""
-    return contract_leg(HostBlocks.from_device(P), k, HostBlocks.from_device(C), ck, cnew).to_device()
+    device tensor C, whose leg cnew takes its place: one planned pair
+    contraction on the device (grouped block GEMMs), written into every
+    charge-conserving block of the new legs.  Plans are kept per operand
+    structure (least recently used first out)."""
+    from .contractions import NativePlan
+    from .planning import ContractionPlan, PairStep
+    if C.rank != 2 or {ck, cnew} != {0, 1}:
+        raise ValueError("absorb_link: C must be a 2-leg tensor and (ck, cnew) its two legs")
+    if C.blocks.legs[ck] != P.blocks.legs[k].dual():
+        raise ValueError("absorb_link: the contracted legs are not dual spaces")
+    key = (P.blocks.key(), C.blocks.key(), k, ck)
+    entry = _ABSORB_PLANS.get(key)
+    if entry is None:
+        N = P.rank
+        legs = list(P.blocks.legs)
+        legs[k] = C.blocks.legs[cnew]
+        out = BlockStructure.invariant(legs)
+        perm = tuple(i if i < k else (N - 1 if i == k else i - 1) for i in range(N))
+        step = PairStep(0, 1, 2, tuple(j for j in range(N) if j != k), (k,), (cnew,), (ck,),
+                        False, False, (perm, ()))
+        entry = (NativePlan(ContractionPlan(3, 2, [step]), [P, C], out), out)
+        _ABSORB_PLANS[key] = entry
+        if len(_ABSORB_PLANS) > ABSORB_PLAN_CACHE:
+            _ABSORB_PLANS.popitem(last=False)
+    else:
+        _ABSORB_PLANS.move_to_end(key)
+    plan, out = entry
+    res = DTensor.empty(out.dims, P.ncod, out)
+    plan.execute([P, C], res)
+    return res
 
 
 def two_site_host(A, Bm, k):
