@@ -266,17 +266,23 @@ def link_split(A, k=None):
     legs onto a new bond in slot k, R :: new <- old (flat legs: the new bond's
     dual, leg k) on the host — the QR link splits of tdvp.jl:291-301 and
     Backend.jl:597-609."""
-    T = HostBlocks.from_device(A)
+    Q, R = qr_isolating(HostBlocks.from_device(A), k)
+    return Q.to_device(), R
+
+
+def qr_isolating(T, k=None):
+    """Host block QR isolating leg k of T (the last leg when k is None): Q keeps
+    T's leg order with leg k replaced by the new bond (degeneracy per sector =
+    the rank bound min(rows, columns) of that sector's matrix), R :: new <- old."""
     N = len(T.legs)
     if k is None or k == N - 1:
-        Q, R = qr_left(T)
-        return Q.to_device(), R
+        return qr_left(T)
     perm = [j for j in range(N) if j != k] + [k]
     Q, R = qr_left(T.permute(perm))
     back = [0] * N
     for pos, j in enumerate(perm):
         back[j] = pos
-    return Q.permute(back, T.ncod).to_device(), R
+    return Q.permute(back, T.ncod), R
 
 
 def to_structure(T, struct, ncod):

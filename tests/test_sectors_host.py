@@ -25,6 +25,41 @@ def test_block_qr_reproduces_and_is_isometric():
     assert np.allclose(q.conj().T @ q, np.eye(q.shape[1]), atol=1e-12)
 
 
+def test_block_qr_isolating_a_middle_leg_matches_dense_qr():
+    """The QR of a link split that isolates a child leg (tdvp.jl:291-301,
+    Backend.jl:602-609) on block tensors: Q's column space over the other legs
+    is the dense QR's, Q . R rebuilds T, and a sector with fewer rows than
+    columns keeps a bond degeneracy equal to its row count."""
+    rng = np.random.default_rng(2)
+    L = lambda *s: SectorLeg([((q,), d) for q, d in s])
+    # flat charges: leg0 + leg1 + leg2 + leg3 = 0; the charge-2 sector of leg 1 (isolated)
+    # meets (leg0, leg3) = (0, -2) and (1, -3): 2 + 1 = 3 rows against 4 columns
+    legs = [L((0, 2), (1, 1)), L((0, 2), (1, 3), (2, 4)), L((0, 1)), L((-1, 3), (-2, 1), (-3, 1))]
+    T = S.random_blocks(rng, legs, 3)
+    k = 1
+    Q, R = S.qr_isolating(T, k)
+    assert [l.dim for l in Q.legs[:k] + Q.legs[k + 1:]] == [l.dim for l in T.legs[:k] + T.legs[k + 1:]]
+    back = S.contract_leg(Q, k, R, 0, 1)                  # R :: new <- old on leg k
+    assert np.abs(back.dense() - T.dense()).max() < 1e-12
+    # rank bound per sector: rows over the other legs with the complementary charge
+    rows = {}
+    for key, blk in T.blocks.items():
+        rows[key[k]] = rows.get(key[k], 0) + blk.size // blk.shape[k]
+    degs = {q: d for q, d in Q.legs[k].sectors}
+    for s_, (q, d) in enumerate(T.legs[k].sectors):
+        assert degs.get(q, 0) == min(d, rows.get(s_, 0)), (q, d, rows.get(s_))
+    assert Q.legs[k].dim < T.legs[k].dim                   # the deficient sector was reduced
+    # dense comparison: isolate leg k as columns
+    perm = [j for j in range(4) if j != k] + [k]
+    Td = np.transpose(T.dense(), perm).reshape(-1, T.legs[k].dim, order="F")
+    Qd = np.transpose(Q.dense(), perm).reshape(-1, Q.legs[k].dim, order="F")
+    assert np.allclose(Qd.conj().T @ Qd, np.eye(Qd.shape[1]), atol=1e-12)
+    U, sv, _ = np.linalg.svd(Td, full_matrices=False)
+    r = int(np.sum(sv > 1e-10 * sv[0]))
+    assert r == Qd.shape[1]
+    assert np.abs(Qd @ Qd.conj().T - U[:, :r] @ U[:, :r].conj().T).max() < 1e-10
+
+
 def test_block_svd_matches_dense_spectrum():
     rng = np.random.default_rng(1)
     topo = Tr.star_topology(2, 1)
