@@ -46,3 +46,15 @@ def test_binding_loads_and_reports_errors_without_gpu():
     assert L.graft_ctx_set_stream(None, None) == 1
     assert b"null" in L.graft_last_error()
     assert L.graft_num_tile_configs() >= 1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus N with N > 1 prints one JSON line saying
+    so and exits with status 2, before any device work."""
+    import json
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2, (r.returncode, r.stderr[-400:])
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert line["n_gpus"] == 2 and "one GPU" in line["error"]
