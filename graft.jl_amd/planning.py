@@ -252,10 +252,28 @@ def sector_pair_profile(legsA, oA, cA, legsB, oB):
     """Backend.pair_cost block loop (Backend.jl:328-368) in the flat-leg model.
 
     Returns (sector_flops = sum_q 2 m_q k_q n_q, output legs, output stored
-    elements, largest GEMM block, number of blocks)."""
-    mdist = charge_distribution([legsA[i] for i in oA]) if oA else None
-    kdist = charge_distribution([legsA[i] for i in cA]) if cA else None
-    ndist = charge_distribution([legsB[i] for i in oB]) if oB else None
+    elements, largest GEMM block, number of blocks).  Memoized on the three
+    leg tuples it depends on: the planner's search meets most of them again."""
+    mlegs = tuple(legsA[i] for i in oA)
+    klegs = tuple(legsA[i] for i in cA)
+    nlegs = tuple(legsB[i] for i in oB)
+    key = (mlegs, klegs, nlegs)
+    hit = _PAIR_PROFILE.get(key)
+    if hit is None:
+        if len(_PAIR_PROFILE) > 200000:
+            _PAIR_PROFILE.clear()
+        hit = _PAIR_PROFILE[key] = _pair_profile(mlegs, klegs, nlegs)
+    flops, out_legs, out_el, largest, nblocks = hit
+    return flops, list(out_legs), out_el, largest, nblocks
+
+
+_PAIR_PROFILE = {}
+
+
+def _pair_profile(mlegs, klegs, nlegs):
+    mdist = charge_distribution(mlegs) if mlegs else None
+    kdist = charge_distribution(klegs) if klegs else None
+    ndist = charge_distribution(nlegs) if nlegs else None
     mdist = mdist or {0: 1}
     kdist = kdist or {0: 1}
     ndist = ndist or {0: 1}
@@ -273,8 +291,7 @@ def sector_pair_profile(legsA, oA, cA, legsB, oB):
         if n:
             out_el += m * n
             largest = max(largest, float(m * n))
-    out_legs = [legsA[i] for i in oA] + [legsB[i] for i in oB]
-    return flops, out_legs, out_el, largest, nblocks
+    return flops, mlegs + nlegs, out_el, largest, nblocks
 
 
 # ---------------------------------------------------------------------------
