@@ -32,6 +32,11 @@ How each function is pinned (so that a plausible mistake fails a check):
   lib/GraftPlanning/test/runtests.jl:167-204).
 * ``exact_groundstate`` / ``exact_evolve`` — eigendecomposition identities.
 * ``execute_pair_steps`` — equals ``ncon`` on the same network.
+* ``tdvp1_step`` — with every bond at full rank it equals exact propagation
+  exp(dz H) (each site step and the link step after it act on one subspace
+  and cancel, the last site step is the whole propagator); with truncated
+  bonds it conserves the norm and the energy, and the order-2 step is
+  time-reversible (a step of -dz undoes a step of dz).
 
 Nothing in this module is "parity unpinned".
 """
@@ -908,3 +913,78 @@ def dense_energy(t, psi, terms):
     v = ttns_to_dense(t, psi, [True] * t.n)
     H = dense_hamiltonian(t.n, terms)
     return float((np.vdot(v, H @ v) / np.vdot(v, v)).real)
+
+
+# ---------------------------------------------------------------------------
+# TDVP1 — tdvp.jl:171-301: the tree projector-splitting sweep on dense
+# tensors.  Every local map is P^dagger H P with P the isometric embedding of
+# the site (link) tensor, exponentiated exactly (no Krylov iteration).
+# ---------------------------------------------------------------------------
+
+
+def _expm_apply(h, dz, x):
+    """exp(dz h) x for a hermitian matrix h, by eigendecomposition."""
+    w, U = np.linalg.eigh((h + h.conj().T) / 2)
+    return U @ (np.exp(dz * w) * (U.conj().T @ x))
+
+
+def _local_exp(P, H, dz, X):
+    x = X.reshape(-1, order="F")
+    return _expm_apply(P.conj().T @ H @ P, dz, x).reshape(X.shape, order="F")
+
+
+def _evolve_link_and_move(t, psi, H, u, v, dz, phys):
+    """_evolve_link_and_move! with the QR splits (tdvp.jl:257-301): split the
+    centre u towards v, evolve the link tensor with exp(-dz h0), absorb it into v."""
+    psi = list(psi)
+    if t.parent[u] == v:
+        Q, C = left_orth(psi[u])                       # C :: V_new <- V_e
+        psi[u] = Q
+        C = _local_exp(embedding_h0(t, psi, phys, u, v), H, -dz, C)
+        psi[v] = absorb_on_leg(psi[v], C, childslot(t, v, u))
+    else:
+        k = childslot(t, u, v)
+        Q, Cd = left_orth(np.moveaxis(psi[u], k, -1))  # Cd :: Y <- V_e
+        psi[u] = np.moveaxis(Q, -1, k)
+        C = _local_exp(embedding_h0(t, psi, phys, v, u), H, -dz, Cd.T)   # :: V_e <- Y
+        B = psi[v]
+        psi[v] = np.tensordot(B, C, axes=([B.ndim - 1], [0]))
+    return psi
+
+
+def _tdvp1_sweep(t, psi, center, H, dz, phys, rev, terminal_dz=None, skip_initial_site=False):
+    """_tdvp1_sweep! (tdvp.jl:197-251): post-order (reversed when rev) site
+    updates; a link crossed child -> parent (parent -> child when rev) is
+    evolved backward, any other crossing is a plain QR gauge move."""
+    order = postorder(t)
+    if rev:
+        order = order[::-1]
+    psi = move_center(t, psi, center, order[0])
+    for i, n in enumerate(order):
+        if not (skip_initial_site and i == 0):
+            site_dz = terminal_dz if (terminal_dz is not None and i == len(order) - 1) else dz
+            psi[n] = _local_exp(embedding_h1(t, psi, phys, n), H, site_dz, psi[n])
+        if i == len(order) - 1:
+            break
+        seg = path_between(t, n, order[i + 1])
+        for u, v in zip(seg[:-1], seg[1:]):
+            if (t.parent[u] == v) != rev:
+                psi = _evolve_link_and_move(t, psi, H, u, v, dz, phys)
+            else:
+                psi = move_center(t, psi, u, v)
+    return psi, order[-1]
+
+
+def tdvp1_step(t, psi, center, H, dz, order=2, phys=None):
+    """step! of TDVP1 (tdvp.jl:171-195) on dense tensors with the dense
+    Hamiltonian H: order 1 is one forward sweep; order 2 a forward half-step
+    sweep whose last site takes the full step, then a reverse half-step sweep
+    that skips its first site.  Returns (tensors, centre)."""
+    phys = [True] * t.n if phys is None else list(phys)
+    psi = list(psi)
+    if order == 1:
+        return _tdvp1_sweep(t, psi, center, H, dz, phys, rev=False)
+    if order == 2:
+        psi, center = _tdvp1_sweep(t, psi, center, H, dz / 2, phys, rev=False, terminal_dz=dz)
+        return _tdvp1_sweep(t, psi, center, H, dz / 2, phys, rev=True, skip_initial_site=True)
+    raise ValueError("order must be 1 or 2")

@@ -119,3 +119,29 @@ def test_cbe_directions_match_dense_projected_core(g):
     assert s[1] - s[2] > 1e-6 * s[0]
     assert np.all(ov > 1 - 1e-9)
     assert np.abs(Qn.conj().T @ Eh).max() < 1e-12
+
+
+def test_tdvp1_matches_oracle_at_truncated_bonds(g):
+    """7-site binary TFI tree at D=3 (truncated bonds, where TDVP1 is not the
+    exact propagator): the device TDVP1 of order 2 (Krylov exponentials to
+    1e-12, tdvp.jl:171-301) and the oracle's sweep with exact local
+    exponentials give the same state after 3 steps."""
+    from graft_amd import evolution as EV, networks as N, operators as O, trees as Tr
+    from graft_amd.tensors import DTensor
+    topo = Tr.binary_topology(2)
+    t = ref.Tree(list(topo.ids), list(topo.parent), [list(c) for c in topo.children])
+    assert t.parent[0] < 0
+    _say("TDVP1 vs oracle TDVP1, binary(2) TFI, D=3")
+    host = ref.random_ttns(np.random.default_rng(9), t, 2, 3)
+    psi = N.TTNS(topo, [DTensor.from_numpy(a, a.ndim - 1) for a in host], 0)
+    H = O.tfi_ttno(topo, 1.0, 0.9)
+    Hd = ref.dense_hamiltonian(t.n, ref.tfi_terms(t, 1.0, 0.9))
+    ev = EV.TDVP1(order=2)
+    p, c = list(host), 0
+    for _ in range(3):
+        EV.step_(ev, psi, H, -0.05j)
+        p, c = ref.tdvp1_step(t, p, c, Hd, -0.05j, 2)
+    want = ref.ttns_to_dense(t, p, [True] * t.n)
+    v0 = ref.ttns_to_dense(t, host, [True] * t.n)
+    assert np.linalg.norm(want - ref.exact_evolve(Hd, v0, -0.15j)) > 1e-3   # truncation matters
+    assert np.linalg.norm(_dense(t, psi) - want) < 1e-9

@@ -240,6 +240,46 @@ def test_exact_evolve_unitary_and_groundstate():
 # --------------------------------------------------------------------------
 
 
+def _tfi_tdvp_setup(D, seed=1):
+    t = ref.binary_tree(2)
+    Hd = ref.dense_hamiltonian(t.n, ref.tfi_terms(t, 1.0, 0.9))
+    psi = ref.random_ttns(np.random.default_rng(seed), t, 2, D)
+    return t, Hd, psi, ref.ttns_to_dense(t, psi, [True] * t.n)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_oracle_tdvp1_is_exact_at_full_bonds(order):
+    """7-site binary TFI tree at D=8 (every bond holds its subtree's whole
+    space): TDVP1 of either order equals exp(dz H) (test/runtests.jl:910-925).
+    A link step evolved forward, or an order-2 sweep without the full terminal
+    step, misses this by more than 0.07."""
+    t, Hd, psi, v0 = _tfi_tdvp_setup(8)
+    p, c = list(psi), 0
+    for _ in range(3):
+        p, c = ref.tdvp1_step(t, p, c, Hd, -0.05j, order)
+    v = ref.ttns_to_dense(t, p, [True] * t.n)
+    assert np.linalg.norm(v - ref.exact_evolve(Hd, v0, -0.15j)) < 1e-12
+    assert abs(np.linalg.norm(p[c]) - 1) < 1e-12          # the returned centre carries the norm
+
+
+def test_oracle_tdvp1_conserves_and_is_reversible_at_truncated_bonds():
+    """D=3 (truncated): the projected evolution leaves the exact one, yet every
+    sub-step is the exponential of a hermitian map, so the norm and the energy
+    are conserved, and the symmetric order-2 step is undone by a step of -dz."""
+    t, Hd, psi, v0 = _tfi_tdvp_setup(3)
+    E0 = np.vdot(v0, Hd @ v0).real
+    p, c = list(psi), 0
+    for _ in range(3):
+        p, c = ref.tdvp1_step(t, p, c, Hd, -0.05j, 2)
+    v = ref.ttns_to_dense(t, p, [True] * t.n)
+    assert np.linalg.norm(v - ref.exact_evolve(Hd, v0, -0.15j)) > 1e-3
+    assert abs(np.linalg.norm(v) - 1) < 1e-12
+    assert abs(np.vdot(v, Hd @ v).real - E0) < 1e-12 * abs(E0)
+    p, c = ref.tdvp1_step(t, list(psi), 0, Hd, -0.05j, 2)
+    p, c = ref.tdvp1_step(t, p, c, Hd, 0.05j, 2)
+    assert np.linalg.norm(ref.ttns_to_dense(t, p, [True] * t.n) - v0) < 1e-12
+
+
 def test_fermion_operators_anticommute():
     t = ref.star_tree(2, 1)
     _, c = ref.hubbard_dense(t)
