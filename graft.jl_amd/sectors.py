@@ -6,8 +6,10 @@ flat charges sum to zero, packed in block order).  The effective maps and
 environments run as grouped block GEMMs in libgraft_heff.so.  This module
 holds the gauge factorizations of such tensors (block QR, truncated block
 SVD) and their construction; the reference performs these in
-MatrixAlgebraKit through TensorKit (Backend.jl:466-609).  They run once per
-site move on small per-sector matrices and are outside the H_eff hot path.
+MatrixAlgebraKit through TensorKit (Backend.jl:466-609).  The factorizations
+run on the host, once per site move, on small per-sector matrices, outside
+the H_eff hot path; absorbing a link factor into a neighbour is one planned
+pair contraction on the device.
 
 Charge conventions (TensorKit, flat-leg picture): a codomain leg carries the
 charges of its space V, a domain leg those of dual(V) (negated).  A TTNS
