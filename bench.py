@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 MI355X_FP64_MFMA_TFLOPS = 78.6
+THREE_M_CONFIGS = (7, 8, 9)     # tile configurations of the three-product complex form
 MI355X_HBM_GBPS = 8000.0
 # HBM bytes per launch of the dominant step, from the two counter runs
 # (FETCH_SIZE and WRITE_SIZE, scaled by a 1 GiB copy) summarized in this file.
@@ -685,6 +686,11 @@ def main():
                    "mfma_fraction_whole_apply": apply_tflops / MI355X_FP64_MFMA_TFLOPS},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": MI355X_FP64_MFMA_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / MI355X_FP64_MFMA_TFLOPS,
+                     # achieved/frac count the algorithmic 8 FLOP per complex MAC; the 3M
+                     # tile configurations issue 6 real MFMA FLOP per complex MAC
+                     "complex_form": "3M" if info[dom]["config"] in THREE_M_CONFIGS else "4M",
+                     "executed_frac": (achieved / MI355X_FP64_MFMA_TFLOPS
+                                       * (0.75 if info[dom]["config"] in THREE_M_CONFIGS else 1.0)),
                      "traffic": profiled_traffic(dom, args.D),
                      "algorithmic_bytes": 16.0 * (info[dom]["m"] * info[dom]["k"]
                                                   + info[dom]["k"] * info[dom]["n"]

@@ -86,6 +86,7 @@ def lib():
                               ctypes.POINTER(vp)],
         "graft_plan_destroy": [vp],
         "graft_plan_info": [vp, ctypes.POINTER(u64), dp, ctypes.POINTER(u64)],
+        "graft_plan_device_bytes": [vp, ctypes.POINTER(u64)],
         "graft_plan_step_info": [vp, i32, ctypes.POINTER(i64), ctypes.POINTER(i64), dp,
                                  ctypes.POINTER(i32)],
         "graft_plan_set_tile_config": [vp, i32, i32],

@@ -228,15 +228,24 @@ NO_TRUNCATION = TruncationScheme()
 
 
 def truncation_rank(s, ts):
-    """Number of singular values kept (sorted descending) under truncrank & trunctol & truncerror."""
+    """Number of singular values kept under truncrank & trunctol & truncerror
+    (Backend.jl:476-483) for a spectrum sorted in descending order.
+
+    The rules are MatrixAlgebraKit's documented ones (the package itself,
+    0.6.9, is not in the reference tree): truncrank(maxdim) keeps the maxdim
+    largest; trunctol(atol, rtol) keeps every value >= max(atol, rtol * ||s||_2);
+    truncerror(rtol=discarded_weight) keeps the fewest largest values whose
+    discarded 2-norm is <= discarded_weight * ||s||_2; ``&`` keeps what every
+    rule keeps.  At least one value is kept, so a bond never becomes empty."""
+    s = np.asarray(s, dtype=np.float64)
     n = len(s)
     keep = min(n, ts.maxdim)
+    tot = math.sqrt(float(np.sum(s ** 2))) if n else 0.0
     if ts.atol > 0 or ts.rtol > 0:
-        thr = max(ts.atol, ts.rtol * (s[0] if n else 0.0))
-        keep = min(keep, int(np.sum(s > thr)))
+        thr = max(ts.atol, ts.rtol * tot)
+        keep = min(keep, int(np.sum(s >= thr)))
     if ts.discarded_weight > 0 and n:
-        tot = math.sqrt(float(np.sum(s ** 2)))
-        tail = np.sqrt(np.cumsum((s ** 2)[::-1]))[::-1]    # tail[i] = norm of s[i:]
+        tail = np.sqrt(np.cumsum((s ** 2)[::-1]))[::-1]    # tail[i] = 2-norm of s[i:]
         k = n
         for i in range(n):
             if tail[i] <= ts.discarded_weight * tot:
@@ -247,45 +256,123 @@ def truncation_rank(s, ts):
 
 
 GRAM_SVD_MIN = 256   # smaller matrix side from which split_svd takes the Gram route
+GRAM_ACCEPT = 1e-6   # a Gram level keeps the eigenvalues >= this fraction of its largest
+TAIL_FLOOR = 1e-7    # deflate until what is left lies below this fraction of sigma_1
+GRAM_MAX_LEVELS = 8
+
+
+def gram_spectrum(M, wide):
+    """Full singular spectrum of M (descending) and an orthonormal basis of
+    its smaller side ordered like it, by deflated Gram eigendecompositions.
+
+    Level 0 diagonalizes G = M M^H (``wide``) or M^H M.  An eigenvalue of a
+    Gram matrix carries an absolute error ~ eps * (its largest eigenvalue), so
+    a level keeps only the eigenvalues within GRAM_ACCEPT of its largest
+    (singular values within 1e-3 of its top: relative error <= ~1e-10); the
+    rest of the basis R is deflated: the next level diagonalizes the Gram
+    matrix of R^H M (or M R), formed from M itself, whose top is the first
+    value not yet kept.  Levels continue while the remainder's top exceeds
+    TAIL_FLOOR * sigma_1.  Every value sigma >= 1e-5 sigma_1 is then accurate
+    to 1e-10 relative (the product R^H M rounds at eps * sigma_1, as a
+    backward-stable SVD does), and smaller ones to ~1e-14 sigma_1 absolute."""
+    G = matmul(M, M, "N", "H") if wide else matmul(M, M, "H", "N")
+    side = G.shape[0]
+    vals, vecs = [], []
+    R = None                              # remaining basis (None = identity)
+    top0 = None
+    for level in range(GRAM_MAX_LEVELS):
+        G = (G + G.conj().T) / 2
+        # unit scale: the device eigensolver's eigenvalue error does not follow
+        # a matrix whose norm is far below one (a deflated level's is 1e-6 and less)
+        scale = float(torch.diagonal(G).real.max())
+        scale = scale if scale > 0 else 1.0
+        lam, W = torch.linalg.eigh(G / scale)
+        lam = torch.flip(lam, (0,)).clamp(min=0.0) * scale
+        W = torch.flip(W, (1,))
+        Rw = W if R is None else matmul(R, W)
+        lam_h = lam.detach().cpu().numpy()
+        if top0 is None:
+            top0 = float(lam_h[0]) if len(lam_h) else 0.0
+        acc = max(1, int(np.sum(lam_h >= GRAM_ACCEPT * lam_h[0])))
+        last = (acc == len(lam_h) or level == GRAM_MAX_LEVELS - 1
+                or lam_h[acc] < TAIL_FLOOR ** 2 * top0)
+        if last:
+            vals.append(lam_h)
+            vecs.append(Rw)
+            break
+        vals.append(lam_h[:acc])
+        vecs.append(Rw[:, :acc])
+        # The deflated basis must be orthonormal and orthogonal to every kept
+        # vector to working precision: a tail vector with a component theta
+        # along a kept one of value s_j lifts its own squared value by about
+        # (theta s_j)^2.  Two projections off the kept vectors, then a
+        # Householder QR.
+        R = Rw[:, acc:].contiguous()
+        K = torch.cat(vecs, dim=1) if len(vecs) > 1 else vecs[0]
+        for _ in range(2):
+            R = R - matmul(K, matmul(K, R, "H"))
+        R, _ = torch.linalg.qr(R, mode="reduced")
+        B = matmul(R, M, "H", "N") if wide else matmul(M, R)
+        G = matmul(B, B, "N", "H") if wide else matmul(B, B, "H", "N")
+    lam_all = np.concatenate(vals)
+    Q = torch.cat(vecs, dim=1) if len(vecs) > 1 else vecs[0]
+    order = np.argsort(-lam_all, kind="stable")
+    if np.any(order != np.arange(len(order))):
+        lam_all = lam_all[order]
+        Q = Q[:, torch.as_tensor(order, device=Q.device)]
+    assert Q.shape == (side, side)
+    return np.sqrt(lam_all), Q
+
+
+def svd_spectrum(M, method=None):
+    """Full singular spectrum of a torch matrix (numpy, descending) and the
+    state ``svd_kept`` needs to produce the kept factors: the library SVD
+    ("direct"), or from GRAM_SVD_MIN on the deflated Gram spectrum and basis."""
+    rows, cols = M.shape
+    if method is None:
+        method = "gram" if min(rows, cols) >= GRAM_SVD_MIN else "direct"
+    if method == "direct":
+        U, S, Vh = torch.linalg.svd(M, full_matrices=False)
+        return S.detach().cpu().numpy(), ("direct", U, S, Vh)
+    wide = rows <= cols
+    s, Q = gram_spectrum(M, wide)
+    return s, ("gram", wide, Q)
+
+
+def svd_kept(M, state, r):
+    """The r leading singular triplets (U, S, Vh) of M from ``svd_spectrum``'s
+    state: slices of the library SVD, or an exact SVD of the projection of M
+    onto the top-r Gram basis (the library SVD when r exceeds half the
+    smaller side, where the projection saves nothing)."""
+    if state[0] == "direct":
+        _, U, S, Vh = state
+        return U[:, :r], S[:r], Vh[:r, :]
+    _, wide, Q = state
+    if 2 * r > min(M.shape):
+        U, S, Vh = torch.linalg.svd(M, full_matrices=False)
+        return U[:, :r], S[:r], Vh[:r, :]
+    Qr = Q[:, :r]
+    if wide:
+        Ub, S, Vh = torch.linalg.svd(matmul(Qr, M, "H"), full_matrices=False)
+        return matmul(Qr, Ub), S, Vh
+    U, S, Vb = torch.linalg.svd(matmul(M, Qr), full_matrices=False)
+    return U, S, matmul(Vb, Qr, "N", "H")
 
 
 def split_svd(M, ts=NO_TRUNCATION, method=None):
     """Truncated SVD of a torch matrix -> (U, S, Vh, discarded norm, full spectrum).
 
     ``method`` "direct": the library SVD.  "gram" (the default from
-    GRAM_SVD_MIN on): the full spectrum from the Hermitian eigendecomposition
-    of the smaller Gram matrix G = M M^H (or M^H M), then the kept part from an
-    exact SVD of the projection Q_r^H M onto the top-r eigenvectors.  The kept
-    singular values and vectors are those of M to working precision when the
-    spectrum has a gap at the cut; the discarded tail (sqrt of Gram eigenvalues)
-    carries absolute error ~ eps * sigma_1, so a tail value sigma_i has relative
-    error ~ eps (sigma_1 / sigma_i)^2.  On the GPU the library SVD of a
-    2048 x 2048 complex matrix costs 18x the Gram route (MI355X, ROCm 7)."""
-    rows, cols = M.shape
-    if method is None:
-        method = "gram" if min(rows, cols) >= GRAM_SVD_MIN else "direct"
-    if method == "direct":
-        U, S, Vh = torch.linalg.svd(M, full_matrices=False)
-        s = S.detach().cpu().numpy()
-        r = truncation_rank(s, ts)
-        disc = float(np.sqrt(np.sum(s[r:] ** 2)))
-        return U[:, :r], S[:r], Vh[:r, :], disc, s
-    wide = rows <= cols
-    G = matmul(M, M, "N", "H") if wide else matmul(M, M, "H", "N")
-    lam, Q = torch.linalg.eigh(G)
-    lam = torch.flip(lam, (0,)).clamp(min=0.0)
-    Q = torch.flip(Q, (1,))
-    s = np.sqrt(lam.detach().cpu().numpy())
+    GRAM_SVD_MIN on): the full spectrum from ``gram_spectrum`` (deflated
+    Hermitian eigendecompositions on the smaller side: every value down to
+    1e-5 sigma_1 to 1e-10 relative), then the kept part from an exact SVD of
+    the projection Q_r^H M onto the top-r basis vectors.  The kept singular
+    values and vectors are those of M to working precision when the spectrum
+    has a gap at the cut.  On the GPU the library SVD of a 2048 x 2048
+    complex matrix costs 18x a Gram eigendecomposition (MI355X, ROCm 7)."""
+    s, state = svd_spectrum(M, method)
     r = truncation_rank(s, ts)
-    if 2 * r > min(rows, cols):          # little truncation: the projection saves nothing
-        return split_svd(M, ts, "direct")
-    Qr = Q[:, :r]
-    if wide:
-        Ub, S, Vh = torch.linalg.svd(matmul(Qr, M, "H"), full_matrices=False)
-        U = matmul(Qr, Ub)
-    else:
-        U, S, Vb = torch.linalg.svd(matmul(M, Qr), full_matrices=False)
-        Vh = matmul(Vb, Qr, "N", "H")
+    U, S, Vh = svd_kept(M, state, r)
     s = s.copy()
     s[:r] = S.detach().cpu().numpy()
     disc = float(np.sqrt(np.sum(s[r:] ** 2)))

@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+from collections import OrderedDict
 
 import numpy as np
 import torch
@@ -49,6 +50,9 @@ class NativePlan:
         outn = ctypes.c_uint64()
         nat.check(nat.lib().graft_plan_info(h, ctypes.byref(ws), ctypes.byref(macs), ctypes.byref(outn)))
         self.workspace_bytes = ws.value
+        db = ctypes.c_uint64()
+        nat.check(nat.lib().graft_plan_device_bytes(h, ctypes.byref(db)))
+        self.device_bytes = db.value    # block tables and fused-step maps held by the plan
         self.macs = macs.value          # complex multiply-adds over the executed blocks
         self.output_elements = outn.value
         self.out_dims = out_dims
@@ -158,7 +162,8 @@ class EnvCache:
         self.tune_orders = tune_orders   # measured pairwise order for heavy dense maps
         self.envs = {}
         self.node_generations = [0] * nnodes(topo)
-        self.plans = {}
+        self.plans = OrderedDict()       # least recently used first
+        self.plan_bytes = 0              # device bytes held by the cached plans
         self.rootcaps = {}
         self._sides = {}
         self.env_builds = 0
@@ -197,6 +202,7 @@ class EnvCache:
         p = self.plans.get(key)
         if p is not None:
             self.plan_hits += 1
+            self.plans.move_to_end(key)
             return p
         self.plan_misses += 1
         dims = [list(op.dims) for op in operands]
@@ -216,7 +222,20 @@ class EnvCache:
         if out_structure is None and legs is not None:
             p.out_structure = _native_output_structure(p, plan, legs, spec.conjs)
         self.plans[key] = p
+        self.plan_bytes += p.device_bytes
+        self._evict_plans()
         return p
+
+    def _evict_plans(self):
+        """Bound the cached plans by count and device bytes, least recently used
+        out first.  Every truncated split or bond expansion gives the sector
+        maps new block structures, so a long sweep would otherwise keep every
+        superseded plan (and its per-element maps) alive.  A plan still held by
+        a live map is freed when that map goes."""
+        while len(self.plans) > 1 and (len(self.plans) > PLAN_CACHE_MAX
+                                       or self.plan_bytes > PLAN_CACHE_BYTES):
+            _, old = self.plans.popitem(last=False)
+            self.plan_bytes -= old.device_bytes
 
     def root_cap(self, dims, legs=None):
         """_root_cap! — envcache.jl:626-655 (ones tensor closing unit legs).
@@ -240,6 +259,9 @@ class EnvCache:
             self.rootcaps[key] = c
         return c
 
+
+PLAN_CACHE_MAX = 512            # plans kept per EnvCache ...
+PLAN_CACHE_BYTES = 4 << 30      # ... and the device bytes they may hold
 
 _TUNED_KINDS = ("h1", "h2", "h0")
 _TUNED_ORDERS = {}           # (spec, log2 leg dims) -> measured fastest tree

@@ -256,6 +256,7 @@ struct graft_plan_impl {
   double macs = 0.0;
   void* d_meta = nullptr;  // one allocation holding every step's items/segs/koff
   void* d_maps = nullptr;  // element -> fused-offset maps of every fused step
+  uint64_t meta_bytes = 0, map_bytes = 0;
   int device = 0;
 };
 
@@ -1258,6 +1259,7 @@ static int upload_step_meta(graft_plan_impl& p) {
     p.d_meta = nullptr;
   }
   GRAFT_CHECK_HIP(hipMalloc(&p.d_meta, total));
+  p.meta_bytes = total;
   std::vector<char> host(total, 0);
   size_t off = 0;
   for (auto& ex : p.steps) {
@@ -1318,6 +1320,7 @@ static int build_maps(graft_ctx* ctx, graft_plan_impl& p) {
     }
   if (total == 0) return GRAFT_OK;
   GRAFT_CHECK_HIP(hipMalloc(&p.d_maps, total));
+  p.map_bytes = total;
   uint32_t* tmp = nullptr;
   GRAFT_CHECK_HIP(hipMalloc(&tmp, al(tmp_n)));
   char* base = (char*)p.d_maps;
@@ -1724,6 +1727,12 @@ int graft_plan_info(graft_plan* plan, uint64_t* workspace_bytes, double* macs,
   if (workspace_bytes) *workspace_bytes = plan->impl.ws_bytes;
   if (macs) *macs = plan->impl.macs;
   if (output_elements) *output_elements = (uint64_t)plan->impl.slots[plan->impl.output_slot].total;
+  return GRAFT_OK;
+}
+
+int graft_plan_device_bytes(graft_plan* plan, uint64_t* bytes) {
+  GRAFT_REQUIRE(plan != nullptr && bytes != nullptr, "null plan or output");
+  *bytes = plan->impl.meta_bytes + plan->impl.map_bytes;
   return GRAFT_OK;
 }
 

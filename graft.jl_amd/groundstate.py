@@ -2,7 +2,10 @@
 
 ``dmrg2_`` is dmrg2! (GroundState.jl:102-143): post-order bond sweep and its
 reverse, a Lanczos ground state of the two-site effective Hamiltonian on every
-bond, and a truncated split.  The local solve goes through
+bond, and a truncated split.  Before each sweep a binary root without a
+physical leg has both child bonds completed by zero-weight directions
+(GroundState.jl:119-120), else each root-edge split stays rank-limited by the
+other root bond.  The local solve goes through
 ``groundstate_eigsolve_backend`` (GroundState.jl:25-26), here the device
 Lanczos of ``krylov.eigsolve``.
 """
@@ -10,11 +13,14 @@ from __future__ import annotations
 
 import time
 
+import numpy as np
+import torch
+
 from . import backend as B
 from . import krylov
 from .contractions import EnvCache, eff_h2, split_two_site_, two_site_tensor
 from .networks import move_center_
-from .trees import postorder
+from .trees import childslot, postorder
 
 
 def groundstate_eigsolve_backend(effective, x, howmany, which, precision="fp64", **kw):
@@ -46,6 +52,70 @@ class _Phases:
         self.t = now
 
 
+def _physless_root_two_site_targets(psi, trunc):
+    """_physless_root_two_site_targets (expansion.jl:542-549): at a binary root
+    without a physical leg, both root-child bonds are opened to the ordinary
+    two-site cap, bounded by each child's codomain dimension."""
+    t = psi.topo
+    r = t.root
+    kids = t.children[r]
+    if psi.hasphys[r] or len(kids) != 2:
+        return []
+    target = min(trunc.maxdim, min(int(np.prod(psi.tensors[n].dims[:-1])) for n in kids))
+    return [(n, target) for n in kids]
+
+
+def _null_enrich_split(A, maxdim):
+    """_null_enrich_split (expansion.jl:560-575): A == U . R with U isometric and
+    up to ``maxdim`` columns: the column space of A completed by orthonormal
+    directions of its complement (zero weight, so the state is unchanged).
+    The complement directions come from a fixed-seed random probe projected
+    off range(A) (any orthonormal complement basis serves: the following
+    variational updates choose which directions acquire weight).  Returns
+    (U as a site tensor, R :: new <- old) or (A, None) when there is no room."""
+    if A.blocks is not None:
+        raise NotImplementedError("the physless-root bootstrap is built for dense states")
+    M = B.matrix_view(A, A.rank - 1)
+    rows, old = M.shape
+    room = min(maxdim, rows) - old
+    if room <= 0:
+        return A, None
+    Q, _ = B.thin_qr(M)
+    g = torch.Generator(device=M.device)
+    g.manual_seed(0x5EED)
+    G = torch.randn((rows, room), dtype=M.dtype, device=M.device, generator=g)
+    for _ in range(2):                        # twice is enough (Kahan, Parlett)
+        G = G - B.matmul(Q, B.matmul(Q, G, "H"))
+    E, _ = torch.linalg.qr(G, mode="reduced")
+    U = torch.cat([Q, E], dim=1)
+    R = B.matmul(U, M, "H")
+    Ut = B.from_matrix(U, A.dims[:-1] + (U.shape[1],), A.ncod)
+    Rt = B.from_matrix(R, (R.shape[0], R.shape[1]), 1)
+    return Ut, Rt
+
+
+def _bootstrap_physless_root_(psi, cache, targets):
+    """_bootstrap_physless_root! (expansion.jl:584-602): complete both root-child
+    subspaces before the sweep, so that neither root-edge split is rank-limited
+    by the other root bond."""
+    t = psi.topo
+    r = t.root
+    for n, target in targets:
+        if t.parent[n] != r:
+            raise ValueError("root bootstrap target is not a root child")
+        if psi.tensors[n].dims[-1] >= target:
+            continue
+        move_center_(psi, n, cache)
+        U, R = _null_enrich_split(psi.tensors[n], target)
+        if R is None:
+            continue
+        psi.tensors[n] = U
+        psi.tensors[r] = B.absorb_on_leg(psi.tensors[r], R, childslot(t, r, n))
+        psi.center = r
+        cache.invalidate_edge_(n, r)
+    return psi
+
+
 def dmrg2_(psi, H, trunc=B.NO_TRUNCATION, nsweeps=10, tol=1e-10, krylovdim=20, eig_tol=1e-12,
            precision="fp64", verbose=False, callback=None, timings=None):
     """Two-site DMRG; returns (psi, energies) with one energy per sweep.
@@ -64,6 +134,7 @@ def dmrg2_(psi, H, trunc=B.NO_TRUNCATION, nsweeps=10, tol=1e-10, krylovdim=20, e
     schedule = [(n, "m") for n in bonds] + [(n, "n") for n in reversed(bonds)]
     for sweep in range(1, nsweeps + 1):
         t0 = time.perf_counter()
+        _bootstrap_physless_root_(psi, cache, _physless_root_two_site_targets(psi, trunc))
         E = float("nan")
         ph = _Phases(timings) if timings is not None else None
         for n, center_on in schedule:

@@ -21,7 +21,7 @@ import numpy as np
 
 from .networks import TTNO
 from .tensors import DTensor
-from .trees import edges, nchildren, nnodes
+from .trees import edges, nchildren, nnodes, postorder
 
 SX = np.array([[0, 1], [1, 0]], dtype=complex)
 SZ = np.array([[1, 0], [0, -1]], dtype=complex)
@@ -71,9 +71,99 @@ def nn_ttno_host(topo, d, onsite, pair_ops, pair_coef):
     return W
 
 
+def terms_ttno_host(topo, phys, terms):
+    """Host arrays of the automaton TTNO for a sum of one- and two-site terms on
+    any sites of the tree; nodes with ``phys[n] == 0`` carry no physical leg
+    (W :: children... <- parent, as ttno.jl:1-9 allows).
+
+    ``terms``: list of (coef, [(site, op), ...]) with one or two distinct sites.
+    The virtual leg on edge (c -> parent) carries: 0 identity, 1 done, and one
+    channel per two-site term with exactly one site below c (its operator is
+    placed, the partner is still to come).  A term completes, with its
+    coefficient, at the lowest node whose subtree holds all its sites."""
+    n = nnodes(topo)
+    below = [set() for _ in range(n)]
+    for v in postorder(topo):
+        below[v].add(v)
+        for c in topo.children[v]:
+            below[v] |= below[c]
+    for coef, ops in terms:
+        sites = [s for s, _ in ops]
+        if not 1 <= len(sites) <= 2 or len(set(sites)) != len(sites):
+            raise ValueError("terms_ttno_host: every term acts on one or two distinct sites")
+        if any(not phys[s] for s in sites):
+            raise ValueError("terms_ttno_host: a term acts on a node without a physical leg")
+    chan = {}
+    for c in range(n):
+        if topo.parent[c] < 0:
+            continue
+        ch = {"id": 0, "done": 1}
+        for i, (_, ops) in enumerate(terms):
+            if len(ops) == 2 and sum(s in below[c] for s, _ in ops) == 1:
+                ch[i] = len(ch)
+        chan[c] = ch
+    W = []
+    for v in range(n):
+        kids = topo.children[v]
+        K = len(kids)
+        d = phys[v]
+        root = topo.parent[v] < 0
+        up_dim = 1 if root else len(chan[v])
+        T = np.zeros([len(chan[c]) for c in kids] + ([d, d] if d else []) + [up_dim], dtype=complex)
+        I = np.eye(d, dtype=complex) if d else None
+
+        def put(children, up, op, coef=1.0):
+            if root:
+                if up != "done":
+                    return
+                u = 0
+            else:
+                u = chan[v][up]
+            idx = tuple(children) + ((slice(None), slice(None)) if d else ()) + (u,)
+            T[idx] += coef * (op if d else 1.0)
+
+        idle = [0] * K
+        put(idle, "id", I)                                  # identity passes up
+        for k in range(K):
+            ch = list(idle)
+            ch[k] = 1
+            put(ch, "done", I)                              # a completed subtree passes up
+        for i, (coef, ops) in enumerate(terms):
+            inside = [(s, o) for s, o in ops if s in below[v]]
+            if len(inside) < len(ops):
+                if len(inside) == 1 and not root:           # carry the open term upwards
+                    s, o = inside[0]
+                    if s == v:
+                        put(idle, i, o)
+                    else:
+                        k = next(k for k, c in enumerate(kids) if s in below[c])
+                        ch = list(idle)
+                        ch[k] = chan[kids[k]][i]
+                        put(ch, i, I)
+                continue
+            if any(all(s in below[c] for s, _ in ops) for c in kids):
+                continue                                    # completes further down
+            ch = list(idle)
+            op = I
+            for s, o in ops:
+                if s == v:
+                    op = o
+                else:
+                    k = next(k for k, c in enumerate(kids) if s in below[c])
+                    ch[k] = chan[kids[k]][i]
+            put(ch, "done", op, coef)
+        W.append(T)
+    return W
+
+
 def _upload(topo, W, hermitian=True):
-    tens = [DTensor.from_numpy(a, nchildren(topo, n) + 1) for n, a in enumerate(W)]
+    tens = [DTensor.from_numpy(a, nchildren(topo, n) + (1 if a.ndim == nchildren(topo, n) + 3 else 0))
+            for n, a in enumerate(W)]
     return TTNO(topo, tens, ishermitian=hermitian)
+
+
+def terms_ttno(topo, phys, terms, hermitian=True):
+    return _upload(topo, terms_ttno_host(topo, phys, terms), hermitian)
 
 
 def tfi_host(topo, J=1.0, g=0.9):

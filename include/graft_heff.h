@@ -148,6 +148,11 @@ int graft_plan_destroy(graft_plan* plan);
  * all GEMM blocks), and the number of output elements. */
 int graft_plan_info(graft_plan* plan, uint64_t* workspace_bytes, double* macs,
                     uint64_t* output_elements);
+/* Device bytes the plan itself holds (block tables and the per-element maps
+ * of fused sector steps; the workspace arena belongs to the context).  The
+ * reference's plans are host-only and carry no such figure; a plan cache
+ * bounds its device memory with it. */
+int graft_plan_device_bytes(graft_plan* plan, uint64_t* bytes);
 /* Per-step GEMM profile: m, k, n of the largest block, block count and MACs. */
 int graft_plan_step_info(graft_plan* plan, int32_t step, int64_t* mkn3, int64_t* nblocks,
                          double* macs, int32_t* tile_config);

@@ -42,7 +42,6 @@ Nothing in this module is "parity unpinned".
 """
 from __future__ import annotations
 
-import itertools
 import math
 from dataclasses import dataclass, field
 
@@ -296,9 +295,14 @@ def ttns_to_dense(t, tensors, phys):
     return res.reshape(-1, order="F")
 
 
-def ttno_to_dense(t, W):
-    """Dense operator of a TTNO closed by a ones cap — GraftTestUtils.jl:272-304."""
-    sites = list(range(t.n))
+def ttno_to_dense(t, W, phys=None):
+    """Dense operator of a TTNO closed by a ones cap — GraftTestUtils.jl:272-304.
+
+    ``phys[i]`` false marks a node without physical legs (W :: children <- parent);
+    the dense operator then acts on the physical nodes in node order."""
+    phys = [True] * t.n if phys is None else list(phys)
+    sites = [i for i in range(t.n) if phys[i]]
+    pos = {s: k for k, s in enumerate(sites)}
     ns = len(sites)
     labels = []
     tensors = []
@@ -307,8 +311,9 @@ def ttno_to_dense(t, W):
         K = len(t.children[i])
         for k, c in enumerate(t.children[i]):
             idx[k] = c + 1
-        idx[K] = -(i + 1)
-        idx[K + 1] = -(ns + i + 1)
+        if phys[i]:
+            idx[K] = -(pos[i] + 1)
+            idx[K + 1] = -(ns + pos[i] + 1)
         if t.parent[i] < 0:
             lbl = t.n + 1
             idx[-1] = lbl
@@ -745,37 +750,6 @@ def execute_pair_steps(steps, operands, nslots, output_slot):
         slots[a] = None
         slots[b] = None
     return slots[output_slot]
-
-
-# ---------------------------------------------------------------------------
-# Small Krylov helpers used by solver-level checks (dense reference only).
-# ---------------------------------------------------------------------------
-
-
-def lowest_eigpair_dense(Hmat):
-    w, v = np.linalg.eigh((Hmat + Hmat.conj().T) / 2)
-    return w[0], v[:, 0]
-
-
-def brute_force_orders(n_tensors):
-    """All ordered binary trees over n leaves (for planner optimality checks)."""
-    def trees(items):
-        if len(items) == 1:
-            yield items[0]
-            return
-        first = items[0]
-        rest = items[1:]
-        for r in range(0, len(rest)):
-            for left_rest in itertools.combinations(rest, r):
-                left = (first,) + left_rest
-                right = tuple(x for x in rest if x not in left_rest)
-                if not right:
-                    continue
-                for lt in trees(list(left)):
-                    for rt in trees(list(right)):
-                        yield (lt, rt)
-
-    return list(trees(list(range(n_tensors))))
 
 
 # ---------------------------------------------------------------------------

@@ -343,6 +343,52 @@ def test_dmrg2_star_matches_exact_diagonalization(g):
     assert abs(Es[-1] - E0) < 1e-10, (Es, E0)
 
 
+def test_dmrg2_physless_binary_root_bootstrap(g):
+    """test/runtests.jl:781-804: binary(2) with sites on the leaves only (root
+    and inner nodes carry no physical leg), Heisenberg chain through the
+    leaves, D=2 start, maxdim 4.  The root bootstrap alone keeps the state
+    (1e-12) and opens both root bonds to 4; dmrg2_ then reaches the exact
+    ground energy (1e-10) with root bonds [4, 4]."""
+    from graft_amd import (backend as B, contractions as C, groundstate as GS, networks as N,
+                           operators as O, trees as Tr)
+    topo = Tr.binary_topology(2)
+    t = _oracle_tree(topo)
+    leaves = [i for i in range(t.n) if not t.children[i]]
+    phys = [2 if i in leaves else 0 for i in range(t.n)]
+    pos = {s: k for k, s in enumerate(leaves)}
+    terms = []
+    for a, b in zip(leaves[:-1], leaves[1:]):
+        terms += [(0.5, [(a, ref.SP), (b, ref.SM)]), (0.5, [(a, ref.SM), (b, ref.SP)]),
+                  (0.25, [(a, ref.SZ), (b, ref.SZ)])]
+    Hd = ref.dense_hamiltonian(len(leaves), [(c, [(pos[s], o) for s, o in ops]) for c, ops in terms])
+    E0 = np.linalg.eigvalsh(Hd)[0]
+    H = O.terms_ttno(topo, phys, terms)
+    _say("dmrg2 physless binary(2) root bootstrap")
+    host, hasphys = N.random_ttns_host(np.random.default_rng(20260710), topo,
+                                       {i: 2 for i in leaves}, 2)
+    flags = [bool(p) for p in phys]
+
+    def state(psi):
+        return N.TTNS(topo, [a.clone() for a in psi.tensors], psi.center, psi.hasphys)
+
+    psi0 = N.TTNS(topo, [g.tensors.DTensor.from_numpy(a, a.ndim - 1) for a in host], topo.root,
+                  hasphys)
+    kids = topo.children[topo.root]
+    assert GS._physless_root_two_site_targets(psi0, B.TruncationScheme()) == [(k, 4) for k in kids]
+    v0 = ref.ttns_to_dense(t, host, flags)
+    boot = state(psi0)
+    GS._bootstrap_physless_root_(boot, C.EnvCache(topo),
+                                 GS._physless_root_two_site_targets(boot, B.TruncationScheme(maxdim=4)))
+    assert [boot.tensors[k].dims[-1] for k in kids] == [4, 4]
+    boot.check_arrows()
+    v1 = ref.ttns_to_dense(t, [a.to_numpy() for a in boot.tensors], flags)
+    assert np.linalg.norm(v1 - v0) < 1e-12
+    psi = state(psi0)
+    _, Es = GS.dmrg2_(psi, H, trunc=B.TruncationScheme(maxdim=4), nsweeps=4)
+    assert [psi.tensors[k].dims[-1] for k in kids] == [4, 4]
+    assert abs(Es[-1] - E0) < 1e-10, (Es, E0)
+
+
 def test_tdvp1_full_rank_equals_exact_evolution(g):
     from graft_amd import evolution as EV, networks as N, operators as O, trees as Tr
     topo = Tr.star_topology(3, 2)

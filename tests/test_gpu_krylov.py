@@ -180,32 +180,52 @@ def test_device_exponentiate_matches_host_driver_and_dense(g):
         assert infod.numiter == infoh.numiter
 
 
-@pytest.mark.parametrize("shape", [(600, 900), (1100, 700)])
-def test_gram_route_svd_matches_dense_svd(g, shape):
-    """The Gram-route truncated SVD against LAPACK: kept singular values and
-    the truncated reconstruction, on a matrix with a decaying spectrum."""
+@pytest.mark.parametrize("shape,decay", [((600, 900), 25.0), ((1100, 700), 25.0),
+                                         ((700, 2000), 40.0 / np.log(10.0))])
+def test_gram_route_svd_matches_dense_svd(g, shape, decay):
+    """The Gram-route truncated SVD against LAPACK on matrices whose spectra
+    decay over 11 to 15 decades: every singular value >= 1e-5 sigma_1 to 1e-10
+    relative (the truncation spectrum), all of them to 1e-13 sigma_1
+    absolute, the kept values to 1e-12, the discarded norm, and the truncated
+    reconstruction; cuts by maxdim, by atol and by discarded weight that fall
+    deep inside the tail keep LAPACK's rank."""
     import torch
     from graft_amd import backend as B
     rows, cols = shape
-    _say("gram svd", shape)
+    _say("gram svd", shape, decay)
     rng = np.random.default_rng(rows)
-    U0, _ = np.linalg.qr(ref.random_complex(rng, (rows, min(shape))))
-    V0, _ = np.linalg.qr(ref.random_complex(rng, (cols, min(shape))))
-    sig = np.exp(-np.arange(min(shape)) / 25.0)
+    n = min(shape)
+    U0, _ = np.linalg.qr(ref.random_complex(rng, (rows, n)))
+    V0, _ = np.linalg.qr(ref.random_complex(rng, (cols, n)))
+    sig = np.exp(-np.arange(n) / decay)
     M = (U0 * sig) @ V0.conj().T
     Md = torch.from_numpy(M).cuda()
+    s_ref = np.linalg.svd(M, compute_uv=False)
     ts = B.TruncationScheme(maxdim=64)
     U, S, Vh, disc, spec = B.split_svd(Md, ts, method="gram")
-    s_ref = np.linalg.svd(M, compute_uv=False)
     assert U.shape == (rows, 64) and Vh.shape == (64, cols)
+    big = s_ref >= 1e-5 * s_ref[0]
+    assert np.abs(spec[big] / s_ref[big] - 1).max() < 1e-10
+    assert np.abs(spec - s_ref).max() < 1e-13 * s_ref[0]
     assert np.abs(S.cpu().numpy() - s_ref[:64]).max() < 1e-12 * s_ref[0]
-    assert np.abs(spec[64:] - s_ref[64:]).max() < 1e-6 * s_ref[0]     # tail: absolute ~sqrt(n eps)
-    assert abs(disc - np.sqrt(np.sum(s_ref[64:] ** 2))) < 1e-10
+    assert abs(disc - np.sqrt(np.sum(s_ref[64:] ** 2))) < 1e-10 * max(disc, 1e-300)
     low = (U * S) @ Vh
     want = (U0[:, :64] * sig[:64]) @ V0[:, :64].conj().T
     assert np.abs(low.cpu().numpy() - want).max() < 1e-11
     Ud, Sd, Vhd, discd, _ = B.split_svd(Md, ts, method="direct")
     assert np.abs(Sd.cpu().numpy() - S.cpu().numpy()).max() < 1e-12
+    # cuts inside the tail: thresholds midway (geometrically) between two values
+    mid = lambda i: float(np.sqrt(sig[i] * sig[i + 1]))
+    for cut in (B.TruncationScheme(atol=mid(150)),
+                B.TruncationScheme(maxdim=n, rtol=mid(120) / np.linalg.norm(sig)),
+                B.TruncationScheme(discarded_weight=float(np.sqrt(np.sum(sig[100:] ** 2))
+                                                          / np.linalg.norm(sig)) * 1.0001)):
+        r_ref = B.truncation_rank(s_ref, cut)
+        r = B.truncation_rank(spec, cut)
+        assert r == r_ref, (cut, r, r_ref)
+        Ug, Sg, _, dg, _ = B.split_svd(Md, cut, method="gram")
+        assert Ug.shape[1] == r_ref
+        assert abs(dg - np.sqrt(np.sum(s_ref[r_ref:] ** 2))) < 1e-10 * max(dg, 1e-300)
 
 
 @pytest.mark.parametrize("shape", [(7, 5, 9), (300, 40000, 48)])

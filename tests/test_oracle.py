@@ -69,6 +69,48 @@ def test_ttno_builders_equal_dense(topo_fn):
     assert np.abs(ref.ttno_to_dense(t, O.tfi_random_host(topo, Js, hs)) - Hd).max() < 1e-12
 
 
+def _leaf_heisenberg_chain(t, sites, J=1.0):
+    """heisenberg_chain(sites) (test/runtests.jl:32-41) on consecutive listed sites."""
+    terms = []
+    for a, b in zip(sites[:-1], sites[1:]):
+        terms += [(J / 2, [(a, ref.SP), (b, ref.SM)]), (J / 2, [(a, ref.SM), (b, ref.SP)]),
+                  (J / 4, [(a, ref.SZ), (b, ref.SZ)])]
+    return terms
+
+
+@pytest.mark.parametrize("topo_fn", [lambda: Tr.star_topology(3, 1), lambda: Tr.binary_topology(2),
+                                     lambda: Tr.mps_topology(4)])
+def test_terms_ttno_equals_dense(topo_fn):
+    """The general term automaton on every node equals the dense Hamiltonian,
+    for edge terms and for terms between distant sites."""
+    topo = topo_fn()
+    t = _product_tree(topo)
+    n = t.n
+    rng = np.random.default_rng(9)
+    terms = ref.tfi_terms(t, 1.0, 0.9) + _leaf_heisenberg_chain(t, list(rng.permutation(n)))
+    W = O.terms_ttno_host(topo, [2] * n, terms)
+    assert np.abs(ref.ttno_to_dense(t, W) - ref.dense_hamiltonian(n, terms)).max() < 1e-12
+
+
+def test_terms_ttno_physless_internal_nodes():
+    """Sites only on the leaves of binary(2) (root and inner nodes have no
+    physical leg), Heisenberg chain through the leaves as in
+    test/runtests.jl:781-804: the TTNO equals the dense chain Hamiltonian."""
+    topo = Tr.binary_topology(2)
+    t = _product_tree(topo)
+    leaves = [i for i in range(t.n) if not t.children[i]]
+    phys = [2 if i in leaves else 0 for i in range(t.n)]
+    pos = {s: k for k, s in enumerate(leaves)}
+    terms = _leaf_heisenberg_chain(t, leaves)
+    W = O.terms_ttno_host(topo, phys, terms)
+    assert all(W[i].ndim == len(t.children[i]) + (3 if phys[i] else 1) for i in range(t.n))
+    local = [(c, [(pos[s], o) for s, o in ops]) for c, ops in terms]
+    Hd = ref.dense_hamiltonian(len(leaves), local)
+    assert np.abs(ref.ttno_to_dense(t, W, [bool(p) for p in phys]) - Hd).max() < 1e-12
+    # the chain's ground energy (4-site open Heisenberg, J=1: -(3 + 2 sqrt 3)/4)
+    assert abs(np.linalg.eigvalsh(Hd)[0] + (3 + 2 * np.sqrt(3)) / 4) < 1e-12
+
+
 def test_ttno_bond_dims():
     topo = Tr.binary_topology(2)
     assert O.heisenberg_host(topo)[1].shape[-1] == 5
