@@ -19,6 +19,7 @@ import torch
 
 from . import backend as B
 from . import krylov
+from . import sector_linalg as SL
 from . import sectors as S
 from . import tensors as T
 from .contractions import EnvCache, eff_h0, eff_h1, eff_h2, two_site_tensor
@@ -273,7 +274,7 @@ def _evolve_sector_link_(ev, psi, H, u, v, dz, herm):
         psi.tensors[u] = Q
         cache.invalidate_node_(u)
         k0 = eff_h0(cache, psi, H, u, v, precision=ev.precision)
-        C = S.to_structure(R, k0.blocks, 1)
+        C = SL.to_structure(R, k0.blocks, 1)
         C, _ = evolution_exponentiate_backend(k0, -dz, C, **kw)
         psi.tensors[v] = S.absorb_link(psi.tensors[v], childslot(t, v, u), C, 1, 0)
     else:
@@ -282,7 +283,7 @@ def _evolve_sector_link_(ev, psi, H, u, v, dz, herm):
         cache.invalidate_node_(u)
         psi.center = u
         k0 = eff_h0(cache, psi, H, v, u, precision=ev.precision)
-        C = S.to_structure(R.permute((1, 0), 1), k0.blocks, 1)    # :: V_e <- Y
+        C = SL.to_structure(SL.permute2(R), k0.blocks, 1)       # :: V_e <- Y
         C, _ = evolution_exponentiate_backend(k0, -dz, C, **kw)
         P = psi.tensors[v]
         psi.tensors[v] = S.absorb_link(P, P.rank - 1, C, 0, 1)

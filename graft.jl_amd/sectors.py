@@ -6,10 +6,11 @@ flat charges sum to zero, packed in block order).  The effective maps and
 environments run as grouped block GEMMs in libgraft_heff.so.  This module
 holds the gauge factorizations of such tensors (block QR, truncated block
 SVD) and their construction; the reference performs these in
-MatrixAlgebraKit through TensorKit (Backend.jl:466-609).  The factorizations
-run on the host, once per site move, on small per-sector matrices, outside
-the H_eff hot path; absorbing a link factor into a neighbour is one planned
-pair contraction on the device.
+MatrixAlgebraKit through TensorKit (Backend.jl:466-609).  The sweeps call the
+device forms (``sector_linalg``: per-sector matrices gathered on the device,
+factorized there); the host forms here build initial states and serve as the
+reference the device forms are tested against.  Absorbing a link factor into
+a neighbour is one planned pair contraction on the device.
 
 Charge conventions (TensorKit, flat-leg picture): a codomain leg carries the
 charges of its space V, a domain leg those of dual(V) (negated).  A TTNS
@@ -250,12 +251,12 @@ def move_center_edge_(psi, m, cache):
     if t.parent[n] == m:
         Q, R = link_split(psi.tensors[n])                   # R :: new <- old
         psi.tensors[n] = Q
-        psi.tensors[m] = absorb_link(psi.tensors[m], childslot(t, m, n), R.to_device(), 1, 0)
+        psi.tensors[m] = absorb_link(psi.tensors[m], childslot(t, m, n), R, 1, 0)
     else:
         Q, R = link_split(psi.tensors[n], childslot(t, n, m))
         psi.tensors[n] = Q
         P = psi.tensors[m]
-        psi.tensors[m] = absorb_link(P, P.rank - 1, R.to_device(), 1, 0)
+        psi.tensors[m] = absorb_link(P, P.rank - 1, R, 1, 0)
     psi.center = m
     if cache is not None:
         cache.invalidate_edge_(n, m)
@@ -266,10 +267,10 @@ def link_split(A, k=None):
     """Block QR of a device sector tensor that isolates leg k (the last leg when
     k is None): A = Q . R per sector of that leg, Q isometric from the other
     legs onto a new bond in slot k, R :: new <- old (flat legs: the new bond's
-    dual, leg k) on the host — the QR link splits of tdvp.jl:291-301 and
-    Backend.jl:597-609."""
-    Q, R = qr_isolating(HostBlocks.from_device(A), k)
-    return Q.to_device(), R
+    dual, leg k), both on the device — the QR link splits of tdvp.jl:291-301
+    and Backend.jl:597-609.  ``qr_isolating`` below is the host form."""
+    from .sector_linalg import qr_isolating as device_qr
+    return device_qr(A, k)
 
 
 def qr_isolating(T, k=None):
@@ -301,8 +302,14 @@ def to_structure(T, struct, ncod):
     return DTensor.from_blocks(struct, data, ncod)
 
 
-_ABSORB_PLANS = OrderedDict()
+_ABSORB_PLANS = OrderedDict()      # pair plans of link absorbs and two-site tensors
 ABSORB_PLAN_CACHE = 256
+
+
+def _remember_plan(key, entry):
+    _ABSORB_PLANS[key] = entry
+    if len(_ABSORB_PLANS) > ABSORB_PLAN_CACHE:
+        _ABSORB_PLANS.popitem(last=False)
 
 
 def absorb_link(P, k, C, ck, cnew):
@@ -328,9 +335,7 @@ def absorb_link(P, k, C, ck, cnew):
         step = PairStep(0, 1, 2, tuple(j for j in range(N) if j != k), (k,), (cnew,), (ck,),
                         False, False, (perm, ()))
         entry = (NativePlan(ContractionPlan(3, 2, [step]), [P, C], out), out)
-        _ABSORB_PLANS[key] = entry
-        if len(_ABSORB_PLANS) > ABSORB_PLAN_CACHE:
-            _ABSORB_PLANS.popitem(last=False)
+        _remember_plan(key, entry)
     else:
         _ABSORB_PLANS.move_to_end(key)
     plan, out = entry
@@ -375,36 +380,39 @@ def two_site_tensor(psi, n, m):
     from .trees import childslot
     k = childslot(psi.topo, m, n)
     A, Bm = psi.tensors[n], psi.tensors[m]
-    pn = A.rank - 1
-    bopen = tuple(j for j in range(Bm.rank) if j != k)
-    legs = A.blocks.legs[:-1] + [Bm.blocks.legs[j] for j in bopen]
-    out = BlockStructure.invariant(legs)
-    step = PairStep(0, 1, 2, tuple(range(pn)), (pn,), bopen, (k,), False, False,
-                    (tuple(range(len(legs))), ()))
-    plan = NativePlan(ContractionPlan(3, 2, [step]), [A, Bm], out)
-    theta = DTensor.empty(out.dims, len(legs), out)
+    key = ("theta", A.blocks.key(), Bm.blocks.key(), k)
+    entry = _ABSORB_PLANS.get(key)
+    if entry is None:
+        pn = A.rank - 1
+        bopen = tuple(j for j in range(Bm.rank) if j != k)
+        legs = A.blocks.legs[:-1] + [Bm.blocks.legs[j] for j in bopen]
+        out = BlockStructure.invariant(legs)
+        step = PairStep(0, 1, 2, tuple(range(pn)), (pn,), bopen, (k,), False, False,
+                        (tuple(range(len(legs))), ()))
+        entry = (NativePlan(ContractionPlan(3, 2, [step]), [A, Bm], out), out)
+        _remember_plan(key, entry)
+    else:
+        _ABSORB_PLANS.move_to_end(key)
+    plan, out = entry
+    theta = DTensor.empty(out.dims, len(out.legs), out)
     plan.execute([A, Bm], theta)
     return theta
 
 
 def split_two_site_(psi, theta, n, m, trunc_rank, center_on="n"):
+    """split_two_site! (effective.jl:852-876) for sector states: the truncated
+    block SVD runs on the device per row charge; the singular values go to the
+    new centre and the bond takes child slot k of A_m."""
+    from .sector_linalg import svd_split as device_svd
     from .trees import childslot
     k = childslot(psi.topo, m, n)
     pn = psi.tensors[n].ncod
     Km = psi.tensors[m].ncod
-    T = HostBlocks.from_device(theta)
-    U, S, Vh, spectrum, disc, bond = svd_split(T, pn, trunc_rank)
-    if center_on == "n":
-        U = scale_bond(U, S, len(U.legs) - 1)
-    else:
-        Vh = scale_bond(Vh, S, 0)
-    # Vh legs: (bond, m legs except slot k ...); move the bond to slot k
-    N = len(Vh.legs)
-    p1 = [0 if j == k else 1 + (j if j < k else j - 1) for j in range(Km)]
-    Am = Vh.permute(p1 + [N - 1], Km)
-    U.ncod = pn
-    psi.tensors[n] = U.to_device()
-    psi.tensors[m] = Am.to_device()
+    U, S, Am, spectrum, disc, bond = device_svd(theta, pn, trunc_rank,
+                                                 absorb="u" if center_on == "n" else "vh",
+                                                 bond_slot=k, vh_ncod=Km)
+    psi.tensors[n] = U
+    psi.tensors[m] = Am
     psi.center = n if center_on == "n" else m
     return psi, dict(discarded=disc, spectrum=spectrum, kept=int(sum(d for _, d in bond.sectors)))
 
