@@ -100,6 +100,7 @@ def lib():
         "graft_plan_execute": [vp, vp, pp, vp, i32],
         "graft_plan_execute_timed": [vp, vp, pp, vp, i32, ctypes.POINTER(ctypes.c_float)],
         "graft_heff_apply": [vp, vp, vp, pp, vp, i32],
+        "graft_heff_apply_noalloc": [vp, vp, vp, pp, vp, i32],
         "graft_env_update": [vp, vp, pp, vp],
         "graft_contract_pair": [vp, ctypes.POINTER(PairStepC), ctypes.POINTER(TensorDescC), vp,
                                 ctypes.POINTER(TensorDescC), vp, vp],

@@ -13,6 +13,7 @@
 #include <map>
 #include <unordered_map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -1383,11 +1384,11 @@ static int ensure_workspace(graft_ctx* ctx, uint64_t bytes) {
 }
 
 static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void* const* inputs,
-                    void* output, int precision) {
+                    void* output, int precision, char* ws) {
   auto ptr = [&](int slot) -> void* {
     if (slot < p.ninputs) return const_cast<void*>(inputs[slot]);
     if (slot == p.output_slot) return output;
-    return (char*)ctx->ws + p.ws_offset[slot];
+    return ws + p.ws_offset[slot];
   };
   void* A = ptr(ex.a);
   void* B = ptr(ex.b);
@@ -1396,12 +1397,12 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     set_error("null operand pointer");
     return GRAFT_E_INVALID;
   }
-  void* partial = ex.ksplit > 1 ? (char*)ctx->ws + p.ws_partial : nullptr;
+  void* partial = ex.ksplit > 1 ? ws + p.ws_partial : nullptr;
   void* Cdst = C;
   if (ex.fused) {
     // gather both operands into the per-charge matrices, multiply, scatter below
     auto al = [](int64_t x) { return (size_t)((x * 16 + 255) / 256) * 256; };
-    char* fb = (char*)ctx->ws + p.ws_fused;
+    char* fb = ws + p.ws_fused;
     void* Af = fb;
     void* Bf = fb + al(ex.fa);
     void* Cf = fb + al(ex.fa) + al(ex.fb);
@@ -1455,13 +1456,42 @@ static int execute_plan(graft_ctx* ctx, graft_plan_impl& p, const void* const* i
   rc = ensure_workspace(ctx, p.ws_bytes);
   if (rc) return rc;
   for (auto& ex : p.steps) {
-    rc = run_step(ctx, p, ex, inputs, output, precision);
+    rc = run_step(ctx, p, ex, inputs, output, precision, (char*)ctx->ws);
     if (rc) return rc;
   }
   return GRAFT_OK;
 }
 
 }  // namespace graft
+
+// Workspace of graft_heff_apply_noalloc: a file-scope device array, so that
+// entry point never allocates.  Each device holds its own copy from module
+// load until the process ends; its address is looked up once per device when
+// a context is created there (the array's name is only a host-side handle).
+static constexpr uint64_t kNoallocScratchBytes = 128ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+static constexpr int kMaxDevices = 64;
+static constexpr int kNoallocMaxInputs = 32;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+static int init_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    graft::set_error("device index out of range for the static scratch table");
+    return GRAFT_E_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return GRAFT_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr) {
+    (void)hipGetLastError();
+    graft::set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    return GRAFT_E_HIP;
+  }
+  g_scratch_ptr[device] = ptr;
+  return GRAFT_OK;
+}
 
 using namespace graft;
 
@@ -1476,6 +1506,8 @@ int graft_ctx_create(int device, void* hip_stream, graft_ctx** out) {
   c->device = device;
   c->stream = (hipStream_t)hip_stream;
   GRAFT_CHECK_HIP(hipSetDevice(device));
+  const int rs = init_scratch(device);
+  if (rs) return rs;
   GRAFT_CHECK_HIP(hipMalloc((void**)&c->d_partials,
                             sizeof(double) * 2 * kReduceBlocks * kMaxMultiVec));
   GRAFT_CHECK_HIP(hipMalloc((void**)&c->d_result, sizeof(double) * 2 * kMaxMultiVec));
@@ -1843,7 +1875,7 @@ int graft_plan_execute_timed(graft_ctx* ctx, graft_plan* plan, const void* const
   for (auto& e : ev) GRAFT_CHECK_HIP(hipEventCreate(&e));
   GRAFT_CHECK_HIP(hipEventRecord(ev[0], ctx->stream));
   for (size_t s = 0; s < ns && !rc; ++s) {
-    rc = run_step(ctx, p, p.steps[s], inputs, output, precision);
+    rc = run_step(ctx, p, p.steps[s], inputs, output, precision, (char*)ctx->ws);
     if (!rc) GRAFT_CHECK_HIP(hipEventRecord(ev[s + 1], ctx->stream));
   }
   if (!rc) {
@@ -1888,10 +1920,11 @@ int graft_plan_autotune(graft_ctx* ctx, graft_plan* plan, const void* const* inp
         if ((c == kSkinnyConfig || c == kBlockStreamConfig) && o == 1) continue;
         ex.tr = (o == 1);
         // warm-up, then time
-        rc = run_step(ctx, p, ex, inputs, output, precision);
+        rc = run_step(ctx, p, ex, inputs, output, precision, (char*)ctx->ws);
         if (rc) break;
         GRAFT_CHECK_HIP(hipEventRecord(e0, ctx->stream));
-        for (int r = 0; r < repeats && !rc; ++r) rc = run_step(ctx, p, ex, inputs, output, precision);
+        for (int r = 0; r < repeats && !rc; ++r)
+          rc = run_step(ctx, p, ex, inputs, output, precision, (char*)ctx->ws);
         GRAFT_CHECK_HIP(hipEventRecord(e1, ctx->stream));
         GRAFT_CHECK_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -1931,6 +1964,35 @@ int graft_heff_apply(graft_ctx* ctx, graft_plan* plan, const void* x, const void
   inputs[0] = x;
   for (int i = 1; i < n; ++i) inputs[i] = statics[i - 1];
   return execute_plan(ctx, plan->impl, inputs.data(), y, precision);
+}
+
+int graft_heff_apply_noalloc(graft_ctx* ctx, graft_plan* plan, const void* x,
+                             const void* const* statics, void* y, int32_t precision) {
+  GRAFT_REQUIRE(ctx != nullptr && plan != nullptr, "null argument");
+  graft_plan_impl& p = plan->impl;
+  const int n = p.ninputs;
+  GRAFT_REQUIRE(n >= 1 && x != nullptr && (n == 1 || statics != nullptr) && y != nullptr,
+                "null operand");
+  GRAFT_REQUIRE(n <= kNoallocMaxInputs, "too many plan inputs for the allocation-free apply");
+  GRAFT_REQUIRE(ctx->device >= 0 && ctx->device < kMaxDevices, "device index out of range");
+  if (p.ws_bytes > kNoallocScratchBytes) {
+    set_error("the plan needs " + std::to_string(p.ws_bytes) +
+              " workspace bytes; the allocation-free apply has a static scratch of " +
+              std::to_string(kNoallocScratchBytes) + " bytes");
+    return GRAFT_E_NOMEM;
+  }
+  char* ws = (char*)g_scratch_ptr[ctx->device];
+  GRAFT_REQUIRE(ws != nullptr, "the static scratch of this device is not set up");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  const void* inputs[kNoallocMaxInputs];
+  inputs[0] = x;
+  for (int i = 1; i < n; ++i) inputs[i] = statics[i - 1];
+  for (auto& ex : p.steps) {
+    rc = run_step(ctx, p, ex, inputs, y, precision, ws);
+    if (rc) return rc;
+  }
+  return GRAFT_OK;
 }
 
 int graft_env_update(graft_ctx* ctx, graft_plan* plan, const void* const* operands,

@@ -194,6 +194,16 @@ int graft_plan_execute_timed(graft_ctx* ctx, graft_plan* plan, const void* const
 /* eff_h1 / eff_h2 / eff_h0 apply: inputs = (x, statics...). */
 int graft_heff_apply(graft_ctx* ctx, graft_plan* plan, const void* x,
                      const void* const* statics, void* y, int32_t precision);
+/* graft_heff_apply that allocates nothing: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The plan's
+ * intermediates, split-K partials and fused sector matrices live in a static
+ * device array of 128 MiB per device instead of the context's workspace
+ * arena; a plan needing more returns GRAFT_E_NOMEM.  Every call on one device
+ * shares that array and leaves its contents behind, so calls that overlap on
+ * one device (several streams or threads) race. */
+int graft_heff_apply_noalloc(graft_ctx* ctx, graft_plan* plan, const void* x,
+                             const void* const* statics, void* y, int32_t precision);
 /* env(u -> v) from (ket, envs..., cap?, W, bra). */
 int graft_env_update(graft_ctx* ctx, graft_plan* plan, const void* const* operands,
                      void* env_out);
