@@ -744,3 +744,85 @@ def eff_h2(cache, psi, H, n, m, optimize=True, memory_weight=1.0, sector_aware=T
     native = cache.get_plan("h2", spec, [proto] + statics, optimize, memory_weight, sector_aware,
                             memory_cap_bytes, out_structure=out_structure)
     return EffectiveMap(native, statics, proto, "h2", precision, spec, cache)
+
+
+# ---------------------------------------------------------------------------
+# Factor frame of a directed edge — two_site_factor_frame.jl:118-160, 273-426
+# ---------------------------------------------------------------------------
+
+
+def _site_factor_spec(cache, psi, H, site, peer, X):
+    """_site_factor_action (two_site_factor_frame.jl:273-352) in X's own leg
+    order: X (a tensor in the site's leg layout) contracted with W_site, every
+    environment of the site except the one from ``peer`` and the root cap.
+    Open legs: the site's external bra legs (every leg but the one towards
+    ``peer``, in leg order), then X's leg towards ``peer`` (ket) and W's leg
+    towards ``peer`` (operator channel): (N-1) codomain legs, 2 domain legs."""
+    t = psi.topo
+    hp = psi.hasphys[site]
+    W = H.tensors[site]
+    K = nchildren(t, site)
+    Nx = X.rank
+    edge = _stateleg(t, hp, site, peer)
+    opedge = _opleg(t, hp, site, peer)
+    ext = [l for l in range(Nx) if l != edge]
+    pos = {l: i for i, l in enumerate(ext)}
+    envlist = [(w, env_(cache, psi, H, w, site)) for w in neighbors(t, site) if w != peer]
+    xidx = [0] * Nx
+    widx = [0] * W.rank
+    labels = [xidx, widx]
+    conjs = [False, False]
+    statics = [W]
+    children, parents, caps = [], [], []
+    nxt = [0]
+
+    def fresh():
+        nxt[0] += 1
+        return nxt[0]
+
+    xidx[edge] = -Nx
+    widx[opedge] = -(Nx + 1)
+    if hp:
+        pin = fresh()
+        xidx[K] = pin
+        widx[K + 1] = pin
+        widx[K] = -(pos[K] + 1)
+    for w, E in envlist:
+        lx = _stateleg(t, hp, site, w)
+        kk, oo = fresh(), fresh()
+        xidx[lx] = kk
+        widx[_opleg(t, hp, site, w)] = oo
+        labels.append([kk, oo, -(pos[lx] + 1)])
+        conjs.append(False)
+        statics.append(E)
+        (parents if t.parent[site] == w else children).append(len(labels) - 1)
+    if t.parent[site] < 0:
+        ka, ko = fresh(), fresh()
+        xidx[Nx - 1] = ka
+        widx[-1] = ko
+        labels.append([ka, ko, -(pos[Nx - 1] + 1)])
+        conjs.append(False)
+        cl = None
+        if X.blocks is not None:
+            xl = _legs_of(X)[-1]
+            cl = [xl.dual(), _legs_of(W)[-1].dual(), xl]
+        statics.append(cache.root_cap((X.dims[-1], W.dims[-1], X.dims[-1]), cl))
+        caps.append(len(labels) - 1)
+    preferred = children + [1] + parents + caps
+    spec = ContractionSpec(labels, conjs, Nx + 1, (Nx - 1, 2), 0, preferred)
+    return spec, statics
+
+
+def site_factor_action(cache, psi, H, site, peer, X=None):
+    """One half of the factor frame of edge site -> peer
+    (oriented_two_site_factor_frame, two_site_factor_frame.jl:404-426): a
+    device tensor with legs (external bra legs of ``site``..., X's edge leg,
+    W's edge leg).  With A = X at ``site`` and the partner tensor at ``peer``,
+    the two-site map applied to their product is the contraction of the two
+    halves over both edge legs, so H_eff(Theta) is never formed."""
+    X = psi.tensors[site] if X is None else X
+    spec, statics = _site_factor_spec(cache, psi, H, site, peer, X)
+    plan = cache.get_plan("factor", spec, [X] + statics)
+    out = DTensor.empty(plan.out_dims, X.rank - 1, plan.out_structure)
+    plan.execute([X] + statics, out)
+    return out

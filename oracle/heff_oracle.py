@@ -261,14 +261,34 @@ _LETTERS = "abcdefghijklmnopqrstuvwxyzABCDEFGHIJKLMNOPQRSTUVWXYZ"
 
 
 def ncon(tensors, labels, conjs=None):
+    """Contract a labelled network pairwise with numpy.tensordot along the
+    order numpy.einsum_path picks (every label joins at most two tensors, so
+    each pair is one tensordot); open labels -1, -2, ... give the output order."""
     conjs = conjs or [False] * len(tensors)
     alllab = sorted({l for ls in labels for l in ls}, key=lambda x: (x < 0, abs(x)))
     sym = {l: _LETTERS[i] for i, l in enumerate(alllab)}
     opens = sorted([l for l in alllab if l < 0], key=lambda x: -x)
     subs = ",".join("".join(sym[l] for l in ls) for ls in labels)
     out = "".join(sym[l] for l in opens)
-    ops = [np.conj(t) if c else t for t, c in zip(tensors, conjs)]
-    return np.einsum(f"{subs}->{out}", *ops, optimize="greedy")
+    ops = [np.conj(t) if c else np.asarray(t) for t, c in zip(tensors, conjs)]
+    labs = [list(l) for l in labels]
+    if len(ops) > 1:
+        path = np.einsum_path(f"{subs}->{out}", *ops, optimize="greedy")[0][1:]
+        for group in path:
+            idx = sorted(group, reverse=True)
+            picked = [(ops.pop(i), labs.pop(i)) for i in idx][::-1]
+            A, la = picked[0]
+            for Bt, lb in picked[1:]:             # a group of more than two: left to right
+                shared = [x for x in la if x in lb]
+                A = np.tensordot(A, Bt, axes=([la.index(x) for x in shared],
+                                              [lb.index(x) for x in shared]))
+                la = [x for x in la if x not in shared] + [x for x in lb if x not in shared]
+            ops.append(A)
+            labs.append(la)
+    res, lr = ops[0], labs[0]
+    if len(lr) != len(set(lr)):
+        raise ValueError("ncon: a label is repeated within one tensor (trace) — not supported")
+    return np.transpose(res, [lr.index(l) for l in opens]) if opens else res.reshape(())
 
 
 # ---------------------------------------------------------------------------

@@ -77,48 +77,62 @@ def test_cbe_grows_bonds_and_tracks_exact_evolution(g):
     assert abs(np.linalg.norm(_dense(t, p3)) - 1) < 1e-10
 
 
-def test_cbe_directions_match_dense_projected_core(g):
+@pytest.mark.parametrize("arm,D,active,solver", [
+    (2, 2, "n", None), (5, 16, "n", None), (5, 16, "m", None), (5, 16, "n", "gram_u"),
+    (5, 16, "m", "gram_v"), (5, 16, "m", "block_krylov"), (5, 16, "n", "exact_svd")])
+def test_cbe_directions_match_dense_projected_core(g, arm, D, active, solver):
     """The spawned directions span the leading left singular space of the
-    double-complement core computed densely by the oracle."""
+    double-complement core computed densely by the oracle (two-site map
+    applied to Theta, both projections), for either endpoint of the
+    centre-arm edge active, through every solver (small-side Gram on either
+    side, block Krylov, exact SVD); they are orthogonal to the active range Q."""
     import torch
-    from graft_amd import backend as B, contractions as C, evolution as EV, networks as N, trees as Tr
-    topo, t, psi, H, _ = _setup(2, 7)
-    _say("CBE directions vs dense core")
+    from graft_amd import backend as B, contractions as C, evolution as EV, networks as N
+    from graft_amd import operators as O, trees as Tr
+    topo = Tr.star_topology(3, arm)
+    t = ref.Tree(list(topo.ids), list(topo.parent), [list(c) for c in topo.children])
+    psi = N.random_ttns(np.random.default_rng(7), topo, 2, D)
+    H = O.tfi_ttno(topo, 1.0, 0.9)
+    _say("CBE directions vs dense core, star(3,%d) D=%d active %s solver %s" % (arm, D, active, solver))
     n = topo.children[topo.root][0]          # first arm node, child of the centre
     m = topo.root
+    u = n if active == "n" else m
     cache = C.EnvCache(topo)
-    N.move_center_(psi, n, cache)
-    ev = EV.TDVP1_CBE(trunc=B.TruncationScheme(maxdim=16),
+    N.move_center_(psi, u, cache)
+    ev = EV.TDVP1_CBE(trunc=B.TruncationScheme(maxdim=10 * D),
                       cbe=EV.PredictorCBE(max_add=8, neigs=2, spawn_threshold=1e-12))
     ev.cache = cache
     host = [a.to_numpy() for a in psi.tensors]
-    A = host[n]
-    Ma = A.reshape(-1, A.shape[-1], order="F")
-    Qn, _ = np.linalg.qr(Ma)
-    E = EV._cbe_directions(ev, psi, H, n, m, True, torch.from_numpy(Qn).cuda(), -0.02j, 8)
-    # oracle: dense projected core
-    W = __import__("graft_amd.operators", fromlist=["x"]).tfi_host(topo, 1.0, 0.9)
+    k = [c for c in topo.children[m]].index(n)
+    An, Am = host[n], host[m]
+    permm = [j for j in range(Am.ndim) if j != k] + [k]
+    Mn = An.reshape(-1, An.shape[-1], order="F")
+    Mm = np.transpose(Am, permm).reshape(-1, Am.shape[k], order="F")
+    Qa, _ = np.linalg.qr(Mn if active == "n" else Mm)
+    E = EV._cbe_directions(ev, psi, H, n, m, active == "n", torch.from_numpy(Qa).cuda(), -0.02j, 8,
+                           solver=solver)
+    # oracle: the two-site map on Theta, then both complements
+    W = O.tfi_host(topo, 1.0, 0.9)
     envs = ref.all_envs(t, host, W)
     theta = ref.two_site_tensor(t, host, n, m)
     Y = ref.h2_apply(t, W, envs, n, m, theta)
-    pn = A.ndim - 1
+    pn = An.ndim - 1
     Ym = Y.reshape(int(np.prod(Y.shape[:pn])), -1, order="F")
-    k = [c for c in topo.children[m]].index(n)
-    Am = host[m]
-    perm = [j for j in range(Am.ndim) if j != k] + [k]
-    Bp = np.transpose(Am, perm).reshape(-1, Am.shape[k], order="F")
-    Z = Ym - (Ym @ Bp.conj()) @ Bp.T
-    Z = Z - Qn @ (Qn.conj().T @ Z)
-    U, s, _ = np.linalg.svd(Z)
+    Z, Pp = (Ym, Mm) if active == "n" else (Ym.T, Mn)
+    Z = Z - (Z @ Pp.conj()) @ Pp.T
+    Z = Z - Qa @ (Qa.conj().T @ Z)
+    U, s, _ = np.linalg.svd(Z, full_matrices=False)
     r = E.shape[1]
     assert r == 2
+    want = solver or ("gram_v" if Z.shape[1] <= Z.shape[0] else "gram_u")
+    assert ev.last_cbe_info["solver"].startswith(want), (ev.last_cbe_info["solver"], want)
     assert abs(ev.last_cbe_info["sigma_max"] - s[0]) < 1e-10 * s[0]
     # same 2-dimensional subspace (the gap to the third value keeps it well defined)
     Eh = E.cpu().numpy()
     ov = np.linalg.svd(U[:, :2].conj().T @ Eh, compute_uv=False)
     assert s[1] - s[2] > 1e-6 * s[0]
     assert np.all(ov > 1 - 1e-9)
-    assert np.abs(Qn.conj().T @ Eh).max() < 1e-12
+    assert np.abs(Qa.conj().T @ Eh).max() < 1e-12
 
 
 def test_tdvp1_matches_oracle_at_truncated_bonds(g):

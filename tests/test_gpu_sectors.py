@@ -216,3 +216,92 @@ def test_sector_fusion_modes_agree(g, mode):
         assert _rel(y_mode, y_auto) < 1e-13
     finally:
         ctx.set_sector_fusion(1)
+
+
+def test_sector_cbe_split_directions_match_dense_core(g):
+    """One sector CBE link split (the reference's exact sector-aware selector,
+    cbe.jl:459-463): U R reproduces the active tensor, U is isometric, and the
+    spawned directions span the leading left singular space of the dense
+    double-complement core the oracle forms from the two-site map on the
+    embedded tensors."""
+    import torch
+    from graft_amd import backend as B, contractions as C, evolution as EV, networks as N
+    from graft_amd import operators as O, sectors as S, trees as Tr
+    topo = Tr.binary_topology(2)
+    t = _tree(topo)
+    _say("sector CBE split vs dense core, binary(2) hubbard, D=8")
+    psi, _ = S.random_hubbard_ttns(np.random.default_rng(21), topo, 8, sigma=1.6)
+    Hs = O.hubbard_sector_ttno(topo, 1.0, 4.0)
+    Wd = O.hubbard_host(topo, 1.0, 4.0)
+    n, m = 1, 0
+    cache = C.EnvCache(topo)
+    N.move_center_(psi, n, cache)
+    ev = EV.TDVP1_CBE(trunc=B.TruncationScheme(maxdim=64),
+                      cbe=EV.PredictorCBE(max_add=8, neigs=3, spawn_threshold=1e-12))
+    ev.cache = cache
+    dense = _dense_state(psi)
+    A = dense[n]
+    U, R = EV._cbe_split(ev, psi, Hs, n, m, -0.05j)
+    assert ev.last_cbe_info["selected"] == 3
+    Ud, Rd = U.to_numpy(), R.to_numpy()
+    Um = Ud.reshape(-1, Ud.shape[-1], order="F")
+    assert np.abs(Um.conj().T @ Um - np.eye(Um.shape[1])).max() < 1e-12
+    rec = np.tensordot(Ud, Rd, axes=([Ud.ndim - 1], [0]))
+    assert _rel(rec, A) < 1e-12
+    # oracle: two-site map on Theta, double complement, leading left vectors
+    envs = ref.all_envs(t, dense, Wd)
+    theta = ref.two_site_tensor(t, dense, n, m)
+    Y = ref.h2_apply(t, Wd, envs, n, m, theta)
+    pn = A.ndim - 1
+    Ym = Y.reshape(int(np.prod(Y.shape[:pn])), -1, order="F")
+    Am = dense[m]
+    k = list(topo.children[m]).index(n)
+    perm = [j for j in range(Am.ndim) if j != k] + [k]
+    Pp = np.transpose(Am, perm).reshape(-1, Am.shape[k], order="F")
+    Ma = A.reshape(-1, A.shape[-1], order="F")
+    Qd, sv, _ = np.linalg.svd(Ma, full_matrices=False)
+    Qd = Qd[:, sv > 1e-12 * sv[0]]
+    Z = Ym - (Ym @ Pp.conj()) @ Pp.T
+    Z = Z - Qd @ (Qd.conj().T @ Z)
+    Uz, s, _ = np.linalg.svd(Z, full_matrices=False)
+    assert s[2] - s[3] > 1e-6 * s[0]
+    assert abs(ev.last_cbe_info["sigma_max"] - s[0]) < 1e-10 * s[0]
+    E = Um - Qd @ (Qd.conj().T @ Um)                     # the spawned part of U
+    Es, es, _ = np.linalg.svd(E, full_matrices=False)
+    Es = Es[:, es > 0.5]
+    assert Es.shape[1] == 3
+    ov = np.linalg.svd(Uz[:, :3].conj().T @ Es, compute_uv=False)
+    assert np.all(ov > 1 - 1e-9), ov
+
+
+def test_sector_tdvp1_cbe_matches_dense_tdvp1_cbe(g):
+    """TDVP1_CBE on a U(1) x U(1) Hubbard state (binary(2), D=8, maxdim 16)
+    against TDVP1_CBE on the dense embedding of the same tensors: the dense
+    core is block diagonal in charge, so both spawn the same directions and
+    reach one physical state; bonds grow, norm and energy are conserved."""
+    from graft_amd import backend as B, contractions as C, evolution as EV, networks as N
+    from graft_amd import operators as O, sectors as S, trees as Tr
+    from graft_amd.tensors import DTensor
+    topo = Tr.binary_topology(2)
+    t = _tree(topo)
+    _say("sector TDVP1_CBE vs dense TDVP1_CBE, binary(2) hubbard, D=8")
+    psi, _ = S.random_hubbard_ttns(np.random.default_rng(22), topo, 8, sigma=1.6)
+    Hs = O.hubbard_sector_ttno(topo, 1.0, 4.0)
+    Wd = O.hubbard_host(topo, 1.0, 4.0)
+    Hd = N.TTNO(topo, [DTensor.from_numpy(w, w.ndim - 2) for w in Wd], ishermitian=True)
+    psi_d = N.TTNS(topo, [DTensor.from_numpy(a, a.ndim - 1) for a in _dense_state(psi)], psi.center)
+    E0 = C.expect(psi, Hs, C.EnvCache(topo)).real
+    mk = lambda: EV.TDVP1_CBE(order=2, trunc=B.TruncationScheme(maxdim=16),
+                              cbe=EV.PredictorCBE(max_add=4, neigs=2))
+    evs, evd = mk(), mk()
+    b0 = sum(psi.bond_dims())
+    for _ in range(2):
+        EV.step_(evs, psi, Hs, -0.05j)
+        EV.step_(evd, psi_d, Hd, -0.05j)
+    assert sum(psi.bond_dims()) > b0
+    assert psi.bond_dims() == psi_d.bond_dims()
+    vs, vd = _full_vector(t, psi), _full_vector(t, psi_d)
+    assert 1 - abs(np.vdot(vs, vd)) < 1e-9
+    assert abs(np.linalg.norm(vs) - 1) < 1e-10
+    E1 = C.expect(psi, Hs, C.EnvCache(topo)).real
+    assert abs(E1 - E0) < 1e-9 * abs(E0), (E0, E1)
