@@ -94,10 +94,74 @@ def cpu_baseline(D_cpu, depth=5, seed=7, budget_s=20.0):
         a = time.perf_counter()
         ref.execute_pair_steps(steps, ops, plan.nslots, plan.output_slot)
         times.append(time.perf_counter() - a)
-        if time.perf_counter() - t0 > budget_s or len(times) >= 5:
+        if time.perf_counter() - t0 >= budget_s or len(times) >= 5:
             break
     sec = float(np.median(times))
     return dict(seconds_per_apply=sec, flops=8.0 * plan.flops, D=D_cpu, samples=len(times))
+
+
+def cpu_gpu_sweep(maxdim=32, nsweeps=4, seed=1):
+    """The reference's CPU-runnable workload, measured on both sides: dmrg2 on
+    the transverse-field Ising model (J=1, h=0.9) on the star of a centre and
+    3 arms of 2 sites, random bond-2 start, maxdim 32, ``nsweeps`` sweeps, on
+    the host (the oracle's sweep: the same pairwise maps in numpy, Lanczos to
+    1e-12, LAPACK SVD) and on the GPU (dmrg2_), against exact diagonalization."""
+    import torch
+    from oracle import heff_oracle as ref
+    import __graft_entry__
+    __graft_entry__.load_package()
+    from graft_amd import backend as B, groundstate as GS, networks as N, operators as O, trees as Tr
+    topo = Tr.star_topology(3, 2)
+    t = ref.Tree(list(topo.ids), list(topo.parent), [list(c) for c in topo.children])
+    host, _ = N.random_ttns_host(np.random.default_rng(seed), topo, 2, 2)
+    W = O.tfi_host(topo, 1.0, 0.9)
+    E0 = float(np.linalg.eigvalsh(ref.dense_hamiltonian(t.n, ref.tfi_terms(t, 1.0, 0.9)))[0])
+    t0 = time.perf_counter()
+    _, _, Ec = ref.dmrg2_sweeps(t, host, W, topo.root, maxdim, nsweeps=nsweeps)
+    cpu_s = time.perf_counter() - t0
+    psi = N.ttns_from_host(topo, host, topo.root)
+    H = O.tfi_ttno(topo, 1.0, 0.9)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, Eg = GS.dmrg2_(psi, H, trunc=B.TruncationScheme(maxdim=maxdim), nsweeps=nsweeps, tol=0.0)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    return {"workload": f"dmrg2, TFI J=1 h=0.9 on star 3x2 (7 sites), bond-2 start, maxdim {maxdim}, "
+                        f"{nsweeps} sweeps",
+            "cpu_s": cpu_s, "gpu_s": gpu_s, "cpu_energy_err": abs(Ec[-1] - E0),
+            "gpu_energy_err": abs(Eg[-1] - E0), "cpu_threads": blas_threads()}
+
+
+def cpu_protocol(h2, x, D_cpu, D_one, flops_apply, samples=5):
+    """The CPU baseline protocol: one GPU apply and one host apply of the same
+    pairwise plan alternate (median of ``samples`` each), with all host cores at
+    D_cpu and, separately, one core at D_one; host rates are FLOP-normalized to
+    the GPU workload's apply."""
+    import torch
+    from threadpoolctl import threadpool_limits
+    gpu_ms, cpu = [], {}
+    y = h2(x)
+    for label, D, threads in (("all_cores", D_cpu, None), ("one_core", D_one, 1)):
+        times = []
+        for i in range(samples):
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            y = h2(x, out=y)
+            torch.cuda.synchronize()
+            gpu_ms.append((time.perf_counter() - a) * 1e3)
+            if threads is None:
+                c = cpu_baseline(D, budget_s=0.0)
+            else:
+                with threadpool_limits(limits=threads, user_api="blas"):
+                    c = cpu_baseline(D, budget_s=0.0)
+            times.append(c["seconds_per_apply"])
+        sec = float(np.median(times))
+        rate = c["flops"] / sec
+        cpu[label] = {"D": D, "threads": threads or blas_threads(), "seconds_per_apply": sec,
+                      "gflops": rate / 1e9, "applies_per_s_normalized": rate / flops_apply,
+                      "samples": samples}
+    cpu["gpu_ms_interleaved_median"] = float(np.median(gpu_ms))
+    return cpu
 
 
 def precision_lockstep(D, nsteps, dt=0.05, seed=3, arms=3, armlen=10,
@@ -516,6 +580,12 @@ def main():
     ap.add_argument("--D", type=int, default=256)
     ap.add_argument("--cpu-D", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-one-D", type=int, default=64,
+                    help="bond dimension of the one-core host apply")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="apply mode: skip the sweep figures (Hubbard sector vs dense, host vs GPU)")
+    ap.add_argument("--sweep-hub-D", type=int, default=128,
+                    help="apply mode: maxdim of the Hubbard sector vs dense FP64 sweep")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the heuristic tile configurations (default: autotune every step)")
     ap.add_argument("--mode", choices=["apply", "sweep", "hubbard", "hubbard-apply", "hubbard-tdvp",
@@ -653,16 +723,21 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        c = cpu_baseline(args.cpu_D)
-        cpu_rate = c["flops"] / c["seconds_per_apply"]
-        cpu = {"value": cpu_rate / flops_apply, "unit": "applies/s", "cores": blas_threads(),
-               "kind": "port",
+        t_cpu = time.perf_counter()
+        proto = cpu_protocol(h2, x, args.cpu_D, args.cpu_one_D, flops_apply)
+        allc = proto["all_cores"]
+        cpu = {"value": allc["applies_per_s_normalized"], "unit": "applies/s",
+               "cores": allc["threads"], "kind": "port",
                "sample": (f"oracle (numpy/OpenBLAS zgemm, same pairwise plan) eff_h2 at the root edge "
-                          f"with D={c['D']}: {c['seconds_per_apply']:.3f} s/apply "
-                          f"({cpu_rate / 1e9:.1f} GFLOP/s), FLOP-normalized to the D={args.D} apply; "
-                          f"median of {c['samples']}; host {platform.processor() or platform.machine()}, "
-                          f"{os.cpu_count()} CPUs")}
-
+                          f"with D={allc['D']}: {allc['seconds_per_apply']:.3f} s/apply "
+                          f"({allc['gflops']:.1f} GFLOP/s), FLOP-normalized to the D={args.D} apply; "
+                          f"median of {allc['samples']}, alternating with GPU applies; one core at "
+                          f"D={proto['one_core']['D']}: {proto['one_core']['gflops']:.1f} GFLOP/s; "
+                          f"host {platform.processor() or platform.machine()}, {os.cpu_count()} CPUs"),
+               "protocol": proto}
+        if not args.no_sweep:
+            cpu["sweep"] = cpu_gpu_sweep()
+        print(f"# cpu baseline {time.perf_counter() - t_cpu:.1f} s", file=sys.stderr, flush=True)
     out = {
         "metric": "H_eff applies/s + % MFMA peak; DMRG sweep s at fixed D vs plain FP64 GPU & CPU",
         "value": applies_per_s,
@@ -699,6 +774,21 @@ def main():
                                f"n={info[dom]['n']})"},
         "cpu_baseline": cpu,
     }
+    sweep = None
+    if not args.no_sweep:
+        t_sw = time.perf_counter()
+        hubbard_bench(8, 1)        # first use of the factorization libraries, outside the figure
+        r = hubbard_bench(args.sweep_hub_D, 1)
+        sweep = {"workload": f"one dmrg2_ sweep, Hubbard t=1 U=4 half filling, 16-site tree, "
+                             f"maxdim {r['D']}",
+                 "sector_batched_fp64_s": r["sector_sweep_s"][-1],
+                 "plain_dense_fp64_s": r["dense_sweep_s"][-1], "speedup": r["speedup"][-1],
+                 "sector_energy": r["sector_energies"][-1],
+                 "sector_phases": r["sector_phases"][-1],
+                 "apply_ms_sector": r["apply_sector"]["ms"], "apply_ms_dense": r["apply_dense"]["ms"]}
+        print(f"# sweeps {time.perf_counter() - t_sw:.1f} s", file=sys.stderr, flush=True)
+
+    out["sweep"] = sweep
     print(json.dumps(out), flush=True)
 
 

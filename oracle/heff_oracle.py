@@ -260,6 +260,9 @@ def tfi_chain_exact_energy(n, J, g):
 _LETTERS = "abcdefghijklmnopqrstuvwxyzABCDEFGHIJKLMNOPQRSTUVWXYZ"
 
 
+_PATHS = {}      # contraction order per network and operand shapes
+
+
 def ncon(tensors, labels, conjs=None):
     """Contract a labelled network pairwise with numpy.tensordot along the
     order numpy.einsum_path picks (every label joins at most two tensors, so
@@ -273,7 +276,10 @@ def ncon(tensors, labels, conjs=None):
     ops = [np.conj(t) if c else np.asarray(t) for t, c in zip(tensors, conjs)]
     labs = [list(l) for l in labels]
     if len(ops) > 1:
-        path = np.einsum_path(f"{subs}->{out}", *ops, optimize="greedy")[0][1:]
+        pkey = (subs, out, tuple(o.shape for o in ops))
+        path = _PATHS.get(pkey)
+        if path is None:
+            path = _PATHS[pkey] = np.einsum_path(f"{subs}->{out}", *ops, optimize="greedy")[0][1:]
         for group in path:
             idx = sorted(group, reverse=True)
             picked = [(ops.pop(i), labs.pop(i)) for i in idx][::-1]
@@ -982,3 +988,126 @@ def tdvp1_step(t, psi, center, H, dz, order=2, phys=None):
         psi, center = _tdvp1_sweep(t, psi, center, H, dz / 2, phys, rev=False, terminal_dz=dz)
         return _tdvp1_sweep(t, psi, center, H, dz / 2, phys, rev=True, skip_initial_site=True)
     raise ValueError("order must be 1 or 2")
+
+
+# ---------------------------------------------------------------------------
+# Two-site DMRG on the host — GroundState.jl:102-143 (the CPU sweep baseline).
+# Local problems by restarted Lanczos on the oracle's h2 map (krylovdim 20,
+# absolute residual 1e-12); environments cached per directed edge and dropped when a
+# tensor on their side changes (envcache.jl:812-844); truncated SVD split.
+# ---------------------------------------------------------------------------
+
+
+def _side(t, u, v):
+    """Nodes on u's side of edge (u, v)."""
+    acc, stack = set(), [(u, v)]
+    while stack:
+        i, frm = stack.pop()
+        acc.add(i)
+        for w in neighbors(t, i):
+            if w != frm:
+                stack.append((w, i))
+    return acc
+
+
+def _env(t, psi, W, envs, u, v):
+    if (u, v) not in envs:
+        for w in neighbors(t, u):
+            if w != v:
+                _env(t, psi, W, envs, w, u)
+        envs[(u, v)] = build_env(t, psi, W, psi, u, v, envs)
+    return envs[(u, v)]
+
+
+def lanczos_lowest(mv, x0, krylovdim=20, tol=1e-12, maxiter=100):
+    """Lowest eigenpair of a hermitian map by restarted Lanczos: full
+    reorthogonalization, restart from the Ritz vector, until the Ritz residual
+    |beta * last component| <= tol (absolute, as KrylovKit compares it)."""
+    v = x0 / np.linalg.norm(x0)
+    theta = 0.0
+    for _ in range(maxiter):
+        V = [v]
+        T = np.zeros((krylovdim + 1, krylovdim), dtype=complex)
+        k = 0
+        for j in range(krylovdim):
+            w = mv(V[j])
+            for _ in range(2):
+                c = np.array([np.vdot(u, w) for u in V])
+                w = w - sum(ci * u for ci, u in zip(c, V))
+                T[: len(V), j] += c
+            beta = np.linalg.norm(w)
+            T[j + 1, j] = beta
+            k = j + 1
+            if beta < 1e-14 * max(1.0, np.abs(T[: k, : k]).max()):
+                break
+            V.append(w / beta)
+        Tk = T[:k, :k]
+        lam, U = np.linalg.eigh((Tk + Tk.conj().T) / 2)
+        theta, y = float(lam[0]), U[:, 0]
+        v = sum(yi * u for yi, u in zip(y, V[:k]))
+        v = v / np.linalg.norm(v)
+        if abs(T[k, k - 1] * y[-1]) <= tol or k < krylovdim:
+            break
+    return theta, v
+
+
+def dmrg2_sweeps(t, psi, W, center, maxdim, nsweeps=1, krylovdim=20, tol=1e-12):
+    """dmrg2! sweeps (post-order bonds, then reversed) with maxdim truncation.
+    Returns (tensors, centre, energies per sweep)."""
+    psi = list(psi)
+    sides = {}
+    envs = {}
+
+    def drop(nodes):
+        for key in [k for k in envs if nodes & sides.setdefault(k, _side(t, *k))]:
+            del envs[key]
+
+    bonds = [n for n in postorder(t) if t.parent[n] >= 0]
+    schedule = [(n, "m") for n in bonds] + [(n, "n") for n in reversed(bonds)]
+    energies = []
+    for _ in range(nsweeps):
+        E = float("nan")
+        for n, center_on in schedule:
+            m = t.parent[n]
+            moved = set(path_between(t, center, n))
+            psi = move_center(t, psi, center, n)
+            center = n
+            drop(moved)
+            for w in neighbors(t, n):
+                if w != m:
+                    _env(t, psi, W, envs, w, n)
+            for w in neighbors(t, m):
+                if w != n:
+                    _env(t, psi, W, envs, w, m)
+            theta = two_site_tensor(t, psi, n, m)
+            shape = theta.shape
+            size = theta.size
+
+            def mv(x):
+                y = h2_apply(t, W, envs, n, m, np.asarray(x).reshape(shape, order="F"))
+                return y.reshape(-1, order="F")
+
+            if size <= krylovdim + 1:
+                Hm = np.stack([mv(e) for e in np.eye(size, dtype=complex)], axis=1)
+                w_, v_ = np.linalg.eigh((Hm + Hm.conj().T) / 2)
+                E, vec = float(w_[0]), v_[:, 0]
+            else:
+                E, vec = lanczos_lowest(mv, theta.reshape(-1, order="F"), krylovdim, tol)
+            theta = vec.reshape(shape, order="F")
+            pn = psi[n].ndim - 1
+            M = theta.reshape(int(np.prod(shape[:pn])), -1, order="F")
+            U, S, Vh = np.linalg.svd(M, full_matrices=False)
+            r = max(1, min(maxdim, int(np.sum(S > 0))))
+            U, S, Vh = U[:, :r], S[:r], Vh[:r]
+            if center_on == "n":
+                U = U * S[None, :]
+            else:
+                Vh = S[:, None] * Vh
+            psi[n] = U.reshape(shape[:pn] + (r,), order="F")
+            k = childslot(t, m, n)
+            Bm = Vh.reshape((r,) + shape[pn:], order="F")
+            psi[m] = np.moveaxis(Bm, 0, k)
+            center = n if center_on == "n" else m
+            drop({n, m})
+        energies.append(E)
+    return psi, center, energies

@@ -405,3 +405,14 @@ def test_oracle_energy_gradient_against_dense_and_finite_differences():
         an = np.real(np.sum((Wi[i] - W0[i]) * Rs[i])) / Z.real
         Hi = ref.dense_hamiltonian(t.n, [(-1.0, [(i, ref.SX)])])
         assert abs(an - np.vdot(v, Hi @ v).real / nv) < 1e-12, i
+
+
+def test_oracle_dmrg2_reaches_exact_ground_energy():
+    """The host dmrg2 sweeps (the CPU sweep baseline) against exact
+    diagonalization: TFI on the 3x2 star, maxdim 32 (runtests.jl:687-700)."""
+    t = ref.star_tree(3, 2)
+    psi = ref.random_ttns(np.random.default_rng(1), t, 2, 2)
+    W = O.tfi_host(Tr.star_topology(3, 2), 1.0, 0.9)
+    E0 = np.linalg.eigvalsh(ref.dense_hamiltonian(t.n, ref.tfi_terms(t, 1.0, 0.9)))[0]
+    _, _, Es = ref.dmrg2_sweeps(t, psi, W, 0, 32, nsweeps=4)
+    assert abs(Es[-1] - E0) < 1e-10, (Es, E0)
