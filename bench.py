@@ -545,6 +545,19 @@ def hubbard_apply_bench(D, reps=20, sigma=1.6, seed=5):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
+    # the tile autotuner on this plan (every configuration and orientation per step)
+    torch.cuda.synchronize()
+    t_at = time.perf_counter()
+    h2.autotune(x)
+    torch.cuda.synchronize()
+    t_at = time.perf_counter() - t_at
+    step_ms_tuned = h2.native.execute_timed(h2.operands(x), y)
+    e0.record()
+    for _ in range(reps):
+        y = h2(x, out=y)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_tuned = e0.elapsed_time(e1) / reps
     # two Lanczos cycles of 20 applies: device driver (expansions replayed as
     # HIP graphs) against the per-call driver, wall time per apply
     from graft_amd import krylov as K
@@ -557,8 +570,10 @@ def hubbard_apply_bench(D, reps=20, sigma=1.6, seed=5):
         lanczos["device_graph" if device else "per_call"] = {
             "ms_per_apply": (time.perf_counter() - t) * 1e3 / info.numops, "applies": info.numops,
             "ritz_value": float(vals[0])}
-    return {"D": D, "ms_per_apply": ms, "flops": h2.flops, "tflops": h2.flops / ms / 1e9,
-            "step_ms": step_ms, "steps": h2.native.step_info(), "theta_elements": int(x.numel),
+    return {"D": D, "ms_per_apply": ms_tuned, "ms_per_apply_untuned": ms, "autotune_s": t_at,
+            "flops": h2.flops, "tflops": h2.flops / ms_tuned / 1e9,
+            "step_ms": step_ms_tuned, "step_ms_untuned": step_ms, "steps": h2.native.step_info(),
+            "theta_elements": int(x.numel),
             "blocks": len(x.blocks.blocks) if hasattr(x.blocks, "blocks") else None,
             "lanczos": lanczos}
 

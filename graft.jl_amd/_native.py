@@ -118,6 +118,7 @@ def lib():
         "graft_graph_capture_begin": [vp],
         "graft_ctx_reserve_workspace": [vp, ctypes.c_uint64],
         "graft_ctx_set_sector_fusion": [vp, i32],
+        "graft_ctx_set_fused_map_loads": [vp, i32],
         "graft_graph_capture_end": [vp, ctypes.POINTER(vp)],
         "graft_graph_launch": [vp, vp],
         "graft_graph_destroy": [vp],
@@ -223,6 +224,11 @@ class Context:
     def set_sector_fusion(self, mode):
         """0 per-block GEMM items, 1 automatic fused per-charge matrices, 2 always fused."""
         check(lib().graft_ctx_set_sector_fusion(self.handle, int(mode)))
+
+    def set_fused_map_loads(self, on):
+        """1 (default): fused FP64 sector steps read and write the blocks through the
+        element maps inside the GEMM; 0: gather, GEMM, scatter."""
+        check(lib().graft_ctx_set_fused_map_loads(self.handle, int(bool(on))))
 
     def reserve_workspace(self, nbytes):
         check(lib().graft_ctx_reserve_workspace(self.handle, int(nbytes)))

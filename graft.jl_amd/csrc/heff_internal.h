@@ -46,18 +46,29 @@ struct TileConfig {
   int bm, bn, bk, wm, wn;
   int m3;  // three real products per complex product (Karatsuba form)
 };
-constexpr int kNumTileConfigs = 10;
+constexpr int kNumTileConfigs = 11;
 extern const TileConfig kTileConfigs[kNumTileConfigs];
+
+// Element maps of a fused sector step read / written inside the GEMM: entry
+// i of a fused matrix lives at blocked element a[i] / b[i] / c[i]
+// (kMapZero: absent — zero when read, skipped when stored).
+struct FusedMaps {
+  const uint32_t* a;
+  const uint32_t* b;
+  const uint32_t* c;
+};
 
 // Launch one grouped strided complex GEMM (precision GRAFT_PREC_*).
 // ksplit > 1 (single-block items only) splits K over ksplit blocks per
 // tile; partial must hold ksplit * M * N complex values.  tr selects the
 // transposed MFMA orientation (coalesced stores when C's unit-stride leg is
-// an m leg).
+// an m leg).  maps (FP64, ksplit 1 only): the item/segment offsets address
+// fused matrices and A, B, C are the blocked tensors behind the maps.
 hipError_t launch_zgemm(int config, int precision, const void* A, const void* B, void* C,
                         const GemmItem* items, int nitems, const GemmSeg* segs,
                         const uint32_t* koff, int total_tiles, int conjA, int conjB,
-                        int ksplit, void* partial, bool tr, hipStream_t stream);
+                        int ksplit, void* partial, bool tr, hipStream_t stream,
+                        const FusedMaps* maps = nullptr);
 int tiles_for(int config, int M, int N);
 // FP32 / BF16x3 products with the same addressing (zgemm_lowp_kernels.hip).
 hipError_t launch_zgemm_lowp(int config, int precision, const void* A, const void* B, void* C,

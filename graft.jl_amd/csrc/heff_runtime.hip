@@ -242,6 +242,8 @@ struct StepExec {
   uint32_t* d_ma = nullptr;              // per-element fused offsets (built once per plan)
   uint32_t* d_mb = nullptr;
   uint32_t* d_mc = nullptr;
+  uint32_t* d_mci = nullptr;             // fused C offset -> blocked element (mapped stores)
+  bool c_complete = false;               // every blocked C element has a fused entry
   uint64_t fused_bytes = 0;
 };
 
@@ -280,6 +282,7 @@ struct graft_ctx {
   double* d_comb = nullptr;      // block-combine coefficients
   bool capturing = false;        // a graph capture is open on the stream
   int sector_fuse = 1;  // 0 per-block GEMM, 1 automatic, 2 fused / 3 per-block streaming for every multi-block step
+  int fused_map_loads = 1;  // fused FP64 MFMA steps read / write the blocks through the maps
   uint64_t ws_generation = 0;    // bumped whenever the workspace moves
 };
 
@@ -646,6 +649,8 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
     }
     ex.pc.push_back(pbk);
   }
+  ex.c_complete = std::all_of(ex.pc.begin(), ex.pc.end(),
+                              [](const PermBlock& b) { return b.other >= 0; });
   ex.fa = fa;
   ex.fb = fb;
   ex.fc = fc;
@@ -1316,7 +1321,7 @@ static int build_maps(graft_ctx* ctx, graft_plan_impl& p) {
   auto al = [](int64_t n) { return ((size_t)n * sizeof(uint32_t) + 255) & ~size_t(255); };
   for (auto& ex : p.steps)
     if (ex.fused) {
-      total += al(ex.fa) + al(ex.fb) + al(ex.totC);
+      total += al(ex.fa) + al(ex.fb) + al(ex.totC) + (ex.c_complete ? al(ex.fc) : 0);
       tmp_n = std::max(tmp_n, std::max(ex.totA, ex.totB));
     }
   if (total == 0) return GRAFT_OK;
@@ -1334,6 +1339,11 @@ static int build_maps(graft_ctx* ctx, graft_plan_impl& p) {
     base += al(ex.fb);
     ex.d_mc = (uint32_t*)base;
     base += al(ex.totC);
+    ex.d_mci = nullptr;
+    if (ex.c_complete) {
+      ex.d_mci = (uint32_t*)base;
+      base += al(ex.fc);
+    }
     // (an operand without blocks has no forward map: its inverse is all zero)
     const int64_t nA = ex.pa.empty() ? 0 : ex.totA, nB = ex.pb.empty() ? 0 : ex.totB;
     e = launch_perm_map(ex.d_pa, (int)ex.pa.size(), ex.d_ca, nA, tmp, ctx->stream);
@@ -1343,6 +1353,8 @@ static int build_maps(graft_ctx* ctx, graft_plan_impl& p) {
     if (e == hipSuccess) e = launch_invert_map(tmp, nB, ex.d_mb, ex.fb, ctx->stream);
     if (e == hipSuccess)
       e = launch_perm_map(ex.d_pc, (int)ex.pc.size(), ex.d_cc, ex.totC, ex.d_mc, ctx->stream);
+    if (e == hipSuccess && ex.d_mci)
+      e = launch_invert_map(ex.d_mc, ex.totC, ex.d_mci, ex.fc, ctx->stream);
   }
   // the maps read the block tables, which a later autotune may re-upload
   const hipError_t es = hipStreamSynchronize(ctx->stream);
@@ -1399,6 +1411,26 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
   }
   void* partial = ex.ksplit > 1 ? ws + p.ws_partial : nullptr;
   void* Cdst = C;
+  if (ex.fused && ctx->fused_map_loads && precision == GRAFT_PREC_FP64 && ex.d_mci != nullptr &&
+      ex.config < kNumTileConfigs && ex.ksplit == 1) {
+    // The GEMM writes the result blocks through the element map (no scatter
+    // pass).  An operand the tiles read at most twice is also read through
+    // its map; one read more often is gathered once into its fused matrix.
+    const TileConfig& tc = kTileConfigs[ex.config];
+    const bool ma = (ex.maxN + tc.bn - 1) / tc.bn <= 2;
+    const bool mb = (ex.maxM + tc.bm - 1) / tc.bm <= 2;
+    auto al = [](int64_t x) { return (size_t)((x * 16 + 255) / 256) * 256; };
+    char* fb = ws + p.ws_fused;
+    void* Af = fb;
+    void* Bf = fb + al(ex.fa);
+    if (!ma) GRAFT_CHECK_HIP(launch_map_gather(A, Af, ex.d_ma, ex.fa, ctx->stream));
+    if (!mb) GRAFT_CHECK_HIP(launch_map_gather(B, Bf, ex.d_mb, ex.fb, ctx->stream));
+    const FusedMaps maps{ma ? ex.d_ma : nullptr, mb ? ex.d_mb : nullptr, ex.d_mci};
+    GRAFT_CHECK_HIP(launch_zgemm(ex.config, precision, ma ? A : Af, mb ? B : Bf, C, ex.d_items,
+                                 (int)ex.items.size(), ex.d_segs, ex.d_koff, ex.total_tiles,
+                                 ex.conjA, ex.conjB, 1, nullptr, ex.tr, ctx->stream, &maps));
+    return GRAFT_OK;
+  }
   if (ex.fused) {
     // gather both operands into the per-charge matrices, multiply, scatter below
     auto al = [](int64_t x) { return (size_t)((x * 16 + 255) / 256) * 256; };
@@ -1557,6 +1589,12 @@ int graft_ctx_set_sector_fusion(graft_ctx* ctx, int32_t mode) {
   GRAFT_REQUIRE(ctx != nullptr, "null context");
   GRAFT_REQUIRE(mode >= 0 && mode <= 3, "sector fusion mode must be 0, 1, 2 or 3");
   ctx->sector_fuse = mode;
+  return GRAFT_OK;
+}
+
+int graft_ctx_set_fused_map_loads(graft_ctx* ctx, int32_t on) {
+  GRAFT_REQUIRE(ctx != nullptr, "null context");
+  ctx->fused_map_loads = on ? 1 : 0;
   return GRAFT_OK;
 }
 

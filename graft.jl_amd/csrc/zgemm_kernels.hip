@@ -34,6 +34,7 @@ const TileConfig kTileConfigs[kNumTileConfigs] = {
     {64, 64, 16, 32, 32, 1},    // 7: three-product form
     {128, 64, 16, 64, 32, 1},   // 8: three-product form
     {64, 128, 16, 32, 64, 1},   // 9: three-product form
+    {64, 256, 8, 64, 64, 0},    // 10: n up to 256 in one tile (each A panel read once)
 };
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -60,13 +61,20 @@ __device__ __forceinline__ uint32_t decode_offset(uint32_t idx, int rank, const 
 // Cr = T1 - T2, Ci = T3 - T1 - T2 — three real MFMAs per four complex k and
 // 16x16 block instead of four (Karatsuba / Gauss).  In FP64 its rounding
 // error stays O(eps (|Ar| + |Ai|)(|Br| + |Bi|)) per term.
-template <int BM, int BN, int BK, int WM, int WN, bool TR, bool M3>
+//
+// MAP (fused sector steps): the item/segment offsets address per-charge fused
+// matrices.  An operand whose map is given is never formed: maps.a / maps.b
+// give the blocked element of every fused A / B entry (kMapZero: no block,
+// the entry is zero) and maps.c the blocked element each fused C entry is
+// stored to (kMapZero: not stored); an operand without a map is read from
+// its gathered fused matrix as usual.
+template <int BM, int BN, int BK, int WM, int WN, bool TR, bool M3, bool MAP>
 __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
     zgemm_f64_kernel(const double2* __restrict__ Abase, const double2* __restrict__ Bbase,
                      double2* __restrict__ Cbase, const GemmItem* __restrict__ items, int nitems,
                      const GemmSeg* __restrict__ segs, const uint32_t* __restrict__ koff,
                      int total_tiles, int conjA, int conjB, int ksplit,
-                     double2* __restrict__ partial) {
+                     double2* __restrict__ partial, FusedMaps maps) {
   constexpr int NWN = BN / WN;
   constexpr int NT = 64 * (BM / WM) * NWN;
   constexpr int EA = BM * BK / NT;
@@ -135,8 +143,29 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
   const int seg_end = it->seg_begin + it->seg_count;
   for (int s = it->seg_begin; s < seg_end; ++s) {
     const GemmSeg* sg = segs + s;
-    const double2* A = Abase + sg->a_off;
-    const double2* B = Bbase + sg->b_off;
+    const bool mapa = MAP && maps.a != nullptr, mapb = MAP && maps.b != nullptr;
+    const double2* A = Abase + (mapa ? 0 : sg->a_off);
+    const double2* B = Bbase + (mapb ? 0 : sg->b_off);
+    const uint32_t* mA = mapa ? maps.a + sg->a_off : nullptr;
+    const uint32_t* mB = mapb ? maps.b + sg->b_off : nullptr;
+    auto ldA = [&](uint64_t i) -> double2 {
+      if constexpr (MAP) {
+        if (mapa) {
+          const uint32_t e = mA[i];
+          return e < kMapZero ? A[e] : double2{0.0, 0.0};
+        }
+      }
+      return A[i];
+    };
+    auto ldB = [&](uint64_t i) -> double2 {
+      if constexpr (MAP) {
+        if (mapb) {
+          const uint32_t e = mB[i];
+          return e < kMapZero ? B[e] : double2{0.0, 0.0};
+        }
+      }
+      return B[i];
+    };
     const int K = sg->K;
     const uint32_t* kA = koff + sg->koff_idx;
     const uint32_t* kB = kA + K;
@@ -182,12 +211,12 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
         const uint32_t ko = (q < kend) ? kA[q] : 0u;
 #pragma unroll
         for (int e = 0; e < EA; ++e)
-          ra[e] = (q < kend && rowA[e] != kBadRow) ? A[(uint64_t)rowA[e] + ko] : double2{0.0, 0.0};
+          ra[e] = (q < kend && rowA[e] != kBadRow) ? ldA((uint64_t)rowA[e] + ko) : double2{0.0, 0.0};
       } else {
 #pragma unroll
         for (int e = 0; e < EA; ++e) {
           const int q = k0 + tid / BM + e * (NT / BM);
-          ra[e] = (q < kend && rowA[0] != kBadRow) ? A[(uint64_t)rowA[0] + kA[q]] : double2{0.0, 0.0};
+          ra[e] = (q < kend && rowA[0] != kBadRow) ? ldA((uint64_t)rowA[0] + kA[q]) : double2{0.0, 0.0};
         }
       }
       if (bkf) {
@@ -195,12 +224,12 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
         const uint32_t ko = (q < kend) ? kB[q] : 0u;
 #pragma unroll
         for (int e = 0; e < EB; ++e)
-          rb[e] = (q < kend && colB[e] != kBadRow) ? B[(uint64_t)colB[e] + ko] : double2{0.0, 0.0};
+          rb[e] = (q < kend && colB[e] != kBadRow) ? ldB((uint64_t)colB[e] + ko) : double2{0.0, 0.0};
       } else {
 #pragma unroll
         for (int e = 0; e < EB; ++e) {
           const int q = k0 + tid / BN + e * (NT / BN);
-          rb[e] = (q < kend && colB[0] != kBadRow) ? B[(uint64_t)colB[0] + kB[q]] : double2{0.0, 0.0};
+          rb[e] = (q < kend && colB[0] != kBadRow) ? ldB((uint64_t)colB[0] + kB[q]) : double2{0.0, 0.0};
         }
       }
     };
@@ -335,7 +364,19 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
       }
     return;
   }
-  double2* C = Cbase + it->c_off;
+  const bool mapc = MAP && maps.c != nullptr;
+  double2* C = Cbase + (mapc ? 0 : it->c_off);
+  const uint32_t* mC = mapc ? maps.c + it->c_off : nullptr;
+  auto stC = [&](uint64_t i, double2 v) {
+    if constexpr (MAP) {
+      if (mapc) {
+        const uint32_t e = mC[i];
+        if (e < kMapZero) C[e] = v;
+        return;
+      }
+    }
+    C[i] = v;
+  };
   if constexpr (TR) {
     uint32_t rowm[FM];
 #pragma unroll
@@ -353,7 +394,7 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
 #pragma unroll
         for (int a = 0; a < FM; ++a) {
           if (rowm[a] == kBadRow) continue;
-          C[(uint64_t)rowm[a] + co] = cval(a, b, r);
+          stC((uint64_t)rowm[a] + co, cval(a, b, r));
         }
       }
   } else {
@@ -373,7 +414,7 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
 #pragma unroll
         for (int b = 0; b < FN; ++b) {
           if (coln[b] == kBadRow) continue;
-          C[(uint64_t)ro + coln[b]] = cval(a, b, r);
+          stC((uint64_t)ro + coln[b], cval(a, b, r));
         }
       }
   }
@@ -384,29 +425,46 @@ int tiles_for(int config, int M, int N) {
   return ((M + c.bm - 1) / c.bm) * ((N + c.bn - 1) / c.bn);
 }
 
+template <int BM, int BN, int BK, int WM, int WN, bool M3, bool MAP>
+static hipError_t launch_f64_map(const void* A, const void* B, void* C, const GemmItem* items,
+                                 int nitems, const GemmSeg* segs, const uint32_t* koff,
+                                 int total_tiles, int conjA, int conjB, int ksplit, void* partial,
+                                 bool tr, FusedMaps maps, hipStream_t stream) {
+  constexpr int NT = 64 * (BM / WM) * (BN / WN);
+  if (tr)
+    hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, BK, WM, WN, true, M3, MAP>),
+                       dim3(total_tiles * ksplit), dim3(NT), 0, stream, (const double2*)A,
+                       (const double2*)B, (double2*)C, items, nitems, segs, koff, total_tiles,
+                       conjA, conjB, ksplit, (double2*)partial, maps);
+  else
+    hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, BK, WM, WN, false, M3, MAP>),
+                       dim3(total_tiles * ksplit), dim3(NT), 0, stream, (const double2*)A,
+                       (const double2*)B, (double2*)C, items, nitems, segs, koff, total_tiles,
+                       conjA, conjB, ksplit, (double2*)partial, maps);
+  return hipGetLastError();
+}
+
 template <int BM, int BN, int BK, int WM, int WN, bool M3 = false>
 static hipError_t launch_f64(const void* A, const void* B, void* C, const GemmItem* items,
                              int nitems, const GemmSeg* segs, const uint32_t* koff,
                              int total_tiles, int conjA, int conjB, int ksplit, void* partial,
-                             bool tr, hipStream_t stream) {
-  constexpr int NT = 64 * (BM / WM) * (BN / WN);
-  if (tr)
-    hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, BK, WM, WN, true, M3>), dim3(total_tiles * ksplit),
-                       dim3(NT), 0, stream, (const double2*)A, (const double2*)B, (double2*)C,
-                       items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit,
-                       (double2*)partial);
-  else
-    hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, BK, WM, WN, false, M3>), dim3(total_tiles * ksplit),
-                       dim3(NT), 0, stream, (const double2*)A, (const double2*)B, (double2*)C,
-                       items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit,
-                       (double2*)partial);
-  return hipGetLastError();
+                             bool tr, const FusedMaps* maps, hipStream_t stream) {
+  if (maps) {
+    if (ksplit != 1) return hipErrorInvalidValue;   // mapped stores need whole tiles
+    return launch_f64_map<BM, BN, BK, WM, WN, M3, true>(A, B, C, items, nitems, segs, koff,
+                                                        total_tiles, conjA, conjB, 1, nullptr, tr,
+                                                        *maps, stream);
+  }
+  return launch_f64_map<BM, BN, BK, WM, WN, M3, false>(A, B, C, items, nitems, segs, koff,
+                                                       total_tiles, conjA, conjB, ksplit, partial,
+                                                       tr, FusedMaps{}, stream);
 }
 
 hipError_t launch_zgemm_tiles(int config, const void* A, const void* B, void* C,
                               const GemmItem* items, int nitems, const GemmSeg* segs,
                               const uint32_t* koff, int total_tiles, int conjA, int conjB,
-                              int ksplit, void* partial, bool tr, hipStream_t stream);
+                              int ksplit, void* partial, bool tr, const FusedMaps* maps,
+                              hipStream_t stream);
 
 // Sum the k-chunk partials of a single-block item into its output layout.
 __global__ void __launch_bounds__(256)
@@ -433,15 +491,17 @@ __global__ void __launch_bounds__(256)
 hipError_t launch_zgemm(int config, int precision, const void* A, const void* B, void* C,
                         const GemmItem* items, int nitems, const GemmSeg* segs,
                         const uint32_t* koff, int total_tiles, int conjA, int conjB,
-                        int ksplit, void* partial, bool tr, hipStream_t stream) {
+                        int ksplit, void* partial, bool tr, hipStream_t stream,
+                        const FusedMaps* maps) {
   if (total_tiles <= 0) return hipSuccess;
   if (precision != GRAFT_PREC_FP64 && precision != GRAFT_PREC_FP32 &&
       precision != GRAFT_PREC_BF16X3)
     return hipErrorNotSupported;
+  if (maps && precision != GRAFT_PREC_FP64) return hipErrorInvalidValue;
   auto tiles = [&](int ks, void* part) {
     if (precision == GRAFT_PREC_FP64)
       return launch_zgemm_tiles(config, A, B, C, items, nitems, segs, koff, total_tiles, conjA,
-                                conjB, ks, part, tr, stream);
+                                conjB, ks, part, tr, maps, stream);
     return launch_zgemm_lowp(config, precision, A, B, C, items, nitems, segs, koff, total_tiles,
                              conjA, conjB, ks, part, tr, stream);
   };
@@ -459,18 +519,20 @@ hipError_t launch_zgemm(int config, int precision, const void* A, const void* B,
 hipError_t launch_zgemm_tiles(int config, const void* A, const void* B, void* C,
                               const GemmItem* items, int nitems, const GemmSeg* segs,
                               const uint32_t* koff, int total_tiles, int conjA, int conjB,
-                              int ksplit, void* partial, bool tr, hipStream_t stream) {
+                              int ksplit, void* partial, bool tr, const FusedMaps* maps,
+                              hipStream_t stream) {
   switch (config) {
-    case 0: return launch_f64<128, 128, 16, 64, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 1: return launch_f64<128, 64, 16, 64, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 2: return launch_f64<64, 64, 16, 32, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 3: return launch_f64<256, 16, 16, 64, 16>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 4: return launch_f64<64, 128, 16, 32, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 5: return launch_f64<32, 32, 16, 16, 16>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 6: return launch_f64<16, 256, 16, 16, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 7: return launch_f64<64, 64, 16, 32, 32, true>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 8: return launch_f64<128, 64, 16, 64, 32, true>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 9: return launch_f64<64, 128, 16, 32, 64, true>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+    case 0: return launch_f64<128, 128, 16, 64, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
+    case 1: return launch_f64<128, 64, 16, 64, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
+    case 2: return launch_f64<64, 64, 16, 32, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
+    case 3: return launch_f64<256, 16, 16, 64, 16>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
+    case 4: return launch_f64<64, 128, 16, 32, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
+    case 5: return launch_f64<32, 32, 16, 16, 16>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
+    case 6: return launch_f64<16, 256, 16, 16, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
+    case 7: return launch_f64<64, 64, 16, 32, 32, true>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
+    case 8: return launch_f64<128, 64, 16, 64, 32, true>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
+    case 9: return launch_f64<64, 128, 16, 32, 64, true>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
+    case 10: return launch_f64<64, 256, 8, 64, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
     default: return hipErrorInvalidValue;
   }
 }

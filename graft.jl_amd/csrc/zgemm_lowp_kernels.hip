@@ -377,6 +377,7 @@ static hipError_t launch_lp_config(int config, const void* A, const void* B, voi
     case 2: case 7: return launch_lp<64, 64, 32, 32, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
     case 3: return launch_lp<256, 16, 64, 16, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
     case 4: case 9: return launch_lp<64, 128, 32, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+    case 10: return launch_lp<64, 256, 64, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
     case 5: return launch_lp<32, 32, 16, 16, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
     case 6: return launch_lp<16, 256, 16, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
     default: return hipErrorInvalidValue;

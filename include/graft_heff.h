@@ -125,6 +125,12 @@ int graft_ctx_reserve_workspace(graft_ctx* ctx, uint64_t bytes);
  * every multi-block step.  TensorKit multiplies per coupled sector the same
  * way (BlockSparse mul!, as in Backend.jl:166-222). */
 int graft_ctx_set_sector_fusion(graft_ctx* ctx, int32_t mode);
+/* How fused FP64 MFMA sector steps reach their operands: 1 (default) the GEMM
+ * reads the blocks and writes the result blocks through per-element maps in
+ * its tile loads and stores (one launch per step); 0 gather the per-charge
+ * matrices, multiply, scatter (three more launches).  Same results.  No
+ * reference counterpart (TensorKit multiplies block by block). */
+int graft_ctx_set_fused_map_loads(graft_ctx* ctx, int32_t on);
 int graft_synchronize(graft_ctx* ctx);
 const char* graft_last_error(void);
 const char* graft_version(void);
