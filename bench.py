@@ -26,7 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 MI355X_FP64_MFMA_TFLOPS = 78.6
-THREE_M_CONFIGS = (7, 8, 9)     # tile configurations of the three-product complex form
+THREE_M_CONFIGS = (7, 8, 9, 11, 12, 13, 14, 15, 16, 17, 18)     # tile configurations of the three-product complex form
 MI355X_HBM_GBPS = 8000.0
 # HBM bytes per launch of the dominant step, from the two counter runs
 # (FETCH_SIZE and WRITE_SIZE, scaled by a 1 GiB copy) summarized in this file.

@@ -12,7 +12,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 SOURCES = ["heff_runtime.hip", "zgemm_kernels.hip", "skinny_kernels.hip", "zgemm_lowp_kernels.hip",
-           "vector_kernels.hip", "sector_kernels.hip"]
+           "vector_kernels.hip", "sector_kernels.hip", "zgemm_db_kernels.hip"]
 OUT = os.path.join(HERE, "libgraft_heff.so")
 OBJDIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -46,7 +46,7 @@ struct TileConfig {
   int bm, bn, bk, wm, wn;
   int m3;  // three real products per complex product (Karatsuba form)
 };
-constexpr int kNumTileConfigs = 11;
+constexpr int kNumTileConfigs = 19;
 extern const TileConfig kTileConfigs[kNumTileConfigs];
 
 // Element maps of a fused sector step read / written inside the GEMM: entry
@@ -68,9 +68,17 @@ hipError_t launch_zgemm(int config, int precision, const void* A, const void* B,
                         const GemmItem* items, int nitems, const GemmSeg* segs,
                         const uint32_t* koff, int total_tiles, int conjA, int conjB,
                         int ksplit, void* partial, bool tr, hipStream_t stream,
-                        const FusedMaps* maps = nullptr);
+                        const FusedMaps* maps = nullptr, int kfast = -1);
 int tiles_for(int config, int M, int N);
 // FP32 / BF16x3 products with the same addressing (zgemm_lowp_kernels.hip).
+// Double-buffered three-product FP64 tiles (configurations 11-18), operands
+// addressed directly: no element maps, no split-K.  Configurations 15-18 run
+// the pipelined kernel when kfast >= 0 (every tile whole, no conjugation,
+// bit 0 / bit 1: every segment loads A / B along k) and the kernel of
+// configuration c - 4 otherwise.
+hipError_t launch_zgemm_db(int config, const void* A, const void* B, void* C, const GemmItem* items,
+                           int nitems, const GemmSeg* segs, const uint32_t* koff, int total_tiles,
+                           int conjA, int conjB, bool tr, int kfast, hipStream_t stream);
 hipError_t launch_zgemm_lowp(int config, int precision, const void* A, const void* B, void* C,
                              const GemmItem* items, int nitems, const GemmSeg* segs,
                              const uint32_t* koff, int total_tiles, int conjA, int conjB,

@@ -214,6 +214,7 @@ struct StepExec {
   int total_tiles = 0;
   int ksplit = 1;             // split-K factor (single-block steps with few tiles)
   bool tr = false;            // transposed MFMA orientation in use
+  std::vector<int> kfast_memo;   // per tile configuration: piped_kfast, computed on first use
   bool tr_auto = false;       // orientation from the output's unit-stride leg
   int tr_pinned = -1;         // -1 = automatic
   int skinny = 0;             // streaming kernel eligible: 1 = A is the large operand, 2 = B
@@ -332,11 +333,38 @@ static int choose_config(int64_t M, int64_t N, int skinny) {
   if (M <= 32 && N <= 32) return 5;
   // Measured on MI355X (autotuner, root-edge eff_h2 at D = 256): the 64x64
   // three-product tiles beat the 128x128 four-product ones on every
-  // MFMA-bound step, so they are the default for large blocks.
-  if (M >= 64 && N >= 64) return 7;
+  // MFMA-bound step, and the double-buffered kernel (pipelined on
+  // whole-tile steps) beats the single-buffered one in every load direction,
+  // so it is the default for large blocks.
+  if (M >= 64 && N >= 64) return 15;
   if (N <= 64 && M >= 128) return 1;
   if (M <= 64 && N >= 128) return 4;
   return 2;
+}
+
+// Load directions of a step for the pipelined tiles of configuration c:
+// bit 0 every segment loads A along k, bit 1 B; -1 unless every tile of c is
+// whole (M, N and K multiples of the tile), there is no conjugation and all
+// segments share their directions.
+static int piped_kfast(StepExec& ex, int c) {
+  if (c < 0 || c >= kNumTileConfigs || ex.conjA || ex.conjB || ex.ksplit != 1) return -1;
+  if (ex.kfast_memo.empty()) ex.kfast_memo.assign(kNumTileConfigs, -2);
+  if (ex.kfast_memo[c] != -2) return ex.kfast_memo[c];
+  const TileConfig& tc = kTileConfigs[c];
+  int kf = -1;
+  bool ok = true;
+  for (const GemmItem& it : ex.items) {
+    if (it.M % tc.bm || it.N % tc.bn) { ok = false; break; }
+    for (int s = it.seg_begin; ok && s < it.seg_begin + it.seg_count; ++s) {
+      const GemmSeg& sg = ex.segs[s];
+      const int f = (sg.a_kfast ? 1 : 0) | (sg.b_kfast ? 2 : 0);
+      if (sg.K % tc.bk || (kf >= 0 && f != kf)) ok = false;
+      kf = f;
+    }
+    if (!ok) break;
+  }
+  ex.kfast_memo[c] = ok ? kf : -1;
+  return ex.kfast_memo[c];
 }
 
 static bool canonical_blocks(const TStruct& T) {
@@ -1464,9 +1492,10 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
                                      ctx->stream));
     }
   } else {
+    const int kfast = precision == GRAFT_PREC_FP64 ? piped_kfast(ex, ex.config) : -1;
     hipError_t e = launch_zgemm(ex.config, precision, A, B, C, ex.d_items, (int)ex.items.size(),
                                 ex.d_segs, ex.d_koff, ex.total_tiles, ex.conjA, ex.conjB,
-                                ex.ksplit, partial, ex.tr, ctx->stream);
+                                ex.ksplit, partial, ex.tr, ctx->stream, nullptr, kfast);
     if (e == hipErrorNotSupported) {
       set_error("precision not supported by this build");
       return GRAFT_E_UNSUPPORTED;

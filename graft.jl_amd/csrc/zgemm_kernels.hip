@@ -35,6 +35,14 @@ const TileConfig kTileConfigs[kNumTileConfigs] = {
     {128, 64, 16, 64, 32, 1},   // 8: three-product form
     {64, 128, 16, 32, 64, 1},   // 9: three-product form
     {64, 256, 8, 64, 64, 0},    // 10: n up to 256 in one tile (each A panel read once)
+    {64, 64, 16, 32, 32, 1},    // 11: three-product form, double-buffered LDS
+    {128, 64, 16, 64, 32, 1},   // 12: three-product form, double-buffered LDS
+    {64, 128, 16, 32, 64, 1},   // 13: three-product form, double-buffered LDS
+    {128, 64, 16, 32, 32, 1},   // 14: three-product form, double-buffered LDS, 8 waves
+    {64, 64, 16, 32, 32, 1},    // 15-18: 11-14 with the pipelined kernel on whole-tile steps
+    {128, 64, 16, 64, 32, 1},   // 16
+    {64, 128, 16, 32, 64, 1},   // 17
+    {128, 64, 16, 32, 32, 1},   // 18
 };
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -464,7 +472,7 @@ hipError_t launch_zgemm_tiles(int config, const void* A, const void* B, void* C,
                               const GemmItem* items, int nitems, const GemmSeg* segs,
                               const uint32_t* koff, int total_tiles, int conjA, int conjB,
                               int ksplit, void* partial, bool tr, const FusedMaps* maps,
-                              hipStream_t stream);
+                              int kfast, hipStream_t stream);
 
 // Sum the k-chunk partials of a single-block item into its output layout.
 __global__ void __launch_bounds__(256)
@@ -492,7 +500,7 @@ hipError_t launch_zgemm(int config, int precision, const void* A, const void* B,
                         const GemmItem* items, int nitems, const GemmSeg* segs,
                         const uint32_t* koff, int total_tiles, int conjA, int conjB,
                         int ksplit, void* partial, bool tr, hipStream_t stream,
-                        const FusedMaps* maps) {
+                        const FusedMaps* maps, int kfast) {
   if (total_tiles <= 0) return hipSuccess;
   if (precision != GRAFT_PREC_FP64 && precision != GRAFT_PREC_FP32 &&
       precision != GRAFT_PREC_BF16X3)
@@ -501,7 +509,7 @@ hipError_t launch_zgemm(int config, int precision, const void* A, const void* B,
   auto tiles = [&](int ks, void* part) {
     if (precision == GRAFT_PREC_FP64)
       return launch_zgemm_tiles(config, A, B, C, items, nitems, segs, koff, total_tiles, conjA,
-                                conjB, ks, part, tr, maps, stream);
+                                conjB, ks, part, tr, maps, ks == 1 ? kfast : -1, stream);
     return launch_zgemm_lowp(config, precision, A, B, C, items, nitems, segs, koff, total_tiles,
                              conjA, conjB, ks, part, tr, stream);
   };
@@ -520,7 +528,7 @@ hipError_t launch_zgemm_tiles(int config, const void* A, const void* B, void* C,
                               const GemmItem* items, int nitems, const GemmSeg* segs,
                               const uint32_t* koff, int total_tiles, int conjA, int conjB,
                               int ksplit, void* partial, bool tr, const FusedMaps* maps,
-                              hipStream_t stream) {
+                              int kfast, hipStream_t stream) {
   switch (config) {
     case 0: return launch_f64<128, 128, 16, 64, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
     case 1: return launch_f64<128, 64, 16, 64, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
@@ -533,6 +541,15 @@ hipError_t launch_zgemm_tiles(int config, const void* A, const void* B, void* C,
     case 8: return launch_f64<128, 64, 16, 64, 32, true>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
     case 9: return launch_f64<64, 128, 16, 32, 64, true>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
     case 10: return launch_f64<64, 256, 8, 64, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
+    case 11: case 12: case 13: case 14: case 15: case 16: case 17: case 18: {
+      // the double-buffered tiles address operands directly; mapped or
+      // split-K steps run the single-buffered kernel of the same geometry
+      if (maps == nullptr && ksplit == 1)
+        return launch_zgemm_db(config, A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, tr, kfast, stream);
+      const int g = config >= 15 ? config - 4 : config;
+      const int same = g == 11 ? 7 : g == 13 ? 9 : 8;
+      return launch_zgemm_tiles(same, A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, -1, stream);
+    }
     default: return hipErrorInvalidValue;
   }
 }

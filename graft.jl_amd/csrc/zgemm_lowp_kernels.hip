@@ -373,10 +373,10 @@ static hipError_t launch_lp_config(int config, const void* A, const void* B, voi
                                    int ksplit, void* partial, bool tr, hipStream_t stream) {
   switch (config) {
     case 0: return launch_lp<128, 128, 64, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 1: case 8: return launch_lp<128, 64, 64, 32, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 2: case 7: return launch_lp<64, 64, 32, 32, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+    case 1: case 8: case 12: case 14: case 16: case 18: return launch_lp<128, 64, 64, 32, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+    case 2: case 7: case 11: case 15: return launch_lp<64, 64, 32, 32, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
     case 3: return launch_lp<256, 16, 64, 16, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 4: case 9: return launch_lp<64, 128, 32, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+    case 4: case 9: case 13: case 17: return launch_lp<64, 128, 32, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
     case 10: return launch_lp<64, 256, 64, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
     case 5: return launch_lp<32, 32, 16, 16, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
     case 6: return launch_lp<16, 256, 16, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);

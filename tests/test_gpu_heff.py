@@ -106,6 +106,48 @@ def test_every_tile_config_gives_the_same_result(g):
             assert _rel(out.to_numpy(), want) < TOL, (cfg, orient)
 
 
+PIPED_CASES = [
+    # (A dims, oindA, cindA, B dims, cindB, oindB, conjA, conjB): whole tiles for
+    # every configuration (M, N multiples of 128, K of 16) in each load direction
+    ((64, 256), (1,), (0,), (64, 128), (0,), (1,), False, False),
+    ((256, 64), (0,), (1,), (64, 128), (0,), (1,), False, False),
+    ((64, 256), (1,), (0,), (128, 64), (1,), (0,), False, False),
+    ((256, 64), (0,), (1,), (128, 64), (1,), (0,), False, False),
+    # permuted legs on both sides (offsets decoded per leg)
+    ((16, 8, 16, 8), (0, 2), (1, 3), (8, 8, 128), (0, 1), (2,), False, False),
+    # the same steps that do not qualify: conjugation, ragged M, ragged K
+    ((64, 256), (1,), (0,), (64, 128), (0,), (1,), True, False),
+    ((64, 200), (1,), (0,), (64, 128), (0,), (1,), False, False),
+    ((40, 256), (1,), (0,), (40, 128), (0,), (1,), False, True),
+]
+
+
+@pytest.mark.parametrize("case", range(len(PIPED_CASES)))
+def test_pipelined_tiles_match_oracle(g, case):
+    """Tile configurations 15-18 run the pipelined kernel on steps whose every
+    tile is whole and the double-buffered one otherwise; both orientations."""
+    from graft_amd.contractions import NativePlan
+    from graft_amd.planning import ContractionPlan, PairStep
+    from graft_amd.tensors import DTensor
+    da, oa, ca, db, cb, ob, conjA, conjB = PIPED_CASES[case]
+    _say("pipelined tiles", PIPED_CASES[case])
+    rng = np.random.default_rng(100 + case)
+    A = ref.random_complex(rng, da)
+    Bm = ref.random_complex(rng, db)
+    no = len(oa) + len(ob)
+    step = PairStep(0, 1, 2, oa, ca, ob, cb, conjA, conjB, (tuple(range(no)), ()))
+    dA, dB = DTensor.from_numpy(A), DTensor.from_numpy(Bm)
+    np_ = NativePlan(ContractionPlan(3, 2, [step]), [dA, dB])
+    want = ref.contract_pair(A, oa, ca, conjA, Bm, cb, ob, conjB, tuple(range(no)))
+    for cfg in (11, 12, 13, 14, 15, 16, 17, 18):
+        for orient in (0, 1):
+            np_.set_tile_config(0, cfg)
+            np_.set_orientation(0, orient)
+            out = DTensor.empty(want.shape)
+            np_.execute([dA, dB], out)
+            assert _rel(out.to_numpy(), want) < TOL, (cfg, orient)
+
+
 @pytest.mark.parametrize("big_is_a", [True, False])
 def test_streaming_kernel_matches_oracle(g, big_is_a):
     """The skinny TTNO-absorb kernel: K <= 16, one open side <= 64, both orientations of the operands."""
