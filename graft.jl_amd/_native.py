@@ -96,6 +96,7 @@ def lib():
         "graft_plan_step_launch": [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                    ctypes.POINTER(i32)],
         "graft_plan_step_elements": [vp, i32, ctypes.POINTER(i64)],
+        "graft_plan_step_gemms": [vp, i32, ctypes.POINTER(i64), ctypes.POINTER(i32)],
         "graft_plan_autotune": [vp, vp, pp, vp, i32, i32],
         "graft_plan_execute": [vp, vp, pp, vp, i32],
         "graft_plan_execute_timed": [vp, vp, pp, vp, i32, ctypes.POINTER(ctypes.c_float)],

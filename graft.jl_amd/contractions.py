@@ -78,6 +78,14 @@ class NativePlan:
                             elements=list(abc)))
         return out
 
+    def step_gemms(self, step):
+        """(M, N, K, segments) of every GEMM item of one step."""
+        n = ctypes.c_int64()
+        nat.check(nat.lib().graft_plan_step_gemms(self.handle, step, ctypes.byref(n), None))
+        buf = (ctypes.c_int32 * (4 * max(1, n.value)))()
+        nat.check(nat.lib().graft_plan_step_gemms(self.handle, step, ctypes.byref(n), buf))
+        return [tuple(buf[4 * i:4 * i + 4]) for i in range(n.value)]
+
     def execute(self, operands, out, precision=nat.PREC_FP64):
         ctx = nat.default_context()
         ins = nat.ptr_array([op.ptr for op in operands])

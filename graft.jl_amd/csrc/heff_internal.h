@@ -49,6 +49,22 @@ struct TileConfig {
 constexpr int kNumTileConfigs = 19;
 extern const TileConfig kTileConfigs[kNumTileConfigs];
 
+// Logical tiles of XCD x: [start[x], start[x + 1]).  Hardware dispatches
+// block b to XCD b % 8, so the grid is 8 * max range and block b runs tile
+// start[b % 8] + b / 8 when that is inside its XCD's range.  Ranges of equal
+// work (k-tiles plus a fixed cost per tile) keep XCDs busy equally on steps
+// whose items differ in K; contiguous ranges keep tiles sharing an operand
+// panel on one L2.
+struct XcdSplit {
+  int32_t start[9];
+};
+XcdSplit uniform_xcd_split(int total_tiles);
+inline int xcd_grid(const XcdSplit& s) {
+  int g = 0;
+  for (int x = 0; x < 8; ++x) g = g > s.start[x + 1] - s.start[x] ? g : s.start[x + 1] - s.start[x];
+  return 8 * g;
+}
+
 // Element maps of a fused sector step read / written inside the GEMM: entry
 // i of a fused matrix lives at blocked element a[i] / b[i] / c[i]
 // (kMapZero: absent — zero when read, skipped when stored).
@@ -68,7 +84,8 @@ hipError_t launch_zgemm(int config, int precision, const void* A, const void* B,
                         const GemmItem* items, int nitems, const GemmSeg* segs,
                         const uint32_t* koff, int total_tiles, int conjA, int conjB,
                         int ksplit, void* partial, bool tr, hipStream_t stream,
-                        const FusedMaps* maps = nullptr, int kfast = -1);
+                        const FusedMaps* maps = nullptr, int kfast = -1,
+                        const XcdSplit* xs = nullptr);
 int tiles_for(int config, int M, int N);
 // FP32 / BF16x3 products with the same addressing (zgemm_lowp_kernels.hip).
 // Double-buffered three-product FP64 tiles (configurations 11-18), operands
@@ -78,7 +95,15 @@ int tiles_for(int config, int M, int N);
 // configuration c - 4 otherwise.
 hipError_t launch_zgemm_db(int config, const void* A, const void* B, void* C, const GemmItem* items,
                            int nitems, const GemmSeg* segs, const uint32_t* koff, int total_tiles,
-                           int conjA, int conjB, bool tr, int kfast, hipStream_t stream);
+                           int conjA, int conjB, bool tr, int kfast, const XcdSplit* xs,
+                           hipStream_t stream);
+// Fused sector steps through element maps on the double-buffered 64x64
+// tiles (configurations 11 and 15); kfast: bit 0 / bit 1 A / B fused
+// matrices row-major (loaded along k).
+hipError_t launch_zgemm_dbmap(int config, const void* A, const void* B, void* C, const GemmItem* items,
+                              int nitems, const GemmSeg* segs, int total_tiles, int conjA, int conjB,
+                              bool tr, int kfast, const FusedMaps& maps, const XcdSplit* xs,
+                              hipStream_t stream);
 hipError_t launch_zgemm_lowp(int config, int precision, const void* A, const void* B, void* C,
                              const GemmItem* items, int nitems, const GemmSeg* segs,
                              const uint32_t* koff, int total_tiles, int conjA, int conjB,

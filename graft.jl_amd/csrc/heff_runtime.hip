@@ -215,6 +215,8 @@ struct StepExec {
   int ksplit = 1;             // split-K factor (single-block steps with few tiles)
   bool tr = false;            // transposed MFMA orientation in use
   std::vector<int> kfast_memo;   // per tile configuration: piped_kfast, computed on first use
+  int xcd_config = -1;           // configuration the XCD split below was computed for
+  XcdSplit xcd{};                // work-balanced tile ranges per XCD
   bool tr_auto = false;       // orientation from the output's unit-stride leg
   int tr_pinned = -1;         // -1 = automatic
   int skinny = 0;             // streaming kernel eligible: 1 = A is the large operand, 2 = B
@@ -365,6 +367,49 @@ static int piped_kfast(StepExec& ex, int c) {
   }
   ex.kfast_memo[c] = ok ? kf : -1;
   return ex.kfast_memo[c];
+}
+
+// Tile ranges of equal work per XCD for the current tile configuration:
+// a tile of an item costs its k-tiles plus a fixed 2 (start-up, epilogue).
+// Steps with one item keep equal tile counts.
+static const XcdSplit& xcd_split(StepExec& ex) {
+  if (ex.xcd_config == ex.config) return ex.xcd;
+  ex.xcd_config = ex.config;
+  ex.xcd = uniform_xcd_split(ex.total_tiles);
+  if (ex.items.size() < 2 || ex.config < 0 || ex.config >= kNumTileConfigs) return ex.xcd;
+  const int bk = kTileConfigs[ex.config].bk;
+  std::vector<double> w(ex.items.size());
+  double total = 0.0;
+  for (size_t i = 0; i < ex.items.size(); ++i) {
+    const GemmItem& it = ex.items[i];
+    int64_t kt = 0;
+    for (int s = it.seg_begin; s < it.seg_begin + it.seg_count; ++s) kt += (ex.segs[s].K + bk - 1) / bk;
+    w[i] = (double)(kt + 2);
+    const int next = i + 1 < ex.items.size() ? ex.items[i + 1].tile_begin : ex.total_tiles;
+    total += w[i] * (next - it.tile_begin);
+  }
+  double cum = 0.0;
+  size_t i = 0;
+  for (int x = 1; x < 8; ++x) {
+    const double target = total * x / 8.0;
+    while (i < ex.items.size()) {
+      const int next = i + 1 < ex.items.size() ? ex.items[i + 1].tile_begin : ex.total_tiles;
+      const double iw = w[i] * (next - ex.items[i].tile_begin);
+      if (cum + iw >= target) break;
+      cum += iw;
+      ++i;
+    }
+    int t = ex.total_tiles;
+    if (i < ex.items.size()) {
+      const int next = i + 1 < ex.items.size() ? ex.items[i + 1].tile_begin : ex.total_tiles;
+      t = ex.items[i].tile_begin + (int)std::ceil((target - cum) / w[i]);
+      t = std::min(t, next);
+    }
+    ex.xcd.start[x] = std::max(t, ex.xcd.start[x - 1]);
+  }
+  ex.xcd.start[0] = 0;
+  ex.xcd.start[8] = ex.total_tiles;
+  return ex.xcd;
 }
 
 static bool canonical_blocks(const TStruct& T) {
@@ -1454,6 +1499,16 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     if (!ma) GRAFT_CHECK_HIP(launch_map_gather(A, Af, ex.d_ma, ex.fa, ctx->stream));
     if (!mb) GRAFT_CHECK_HIP(launch_map_gather(B, Bf, ex.d_mb, ex.fb, ctx->stream));
     const FusedMaps maps{ma ? ex.d_ma : nullptr, mb ? ex.d_mb : nullptr, ex.d_mci};
+    if (ex.config == 11 || ex.config == 15) {
+      // every component of a fused step loads A and B in one direction
+      const int dirs = ex.segs.empty() ? 0
+                                       : (ex.segs[0].a_kfast ? 1 : 0) | (ex.segs[0].b_kfast ? 2 : 0);
+      GRAFT_CHECK_HIP(launch_zgemm_dbmap(ex.config, ma ? A : Af, mb ? B : Bf, C, ex.d_items,
+                                         (int)ex.items.size(), ex.d_segs, ex.total_tiles,
+                                         ex.conjA, ex.conjB, ex.tr, dirs, maps, &xcd_split(ex),
+                                         ctx->stream));
+      return GRAFT_OK;
+    }
     GRAFT_CHECK_HIP(launch_zgemm(ex.config, precision, ma ? A : Af, mb ? B : Bf, C, ex.d_items,
                                  (int)ex.items.size(), ex.d_segs, ex.d_koff, ex.total_tiles,
                                  ex.conjA, ex.conjB, 1, nullptr, ex.tr, ctx->stream, &maps));
@@ -1495,7 +1550,8 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     const int kfast = precision == GRAFT_PREC_FP64 ? piped_kfast(ex, ex.config) : -1;
     hipError_t e = launch_zgemm(ex.config, precision, A, B, C, ex.d_items, (int)ex.items.size(),
                                 ex.d_segs, ex.d_koff, ex.total_tiles, ex.conjA, ex.conjB,
-                                ex.ksplit, partial, ex.tr, ctx->stream, nullptr, kfast);
+                                ex.ksplit, partial, ex.tr, ctx->stream, nullptr, kfast,
+                                &xcd_split(ex));
     if (e == hipErrorNotSupported) {
       set_error("precision not supported by this build");
       return GRAFT_E_UNSUPPORTED;
@@ -1869,6 +1925,28 @@ int graft_plan_step_elements(graft_plan* plan, int32_t step, int64_t* abc3) {
   abc3[0] = plan->impl.slots[ex.a].total;
   abc3[1] = plan->impl.slots[ex.b].total;
   abc3[2] = plan->impl.slots[ex.dst].total;
+  return GRAFT_OK;
+}
+
+int graft_plan_step_gemms(graft_plan* plan, int32_t step, int64_t* ngemms, int32_t* mnks) {
+  GRAFT_REQUIRE(plan != nullptr && ngemms != nullptr, "null argument");
+  GRAFT_REQUIRE(step >= 0 && step < (int32_t)plan->impl.steps.size(), "step out of range");
+  const StepExec& ex = plan->impl.steps[step];
+  if (mnks == nullptr) {
+    *ngemms = (int64_t)ex.items.size();
+    return GRAFT_OK;
+  }
+  GRAFT_REQUIRE(*ngemms >= (int64_t)ex.items.size(), "shape array too small");
+  for (size_t i = 0; i < ex.items.size(); ++i) {
+    const GemmItem& it = ex.items[i];
+    int64_t k = 0;
+    for (int s = it.seg_begin; s < it.seg_begin + it.seg_count; ++s) k += ex.segs[s].K;
+    mnks[4 * i] = it.M;
+    mnks[4 * i + 1] = it.N;
+    mnks[4 * i + 2] = (int32_t)k;
+    mnks[4 * i + 3] = it.seg_count;
+  }
+  *ngemms = (int64_t)ex.items.size();
   return GRAFT_OK;
 }
 

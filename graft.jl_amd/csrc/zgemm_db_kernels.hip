@@ -42,12 +42,36 @@ __device__ __forceinline__ uint32_t decode_offset_db(uint32_t idx, int rank, con
   return off;
 }
 
+// Item of a tile: the last item whose first tile is <= tile.  Up to 256
+// items are searched in LDS after one cooperative load of their first
+// tiles (one memory round trip instead of one per bisection step).
+template <int NT>
+__device__ __forceinline__ int find_item(const GemmItem* __restrict__ items, int nitems, int tile) {
+  if (nitems == 1) return 0;
+  int lo = 0, hi = nitems - 1;
+  if (nitems <= 256) {
+    __shared__ int tb[256];
+    for (int i = threadIdx.x; i < nitems; i += NT) tb[i] = items[i].tile_begin;
+    __syncthreads();
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tb[mid] <= tile) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+  }
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 template <int BM, int BN, int BK, int WM, int WN, bool TR>
 __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
     zgemm3m_db_kernel(const double2* __restrict__ Abase, const double2* __restrict__ Bbase,
                       double2* __restrict__ Cbase, const GemmItem* __restrict__ items, int nitems,
                       const GemmSeg* __restrict__ segs, const uint32_t* __restrict__ koff,
-                      int total_tiles, int conjA, int conjB) {
+                      int total_tiles, int conjA, int conjB, XcdSplit xs) {
   constexpr int NWN = BN / WN;
   constexpr int NT = 64 * (BM / WM) * NWN;
   constexpr int EA = BM * BK / NT;
@@ -72,19 +96,10 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
 
   // XCD-aware tile order (block b runs on XCD b % 8): each XCD walks a
   // contiguous range of logical tiles, so tiles sharing an A panel share an L2.
-  int tile;
-  {
-    const int b = blockIdx.x;
-    const int x = b & 7, i = b >> 3;
-    const int q = total_tiles >> 3, r = total_tiles & 7;
-    tile = x * q + (x < r ? x : r) + i;
-  }
-  int lo = 0, hi = nitems - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (items[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
-  }
-  const GemmItem* it = items + lo;
+  // work-balanced XCD ranges (blocks past their XCD's range exit at once)
+  const int tile = xs.start[blockIdx.x & 7] + (int)(blockIdx.x >> 3);
+  if (tile >= xs.start[(blockIdx.x & 7) + 1]) return;
+  const GemmItem* it = items + find_item<64 * (BM / WM) * (BN / WN)>(items, nitems, tile);
   const int M = it->M, N = it->N;
   const int mrank = it->mrank, nrank = it->nrank;
   const int local = tile - it->tile_begin;
@@ -357,7 +372,7 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
     zgemm3m_piped_kernel(const double2* __restrict__ Abase, const double2* __restrict__ Bbase,
                          double2* __restrict__ Cbase, const GemmItem* __restrict__ items, int nitems,
                          const GemmSeg* __restrict__ segs, const uint32_t* __restrict__ koff,
-                         int total_tiles) {
+                         int total_tiles, XcdSplit xs) {
   constexpr int NWN = BN / WN;
   constexpr int NT = 64 * (BM / WM) * NWN;
   constexpr int EA = BM * BK / NT;
@@ -381,19 +396,10 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
   const int li = lane & 15;
   const int kr = lane >> 4;
 
-  int tile;
-  {
-    const int b = blockIdx.x;
-    const int x = b & 7, i = b >> 3;
-    const int q = total_tiles >> 3, r = total_tiles & 7;
-    tile = x * q + (x < r ? x : r) + i;
-  }
-  int lo = 0, hi = nitems - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (items[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
-  }
-  const GemmItem* it = items + lo;
+  // work-balanced XCD ranges (blocks past their XCD's range exit at once)
+  const int tile = xs.start[blockIdx.x & 7] + (int)(blockIdx.x >> 3);
+  if (tile >= xs.start[(blockIdx.x & 7) + 1]) return;
+  const GemmItem* it = items + find_item<64 * (BM / WM) * (BN / WN)>(items, nitems, tile);
   const int mrank = it->mrank, nrank = it->nrank;
   const int local = tile - it->tile_begin;
   const int m0 = (local / it->tiles_n) * BM;
@@ -559,19 +565,245 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
   }
 }
 
+// Fused sector steps (per-charge matrices, one segment per item, rank-1
+// row and column legs, contracted index linear in the fused layout) with
+// element maps: an operand with a map is read through it from its blocked
+// tensor, the others from their gathered fused matrices, and the result is
+// stored through the C map.  Same double-buffered pipeline; every element's
+// source index (map value, or fused index) is read one k-tile ahead of its
+// data load, and the C map values of the tile are read before the k loop.
+template <int BM, int BN, int BK, int WM, int WN, bool TR, bool AKF, bool BKF>
+__global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN), 2)
+    zgemm3m_map_kernel(const double2* __restrict__ Abase, const double2* __restrict__ Bbase,
+                       double2* __restrict__ Cbase, const GemmItem* __restrict__ items, int nitems,
+                       const GemmSeg* __restrict__ segs, int total_tiles, int conjA, int conjB,
+                       FusedMaps maps, XcdSplit xs) {
+  constexpr int NWN = BN / WN;
+  constexpr int NT = 64 * (BM / WM) * NWN;
+  constexpr int EA = BM * BK / NT;
+  constexpr int EB = BN * BK / NT;
+  constexpr int FM = WM / 16;
+  constexpr int FN = WN / 16;
+  constexpr int LA = BM + 1;
+  constexpr int LB = BN + 1;
+  static_assert(NT % BM == 0 && NT % BN == 0 && NT % BK == 0, "tile/thread mismatch");
+  static_assert(EA >= 1 && EB >= 1 && BK % 4 == 0, "tile too small for the thread count");
+
+  __shared__ double2 As[2][BK * LA];
+  __shared__ double2 Bs[2][BK * LB];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm0 = (wave / NWN) * WM;
+  const int wn0 = (wave % NWN) * WN;
+  const int li = lane & 15;
+  const int kr = lane >> 4;
+
+  // work-balanced XCD ranges (blocks past their XCD's range exit at once)
+  const int tile = xs.start[blockIdx.x & 7] + (int)(blockIdx.x >> 3);
+  if (tile >= xs.start[(blockIdx.x & 7) + 1]) return;
+  const GemmItem* it = items + find_item<64 * (BM / WM) * (BN / WN)>(items, nitems, tile);
+  const int M = it->M, N = it->N;
+  const int local = tile - it->tile_begin;
+  const int m0 = (local / it->tiles_n) * BM;
+  const int n0 = (local % it->tiles_n) * BN;
+  const bool mapa = maps.a != nullptr, mapb = maps.b != nullptr, mapc = maps.c != nullptr;
+
+  auto m_of = [&](int a, int r) { return m0 + wm0 + a * 16 + (TR ? li : kr + 4 * r); };
+  auto n_of = [&](int b, int r) { return n0 + wn0 + b * 16 + (TR ? kr + 4 * r : li); };
+  // destination of every output element of this thread (kMapZero: not
+  // stored), read before the k loop so the epilogue does not wait on it
+  const uint32_t* mC = mapc ? maps.c + it->c_off : nullptr;
+  uint32_t cdst[FM][FN][4];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m_of(a, r), n = n_of(b, r);
+        const bool ok = m < M && n < N;
+        const uint32_t ci = ok ? (uint32_t)m * it->c_mstr[0] + (uint32_t)n * it->c_nstr[0] : 0u;
+        cdst[a][b][r] = !ok ? kMapZero : mapc ? mC[ci] : (uint32_t)it->c_off + ci;
+      }
+  dbl4 acc[3][FM][FN];
+#pragma unroll
+  for (int t = 0; t < 3; ++t)
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b) acc[t][a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+  const double sA = conjA ? -1.0 : 1.0;
+  const double sB = conjB ? -1.0 : 1.0;
+
+  const GemmSeg* sg = segs + it->seg_begin;
+  const int K = sg->K;
+  const double2* A = mapa ? Abase : Abase + sg->a_off;
+  const double2* B = mapb ? Bbase : Bbase + sg->b_off;
+  const uint32_t* mA = mapa ? maps.a + sg->a_off : nullptr;
+  const uint32_t* mB = mapb ? maps.b + sg->b_off : nullptr;
+  const uint32_t akstr = AKF ? 1u : (uint32_t)M;    // fused A: M x K, k stride
+  const uint32_t bkstr = BKF ? 1u : (uint32_t)N;    // fused B: K x N
+  // row / column and k offset (within a tile) of this thread's stage element e
+  const int pa0 = AKF ? tid / BK : tid % BM, qa0 = AKF ? tid % BK : tid / BM;
+  const int pb0 = BKF ? tid / BK : tid % BN, qb0 = BKF ? tid % BK : tid / BN;
+  auto pA = [&](int e) { return AKF ? pa0 + e * (NT / BK) : pa0; };
+  auto qA = [&](int e) { return AKF ? qa0 : qa0 + e * (NT / BM); };
+  auto pB = [&](int e) { return BKF ? pb0 + e * (NT / BK) : pb0; };
+  auto qB = [&](int e) { return BKF ? qb0 : qb0 + e * (NT / BN); };
+  uint32_t sa[EA], sb[EB];    // source indices of the next fetch (kMapZero: zero)
+  auto sources = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < EA; ++e) {
+      const int m = m0 + pA(e), k = k0 + qA(e);
+      const bool ok = m < M && k < K;
+      const uint32_t fi = ok ? (uint32_t)m * sg->a_mstr[0] + (uint32_t)k * akstr : 0u;
+      sa[e] = !ok ? kMapZero : mapa ? mA[fi] : fi;
+    }
+#pragma unroll
+    for (int e = 0; e < EB; ++e) {
+      const int n = n0 + pB(e), k = k0 + qB(e);
+      const bool ok = n < N && k < K;
+      const uint32_t fi = ok ? (uint32_t)n * sg->b_nstr[0] + (uint32_t)k * bkstr : 0u;
+      sb[e] = !ok ? kMapZero : mapb ? mB[fi] : fi;
+    }
+  };
+  dbl2 ra[EA], rb[EB];
+  uint32_t vA = 0, vB = 0;
+  auto fetch = [&]() {
+    vA = vB = 0;
+#pragma unroll
+    for (int e = 0; e < EA; ++e) {
+      const bool ok = sa[e] < kMapZero;
+      ra[e] = ld2(A + (ok ? sa[e] : 0u));
+      vA |= (uint32_t)ok << e;
+    }
+#pragma unroll
+    for (int e = 0; e < EB; ++e) {
+      const bool ok = sb[e] < kMapZero;
+      rb[e] = ld2(B + (ok ? sb[e] : 0u));
+      vB |= (uint32_t)ok << e;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < EA; ++e) {
+      const bool ok = (vA >> e) & 1u;
+      st2(&As[buf][qA(e) * LA + pA(e)], dbl2{ok ? ra[e].x : 0.0, ok ? sA * ra[e].y : 0.0});
+    }
+#pragma unroll
+    for (int e = 0; e < EB; ++e) {
+      const bool ok = (vB >> e) & 1u;
+      st2(&Bs[buf][qB(e) * LB + pB(e)], dbl2{ok ? rb[e].x : 0.0, ok ? sB * rb[e].y : 0.0});
+    }
+  };
+  auto compute = [&](int buf) {
+    const double2* as = As[buf] + wm0 + li;
+    const double2* bs = Bs[buf] + wn0 + li;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kq = kk + kr;
+      dbl2 va[FM], vb[FN];
+      double su[FM], sv[FN];
+#pragma unroll
+      for (int a = 0; a < FM; ++a) va[a] = ld2(as + kq * LA + a * 16);
+#pragma unroll
+      for (int b = 0; b < FN; ++b) vb[b] = ld2(bs + kq * LB + b * 16);
+#pragma unroll
+      for (int a = 0; a < FM; ++a) su[a] = va[a].x + va[a].y;
+#pragma unroll
+      for (int b = 0; b < FN; ++b) sv[b] = vb[b].x + vb[b].y;
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+          if constexpr (TR) {
+            acc[0][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(vb[b].x, va[a].x, acc[0][a][b], 0, 0, 0);
+            acc[1][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(vb[b].y, va[a].y, acc[1][a][b], 0, 0, 0);
+            acc[2][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(sv[b], su[a], acc[2][a][b], 0, 0, 0);
+          } else {
+            acc[0][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[a].x, vb[b].x, acc[0][a][b], 0, 0, 0);
+            acc[1][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[a].y, vb[b].y, acc[1][a][b], 0, 0, 0);
+            acc[2][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(su[a], sv[b], acc[2][a][b], 0, 0, 0);
+          }
+        }
+    }
+  };
+
+  const int nkt = (K + BK - 1) / BK;
+  sources(0);
+  fetch();
+  sources(BK);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nkt; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < nkt) {
+      fetch();                    // k-tile kt + 1
+      sources((kt + 2) * BK);     // k-tile kt + 2 (out of range: zero)
+    }
+    compute(buf);
+    if (kt + 1 < nkt) store(buf ^ 1);
+    __syncthreads();
+  }
+
+
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t d = cdst[a][b][r];
+        if (d >= kMapZero) continue;
+        const double t1 = acc[0][a][b][r], t2 = acc[1][a][b][r], t3 = acc[2][a][b][r];
+        st2(Cbase + d, dbl2{t1 - t2, t3 - t1 - t2});
+      }
+}
+
+hipError_t launch_zgemm_dbmap(int config, const void* A, const void* B, void* C, const GemmItem* items,
+                              int nitems, const GemmSeg* segs, int total_tiles, int conjA, int conjB,
+                              bool tr, int kfast, const FusedMaps& maps, const XcdSplit* xsp,
+                              hipStream_t stream) {
+  if (config != 11 && config != 15) return hipErrorInvalidValue;
+  const XcdSplit xs = xsp ? *xsp : uniform_xcd_split(total_tiles);
+  const dim3 grid(xcd_grid(xs)), block(256);
+  const double2* a = (const double2*)A;
+  const double2* b = (const double2*)B;
+  double2* c = (double2*)C;
+#define GRAFT_MAPK(TRV, AKV, BKV)                                                                   \
+  hipLaunchKernelGGL((zgemm3m_map_kernel<64, 64, 16, 32, 32, TRV, AKV, BKV>), grid, block, 0,     \
+                     stream, a, b, c, items, nitems, segs, total_tiles, conjA, conjB, maps, xs)
+  switch ((kfast & 3) | (tr ? 4 : 0)) {
+    case 0: GRAFT_MAPK(false, false, false); break;
+    case 1: GRAFT_MAPK(false, true, false); break;
+    case 2: GRAFT_MAPK(false, false, true); break;
+    case 3: GRAFT_MAPK(false, true, true); break;
+    case 4: GRAFT_MAPK(true, false, false); break;
+    case 5: GRAFT_MAPK(true, true, false); break;
+    case 6: GRAFT_MAPK(true, false, true); break;
+    default: GRAFT_MAPK(true, true, true); break;
+  }
+#undef GRAFT_MAPK
+  return hipGetLastError();
+}
+
 template <int BM, int BN, int BK, int WM, int WN>
 static hipError_t launch_db(const void* A, const void* B, void* C, const GemmItem* items,
                             int nitems, const GemmSeg* segs, const uint32_t* koff, int total_tiles,
-                            int conjA, int conjB, bool tr, int kfast, hipStream_t stream) {
+                            int conjA, int conjB, bool tr, int kfast, const XcdSplit* xsp,
+                            hipStream_t stream) {
   constexpr int NT = 64 * (BM / WM) * (BN / WN);
-  const dim3 grid(total_tiles), block(NT);
+  const XcdSplit xs = xsp ? *xsp : uniform_xcd_split(total_tiles);
+  const dim3 grid(xcd_grid(xs)), block(NT);
   const double2* a = (const double2*)A;
   const double2* b = (const double2*)B;
   double2* c = (double2*)C;
   if (kfast >= 0 && !conjA && !conjB) {
 #define GRAFT_PIPED(TRV, AKV, BKV)                                                                  \
   hipLaunchKernelGGL((zgemm3m_piped_kernel<BM, BN, BK, WM, WN, TRV, AKV, BKV>), grid, block, 0,     \
-                     stream, a, b, c, items, nitems, segs, koff, total_tiles)
+                     stream, a, b, c, items, nitems, segs, koff, total_tiles, xs)
     switch (kfast | (tr ? 4 : 0)) {
       case 0: GRAFT_PIPED(false, false, false); break;
       case 1: GRAFT_PIPED(false, true, false); break;
@@ -587,22 +819,23 @@ static hipError_t launch_db(const void* A, const void* B, void* C, const GemmIte
   }
   if (tr)
     hipLaunchKernelGGL((zgemm3m_db_kernel<BM, BN, BK, WM, WN, true>), grid, block, 0, stream, a, b,
-                       c, items, nitems, segs, koff, total_tiles, conjA, conjB);
+                       c, items, nitems, segs, koff, total_tiles, conjA, conjB, xs);
   else
     hipLaunchKernelGGL((zgemm3m_db_kernel<BM, BN, BK, WM, WN, false>), grid, block, 0, stream, a, b,
-                       c, items, nitems, segs, koff, total_tiles, conjA, conjB);
+                       c, items, nitems, segs, koff, total_tiles, conjA, conjB, xs);
   return hipGetLastError();
 }
 
 hipError_t launch_zgemm_db(int config, const void* A, const void* B, void* C, const GemmItem* items,
                            int nitems, const GemmSeg* segs, const uint32_t* koff, int total_tiles,
-                           int conjA, int conjB, bool tr, int kfast, hipStream_t stream) {
+                           int conjA, int conjB, bool tr, int kfast, const XcdSplit* xs,
+                           hipStream_t stream) {
   const int kf = config >= 15 ? kfast : -1;
   switch (config) {
-    case 11: case 15: return launch_db<64, 64, 16, 32, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, tr, kf, stream);
-    case 12: case 16: return launch_db<128, 64, 16, 64, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, tr, kf, stream);
-    case 13: case 17: return launch_db<64, 128, 16, 32, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, tr, kf, stream);
-    case 14: case 18: return launch_db<128, 64, 16, 32, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, tr, kf, stream);
+    case 11: case 15: return launch_db<64, 64, 16, 32, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, tr, kf, xs, stream);
+    case 12: case 16: return launch_db<128, 64, 16, 64, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, tr, kf, xs, stream);
+    case 13: case 17: return launch_db<64, 128, 16, 32, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, tr, kf, xs, stream);
+    case 14: case 18: return launch_db<128, 64, 16, 32, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, tr, kf, xs, stream);
     default: return hipErrorInvalidValue;
   }
 }

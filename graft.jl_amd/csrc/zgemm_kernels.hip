@@ -428,6 +428,13 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
   }
 }
 
+XcdSplit uniform_xcd_split(int total_tiles) {
+  XcdSplit s;
+  const int q = total_tiles >> 3, r = total_tiles & 7;
+  for (int x = 0; x <= 8; ++x) s.start[x] = x * q + (x < r ? x : r);
+  return s;
+}
+
 int tiles_for(int config, int M, int N) {
   const TileConfig& c = kTileConfigs[config];
   return ((M + c.bm - 1) / c.bm) * ((N + c.bn - 1) / c.bn);
@@ -472,7 +479,7 @@ hipError_t launch_zgemm_tiles(int config, const void* A, const void* B, void* C,
                               const GemmItem* items, int nitems, const GemmSeg* segs,
                               const uint32_t* koff, int total_tiles, int conjA, int conjB,
                               int ksplit, void* partial, bool tr, const FusedMaps* maps,
-                              int kfast, hipStream_t stream);
+                              int kfast, const XcdSplit* xs, hipStream_t stream);
 
 // Sum the k-chunk partials of a single-block item into its output layout.
 __global__ void __launch_bounds__(256)
@@ -500,7 +507,7 @@ hipError_t launch_zgemm(int config, int precision, const void* A, const void* B,
                         const GemmItem* items, int nitems, const GemmSeg* segs,
                         const uint32_t* koff, int total_tiles, int conjA, int conjB,
                         int ksplit, void* partial, bool tr, hipStream_t stream,
-                        const FusedMaps* maps, int kfast) {
+                        const FusedMaps* maps, int kfast, const XcdSplit* xs) {
   if (total_tiles <= 0) return hipSuccess;
   if (precision != GRAFT_PREC_FP64 && precision != GRAFT_PREC_FP32 &&
       precision != GRAFT_PREC_BF16X3)
@@ -509,7 +516,8 @@ hipError_t launch_zgemm(int config, int precision, const void* A, const void* B,
   auto tiles = [&](int ks, void* part) {
     if (precision == GRAFT_PREC_FP64)
       return launch_zgemm_tiles(config, A, B, C, items, nitems, segs, koff, total_tiles, conjA,
-                                conjB, ks, part, tr, maps, ks == 1 ? kfast : -1, stream);
+                                conjB, ks, part, tr, maps, ks == 1 ? kfast : -1, ks == 1 ? xs : nullptr,
+                                stream);
     return launch_zgemm_lowp(config, precision, A, B, C, items, nitems, segs, koff, total_tiles,
                              conjA, conjB, ks, part, tr, stream);
   };
@@ -528,7 +536,7 @@ hipError_t launch_zgemm_tiles(int config, const void* A, const void* B, void* C,
                               const GemmItem* items, int nitems, const GemmSeg* segs,
                               const uint32_t* koff, int total_tiles, int conjA, int conjB,
                               int ksplit, void* partial, bool tr, const FusedMaps* maps,
-                              int kfast, hipStream_t stream) {
+                              int kfast, const XcdSplit* xs, hipStream_t stream) {
   switch (config) {
     case 0: return launch_f64<128, 128, 16, 64, 64>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
     case 1: return launch_f64<128, 64, 16, 64, 32>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, stream);
@@ -545,10 +553,10 @@ hipError_t launch_zgemm_tiles(int config, const void* A, const void* B, void* C,
       // the double-buffered tiles address operands directly; mapped or
       // split-K steps run the single-buffered kernel of the same geometry
       if (maps == nullptr && ksplit == 1)
-        return launch_zgemm_db(config, A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, tr, kfast, stream);
+        return launch_zgemm_db(config, A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, tr, kfast, xs, stream);
       const int g = config >= 15 ? config - 4 : config;
       const int same = g == 11 ? 7 : g == 13 ? 9 : 8;
-      return launch_zgemm_tiles(same, A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, -1, stream);
+      return launch_zgemm_tiles(same, A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, maps, -1, nullptr, stream);
     }
     default: return hipErrorInvalidValue;
   }

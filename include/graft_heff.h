@@ -170,6 +170,11 @@ int graft_plan_step_launch(graft_plan* plan, int32_t step, int32_t* config, int3
  * step's algorithmic HBM bytes are 16 * (A + B + C).  No counterpart in the
  * reference (its pair_cost, Backend.jl:328-368, counts FLOPs only). */
 int graft_plan_step_elements(graft_plan* plan, int32_t step, int64_t* abc3);
+/* GEMM items of one step (output blocks, or per-charge components of a
+ * fused sector step): call with mnks NULL to get their number, then with
+ * 4 * ngemms int32 receiving (M, N, summed K, segments) per item.  No
+ * reference counterpart: shape statistics for tile selection and tests. */
+int graft_plan_step_gemms(graft_plan* plan, int32_t step, int64_t* ngemms, int32_t* mnks);
 /* Block list of the plan's result (sector tensors): call with NULL arrays to
  * get rank and nblocks, then with arrays of nblocks*rank sector indices and
  * nblocks element offsets (packed, in this order). */
