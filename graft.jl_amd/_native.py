@@ -120,6 +120,7 @@ def lib():
         "graft_ctx_reserve_workspace": [vp, ctypes.c_uint64],
         "graft_ctx_set_sector_fusion": [vp, i32],
         "graft_ctx_set_fused_map_loads": [vp, i32],
+        "graft_ctx_set_lean_stream": [vp, i32],
         "graft_graph_capture_end": [vp, ctypes.POINTER(vp)],
         "graft_graph_launch": [vp, vp],
         "graft_graph_destroy": [vp],
@@ -230,6 +231,11 @@ class Context:
         """1 (default): fused FP64 sector steps read and write the blocks through the
         element maps inside the GEMM; 0: gather, GEMM, scatter."""
         check(lib().graft_ctx_set_fused_map_loads(self.handle, int(bool(on))))
+
+    def set_lean_stream(self, on):
+        """1 (default): per-block streaming sector steps run the lean kernel where it
+        applies; 0: always the general per-block kernel."""
+        check(lib().graft_ctx_set_lean_stream(self.handle, int(bool(on))))
 
     def reserve_workspace(self, nbytes):
         check(lib().graft_ctx_reserve_workspace(self.handle, int(nbytes)))

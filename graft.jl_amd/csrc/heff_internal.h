@@ -144,6 +144,46 @@ hipError_t launch_zstream_blocks(bool big_is_a, const void* A, const void* B, vo
                                  const int32_t* order, const StreamWork* work, int nwork, int conjA,
                                  int conjB, hipStream_t stream);
 
+// Lean form of the per-block streaming step, used when the small operand
+// fits in LDS, every output block has at most kLeanMaxS small-side outputs
+// and kLeanRank big-side legs, and every segment contracts at most
+// kLeanInlineK indices.  Everything a lane needs beyond its row index is precomputed on
+// the host: row digits come from a multiply-high by a per-leg reciprocal
+// (exact while J * dim < 2^32), the output offsets of the small-side index
+// per block, and per segment the small-side offsets inside the small
+// operand, which every workgroup stages in LDS once.
+constexpr int kLeanMaxS = 16;
+constexpr int kLeanSmallMax = 4096;      // small-operand elements staged in LDS (64 KB)
+constexpr int kLeanRank = 4;             // big-side legs (unit legs dropped)
+constexpr int kLeanInlineK = 8;          // contracted extent of a segment
+// Fixed-width records, read as whole 16-byte vectors (no per-field loads).
+struct alignas(16) LeanItem {
+  int64_t c_off;
+  int32_t J, S, seg_begin, seg_count, rank, pad;
+  uint32_t dims[kLeanRank];     // big-side legs
+  uint32_t magic[kLeanRank];    // ceil(2^32 / dims[l]) (unused for the last leg)
+  uint32_t cstr[kLeanRank];     // output strides of the big-side legs
+  uint32_t cso[kLeanMaxS];      // output offset of small-side index s
+};
+struct alignas(16) LeanSeg {
+  int64_t big_off;              // big block start
+  int32_t K, pad;
+  uint32_t bstr[kLeanRank];     // big block strides of the big-side legs
+  uint32_t sso[kLeanMaxS];      // small offset (block start included) of small-side index s
+  uint32_t kb[kLeanInlineK];    // big and small offsets of every contracted index k
+  uint32_t ks[kLeanInlineK];
+};
+// One wave's work: rows j0 .. j0 + 63 of the output blocks items[begin ..
+// begin + count) (the records are stored group by group), which share the
+// big operand's open sector tuple: same rows, same row digits.
+struct LeanWork {
+  int32_t begin, count, j0, pad;
+};
+// S_MAX: largest S over the items, rounded up to 4, 8 or 16.
+hipError_t launch_zstream_lean(bool big_is_a, int smax, const void* A, const void* B, void* C,
+                               const LeanItem* items, const LeanSeg* segs, const LeanWork* work,
+                               int nwork, int nsmall, int conjA, int conjB, hipStream_t stream);
+
 // Block <-> fused-matrix copies of the fused sector path (sector_kernels.hip).
 // A PermBlock maps one dense block (contiguous, first leg fastest) of a
 // block-sparse tensor onto a strided rectangle of a fused per-charge matrix.

@@ -228,6 +228,16 @@ struct StepExec {
   std::vector<StreamWork> swork;      // one entry per wave
   int32_t* d_sorder = nullptr;
   StreamWork* d_swork = nullptr;
+  // Lean form of the per-block streaming step (small operand staged in LDS)
+  bool lean = false;
+  int lean_smax = 0;                  // largest small-side extent of an output block
+  int64_t small_total = 0;            // elements of the small operand
+  std::vector<LeanItem> litems;
+  std::vector<LeanSeg> lsegs;
+  std::vector<LeanWork> lwork;
+  LeanItem* d_litems = nullptr;
+  LeanSeg* d_lsegs = nullptr;
+  LeanWork* d_lwork = nullptr;
   uint64_t partial_bytes = 0; // workspace for the split-K partial tiles
   // Fused sector path: the GEMM items above run on per-charge dense matrices
   // gathered from A and B and scattered into C.
@@ -286,6 +296,7 @@ struct graft_ctx {
   bool capturing = false;        // a graph capture is open on the stream
   int sector_fuse = 1;  // 0 per-block GEMM, 1 automatic, 2 fused / 3 per-block streaming for every multi-block step
   int fused_map_loads = 1;  // fused FP64 MFMA steps read / write the blocks through the maps
+  int lean_stream = 1;      // per-block streaming steps use the lean kernel where eligible
   uint64_t ws_generation = 0;    // bumped whenever the workspace moves
 };
 
@@ -793,6 +804,137 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
   return 0;
 }
 
+// Lean per-block streaming records (see LeanItem): built when the small
+// operand fits in LDS, every output block has at most kLeanMaxS small-side
+// outputs and at most kLeanRank big-side legs of extent > 1, every segment
+// contracts at most kLeanInlineK indices, and every big-side digit decodes
+// exactly by a multiply-high (J * dim < 2^32).  Records are stored group by
+// group (sorder); work entries run rows outer, the blocks of one big-side
+// tuple inner.
+static constexpr int64_t kLeanWalkAbove = 1 << 21;
+static void build_lean(StepExec& ex) {
+  ex.lean = false;
+  ex.litems.clear();
+  ex.lsegs.clear();
+  ex.lwork.clear();
+  auto fail = [&ex]() {
+    ex.litems.clear();
+    ex.lsegs.clear();
+    ex.lwork.clear();
+  };
+  if (ex.bstream == 0 || ex.small_total > kLeanSmallMax || ex.items.empty()) return;
+  const bool bigA = ex.bstream == 1;
+  int smax = 1;
+  for (const GemmItem& it : ex.items) {
+    const int S = bigA ? it.N : it.M;
+    if (S > kLeanMaxS) return;
+    smax = std::max(smax, S);
+  }
+  for (const GemmSeg& sg : ex.segs)
+    if (sg.K > kLeanInlineK) return;
+  ex.litems.reserve(ex.items.size());
+  ex.lsegs.reserve(ex.segs.size());
+  for (int32_t item_id : ex.sorder) {
+    const GemmItem& it = ex.items[item_id];
+    LeanItem li;
+    std::memset(&li, 0, sizeof(li));
+    li.c_off = it.c_off;
+    const int J = bigA ? it.M : it.N;
+    const int S = bigA ? it.N : it.M;
+    const int brank = bigA ? it.mrank : it.nrank;
+    const int srank = bigA ? it.nrank : it.mrank;
+    const int32_t* bdims = bigA ? it.mdims : it.ndims;
+    const int32_t* sdims = bigA ? it.ndims : it.mdims;
+    const uint32_t* cb = bigA ? it.c_mstr : it.c_nstr;
+    const uint32_t* cs = bigA ? it.c_nstr : it.c_mstr;
+    li.J = J;
+    li.S = S;
+    // big-side legs of extent > 1 (positions in the item)
+    int keep[GL], nk = 0;
+    for (int l = 0; l < brank; ++l) {
+      if (bdims[l] == 1) continue;
+      if ((uint64_t)J * (uint64_t)bdims[l] >= (1ull << 32) || nk == kLeanRank) return fail();
+      keep[nk++] = l;
+    }
+    li.rank = nk;
+    for (int q = 0; q < nk; ++q) {
+      const uint32_t d = (uint32_t)bdims[keep[q]];
+      li.dims[q] = d;
+      li.magic[q] = q + 1 < nk ? (uint32_t)(((1ull << 32) + d - 1) / d) : 0u;
+      li.cstr[q] = cb[keep[q]];
+    }
+    // small-side index s -> digits over the small legs
+    int64_t sdig[kLeanMaxS][GL];
+    for (int s0 = 0; s0 < S; ++s0) {
+      int64_t idx = s0;
+      for (int l = 0; l < srank; ++l) {
+        sdig[s0][l] = idx % sdims[l];
+        idx /= sdims[l];
+      }
+    }
+    for (int s0 = 0; s0 < S; ++s0) {
+      uint64_t o = 0;
+      for (int l = 0; l < srank; ++l) o += (uint64_t)sdig[s0][l] * cs[l];
+      li.cso[s0] = (uint32_t)o;
+    }
+    li.seg_begin = (int32_t)ex.lsegs.size();
+    li.seg_count = it.seg_count;
+    for (int g = it.seg_begin; g < it.seg_begin + it.seg_count; ++g) {
+      const GemmSeg& sg = ex.segs[g];
+      LeanSeg ls;
+      std::memset(&ls, 0, sizeof(ls));
+      ls.big_off = bigA ? sg.a_off : sg.b_off;
+      const int64_t soff = bigA ? sg.b_off : sg.a_off;
+      const uint32_t* bstr = bigA ? sg.a_mstr : sg.b_nstr;
+      const uint32_t* sstr = bigA ? sg.b_nstr : sg.a_mstr;
+      for (int q = 0; q < nk; ++q) ls.bstr[q] = bstr[keep[q]];
+      for (int s0 = 0; s0 < S; ++s0) {
+        int64_t o = soff;
+        for (int l = 0; l < srank; ++l) o += sdig[s0][l] * (int64_t)sstr[l];
+        ls.sso[s0] = (uint32_t)o;
+      }
+      ls.K = sg.K;
+      const uint32_t* kA = ex.koff.data() + sg.koff_idx;
+      const uint32_t* kbig = bigA ? kA : kA + sg.K;
+      const uint32_t* ksml = bigA ? kA + sg.K : kA;
+      for (int q = 0; q < sg.K; ++q) {
+        ls.kb[q] = kbig[q];
+        ls.ks[q] = ksml[q];
+      }
+      ex.lsegs.push_back(ls);
+    }
+    ex.litems.push_back(li);
+  }
+  // every block of a group has the big-side legs of its first block, so one
+  // row decode serves the group
+  for (size_t g = 0; g + 1 < ex.spanel.size(); ++g) {
+    const int b = ex.spanel[g], e = ex.spanel[g + 1];
+    const LeanItem& f = ex.litems[b];
+    for (int c = b + 1; c < e; ++c) {
+      const LeanItem& o = ex.litems[c];
+      bool same = o.J == f.J && o.rank == f.rank;
+      for (int q = 0; same && q < f.rank; ++q) same = o.dims[q] == f.dims[q];
+      if (!same) return fail();
+    }
+  }
+  // One block per entry unless that makes more than kLeanWalkAbove entries:
+  // walking the blocks of a group one after another trades memory-level
+  // parallelism for cache reuse, which pays only on very large steps.
+  int64_t single = 0;
+  for (size_t g = 0; g + 1 < ex.spanel.size(); ++g)
+    single += (int64_t)((ex.litems[ex.spanel[g]].J + 63) / 64) * (ex.spanel[g + 1] - ex.spanel[g]);
+  const int panel = (int)std::max<int64_t>(1, (single + kLeanWalkAbove - 1) / kLeanWalkAbove);
+  for (size_t g = 0; g + 1 < ex.spanel.size(); ++g) {
+    const int b = ex.spanel[g], e = ex.spanel[g + 1];
+    const int J = ex.litems[b].J;
+    for (int j0 = 0; j0 < J; j0 += 64)
+      for (int c = b; c < e; c += panel) ex.lwork.push_back({c, std::min(panel, e - c), j0, 0});
+  }
+  if (ex.lwork.size() >= (1u << 30)) return fail();
+  ex.lean_smax = smax;
+  ex.lean = true;
+}
+
 static int compile_step(const graft_pair_step& st, const TStruct& A, const TStruct& B,
                         const TStruct* fixed_out, TStruct& C, StepExec& ex, std::string& err,
                         int fuse_mode) {
@@ -1213,6 +1355,8 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
     ex.spanel.push_back((int32_t)ex.sorder.size());
     ex.bstream = bigside;
     ex.bstream_pref = stream_first;
+    ex.small_total = bigside == 1 ? B.total : A.total;
+    build_lean(ex);
   }
   // Orientation: decided by the largest output block's unit-stride leg.
   {
@@ -1332,6 +1476,8 @@ static int upload_step_meta(graft_plan_impl& p) {
     for (auto* v : {&ex.pa, &ex.pb, &ex.pc}) total += al(v->size() * sizeof(PermBlock));
     for (auto* v : {&ex.ca, &ex.cb, &ex.cc}) total += al(v->size() * sizeof(int32_t));
     total += al(ex.sorder.size() * sizeof(int32_t)) + al(ex.swork.size() * sizeof(StreamWork));
+    total += al(ex.litems.size() * sizeof(LeanItem)) + al(ex.lsegs.size() * sizeof(LeanSeg)) +
+             al(ex.lwork.size() * sizeof(LeanWork));
   }
   if (p.d_meta) {
     (void)hipFree(p.d_meta);
@@ -1378,6 +1524,16 @@ static int upload_step_meta(graft_plan_impl& p) {
     if (sw) std::memcpy(host.data() + off, ex.swork.data(), sw);
     ex.d_swork = sw ? (StreamWork*)((char*)p.d_meta + off) : nullptr;
     off += al(sw);
+    auto put = [&](const void* src, size_t sz) -> void* {
+      if (!sz) return nullptr;
+      std::memcpy(host.data() + off, src, sz);
+      void* d = (char*)p.d_meta + off;
+      off += al(sz);
+      return d;
+    };
+    ex.d_litems = (LeanItem*)put(ex.litems.data(), ex.litems.size() * sizeof(LeanItem));
+    ex.d_lsegs = (LeanSeg*)put(ex.lsegs.data(), ex.lsegs.size() * sizeof(LeanSeg));
+    ex.d_lwork = (LeanWork*)put(ex.lwork.data(), ex.lwork.size() * sizeof(LeanWork));
   }
   GRAFT_CHECK_HIP(hipMemcpy(p.d_meta, host.data(), total, hipMemcpyHostToDevice));
   return GRAFT_OK;
@@ -1527,7 +1683,12 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     B = Bf;
     C = Cf;
   }
-  if (ex.config == kBlockStreamConfig) {
+  if (ex.config == kBlockStreamConfig && ex.lean && ctx->lean_stream) {
+    // HBM-bound, FP64 whatever the precision (as the streaming kernels below)
+    GRAFT_CHECK_HIP(launch_zstream_lean(ex.bstream == 1, ex.lean_smax, A, B, C, ex.d_litems,
+                                        ex.d_lsegs, ex.d_lwork, (int)ex.lwork.size(),
+                                        (int)ex.small_total, ex.conjA, ex.conjB, ctx->stream));
+  } else if (ex.config == kBlockStreamConfig) {
     // HBM-bound, FP64 whatever the precision (as the streaming kernel below)
     GRAFT_CHECK_HIP(launch_zstream_blocks(ex.bstream == 1, A, B, C, ex.d_items, ex.d_segs,
                                           ex.d_koff, ex.d_sorder, ex.d_swork,
@@ -1680,6 +1841,12 @@ int graft_ctx_set_sector_fusion(graft_ctx* ctx, int32_t mode) {
 int graft_ctx_set_fused_map_loads(graft_ctx* ctx, int32_t on) {
   GRAFT_REQUIRE(ctx != nullptr, "null context");
   ctx->fused_map_loads = on ? 1 : 0;
+  return GRAFT_OK;
+}
+
+int graft_ctx_set_lean_stream(graft_ctx* ctx, int32_t on) {
+  GRAFT_REQUIRE(ctx != nullptr, "null context");
+  ctx->lean_stream = on ? 1 : 0;
   return GRAFT_OK;
 }
 

@@ -263,6 +263,144 @@ __global__ void __launch_bounds__(kSkinnyThreads)
   }
 }
 
+// Lean per-block streaming: one lane per big-side row, every small-side
+// output of that row in registers.  The workgroup stages the whole small
+// operand (conjugation applied) in LDS once; its waves then walk work
+// entries with a grid stride.  An entry is 64 rows of one output block (or
+// of every block of one big-side tuple, on very large steps).  Row digits
+// come from a multiply-high by the per-leg reciprocal the host computed.
+// Item and segment records are fixed-width and read as whole 16-byte
+// vectors before any use, so the wave waits once per record, not per field.
+template <bool BIG_IS_A, int SM>
+__global__ void __launch_bounds__(kSkinnyThreads)
+    zstream_lean_kernel(const double2* __restrict__ A, const double2* __restrict__ B,
+                        double2* __restrict__ C, const LeanItem* __restrict__ items,
+                        const LeanSeg* __restrict__ segs, const LeanWork* __restrict__ work,
+                        int nwork, int nsmall, int conjA, int conjB) {
+  extern __shared__ double2 sm[];
+  constexpr int kWaves = kSkinnyThreads / 64;
+  constexpr int R = kLeanRank;
+  static_assert(R == 4 && kLeanInlineK == 8 && SM % 4 == 0, "record layout");
+  const double2* __restrict__ small = BIG_IS_A ? B : A;
+  const double2* __restrict__ big = BIG_IS_A ? A : B;
+  const double sgSml = (BIG_IS_A ? conjB : conjA) ? -1.0 : 1.0;
+  const double sgBig = (BIG_IS_A ? conjA : conjB) ? -1.0 : 1.0;
+  for (int i = threadIdx.x; i < nsmall; i += kSkinnyThreads) {
+    double2 v = small[i];
+    v.y *= sgSml;
+    sm[i] = v;
+  }
+  __syncthreads();
+  const int lane = (int)(threadIdx.x & 63);
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int nw = (int)gridDim.x * kWaves;
+  for (int w = (int)blockIdx.x * kWaves + wave; w < nwork; w += nw) {
+    const LeanWork wk = work[w];
+    const LeanItem* it0 = items + wk.begin;
+    const int J = it0->J, rank = it0->rank;
+    const uint4 dims = *reinterpret_cast<const uint4*>(it0->dims);
+    const uint4 magic = *reinterpret_cast<const uint4*>(it0->magic);
+    const int j = wk.j0 + lane;
+    if (j >= J) continue;
+    // digits of j over the big-side legs (legs past rank get 0)
+    uint32_t d0 = (uint32_t)j, d1 = 0u, d2 = 0u, d3 = 0u;
+    if (rank > 1) {
+      const uint32_t q0 = __umulhi(d0, magic.x);
+      d1 = q0;
+      d0 -= q0 * dims.x;
+      if (rank > 2) {
+        const uint32_t q1 = __umulhi(d1, magic.y);
+        d2 = q1;
+        d1 -= q1 * dims.y;
+        if (rank > 3) {
+          const uint32_t q2 = __umulhi(d2, magic.z);
+          d3 = q2;
+          d2 -= q2 * dims.z;
+        }
+      }
+    }
+    for (int t = 0; t < wk.count; ++t) {
+      const LeanItem* it = it0 + t;
+      const int S = it->S;
+      const uint4 cstr = *reinterpret_cast<const uint4*>(it->cstr);
+      uint4 cso[SM / 4];
+#pragma unroll
+      for (int q = 0; q < SM / 4; ++q) cso[q] = reinterpret_cast<const uint4*>(it->cso)[q];
+      const uint32_t co = d0 * cstr.x + d1 * cstr.y + d2 * cstr.z + d3 * cstr.w;
+      double2 acc[SM];
+#pragma unroll
+      for (int s = 0; s < SM; ++s) acc[s] = double2{0.0, 0.0};
+      const int gend = it->seg_begin + it->seg_count;
+      for (int g = it->seg_begin; g < gend; ++g) {
+        const LeanSeg* sg = segs + g;
+        const uint4 bstr = *reinterpret_cast<const uint4*>(sg->bstr);
+        const uint4 kb0 = reinterpret_cast<const uint4*>(sg->kb)[0];
+        const uint4 kb1 = reinterpret_cast<const uint4*>(sg->kb)[1];
+        const uint4 ks0 = reinterpret_cast<const uint4*>(sg->ks)[0];
+        const uint4 ks1 = reinterpret_cast<const uint4*>(sg->ks)[1];
+        uint4 sso[SM / 4];
+#pragma unroll
+        for (int q = 0; q < SM / 4; ++q) sso[q] = reinterpret_cast<const uint4*>(sg->sso)[q];
+        const int K = sg->K;
+        const uint32_t kbv[8] = {kb0.x, kb0.y, kb0.z, kb0.w, kb1.x, kb1.y, kb1.z, kb1.w};
+        const uint32_t ksv[8] = {ks0.x, ks0.y, ks0.z, ks0.w, ks1.x, ks1.y, ks1.z, ks1.w};
+        const double2* bp =
+            big + sg->big_off + (d0 * bstr.x + d1 * bstr.y + d2 * bstr.z + d3 * bstr.w);
+#pragma unroll
+        for (int k = 0; k < kLeanInlineK; ++k) {
+          if (k < K) {
+            double2 x = bp[kbv[k]];
+            x.y *= sgBig;
+#pragma unroll
+            for (int s = 0; s < SM; ++s) {
+              if (s < S) {
+                const uint32_t so = (&sso[s / 4].x)[s % 4];
+                const double2 v = sm[ksv[k] + so];
+                acc[s].x += x.x * v.x - x.y * v.y;
+                acc[s].y += x.x * v.y + x.y * v.x;
+              }
+            }
+          }
+        }
+      }
+      double2* cp = C + it->c_off + co;
+#pragma unroll
+      for (int s = 0; s < SM; ++s)
+        if (s < S) cp[(&cso[s / 4].x)[s % 4]] = acc[s];
+    }
+  }
+}
+
+hipError_t launch_zstream_lean(bool big_is_a, int smax, const void* A, const void* B, void* C,
+                               const LeanItem* items, const LeanSeg* segs, const LeanWork* work,
+                               int nwork, int nsmall, int conjA, int conjB, hipStream_t stream) {
+  if (nwork <= 0) return hipSuccess;
+  if (nsmall < 0 || nsmall > kLeanSmallMax || smax < 1 || smax > kLeanMaxS) return hipErrorInvalidValue;
+  constexpr int kWaves = kSkinnyThreads / 64;
+  // enough workgroups to fill the chip several times over; each stages the
+  // small operand once, so fewer and longer-lived when it is large
+  const int cap = nsmall > 1024 ? 2048 : 8192;
+  const int grid = std::min((nwork + kWaves - 1) / kWaves, cap);
+  const size_t lds = (size_t)std::max(nsmall, 1) * sizeof(double2);
+  const double2* a = (const double2*)A;
+  const double2* b = (const double2*)B;
+  double2* c = (double2*)C;
+#define GRAFT_LEAN(BIGA, SMV)                                                                      \
+  hipLaunchKernelGGL((zstream_lean_kernel<BIGA, SMV>), dim3(grid), dim3(kSkinnyThreads), lds,     \
+                     stream, a, b, c, items, segs, work, nwork, nsmall, conjA, conjB)
+  if (big_is_a) {
+    if (smax <= 4) GRAFT_LEAN(true, 4);
+    else if (smax <= 8) GRAFT_LEAN(true, 8);
+    else GRAFT_LEAN(true, 16);
+  } else {
+    if (smax <= 4) GRAFT_LEAN(false, 4);
+    else if (smax <= 8) GRAFT_LEAN(false, 8);
+    else GRAFT_LEAN(false, 16);
+  }
+#undef GRAFT_LEAN
+  return hipGetLastError();
+}
+
 hipError_t launch_zstream_blocks(bool big_is_a, const void* A, const void* B, void* C,
                                  const GemmItem* items, const GemmSeg* segs, const uint32_t* koff,
                                  const int32_t* order, const StreamWork* work, int nwork, int conjA,

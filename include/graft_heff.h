@@ -131,6 +131,13 @@ int graft_ctx_set_sector_fusion(graft_ctx* ctx, int32_t mode);
  * matrices, multiply, scatter (three more launches).  Same results.  No
  * reference counterpart (TensorKit multiplies block by block). */
 int graft_ctx_set_fused_map_loads(graft_ctx* ctx, int32_t on);
+/* Which kernel runs the per-block streaming sector steps: 1 (default) the
+ * lean kernel where it applies (small operand of at most 4096 elements,
+ * staged in LDS per workgroup; at most 16 small-side outputs per output
+ * block), 0 always the general per-block kernel.  Same arithmetic, same
+ * summation order.  No reference counterpart (TensorKit multiplies block by
+ * block). */
+int graft_ctx_set_lean_stream(graft_ctx* ctx, int32_t on);
 int graft_synchronize(graft_ctx* ctx);
 const char* graft_last_error(void);
 const char* graft_version(void);
