@@ -196,6 +196,21 @@ struct PermBlock {
   int32_t dims[GRAFT_MAX_RANK];
   int64_t ostr[GRAFT_MAX_RANK];   // fused-matrix stride of every block leg
 };
+// A rectangle of a fused per-charge matrix that no block of the operand
+// covers (a pair of sector tuples without a stored block): it must read as
+// zero when a producing step writes that operand straight into the fused
+// layout.  Element (i, j) lives at base + i * rstr + j * cstr.
+struct HoleRect {
+  int64_t base;
+  int32_t rows, cols;
+  int32_t rstr, cstr;
+};
+hipError_t launch_zero_holes(const HoleRect* holes, int nholes, void* fused, hipStream_t stream);
+// Handoff of a fused step's result into the next fused step's operand
+// layout: map[f] = fwd[src[f]] (kMapZero where src[f] is not an element or
+// the element has no fused entry).
+hipError_t launch_compose_map(const uint32_t* src, int64_t n, const uint32_t* fwd, uint32_t* out,
+                              hipStream_t stream);
 // Per-element offset maps, built once per plan, and the copies that use them.
 // Blocked tensors of the fused path hold fewer than kMapZero elements.
 constexpr uint32_t kMapGap = 0xFFFFFFFFu;    // element between blocks: untouched

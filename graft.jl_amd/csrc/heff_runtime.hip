@@ -12,6 +12,7 @@
 #include <functional>
 #include <map>
 #include <unordered_map>
+#include <unordered_set>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -258,6 +259,17 @@ struct StepExec {
   uint32_t* d_mci = nullptr;             // fused C offset -> blocked element (mapped stores)
   bool c_complete = false;               // every blocked C element has a fused entry
   uint64_t fused_bytes = 0;
+  // Handoff (FP64 mapped path): this step stores its result through d_mho
+  // straight into the fused operand layout of step ho_step (side ho_side:
+  // 0 A, 1 B), whose hole rectangles are zeroed first; that step then reads
+  // the operand in place, with no gather.
+  int ho_step = -1, ho_side = 0;
+  uint32_t* d_mho = nullptr;
+  bool ho_in_a = false, ho_in_b = false;   // an operand arrives in fused layout
+  int ho_from_a = -1, ho_from_b = -1;      // the step that produces it
+  std::vector<HoleRect> holes_a, holes_b;  // fused rectangles no operand block covers
+  HoleRect* d_holes_a = nullptr;
+  HoleRect* d_holes_b = nullptr;
 };
 
 struct graft_plan_impl {
@@ -461,7 +473,7 @@ static std::vector<int32_t> chunk_table(const std::vector<PermBlock>& blocks, in
 // (r, open index), contraction (c, contracted index), columns (s, open index).
 static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruct& B,
                        const TStruct& C, const std::vector<int>& pos_of_out, StepExec& ex,
-                       std::string& err) {
+                       std::string& err, bool want_holes_a, bool want_holes_b) {
   const int nOA = st.n_oindA, nCA = st.n_cindA, nOB = st.n_oindB;
   using Key = SecKey;
   // Tuple ids in order of first appearance over the blocks (deterministic).
@@ -735,6 +747,51 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
   }
   ex.c_complete = std::all_of(ex.pc.begin(), ex.pc.end(),
                               [](const PermBlock& b) { return b.other >= 0; });
+  // Hole rectangles of the fused operands: tuple pairs of a component with no
+  // stored block (only for operands another step may produce).
+  auto holes = [&](bool side_a, std::vector<HoleRect>& out) {
+    out.clear();
+    const int n1 = side_a ? nr : nc, n2 = side_a ? nc : ns;
+    std::unordered_set<int64_t> have;
+    if (side_a) {
+      for (size_t i = 0; i < A.blocks.size(); ++i) have.insert((int64_t)aR[i] * n2 + aC[i]);
+    } else {
+      for (size_t i = 0; i < B.blocks.size(); ++i) have.insert((int64_t)bC[i] * n2 + bS[i]);
+    }
+    std::vector<std::vector<int>> t1(comps.size()), t2(comps.size());
+    for (int x = 0; x < n1; ++x) {
+      const int gi = side_a ? rg[x] : cg[x];
+      if (gi >= 0) t1[gi].push_back(x);
+    }
+    for (int y = 0; y < n2; ++y) {
+      const int gi = side_a ? cg[y] : sg[y];
+      if (gi >= 0) t2[gi].push_back(y);
+    }
+    for (size_t gi = 0; gi < comps.size(); ++gi) {
+      const Comp& g = comps[gi];
+      for (int x : t1[gi])
+        for (int y : t2[gi]) {
+          if (have.count((int64_t)x * n2 + y)) continue;
+          HoleRect h;
+          if (side_a) {   // rows r, columns c of A_g: (m, k) at m*K + k or m + M*k
+            h.rows = (int32_t)tuple_size(A, st.oindA, R.keys[x]);
+            h.cols = (int32_t)tuple_size(A, st.cindA, Cc.keys[y]);
+            h.rstr = a_kf ? (int32_t)g.K : 1;
+            h.cstr = a_kf ? 1 : (int32_t)g.M;
+            h.base = g.aoff + ro[x] * h.rstr + co[y] * h.cstr;
+          } else {        // rows c, columns s of B_g: (k, n) at k + K*n or n + N*k
+            h.rows = (int32_t)tuple_size(A, st.cindA, Cc.keys[x]);
+            h.cols = (int32_t)tuple_size(B, st.oindB, S.keys[y]);
+            h.rstr = b_kf ? 1 : (int32_t)g.N;
+            h.cstr = b_kf ? (int32_t)g.K : 1;
+            h.base = g.boff + co[x] * h.rstr + so[y] * h.cstr;
+          }
+          out.push_back(h);
+        }
+    }
+  };
+  if (want_holes_a) holes(true, ex.holes_a);
+  if (want_holes_b) holes(false, ex.holes_b);
   ex.fa = fa;
   ex.fb = fb;
   ex.fc = fc;
@@ -937,7 +994,7 @@ static void build_lean(StepExec& ex) {
 
 static int compile_step(const graft_pair_step& st, const TStruct& A, const TStruct& B,
                         const TStruct* fixed_out, TStruct& C, StepExec& ex, std::string& err,
-                        int fuse_mode) {
+                        int fuse_mode, bool want_holes_a, bool want_holes_b) {
   const int nOA = st.n_oindA, nCA = st.n_cindA, nOB = st.n_oindB, nCB = st.n_cindB;
   if (nCA != nCB || nOA + nCA != A.rank || nOB + nCB != B.rank) {
     err = "pair step leg counts do not match operand ranks";
@@ -1179,7 +1236,7 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
         }
       }
       auto t2 = clk();
-      if (build_fused(st, A, B, C, pos_of_out, ex, err)) return 1;
+      if (build_fused(st, A, B, C, pos_of_out, ex, err, want_holes_a, want_holes_b)) return 1;
       if (timing)
         std::fprintf(stderr, "  contrib %.2f ms (%zu keys), C blocks %.2f ms, fused build %.2f ms\n",
                      t_contrib, nkeys, t_cblocks, ms_since(t2));
@@ -1478,6 +1535,7 @@ static int upload_step_meta(graft_plan_impl& p) {
     total += al(ex.sorder.size() * sizeof(int32_t)) + al(ex.swork.size() * sizeof(StreamWork));
     total += al(ex.litems.size() * sizeof(LeanItem)) + al(ex.lsegs.size() * sizeof(LeanSeg)) +
              al(ex.lwork.size() * sizeof(LeanWork));
+    total += al(ex.holes_a.size() * sizeof(HoleRect)) + al(ex.holes_b.size() * sizeof(HoleRect));
   }
   if (p.d_meta) {
     (void)hipFree(p.d_meta);
@@ -1534,6 +1592,8 @@ static int upload_step_meta(graft_plan_impl& p) {
     ex.d_litems = (LeanItem*)put(ex.litems.data(), ex.litems.size() * sizeof(LeanItem));
     ex.d_lsegs = (LeanSeg*)put(ex.lsegs.data(), ex.lsegs.size() * sizeof(LeanSeg));
     ex.d_lwork = (LeanWork*)put(ex.lwork.data(), ex.lwork.size() * sizeof(LeanWork));
+    ex.d_holes_a = (HoleRect*)put(ex.holes_a.data(), ex.holes_a.size() * sizeof(HoleRect));
+    ex.d_holes_b = (HoleRect*)put(ex.holes_b.data(), ex.holes_b.size() * sizeof(HoleRect));
   }
   GRAFT_CHECK_HIP(hipMemcpy(p.d_meta, host.data(), total, hipMemcpyHostToDevice));
   return GRAFT_OK;
@@ -1550,7 +1610,8 @@ static int build_maps(graft_ctx* ctx, graft_plan_impl& p) {
   auto al = [](int64_t n) { return ((size_t)n * sizeof(uint32_t) + 255) & ~size_t(255); };
   for (auto& ex : p.steps)
     if (ex.fused) {
-      total += al(ex.fa) + al(ex.fb) + al(ex.totC) + (ex.c_complete ? al(ex.fc) : 0);
+      total += al(ex.fa) + al(ex.fb) + al(ex.totC) + (ex.c_complete ? al(ex.fc) : 0) +
+               (ex.ho_step >= 0 ? al(ex.fc) : 0);
       tmp_n = std::max(tmp_n, std::max(ex.totA, ex.totB));
     }
   if (total == 0) return GRAFT_OK;
@@ -1573,12 +1634,26 @@ static int build_maps(graft_ctx* ctx, graft_plan_impl& p) {
       ex.d_mci = (uint32_t*)base;
       base += al(ex.fc);
     }
+    ex.d_mho = nullptr;
+    if (ex.ho_step >= 0) {
+      ex.d_mho = (uint32_t*)base;
+      base += al(ex.fc);
+    }
     // (an operand without blocks has no forward map: its inverse is all zero)
     const int64_t nA = ex.pa.empty() ? 0 : ex.totA, nB = ex.pb.empty() ? 0 : ex.totB;
+    // a producer handing its result to this step stores through its own
+    // fused-to-blocked map composed with this step's forward operand map
+    auto compose = [&](int from) {
+      if (from < 0 || e != hipSuccess) return;
+      const StepExec& es = p.steps[from];
+      e = launch_compose_map(es.d_mci, es.fc, tmp, es.d_mho, ctx->stream);
+    };
     e = launch_perm_map(ex.d_pa, (int)ex.pa.size(), ex.d_ca, nA, tmp, ctx->stream);
+    if (e == hipSuccess && ex.ho_in_a) compose(ex.ho_from_a);
     if (e == hipSuccess) e = launch_invert_map(tmp, nA, ex.d_ma, ex.fa, ctx->stream);
     if (e == hipSuccess)
       e = launch_perm_map(ex.d_pb, (int)ex.pb.size(), ex.d_cb, nB, tmp, ctx->stream);
+    if (e == hipSuccess && ex.ho_in_b) compose(ex.ho_from_b);
     if (e == hipSuccess) e = launch_invert_map(tmp, nB, ex.d_mb, ex.fb, ctx->stream);
     if (e == hipSuccess)
       e = launch_perm_map(ex.d_pc, (int)ex.pc.size(), ex.d_cc, ex.totC, ex.d_mc, ctx->stream);
@@ -1640,21 +1715,36 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
   }
   void* partial = ex.ksplit > 1 ? ws + p.ws_partial : nullptr;
   void* Cdst = C;
-  if (ex.fused && ctx->fused_map_loads && precision == GRAFT_PREC_FP64 && ex.d_mci != nullptr &&
-      ex.config < kNumTileConfigs && ex.ksplit == 1) {
+  auto mapped = [&](const StepExec& e) {
+    return e.fused && ctx->fused_map_loads && precision == GRAFT_PREC_FP64 && e.d_mci != nullptr &&
+           e.config < kNumTileConfigs && e.ksplit == 1;
+  };
+  if (mapped(ex)) {
     // The GEMM writes the result blocks through the element map (no scatter
     // pass).  An operand the tiles read at most twice is also read through
     // its map; one read more often is gathered once into its fused matrix.
+    // Handoffs: an operand the producing step already wrote in this step's
+    // fused layout is read in place; a result this step hands to the next
+    // fused step is stored in that step's layout (its holes zeroed first).
     const TileConfig& tc = kTileConfigs[ex.config];
-    const bool ma = (ex.maxN + tc.bn - 1) / tc.bn <= 2;
-    const bool mb = (ex.maxM + tc.bm - 1) / tc.bm <= 2;
+    const bool ina = ex.ho_in_a && mapped(p.steps[ex.ho_from_a]);
+    const bool inb = ex.ho_in_b && mapped(p.steps[ex.ho_from_b]);
+    const bool hout = ex.ho_step >= 0 && mapped(p.steps[ex.ho_step]);
+    const bool ma = !ina && (ex.maxN + tc.bn - 1) / tc.bn <= 2;
+    const bool mb = !inb && (ex.maxM + tc.bm - 1) / tc.bm <= 2;
     auto al = [](int64_t x) { return (size_t)((x * 16 + 255) / 256) * 256; };
     char* fb = ws + p.ws_fused;
-    void* Af = fb;
-    void* Bf = fb + al(ex.fa);
-    if (!ma) GRAFT_CHECK_HIP(launch_map_gather(A, Af, ex.d_ma, ex.fa, ctx->stream));
-    if (!mb) GRAFT_CHECK_HIP(launch_map_gather(B, Bf, ex.d_mb, ex.fb, ctx->stream));
-    const FusedMaps maps{ma ? ex.d_ma : nullptr, mb ? ex.d_mb : nullptr, ex.d_mci};
+    void* Af = ina ? A : fb;
+    void* Bf = inb ? B : fb + al(ex.fa);
+    if (!ma && !ina) GRAFT_CHECK_HIP(launch_map_gather(A, Af, ex.d_ma, ex.fa, ctx->stream));
+    if (!mb && !inb) GRAFT_CHECK_HIP(launch_map_gather(B, Bf, ex.d_mb, ex.fb, ctx->stream));
+    if (hout) {
+      const StepExec& et = p.steps[ex.ho_step];
+      const HoleRect* h = ex.ho_side == 0 ? et.d_holes_a : et.d_holes_b;
+      const int nh = (int)(ex.ho_side == 0 ? et.holes_a.size() : et.holes_b.size());
+      GRAFT_CHECK_HIP(launch_zero_holes(h, nh, C, ctx->stream));
+    }
+    const FusedMaps maps{ma ? ex.d_ma : nullptr, mb ? ex.d_mb : nullptr, hout ? ex.d_mho : ex.d_mci};
     if (ex.config == 11 || ex.config == 15) {
       // every component of a fused step loads A and B in one direction
       const int dirs = ex.segs.empty() ? 0
@@ -1967,7 +2057,7 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
     const TStruct* fixed = (st.dst == output_slot && have_out) ? &outfixed : nullptr;
     TStruct C;
     if (compile_step(st, p.slots[st.a], p.slots[st.b], fixed, C, p.steps[s], err,
-                     ctx->sector_fuse)) {
+                     ctx->sector_fuse, st.a >= ninputs, st.b >= ninputs)) {
       set_error("step " + std::to_string(s) + ": " + err);
       return GRAFT_E_INVALID;
     }
@@ -1995,6 +2085,40 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
   GRAFT_REQUIRE(defined[output_slot], "output slot never produced");
   for (int s = ninputs; s < nslots; ++s)
     GRAFT_REQUIRE(s == output_slot || dies[s] >= 0, "an intermediate is never consumed");
+  // Handoffs: a fused step whose result feeds a fused step as an operand
+  // stores it in that step's fused layout (its slot grows to the fused
+  // matrix when the component matrices hold hole rectangles).
+  std::vector<int> producer(nslots, -1);
+  std::vector<int64_t> need(nslots, 0);
+  for (int s = 0; s < nsteps; ++s) producer[steps[s].dst] = s;
+  for (int t = 0; t < nsteps; ++t) {
+    StepExec& et = p.steps[t];
+    if (!et.fused) {
+      et.holes_a.clear();
+      et.holes_b.clear();
+      continue;
+    }
+    for (int side = 0; side < 2; ++side) {
+      const int X = side == 0 ? steps[t].a : steps[t].b;
+      const int s = (X >= ninputs && X != output_slot) ? producer[X] : -1;
+      StepExec* es = s >= 0 ? &p.steps[s] : nullptr;
+      if (!es || !es->fused || !es->c_complete || es->ho_step >= 0) {
+        (side == 0 ? et.holes_a : et.holes_b).clear();
+        continue;
+      }
+      es->ho_step = t;
+      es->ho_side = side;
+      if (side == 0) {
+        et.ho_in_a = true;
+        et.ho_from_a = s;
+        need[X] = std::max(need[X], et.fa);
+      } else {
+        et.ho_in_b = true;
+        et.ho_from_b = s;
+        need[X] = std::max(need[X], et.fb);
+      }
+    }
+  }
   // Workspace: first-fit over liveness intervals [born, dies].
   p.ws_offset.assign(nslots, -1);
   struct Live { int64_t off, size; int b, d; };
@@ -2002,7 +2126,7 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
   uint64_t peak = 0;
   for (int s = ninputs; s < nslots; ++s) {
     if (s == output_slot) continue;
-    const int64_t size = ((p.slots[s].total * 16 + 255) / 256) * 256;
+    const int64_t size = ((std::max(p.slots[s].total, need[s]) * 16 + 255) / 256) * 256;
     const int b = born[s], d = dies[s];
     std::vector<std::pair<int64_t, int64_t>> busy;
     for (auto& l : placed)

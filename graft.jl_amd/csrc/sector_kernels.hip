@@ -85,6 +85,31 @@ __global__ void __launch_bounds__(kPermThreads)
   }
 }
 
+// Zero the hole rectangles of a fused operand matrix: one workgroup per
+// rectangle, threads over its elements.
+__global__ void __launch_bounds__(kPermThreads)
+    zero_holes_kernel(const HoleRect* __restrict__ holes, int nholes, double2* __restrict__ fused) {
+  for (int h = blockIdx.x; h < nholes; h += gridDim.x) {
+    const HoleRect r = holes[h];
+    const int n = r.rows * r.cols;
+    for (int e = threadIdx.x; e < n; e += kPermThreads) {
+      const int j = e / r.rows, i = e - j * r.rows;
+      fused[r.base + (int64_t)i * r.rstr + (int64_t)j * r.cstr] = double2{0.0, 0.0};
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kPermThreads)
+    compose_map_kernel(const uint32_t* __restrict__ src, int64_t n, const uint32_t* __restrict__ fwd,
+                       uint32_t* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t e = src[i];
+    const uint32_t f = e < kMapZero ? fwd[e] : kMapZero;
+    out[i] = f < kMapZero ? f : kMapZero;
+  }
+}
+
 static int perm_grid(int64_t n) {
   int64_t g = (n + kPermThreads - 1) / kPermThreads;
   if (g > 16384) g = 16384;
@@ -108,6 +133,22 @@ hipError_t launch_invert_map(const uint32_t* map, int64_t total, uint32_t* inv, 
   if (total > 0)
     hipLaunchKernelGGL(invert_map_kernel, dim3(perm_grid(total)), dim3(kPermThreads), 0, stream,
                        map, total, inv);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_holes(const HoleRect* holes, int nholes, void* fused, hipStream_t stream) {
+  if (nholes <= 0) return hipSuccess;
+  const int grid = nholes < 65536 ? nholes : 65536;
+  hipLaunchKernelGGL(zero_holes_kernel, dim3(grid), dim3(kPermThreads), 0, stream, holes, nholes,
+                     (double2*)fused);
+  return hipGetLastError();
+}
+
+hipError_t launch_compose_map(const uint32_t* src, int64_t n, const uint32_t* fwd, uint32_t* out,
+                              hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(compose_map_kernel, dim3(perm_grid(n)), dim3(kPermThreads), 0, stream, src, n,
+                     fwd, out);
   return hipGetLastError();
 }
 
