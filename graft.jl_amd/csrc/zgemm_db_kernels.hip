@@ -572,7 +572,7 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
 // stored through the C map.  Same double-buffered pipeline; every element's
 // source index (map value, or fused index) is read one k-tile ahead of its
 // data load, and the C map values of the tile are read before the k loop.
-template <int BM, int BN, int BK, int WM, int WN, bool TR, bool AKF, bool BKF>
+template <int BM, int BN, int BK, int WM, int WN, bool TR, bool AKF, bool BKF, bool PIPE>
 __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN), 2)
     zgemm3m_map_kernel(const double2* __restrict__ Abase, const double2* __restrict__ Bbase,
                        double2* __restrict__ Cbase, const GemmItem* __restrict__ items, int nitems,
@@ -737,15 +737,74 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN), 2)
   sources(BK);
   store(0);
   __syncthreads();
-  for (int kt = 0; kt < nkt; ++kt) {
-    const int buf = kt & 1;
-    if (kt + 1 < nkt) {
-      fetch();                    // k-tile kt + 1
-      sources((kt + 2) * BK);     // k-tile kt + 2 (out of range: zero)
+  if constexpr (!PIPE) {
+    for (int kt = 0; kt < nkt; ++kt) {
+      const int buf = kt & 1;
+      if (kt + 1 < nkt) {
+        fetch();                    // k-tile kt + 1
+        sources((kt + 2) * BK);     // k-tile kt + 2 (out of range: zero)
+      }
+      compute(buf);
+      if (kt + 1 < nkt) store(buf ^ 1);
+      __syncthreads();
     }
-    compute(buf);
-    if (kt + 1 < nkt) store(buf ^ 1);
-    __syncthreads();
+  } else {
+    // Pipelined form (the dense pipelined kernel's schedule): the MFMA
+    // fragments are double-buffered in registers, so the LDS reads of step
+    // kk + 1 are in flight while the MFMAs of step kk issue; the next
+    // k-tile's global loads go out before the MFMA block and are staged
+    // after it.  The last k-tile also loads and stages a zero tile (sources
+    // past K), which keeps the loop body free of branches.
+    constexpr int NKK = BK / 4;
+    static_assert(NKK % 2 == 0, "fragment double-buffering needs an even k-step count");
+    dbl2 fa[2][FM], fb[2][FN];
+    auto frag = [&](int set, int buf, int kk) {
+      const double2* as = As[buf] + (kk * 4 + kr) * LA + wm0 + li;
+      const double2* bs = Bs[buf] + (kk * 4 + kr) * LB + wn0 + li;
+#pragma unroll
+      for (int a = 0; a < FM; ++a) fa[set][a] = ld2(as + a * 16);
+#pragma unroll
+      for (int b = 0; b < FN; ++b) fb[set][b] = ld2(bs + b * 16);
+    };
+    auto mma = [&](int set) {
+      double su[FM], sv[FN];
+#pragma unroll
+      for (int a = 0; a < FM; ++a) su[a] = fa[set][a].x + fa[set][a].y;
+#pragma unroll
+      for (int b = 0; b < FN; ++b) sv[b] = fb[set][b].x + fb[set][b].y;
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+          if constexpr (TR) {
+            acc[0][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[set][b].x, fa[set][a].x, acc[0][a][b], 0, 0, 0);
+            acc[1][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[set][b].y, fa[set][a].y, acc[1][a][b], 0, 0, 0);
+            acc[2][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(sv[b], su[a], acc[2][a][b], 0, 0, 0);
+          } else {
+            acc[0][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[set][a].x, fb[set][b].x, acc[0][a][b], 0, 0, 0);
+            acc[1][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[set][a].y, fb[set][b].y, acc[1][a][b], 0, 0, 0);
+            acc[2][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(su[a], sv[b], acc[2][a][b], 0, 0, 0);
+          }
+        }
+    };
+    frag(0, 0, 0);
+    for (int kt = 0; kt < nkt; ++kt) {
+      const int buf = kt & 1;
+      fetch();                    // k-tile kt + 1 (zeros past K)
+      sources((kt + 2) * BK);     // k-tile kt + 2
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int kk = 0; kk < NKK; ++kk) {
+        if (kk + 1 < NKK) frag((kk + 1) & 1, buf, kk + 1);
+        mma(kk & 1);
+      }
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+      store(buf ^ 1);
+      __syncthreads();
+      frag(0, buf ^ 1, 0);
+    }
   }
 
 
@@ -773,8 +832,12 @@ hipError_t launch_zgemm_dbmap(int config, const void* A, const void* B, void* C,
   const double2* b = (const double2*)B;
   double2* c = (double2*)C;
 #define GRAFT_MAPK(TRV, AKV, BKV)                                                                   \
-  hipLaunchKernelGGL((zgemm3m_map_kernel<64, 64, 16, 32, 32, TRV, AKV, BKV>), grid, block, 0,     \
-                     stream, a, b, c, items, nitems, segs, total_tiles, conjA, conjB, maps, xs)
+  if (config == 15)                                                                                 \
+    hipLaunchKernelGGL((zgemm3m_map_kernel<64, 64, 16, 32, 32, TRV, AKV, BKV, true>), grid, block, \
+                       0, stream, a, b, c, items, nitems, segs, total_tiles, conjA, conjB, maps, xs); \
+  else                                                                                              \
+    hipLaunchKernelGGL((zgemm3m_map_kernel<64, 64, 16, 32, 32, TRV, AKV, BKV, false>), grid, block, \
+                       0, stream, a, b, c, items, nitems, segs, total_tiles, conjA, conjB, maps, xs)
   switch ((kfast & 3) | (tr ? 4 : 0)) {
     case 0: GRAFT_MAPK(false, false, false); break;
     case 1: GRAFT_MAPK(false, true, false); break;
