@@ -374,6 +374,8 @@ def hubbard_bench(D, nsweeps=2, precision="fp64", sigma=1.6, seed=5, dense=True,
         ms = e0.elapsed_time(e1) / reps
         info = h2.native.step_info()
         return {"ms": ms, "flops": h2.flops, "tflops": h2.flops / ms / 1e9,
+                "flops_reference_model": h2.reference_flops,
+                "tflops_reference_model": h2.reference_flops / ms / 1e9,
                 "theta_elements": int(x.numel), "max_blocks_per_step": max(s["blocks"] for s in info)}
 
     out["apply_sector"] = one_apply(psi, Hs, True)
@@ -570,8 +572,11 @@ def hubbard_apply_bench(D, reps=20, sigma=1.6, seed=5):
         lanczos["device_graph" if device else "per_call"] = {
             "ms_per_apply": (time.perf_counter() - t) * 1e3 / info.numops, "applies": info.numops,
             "ritz_value": float(vals[0])}
+    tflops_ref = h2.reference_flops / ms_tuned / 1e9
     return {"D": D, "ms_per_apply": ms_tuned, "ms_per_apply_untuned": ms, "autotune_s": t_at,
             "flops": h2.flops, "tflops": h2.flops / ms_tuned / 1e9,
+            "flops_reference_model": h2.reference_flops, "tflops_reference_model": tflops_ref,
+            "fp64_mfma_fraction_reference_model": tflops_ref / MI355X_FP64_MFMA_TFLOPS,
             "step_ms": step_ms_tuned, "step_ms_untuned": step_ms, "steps": h2.native.step_info(),
             "theta_elements": int(x.numel),
             "blocks": len(x.blocks.blocks) if hasattr(x.blocks, "blocks") else None,

@@ -473,8 +473,21 @@ class EffectiveMap:
 
     @property
     def flops(self):
-        """Real floating-point operations per apply: 8 per complex multiply-add."""
+        """Real floating-point operations per apply: 8 per complex multiply-add
+        over the stored block pairs the plan executes."""
         return 8.0 * self.native.macs
+
+    @property
+    def reference_flops(self):
+        """Real floating-point operations per apply in the reference's cost
+        model: 8 per complex multiply-add, and every coupled-sector block of a
+        sector step counted dense, sum_q m_q k_q n_q (Backend.jl:328-368, the
+        sector_flops = sum_q 2 m_q k_q n_q of TensorKit's mul!; 4 x
+        sector_flops).  For dense maps this equals flops."""
+        sf = getattr(self.native.plan, "sector_flops", math.nan)
+        if self.blocks is not None and math.isfinite(sf):
+            return 4.0 * sf
+        return self.flops
 
     def __call__(self, x, out=None):
         if x.dims != self.xdims:

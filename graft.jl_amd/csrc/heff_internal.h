@@ -156,6 +156,7 @@ constexpr int kLeanMaxS = 16;
 constexpr int kLeanSmallMax = 4096;      // small-operand elements staged in LDS (64 KB)
 constexpr int kLeanRank = 4;             // big-side legs (unit legs dropped)
 constexpr int kLeanInlineK = 8;          // contracted extent of a segment
+constexpr int kLeanRows = 2;             // big-side rows per lane (64 * kLeanRows per entry)
 // Fixed-width records, read as whole 16-byte vectors (no per-field loads).
 struct alignas(16) LeanItem {
   int64_t c_off;
@@ -173,7 +174,7 @@ struct alignas(16) LeanSeg {
   uint32_t kb[kLeanInlineK];    // big and small offsets of every contracted index k
   uint32_t ks[kLeanInlineK];
 };
-// One wave's work: rows j0 .. j0 + 63 of the output blocks items[begin ..
+// One wave's work: rows j0 .. j0 + 64 * kLeanRows - 1 of the output blocks items[begin ..
 // begin + count) (the records are stored group by group), which share the
 // big operand's open sector tuple: same rows, same row digits.
 struct LeanWork {

@@ -979,12 +979,13 @@ static void build_lean(StepExec& ex) {
   // parallelism for cache reuse, which pays only on very large steps.
   int64_t single = 0;
   for (size_t g = 0; g + 1 < ex.spanel.size(); ++g)
-    single += (int64_t)((ex.litems[ex.spanel[g]].J + 63) / 64) * (ex.spanel[g + 1] - ex.spanel[g]);
+    single += (int64_t)((ex.litems[ex.spanel[g]].J + 64 * kLeanRows - 1) / (64 * kLeanRows)) *
+              (ex.spanel[g + 1] - ex.spanel[g]);
   const int panel = (int)std::max<int64_t>(1, (single + kLeanWalkAbove - 1) / kLeanWalkAbove);
   for (size_t g = 0; g + 1 < ex.spanel.size(); ++g) {
     const int b = ex.spanel[g], e = ex.spanel[g + 1];
     const int J = ex.litems[b].J;
-    for (int j0 = 0; j0 < J; j0 += 64)
+    for (int j0 = 0; j0 < J; j0 += 64 * kLeanRows)
       for (int c = b; c < e; c += panel) ex.lwork.push_back({c, std::min(panel, e - c), j0, 0});
   }
   if (ex.lwork.size() >= (1u << 30)) return fail();

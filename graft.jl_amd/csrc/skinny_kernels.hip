@@ -263,14 +263,16 @@ __global__ void __launch_bounds__(kSkinnyThreads)
   }
 }
 
-// Lean per-block streaming: one lane per big-side row, every small-side
-// output of that row in registers.  The workgroup stages the whole small
-// operand (conjugation applied) in LDS once; its waves then walk work
-// entries with a grid stride.  An entry is 64 rows of one output block (or
-// of every block of one big-side tuple, on very large steps).  Row digits
-// come from a multiply-high by the per-leg reciprocal the host computed.
-// Item and segment records are fixed-width and read as whole 16-byte
-// vectors before any use, so the wave waits once per record, not per field.
+// Lean per-block streaming: each lane owns kLeanRows big-side rows (j0 +
+// lane + 64 r) of one output block, every small-side output of those rows in
+// registers; the rows' loads are independent, so a wave keeps kLeanRows times
+// as many loads in flight (the step is latency-bound).  The workgroup stages
+// the whole small operand (conjugation applied) in LDS once; its waves then
+// walk work entries with a grid stride.  An entry is 64 * kLeanRows rows of
+// one output block (or of every block of one big-side tuple, on very large
+// steps).  Row digits come from a multiply-high by the per-leg reciprocal
+// the host computed.  Item and segment records are fixed-width and read as
+// whole 16-byte vectors before any use, so the wave waits once per record.
 template <bool BIG_IS_A, int SM>
 __global__ void __launch_bounds__(kSkinnyThreads)
     zstream_lean_kernel(const double2* __restrict__ A, const double2* __restrict__ B,
@@ -279,8 +281,8 @@ __global__ void __launch_bounds__(kSkinnyThreads)
                         int nwork, int nsmall, int conjA, int conjB) {
   extern __shared__ double2 sm[];
   constexpr int kWaves = kSkinnyThreads / 64;
-  constexpr int R = kLeanRank;
-  static_assert(R == 4 && kLeanInlineK == 8 && SM % 4 == 0, "record layout");
+  constexpr int RR = kLeanRows;
+  static_assert(kLeanRank == 4 && kLeanInlineK == 8 && SM % 4 == 0, "record layout");
   const double2* __restrict__ small = BIG_IS_A ? B : A;
   const double2* __restrict__ big = BIG_IS_A ? A : B;
   const double sgSml = (BIG_IS_A ? conjB : conjA) ? -1.0 : 1.0;
@@ -300,22 +302,30 @@ __global__ void __launch_bounds__(kSkinnyThreads)
     const int J = it0->J, rank = it0->rank;
     const uint4 dims = *reinterpret_cast<const uint4*>(it0->dims);
     const uint4 magic = *reinterpret_cast<const uint4*>(it0->magic);
-    const int j = wk.j0 + lane;
-    if (j >= J) continue;
-    // digits of j over the big-side legs (legs past rank get 0)
-    uint32_t d0 = (uint32_t)j, d1 = 0u, d2 = 0u, d3 = 0u;
-    if (rank > 1) {
-      const uint32_t q0 = __umulhi(d0, magic.x);
-      d1 = q0;
-      d0 -= q0 * dims.x;
-      if (rank > 2) {
-        const uint32_t q1 = __umulhi(d1, magic.y);
-        d2 = q1;
-        d1 -= q1 * dims.y;
-        if (rank > 3) {
-          const uint32_t q2 = __umulhi(d2, magic.z);
-          d3 = q2;
-          d2 -= q2 * dims.z;
+    if (wk.j0 + lane >= J) continue;
+    // digits of each row over the big-side legs (legs past rank get 0); a
+    // row past J takes row j0 + lane's digits and stores nothing
+    bool live[RR];
+    uint32_t d0[RR], d1[RR], d2[RR], d3[RR];
+#pragma unroll
+    for (int r = 0; r < RR; ++r) {
+      const int jr = wk.j0 + lane + 64 * r;
+      live[r] = jr < J;
+      d0[r] = (uint32_t)(live[r] ? jr : wk.j0 + lane);
+      d1[r] = d2[r] = d3[r] = 0u;
+      if (rank > 1) {
+        const uint32_t q0 = __umulhi(d0[r], magic.x);
+        d1[r] = q0;
+        d0[r] -= q0 * dims.x;
+        if (rank > 2) {
+          const uint32_t q1 = __umulhi(d1[r], magic.y);
+          d2[r] = q1;
+          d1[r] -= q1 * dims.y;
+          if (rank > 3) {
+            const uint32_t q2 = __umulhi(d2[r], magic.z);
+            d3[r] = q2;
+            d2[r] -= q2 * dims.z;
+          }
         }
       }
     }
@@ -326,10 +336,11 @@ __global__ void __launch_bounds__(kSkinnyThreads)
       uint4 cso[SM / 4];
 #pragma unroll
       for (int q = 0; q < SM / 4; ++q) cso[q] = reinterpret_cast<const uint4*>(it->cso)[q];
-      const uint32_t co = d0 * cstr.x + d1 * cstr.y + d2 * cstr.z + d3 * cstr.w;
-      double2 acc[SM];
+      double2 acc[RR][SM];
 #pragma unroll
-      for (int s = 0; s < SM; ++s) acc[s] = double2{0.0, 0.0};
+      for (int r = 0; r < RR; ++r)
+#pragma unroll
+        for (int s = 0; s < SM; ++s) acc[r][s] = double2{0.0, 0.0};
       const int gend = it->seg_begin + it->seg_count;
       for (int g = it->seg_begin; g < gend; ++g) {
         const LeanSeg* sg = segs + g;
@@ -344,29 +355,44 @@ __global__ void __launch_bounds__(kSkinnyThreads)
         const int K = sg->K;
         const uint32_t kbv[8] = {kb0.x, kb0.y, kb0.z, kb0.w, kb1.x, kb1.y, kb1.z, kb1.w};
         const uint32_t ksv[8] = {ks0.x, ks0.y, ks0.z, ks0.w, ks1.x, ks1.y, ks1.z, ks1.w};
-        const double2* bp =
-            big + sg->big_off + (d0 * bstr.x + d1 * bstr.y + d2 * bstr.z + d3 * bstr.w);
+        const double2* bp[RR];
+#pragma unroll
+        for (int r = 0; r < RR; ++r)
+          bp[r] = big + sg->big_off +
+                  (d0[r] * bstr.x + d1[r] * bstr.y + d2[r] * bstr.z + d3[r] * bstr.w);
 #pragma unroll
         for (int k = 0; k < kLeanInlineK; ++k) {
           if (k < K) {
-            double2 x = bp[kbv[k]];
-            x.y *= sgBig;
+            double2 x[RR];
+#pragma unroll
+            for (int r = 0; r < RR; ++r) {
+              x[r] = bp[r][kbv[k]];
+              x[r].y *= sgBig;
+            }
 #pragma unroll
             for (int s = 0; s < SM; ++s) {
               if (s < S) {
                 const uint32_t so = (&sso[s / 4].x)[s % 4];
                 const double2 v = sm[ksv[k] + so];
-                acc[s].x += x.x * v.x - x.y * v.y;
-                acc[s].y += x.x * v.y + x.y * v.x;
+#pragma unroll
+                for (int r = 0; r < RR; ++r) {
+                  acc[r][s].x += x[r].x * v.x - x[r].y * v.y;
+                  acc[r][s].y += x[r].x * v.y + x[r].y * v.x;
+                }
               }
             }
           }
         }
       }
-      double2* cp = C + it->c_off + co;
 #pragma unroll
-      for (int s = 0; s < SM; ++s)
-        if (s < S) cp[(&cso[s / 4].x)[s % 4]] = acc[s];
+      for (int r = 0; r < RR; ++r) {
+        if (!live[r]) continue;
+        double2* cp =
+            C + it->c_off + (d0[r] * cstr.x + d1[r] * cstr.y + d2[r] * cstr.z + d3[r] * cstr.w);
+#pragma unroll
+        for (int s = 0; s < SM; ++s)
+          if (s < S) cp[(&cso[s / 4].x)[s % 4]] = acc[r][s];
+      }
     }
   }
 }
