@@ -525,7 +525,7 @@ def precision_bench(D, nsteps, dt=0.05, seed=3, arms=3, armlen=10, precisions=("
     return out
 
 
-def hubbard_apply_bench(D, reps=20, sigma=1.6, seed=5):
+def hubbard_apply_bench(D, reps=20, sigma=1.6, seed=5, autotune=True, lanczos=True):
     """One root-edge U(1) x U(1) eff_h2 on the 16-site Hubbard tree: per-step
     and whole-apply timing of the sector-batched map (the profiled workload)."""
     import torch
@@ -548,28 +548,30 @@ def hubbard_apply_bench(D, reps=20, sigma=1.6, seed=5):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
     # the tile autotuner on this plan (every configuration and orientation per step)
-    torch.cuda.synchronize()
-    t_at = time.perf_counter()
-    h2.autotune(x)
-    torch.cuda.synchronize()
-    t_at = time.perf_counter() - t_at
-    step_ms_tuned = h2.native.execute_timed(h2.operands(x), y)
-    e0.record()
-    for _ in range(reps):
-        y = h2(x, out=y)
-    e1.record()
-    torch.cuda.synchronize()
-    ms_tuned = e0.elapsed_time(e1) / reps
+    t_at, step_ms_tuned, ms_tuned = None, step_ms, ms
+    if autotune:
+        torch.cuda.synchronize()
+        t_at = time.perf_counter()
+        h2.autotune(x)
+        torch.cuda.synchronize()
+        t_at = time.perf_counter() - t_at
+        step_ms_tuned = h2.native.execute_timed(h2.operands(x), y)
+        e0.record()
+        for _ in range(reps):
+            y = h2(x, out=y)
+        e1.record()
+        torch.cuda.synchronize()
+        ms_tuned = e0.elapsed_time(e1) / reps
     # two Lanczos cycles of 20 applies: device driver (expansions replayed as
     # HIP graphs) against the per-call driver, wall time per apply
     from graft_amd import krylov as K
-    lanczos = {}
-    for device in (True, False, True, False):
+    lanczos_runs = {}
+    for device in ((True, False, True, False) if lanczos else ()):
         torch.cuda.synchronize()
         t = time.perf_counter()
         vals, _, info = K.eigsolve(h2, x, 1, "SR", krylovdim=20, tol=1e-14, maxiter=2, device=device)
         torch.cuda.synchronize()
-        lanczos["device_graph" if device else "per_call"] = {
+        lanczos_runs["device_graph" if device else "per_call"] = {
             "ms_per_apply": (time.perf_counter() - t) * 1e3 / info.numops, "applies": info.numops,
             "ritz_value": float(vals[0])}
     tflops_ref = h2.reference_flops / ms_tuned / 1e9
@@ -580,7 +582,7 @@ def hubbard_apply_bench(D, reps=20, sigma=1.6, seed=5):
             "step_ms": step_ms_tuned, "step_ms_untuned": step_ms, "steps": h2.native.step_info(),
             "theta_elements": int(x.numel),
             "blocks": len(x.blocks.blocks) if hasattr(x.blocks, "blocks") else None,
-            "lanczos": lanczos}
+            "lanczos": lanczos_runs}
 
 
 def blas_threads():
@@ -619,6 +621,8 @@ def main():
     ap.add_argument("--prec-lowp", default="fp32,bf16x3",
                     help="precision mode: reduced precisions run beside FP64 (comma list, may be empty)")
     ap.add_argument("--hub-D", type=int, default=64)
+    ap.add_argument("--no-lanczos", action="store_true",
+                    help="hubbard-apply mode: skip the Lanczos cycles (short profiled runs)")
     ap.add_argument("--no-dense", action="store_true", help="hubbard mode: sector form only")
     ap.add_argument("--sweep-D", type=int, default=32)
     ap.add_argument("--sweeps", type=int, default=2)
@@ -647,7 +651,7 @@ def main():
                               "precision": r["precision"]}, "detail": r}), flush=True)
         return
     if args.mode == "hubbard-apply":
-        r = hubbard_apply_bench(args.hub_D)
+        r = hubbard_apply_bench(args.hub_D, autotune=not args.no_autotune, lanczos=not args.no_lanczos)
         print(json.dumps({"metric": "sector-batched eff_h2 ms per apply", "value": r["ms_per_apply"],
                           "unit": "ms", "higher_is_better": False, "n_gpus": 1,
                           "config": {"workload": f"eff_h2 root edge, Hubbard 16-site tree, U(1)xU(1), "

@@ -97,9 +97,10 @@ hipError_t launch_zgemm_db(int config, const void* A, const void* B, void* C, co
                            int nitems, const GemmSeg* segs, const uint32_t* koff, int total_tiles,
                            int conjA, int conjB, bool tr, int kfast, const XcdSplit* xs,
                            hipStream_t stream);
-// Fused sector steps through element maps on the double-buffered 64x64
-// tiles (configurations 11 and 15); kfast: bit 0 / bit 1 A / B fused
-// matrices row-major (loaded along k).
+// Fused sector steps through element maps on the double-buffered tiles:
+// 64x64 (configurations 11, 15) and 128x64 with 8 waves (14, 18); 15 and 18
+// run the pipelined schedule.  kfast: bit 0 / bit 1 A / B fused matrices
+// row-major (loaded along k).
 hipError_t launch_zgemm_dbmap(int config, const void* A, const void* B, void* C, const GemmItem* items,
                               int nitems, const GemmSeg* segs, int total_tiles, int conjA, int conjB,
                               bool tr, int kfast, const FusedMaps& maps, const XcdSplit* xs,

@@ -1746,7 +1746,7 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
       GRAFT_CHECK_HIP(launch_zero_holes(h, nh, C, ctx->stream));
     }
     const FusedMaps maps{ma ? ex.d_ma : nullptr, mb ? ex.d_mb : nullptr, hout ? ex.d_mho : ex.d_mci};
-    if (ex.config == 11 || ex.config == 15) {
+    if (ex.config == 11 || ex.config == 15 || ex.config == 14 || ex.config == 18) {
       // every component of a fused step loads A and B in one direction
       const int dirs = ex.segs.empty() ? 0
                                        : (ex.segs[0].a_kfast ? 1 : 0) | (ex.segs[0].b_kfast ? 2 : 0);

@@ -825,19 +825,23 @@ hipError_t launch_zgemm_dbmap(int config, const void* A, const void* B, void* C,
                               int nitems, const GemmSeg* segs, int total_tiles, int conjA, int conjB,
                               bool tr, int kfast, const FusedMaps& maps, const XcdSplit* xsp,
                               hipStream_t stream) {
-  if (config != 11 && config != 15) return hipErrorInvalidValue;
+  if (config != 11 && config != 15 && config != 14 && config != 18) return hipErrorInvalidValue;
+  const bool wide = config == 14 || config == 18;   // 128 x 64, 8 waves
+  const bool piped = config == 15 || config == 18;
   const XcdSplit xs = xsp ? *xsp : uniform_xcd_split(total_tiles);
-  const dim3 grid(xcd_grid(xs)), block(256);
+  const dim3 grid(xcd_grid(xs)), block(wide ? 512 : 256);
   const double2* a = (const double2*)A;
   const double2* b = (const double2*)B;
   double2* c = (double2*)C;
+#define GRAFT_MAPK1(BMV, TRV, AKV, BKV, PV)                                                          \
+  hipLaunchKernelGGL((zgemm3m_map_kernel<BMV, 64, 16, 32, 32, TRV, AKV, BKV, PV>), grid, block, 0,  \
+                     stream, a, b, c, items, nitems, segs, total_tiles, conjA, conjB, maps, xs)
 #define GRAFT_MAPK(TRV, AKV, BKV)                                                                   \
-  if (config == 15)                                                                                 \
-    hipLaunchKernelGGL((zgemm3m_map_kernel<64, 64, 16, 32, 32, TRV, AKV, BKV, true>), grid, block, \
-                       0, stream, a, b, c, items, nitems, segs, total_tiles, conjA, conjB, maps, xs); \
-  else                                                                                              \
-    hipLaunchKernelGGL((zgemm3m_map_kernel<64, 64, 16, 32, 32, TRV, AKV, BKV, false>), grid, block, \
-                       0, stream, a, b, c, items, nitems, segs, total_tiles, conjA, conjB, maps, xs)
+  if (wide) {                                                                                       \
+    if (piped) GRAFT_MAPK1(128, TRV, AKV, BKV, true); else GRAFT_MAPK1(128, TRV, AKV, BKV, false);  \
+  } else {                                                                                          \
+    if (piped) GRAFT_MAPK1(64, TRV, AKV, BKV, true); else GRAFT_MAPK1(64, TRV, AKV, BKV, false);    \
+  }
   switch ((kfast & 3) | (tr ? 4 : 0)) {
     case 0: GRAFT_MAPK(false, false, false); break;
     case 1: GRAFT_MAPK(false, true, false); break;
@@ -849,6 +853,7 @@ hipError_t launch_zgemm_dbmap(int config, const void* A, const void* B, void* C,
     default: GRAFT_MAPK(true, true, true); break;
   }
 #undef GRAFT_MAPK
+#undef GRAFT_MAPK1
   return hipGetLastError();
 }
 
