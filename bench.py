@@ -335,7 +335,8 @@ def hubbard_tree():
     return Tr.TreeTopology("0", [(str((i - 1) // 2), str(i)) for i in range(1, 16)])
 
 
-def hubbard_bench(D, nsweeps=2, precision="fp64", sigma=1.6, seed=5, dense=True, krylovdim=20):
+def hubbard_bench(D, nsweeps=2, precision="fp64", sigma=1.6, seed=5, dense=True, krylovdim=20,
+                  tune_tiles=False):
     """Spinful Hubbard (t=1, U=4, half filling) on the 16-site tree: dmrg2_ sweeps
     with U(1) x U(1) sector-batched maps against the plain dense FP64 maps on the
     dense embedding of the same initial state, plus one root-edge apply of each."""
@@ -347,7 +348,7 @@ def hubbard_bench(D, nsweeps=2, precision="fp64", sigma=1.6, seed=5, dense=True,
     rng = np.random.default_rng(seed)
     psi, host = S.random_hubbard_ttns(rng, topo, D, sigma=sigma)
     Hs = O.hubbard_sector_ttno(topo, 1.0, 4.0)
-    out = {"D": D, "sigma": sigma, "bond_sectors": {}, "precision": precision}
+    out = {"D": D, "sigma": sigma, "bond_sectors": {}, "precision": precision, "tune_tiles": tune_tiles}
     for c in range(len(topo.ids)):
         if topo.parent[c] >= 0:
             leg = psi.tensors[c].blocks.legs[-1]
@@ -389,7 +390,8 @@ def hubbard_bench(D, nsweeps=2, precision="fp64", sigma=1.6, seed=5, dense=True,
             t0 = time.perf_counter()
             tm = {}
             _, Es = GS.dmrg2_(state, H, trunc=B.TruncationScheme(maxdim=D), nsweeps=1,
-                              krylovdim=krylovdim, precision=prec, timings=tm)
+                              krylovdim=krylovdim, precision=prec, timings=tm,
+                              tune_tiles=tune_tiles and state.tensors[0].blocks is not None)
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
             energies.append(Es[-1])
@@ -630,6 +632,9 @@ def main():
     ap.add_argument("--apply-precision", default="fp64", choices=["fp64", "fp32", "bf16x3"],
                     help="apply mode: product arithmetic of the map (reduced precision is not the "
                          "metric's FP64 form and is reported separately)")
+    ap.add_argument("--tune-tiles", action="store_true",
+                    help="hubbard mode: autotune the tiles of heavy sector maps once per step shape "
+                         "class inside the sweeps")
     ap.add_argument("--phase-times", action="store_true",
                     help="sweep mode: synchronized wall time per dmrg2_ phase")
     ap.add_argument("--calib-copy", action="store_true",
@@ -680,7 +685,8 @@ def main():
                           "detail": r}), flush=True)
         return
     if args.mode == "hubbard":
-        r = hubbard_bench(args.hub_D, args.sweeps, args.precision, dense=not args.no_dense)
+        r = hubbard_bench(args.hub_D, args.sweeps, args.precision, dense=not args.no_dense,
+                          tune_tiles=args.tune_tiles)
         print(json.dumps({"metric": "dmrg2 sweep seconds, sector-batched vs plain dense FP64",
                           "value": r["sector_sweep_s"][-1], "unit": "s", "higher_is_better": False,
                           "n_gpus": 1, "config": {

@@ -165,9 +165,10 @@ def _native_output_structure(native, plan, legs, conjs):
 class EnvCache:
     """Directed-edge environment cache with explicit invalidation events."""
 
-    def __init__(self, topo, tune_orders=False):
+    def __init__(self, topo, tune_orders=False, tune_tiles=False):
         self.topo = topo
         self.tune_orders = tune_orders   # measured pairwise order for heavy dense maps
+        self.tune_tiles = tune_tiles     # autotuned tiles for heavy sector maps (per step shape)
         self.envs = {}
         self.node_generations = [0] * nnodes(topo)
         self.plans = OrderedDict()       # least recently used first
@@ -229,6 +230,9 @@ class EnvCache:
             p = NativePlan(plan, operands, out_structure, out_dims)
         if out_structure is None and legs is not None:
             p.out_structure = _native_output_structure(p, plan, legs, spec.conjs)
+        if (self.tune_tiles and legs is not None and kind in _TUNED_KINDS
+                and plan.device_time >= TUNE_TILES_MIN_SECONDS):
+            _tuned_tiles(p, operands, out_dims)
         self.plans[key] = p
         self.plan_bytes += p.device_bytes
         self._evict_plans()
@@ -274,6 +278,48 @@ PLAN_CACHE_BYTES = 4 << 30      # ... and the device bytes they may hold
 _TUNED_KINDS = ("h1", "h2", "h0")
 _TUNED_ORDERS = {}           # (spec, log2 leg dims) -> measured fastest tree
 TUNE_MIN_SECONDS = 1e-3      # maps modelled below this keep the planner's order
+
+
+_TUNED_TILES = {}            # step shape signature -> (tile configuration, orientation)
+TUNE_TILES_MIN_SECONDS = 2e-4   # sector maps modelled below this keep the heuristic tiles
+
+
+def _step_signature(info):
+    """Shape class of one compiled step: how it runs (per-block items, fused
+    components or streaming), its largest m, k, n and its block count, each
+    rounded to a power of two.  Sweeps meet a new block structure on nearly
+    every bond update; steps of the same class take the same tiles."""
+    lg = lambda v: int(round(math.log2(max(int(v), 1))))
+    return (info["config"] if info["config"] >= 19 else -1, lg(info["m"]), lg(info["k"]),
+            lg(info["n"]), lg(info["blocks"]), lg(info["elements"][2]))
+
+
+def _tuned_tiles(native, operands, out_dims):
+    """Tile configuration and MFMA orientation of every step of a sector map,
+    chosen by the library's autotuner (graft_plan_autotune times every
+    configuration and orientation of a step) the first time a step of its
+    shape class is met, and reused for that class afterwards: the known steps
+    are pinned, so the autotuner times only the new ones."""
+    infos = native.step_info()
+    fresh = []
+    for i, inf in enumerate(infos):
+        hit = _TUNED_TILES.get(_step_signature(inf))
+        if hit is None:
+            fresh.append((i, _step_signature(inf)))
+        else:
+            native.set_tile_config(i, hit[0])
+            native.set_orientation(i, hit[1])
+    if not fresh:
+        return
+    ins = [DTensor.zeros(op.dims, op.ncod, op.blocks) if not isinstance(op, DTensor) else op
+           for op in operands]
+    y = DTensor.empty(out_dims, ins[0].ncod, native.out_structure or ins[0].blocks)
+    native.autotune(ins, y)
+    after = native.step_info()
+    if len(_TUNED_TILES) > 4096:
+        _TUNED_TILES.clear()
+    for i, sig in fresh:
+        _TUNED_TILES[sig] = (after[i]["config"], after[i]["orientation"])
 
 
 def _measured_order(spec, dims, plan, operands, out_dims):

@@ -43,6 +43,7 @@ class TDVP1:
     eager: bool = True
     precision: str = "fp64"
     cache: object = None
+    tune_tiles: bool = False     # autotuned tiles for heavy sector maps, per step shape class
 
 
 @dataclass
@@ -87,6 +88,7 @@ class TDVP1_CBE:
     precision: str = "fp64"
     cache: object = None
     last_cbe_info: dict = None
+    tune_tiles: bool = False     # autotuned tiles for heavy sector maps, per step shape class
 
 
 CBE_BLOCK_EXTRA = 4      # Krylov block size = budget + this
@@ -457,7 +459,7 @@ def _cbe_split_sector(ev, psi, H, u, v, dz):
 def step_(ev, psi, H, dz):
     """step! — tdvp.jl:171-195."""
     if ev.cache is None or ev.cache.topo != psi.topo:
-        ev.cache = EnvCache(psi.topo, tune_orders=True)
+        ev.cache = EnvCache(psi.topo, tune_orders=True, tune_tiles=getattr(ev, "tune_tiles", False))
     if ev.order == 1:
         _tdvp1_sweep_(ev, psi, H, dz, rev=False)
     elif ev.order == 2:

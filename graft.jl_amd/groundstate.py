@@ -117,18 +117,20 @@ def _bootstrap_physless_root_(psi, cache, targets):
 
 
 def dmrg2_(psi, H, trunc=B.NO_TRUNCATION, nsweeps=10, tol=1e-10, krylovdim=20, eig_tol=1e-12,
-           precision="fp64", verbose=False, callback=None, timings=None):
+           precision="fp64", verbose=False, callback=None, timings=None, tune_tiles=False):
     """Two-site DMRG; returns (psi, energies) with one energy per sweep.
 
     ``precision`` selects the product arithmetic of the local Lanczos
     ("fp64", or "fp32" / "bf16x3" with FP64 refinement of every local solve).
     ``timings``: a dict that receives seconds per phase (gauge move, two-site
     tensor, map build with environment updates, eigensolve, split) and the
-    FP64/reduced apply FLOPs; phases are then synchronized, which costs a little."""
+    FP64/reduced apply FLOPs; phases are then synchronized, which costs a little.
+    ``tune_tiles``: autotune the tiles of heavy sector maps once per step shape
+    class (EnvCache tune_tiles); the first map of each class pays the tuning."""
     if not H.ishermitian:
         raise ValueError("dmrg2_: DMRG requires ishermitian(H) == true")
     t = psi.topo
-    cache = EnvCache(t, tune_orders=True)
+    cache = EnvCache(t, tune_orders=True, tune_tiles=tune_tiles)
     energies = []
     bonds = [n for n in postorder(t) if t.parent[n] >= 0]
     schedule = [(n, "m") for n in bonds] + [(n, "n") for n in reversed(bonds)]
