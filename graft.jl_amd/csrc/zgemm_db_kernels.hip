@@ -753,8 +753,8 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN), 2)
     // fragments are double-buffered in registers, so the LDS reads of step
     // kk + 1 are in flight while the MFMAs of step kk issue; the next
     // k-tile's global loads go out before the MFMA block and are staged
-    // after it.  The last k-tile also loads and stages a zero tile (sources
-    // past K), which keeps the loop body free of branches.
+    // after it; the loop body has no branches.  The last k-tile runs outside
+    // the loop (nothing left to load) and only up to K.
     constexpr int NKK = BK / 4;
     static_assert(NKK % 2 == 0, "fragment double-buffering needs an even k-step count");
     dbl2 fa[2][FM], fb[2][FN];
@@ -788,10 +788,10 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN), 2)
         }
     };
     frag(0, 0, 0);
-    for (int kt = 0; kt < nkt; ++kt) {
+    for (int kt = 0; kt + 1 < nkt; ++kt) {
       const int buf = kt & 1;
-      fetch();                    // k-tile kt + 1 (zeros past K)
-      sources((kt + 2) * BK);     // k-tile kt + 2
+      fetch();                    // k-tile kt + 1
+      sources((kt + 2) * BK);     // k-tile kt + 2 (zeros past K)
       asm volatile("" ::: "memory");
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -804,6 +804,17 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN), 2)
       store(buf ^ 1);
       __syncthreads();
       frag(0, buf ^ 1, 0);
+    }
+    // The last k-tile, peeled: its 4-deep k steps wholly past K hold zeros
+    // and are skipped (components have ragged K).
+    {
+      const int nk = (K - (nkt - 1) * BK + 3) / 4;   // live k steps, 1 .. NKK
+#pragma unroll
+      for (int kk = 0; kk < NKK; ++kk) {
+        if (kk >= nk) break;
+        if (kk + 1 < NKK && kk + 1 < nk) frag((kk + 1) & 1, (nkt - 1) & 1, kk + 1);
+        mma(kk & 1);
+      }
     }
   }
 
