@@ -27,8 +27,13 @@ EXACT_TENSOR_LIMIT = 10   # planner.jl:232 (_EXACT_TENSOR_LIMIT)
 
 # Roofline time model of the device executor (MI355X, complex FP64): the
 # rates sector steps reach in practice, not the peaks (8 TB/s, 78.6 TFLOP/s).
-DEVICE_FLOPS = 40e12      # real FLOP/s of a block GEMM step
-DEVICE_BYTES = 3e12       # B/s of a streaming step (operands + result once)
+# Fitted to measurement: every admissible pairwise order of the root-edge
+# Hubbard eff_h2 map at D = 256 and D = 1024 (10 orders, U(1) x U(1)) was
+# compiled and timed on MI355X, and these two rates minimize the mean
+# |log(modelled / measured time)| (0.18; 0.33 for 40 TFLOP/s and 3 TB/s);
+# at both sizes the order the model ranks first is the measured fastest.
+DEVICE_FLOPS = 46e12      # real FLOP/s of a block GEMM step (coupled-sector blocks counted dense)
+DEVICE_BYTES = 2e12       # B/s of a streaming step (stored A + B + C, each once)
 
 
 def step_device_time(sector_flops, elements, scalar_bytes=16):
