@@ -606,6 +606,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-one-D", type=int, default=64,
                     help="bond dimension of the one-core host apply")
+    ap.add_argument("--no-sector-apply", action="store_true",
+                    help="apply mode: skip the D=1024 sector-batched apply figure")
     ap.add_argument("--no-sweep", action="store_true",
                     help="apply mode: skip the sweep figures (Hubbard sector vs dense, host vs GPU)")
     ap.add_argument("--sweep-hub-D", type=int, default=128,
@@ -819,6 +821,20 @@ def main():
         print(f"# sweeps {time.perf_counter() - t_sw:.1f} s", file=sys.stderr, flush=True)
 
     out["sweep"] = sweep
+    sector = None
+    if not args.no_sector_apply:
+        # the sector-batched apply at the batching configuration's size (D=1024,
+        # U(1) x U(1)), heuristic tiles, reference-model FLOPs
+        t_sa = time.perf_counter()
+        r = hubbard_apply_bench(1024, autotune=False, lanczos=False)
+        sector = {"workload": "eff_h2 root edge, Hubbard 16-site tree, U(1)xU(1), D=1024",
+                  "ms_per_apply": r["ms_per_apply"], "step_ms": r["step_ms"],
+                  "flops_reference_model": r["flops_reference_model"],
+                  "tflops_reference_model": r["tflops_reference_model"],
+                  "fp64_mfma_fraction_reference_model": r["fp64_mfma_fraction_reference_model"],
+                  "flops_stored_blocks": r["flops"]}
+        print(f"# sector apply {time.perf_counter() - t_sa:.1f} s", file=sys.stderr, flush=True)
+    out["sector_apply"] = sector
     print(json.dumps(out), flush=True)
 
 
