@@ -121,6 +121,7 @@ def lib():
         "graft_ctx_set_sector_fusion": [vp, i32],
         "graft_ctx_set_fused_map_loads": [vp, i32],
         "graft_ctx_set_lean_stream": [vp, i32],
+        "graft_ctx_set_split_words": [vp, i32],
         "graft_graph_capture_end": [vp, ctypes.POINTER(vp)],
         "graft_graph_launch": [vp, vp],
         "graft_graph_destroy": [vp],
@@ -236,6 +237,11 @@ class Context:
         """1 (default): per-block streaming sector steps run the lean kernel where it
         applies; 0: always the general per-block kernel."""
         check(lib().graft_ctx_set_lean_stream(self.handle, int(bool(on))))
+
+    def set_split_words(self, on):
+        """1 (default): BF16x3 applies hand split-bf16 words from a streaming step to
+        its GEMM; 0: the GEMM converts FP64 operands on every tile load."""
+        check(lib().graft_ctx_set_split_words(self.handle, int(bool(on))))
 
     def reserve_workspace(self, nbytes):
         check(lib().graft_ctx_reserve_workspace(self.handle, int(nbytes)))

@@ -105,10 +105,14 @@ hipError_t launch_zgemm_dbmap(int config, const void* A, const void* B, void* C,
                               int nitems, const GemmSeg* segs, int total_tiles, int conjA, int conjB,
                               bool tr, int kfast, const FusedMaps& maps, const XcdSplit* xs,
                               hipStream_t stream);
+// planes (BF16X3, ksplit 1): A holds split-bf16 word pairs (launch_zskinny
+// planes) and B split-bf16 word quads (launch_split_bplanes) at the element
+// indices of the FP64 layout; the tiles stage them without conversion.
 hipError_t launch_zgemm_lowp(int config, int precision, const void* A, const void* B, void* C,
                              const GemmItem* items, int nitems, const GemmSeg* segs,
                              const uint32_t* koff, int total_tiles, int conjA, int conjB,
-                             int ksplit, void* partial, bool tr, hipStream_t stream);
+                             int ksplit, void* partial, bool tr, hipStream_t stream,
+                             bool planes = false);
 
 // Streaming kernel for skinny steps (skinny_kernels.hip).  It is selected as
 // the virtual tile configuration kSkinnyConfig when one operand's open
@@ -119,9 +123,14 @@ constexpr int kSkinnyMaxS = 64;
 constexpr int kSkinnyAccMaxK = 64;   // accumulator variant: longer K, smaller side
 constexpr int kSkinnyAccMaxS = 16;
 // acc selects the accumulator variant (K > kSkinnyMaxK or S > kSkinnyMaxS).
+// planes: the result is written as split-bf16 word pairs (8 bytes per element,
+// at the element's index) for a BF16x3 GEMM that reads it as its A operand.
 hipError_t launch_zskinny(bool big_is_a, bool acc, const void* A, const void* B, void* C,
                           const GemmItem* item, const GemmSeg* seg, const uint32_t* koff, int grid,
-                          int conjA, int conjB, hipStream_t stream);
+                          int conjA, int conjB, hipStream_t stream, bool planes = false);
+// Split-bf16 words of a complex FP64 B operand (16 bytes per element, four
+// words: the real- and imaginary-part columns of the four-product form).
+hipError_t launch_split_bplanes(const void* B, int64_t n, void* out, int conjB, hipStream_t stream);
 int skinny_grid(int64_t J);
 
 // Per-block streaming kernel for multi-block (sector) steps whose small

@@ -270,6 +270,12 @@ struct StepExec {
   std::vector<HoleRect> holes_a, holes_b;  // fused rectangles no operand block covers
   HoleRect* d_holes_a = nullptr;
   HoleRect* d_holes_b = nullptr;
+  // Split-word handoff (BF16X3 applies): a streaming step whose result is
+  // the A operand of a tiled GEMM step writes it as split-bf16 word pairs
+  // (pl_to: that step) and the GEMM stages them without conversion, its B
+  // split once per apply into the plane region (pl_from: the producer).
+  int pl_to = -1, pl_from = -1;
+  uint64_t bplane_bytes = 0;
 };
 
 struct graft_plan_impl {
@@ -281,6 +287,7 @@ struct graft_plan_impl {
   uint64_t ws_bytes = 0;           // slots + split-K partial region + fused matrices
   uint64_t ws_partial = 0;         // byte offset of the split-K partial region
   uint64_t ws_fused = 0;           // byte offset of the fused-matrix region
+  uint64_t ws_planes = 0;          // byte offset of the split-word B region
   double macs = 0.0;
   void* d_meta = nullptr;  // one allocation holding every step's items/segs/koff
   void* d_maps = nullptr;  // element -> fused-offset maps of every fused step
@@ -309,6 +316,7 @@ struct graft_ctx {
   int sector_fuse = 1;  // 0 per-block GEMM, 1 automatic, 2 fused / 3 per-block streaming for every multi-block step
   int fused_map_loads = 1;  // fused FP64 MFMA steps read / write the blocks through the maps
   int lean_stream = 1;      // per-block streaming steps use the lean kernel where eligible
+  int split_words = 1;      // BF16X3: streaming producers write split-bf16 words for their GEMM
   uint64_t ws_generation = 0;    // bumped whenever the workspace moves
 };
 
@@ -1513,14 +1521,16 @@ static void assign_tiles(StepExec& ex) {
 }
 
 static void update_workspace_size(graft_plan_impl& p) {
-  uint64_t part = 0, fused = 0;
+  uint64_t part = 0, fused = 0, planes = 0;
   for (auto& ex : p.steps) {
     part = std::max(part, ex.partial_bytes);
     fused = std::max(fused, ex.fused_bytes);
+    planes = std::max(planes, ex.bplane_bytes);
   }
   p.ws_partial = ((p.ws_slots + 255) / 256) * 256;
   p.ws_fused = p.ws_partial + ((part + 255) / 256) * 256;
-  p.ws_bytes = p.ws_fused + fused;
+  p.ws_planes = p.ws_fused + ((fused + 255) / 256) * 256;
+  p.ws_bytes = p.ws_planes + planes;
 }
 
 static int upload_step_meta(graft_plan_impl& p) {
@@ -1774,6 +1784,21 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     B = Bf;
     C = Cf;
   }
+  // split-word handoff of a BF16X3 apply: decided at run time for both steps
+  auto planes_ok = [&](const StepExec& prod, const StepExec& cons) {
+    return precision == GRAFT_PREC_BF16X3 && ctx->split_words && prod.config == kSkinnyConfig &&
+           !cons.fused &&
+           cons.config < kNumTileConfigs && cons.ksplit == 1;
+  };
+  const bool pl_out = ex.pl_to >= 0 && planes_ok(ex, p.steps[ex.pl_to]);
+  if (ex.pl_from >= 0 && planes_ok(p.steps[ex.pl_from], ex)) {
+    void* Bw = ws + p.ws_planes;
+    GRAFT_CHECK_HIP(launch_split_bplanes(B, (int64_t)(ex.bplane_bytes / 16), Bw, ex.conjB, ctx->stream));
+    GRAFT_CHECK_HIP(launch_zgemm_lowp(ex.config, precision, A, Bw, C, ex.d_items, (int)ex.items.size(),
+                                      ex.d_segs, ex.d_koff, ex.total_tiles, ex.conjA, 0, 1, nullptr,
+                                      ex.tr, ctx->stream, true));
+    return GRAFT_OK;
+  }
   if (ex.config == kBlockStreamConfig && ex.lean && ctx->lean_stream) {
     // HBM-bound, FP64 whatever the precision (as the streaming kernels below)
     GRAFT_CHECK_HIP(launch_zstream_lean(ex.bstream == 1, ex.lean_smax, A, B, C, ex.d_litems,
@@ -1796,7 +1821,7 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
       const int grid = ex.items.size() == 1 ? ex.total_tiles : skinny_grid(ex.skinny == 1 ? it.M : it.N);
       GRAFT_CHECK_HIP(launch_zskinny(ex.skinny == 1, acc, A, B, C, ex.d_items + i,
                                      ex.d_segs + it.seg_begin, ex.d_koff, grid, ex.conjA, ex.conjB,
-                                     ctx->stream));
+                                     ctx->stream, pl_out));
     }
   } else {
     const int kfast = precision == GRAFT_PREC_FP64 ? piped_kfast(ex, ex.config) : -1;
@@ -1932,6 +1957,12 @@ int graft_ctx_set_sector_fusion(graft_ctx* ctx, int32_t mode) {
 int graft_ctx_set_fused_map_loads(graft_ctx* ctx, int32_t on) {
   GRAFT_REQUIRE(ctx != nullptr, "null context");
   ctx->fused_map_loads = on ? 1 : 0;
+  return GRAFT_OK;
+}
+
+int graft_ctx_set_split_words(graft_ctx* ctx, int32_t on) {
+  GRAFT_REQUIRE(ctx != nullptr, "null context");
+  ctx->split_words = on ? 1 : 0;
   return GRAFT_OK;
 }
 
@@ -2119,6 +2150,19 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
         need[X] = std::max(need[X], et.fb);
       }
     }
+  }
+  // Split-word handoffs: a dense streaming step feeding a tiled GEMM step's A
+  // operand (the TTNO absorb before the environment absorb).
+  for (int t = 0; t < nsteps; ++t) {
+    StepExec& et = p.steps[t];
+    const int X = steps[t].a;
+    const int s = (X >= ninputs && X != output_slot) ? producer[X] : -1;
+    if (s < 0 || et.fused || p.steps[s].fused || p.steps[s].skinny == 0 ||
+        p.steps[s].items.size() != 1)
+      continue;
+    p.steps[s].pl_to = t;
+    et.pl_from = s;
+    et.bplane_bytes = (uint64_t)p.slots[steps[t].b].total * 16;
   }
   // Workspace: first-fit over liveness intervals [born, dies].
   p.ws_offset.assign(nslots, -1);

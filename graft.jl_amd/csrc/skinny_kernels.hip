@@ -29,8 +29,32 @@ __device__ __forceinline__ uint32_t sk_decode(uint32_t idx, int rank, const int3
   return off;
 }
 
+// Split-bf16 operand words (GRAFT_PREC_BF16X3 consumers): an FP32 value x
+// is hi + lo, hi = bf16(x), lo = bf16(x - hi), both rounded to nearest even;
+// an element (re, im) becomes the words (re_hi | im_hi << 16) and
+// (re_lo | im_lo << 16), the layout the bf16 GEMM stages.
+__device__ __forceinline__ uint32_t sk_bf16_bits(float x) {
+  return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)x);
+}
+__device__ __forceinline__ uint2 sk_split_words(double re, double im) {
+  const float r = (float)re, i = (float)im;
+  const __bf16 rh = (__bf16)r, ih = (__bf16)i;
+  const float rl = r - (float)rh, il = i - (float)ih;
+  return uint2{(uint32_t)__builtin_bit_cast(uint16_t, rh) | (sk_bf16_bits(i) << 16),
+               sk_bf16_bits(rl) | (sk_bf16_bits(il) << 16)};
+}
+template <bool PL>
+__device__ __forceinline__ void sk_store(double2* C, uint64_t e, double re, double im) {
+  if constexpr (PL)
+    reinterpret_cast<uint2*>(C)[e] = sk_split_words(re, im);
+  else
+    C[e] = double2{re, im};
+}
+
 // BIG_IS_A: A is the large operand (j = m, s = n); otherwise B is (j = n, s = m).
-template <bool BIG_IS_A>
+// PL: the result is written as split-bf16 word pairs (8 bytes per element at
+// the element's index) for a BF16x3 consumer, instead of complex FP64.
+template <bool BIG_IS_A, bool PL>
 __global__ void __launch_bounds__(kSkinnyThreads)
     zskinny_f64_kernel(const double2* __restrict__ Abase, const double2* __restrict__ Bbase,
                        double2* __restrict__ Cbase, const GemmItem* __restrict__ item,
@@ -101,7 +125,7 @@ __global__ void __launch_bounds__(kSkinnyThreads)
           im += x[k].x * w.y + x[k].y * w.x;
         }
       }
-      C[(uint64_t)co + cso[s]] = double2{re, im};
+      sk_store<PL>(C, (uint64_t)co + cso[s], re, im);
     }
   }
 }
@@ -109,7 +133,7 @@ __global__ void __launch_bounds__(kSkinnyThreads)
 // Variant for longer K (<= kSkinnyAccMaxK) and a very small side
 // (S <= kSkinnyAccMaxS): each thread keeps its S outputs in registers and
 // streams its K inputs one at a time, so registers do not grow with K.
-template <bool BIG_IS_A>
+template <bool BIG_IS_A, bool PL>
 __global__ void __launch_bounds__(kSkinnyThreads)
     zskinny_acc_f64_kernel(const double2* __restrict__ Abase, const double2* __restrict__ Bbase,
                            double2* __restrict__ Cbase, const GemmItem* __restrict__ item,
@@ -172,7 +196,7 @@ __global__ void __launch_bounds__(kSkinnyThreads)
     }
 #pragma unroll
     for (int s = 0; s < kSkinnyAccMaxS; ++s)
-      if (s < S) C[(uint64_t)co + cso[s]] = acc[s];
+      if (s < S) sk_store<PL>(C, (uint64_t)co + cso[s], acc[s].x, acc[s].y);
   }
 }
 
@@ -454,26 +478,49 @@ int skinny_grid(int64_t J) {
 
 hipError_t launch_zskinny(bool big_is_a, bool acc, const void* A, const void* B, void* C,
                           const GemmItem* item, const GemmSeg* seg, const uint32_t* koff, int grid,
-                          int conjA, int conjB, hipStream_t stream) {
+                          int conjA, int conjB, hipStream_t stream, bool planes) {
+  const double2* a = (const double2*)A;
+  const double2* b = (const double2*)B;
+  double2* c = (double2*)C;
+#define GRAFT_SK(KER, BIGA, PLV)                                                                    \
+  hipLaunchKernelGGL((KER<BIGA, PLV>), dim3(grid), dim3(kSkinnyThreads), 0, stream, a, b, c, item, \
+                     seg, koff, conjA, conjB)
+#define GRAFT_SK2(KER, BIGA)                                                                        \
+  if (planes) GRAFT_SK(KER, BIGA, true); else GRAFT_SK(KER, BIGA, false)
   if (acc) {
-    if (big_is_a)
-      hipLaunchKernelGGL(zskinny_acc_f64_kernel<true>, dim3(grid), dim3(kSkinnyThreads), 0, stream,
-                         (const double2*)A, (const double2*)B, (double2*)C, item, seg, koff, conjA,
-                         conjB);
-    else
-      hipLaunchKernelGGL(zskinny_acc_f64_kernel<false>, dim3(grid), dim3(kSkinnyThreads), 0,
-                         stream, (const double2*)A, (const double2*)B, (double2*)C, item, seg, koff,
-                         conjA, conjB);
-    return hipGetLastError();
+    if (big_is_a) { GRAFT_SK2(zskinny_acc_f64_kernel, true); }
+    else { GRAFT_SK2(zskinny_acc_f64_kernel, false); }
+  } else {
+    if (big_is_a) { GRAFT_SK2(zskinny_f64_kernel, true); }
+    else { GRAFT_SK2(zskinny_f64_kernel, false); }
   }
-  if (big_is_a)
-    hipLaunchKernelGGL(zskinny_f64_kernel<true>, dim3(grid), dim3(kSkinnyThreads), 0, stream,
-                       (const double2*)A, (const double2*)B, (double2*)C, item, seg, koff, conjA,
-                       conjB);
-  else
-    hipLaunchKernelGGL(zskinny_f64_kernel<false>, dim3(grid), dim3(kSkinnyThreads), 0, stream,
-                       (const double2*)A, (const double2*)B, (double2*)C, item, seg, koff, conjA,
-                       conjB);
+#undef GRAFT_SK2
+#undef GRAFT_SK
+  return hipGetLastError();
+}
+
+// Split-bf16 words of a complex FP64 operand for the bf16 GEMM's B side,
+// conjugation applied: per element (br_h | -bi_h), (br_l | -bi_l),
+// (bi_h | br_h), (bi_l | br_l) (the real-part and imaginary-part columns
+// of the four-product form).
+__global__ void __launch_bounds__(kSkinnyThreads)
+    split_bplanes_kernel(const double2* __restrict__ B, int64_t n, uint4* __restrict__ out, int conjB) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double2 v = B[i];
+    const uint2 w = sk_split_words(v.x, conjB ? -v.y : v.y);
+    const uint32_t rh = w.x & 0xFFFFu, ih = w.x >> 16, rl = w.y & 0xFFFFu, il = w.y >> 16;
+    out[i] = uint4{rh | ((ih ^ 0x8000u) << 16), rl | ((il ^ 0x8000u) << 16), ih | (rh << 16),
+                   il | (rl << 16)};
+  }
+}
+
+hipError_t launch_split_bplanes(const void* B, int64_t n, void* out, int conjB, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  int64_t g = (n + kSkinnyThreads - 1) / kSkinnyThreads;
+  if (g > 16384) g = 16384;
+  hipLaunchKernelGGL(split_bplanes_kernel, dim3((int)g), dim3(kSkinnyThreads), 0, stream,
+                     (const double2*)B, n, (uint4*)out, conjB);
   return hipGetLastError();
 }
 

@@ -40,15 +40,21 @@ __device__ __forceinline__ uint32_t pack2(__bf16 a, __bf16 b) {
   return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
 }
 
-template <int BM, int BN, int WM, int WN, bool TR, int P>
+// PL (BF16X3 only): A holds split-bf16 word pairs and B word quads at the
+// element indices of the FP64 layout (written by the producing streaming
+// step and by the B split), staged into LDS as they are: no conversion.
+template <int BM, int BN, int WM, int WN, bool TR, int P, bool PL>
 __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
     zgemm_lowp_kernel(const double2* __restrict__ Abase, const double2* __restrict__ Bbase,
                       double2* __restrict__ Cbase, const GemmItem* __restrict__ items, int nitems,
                       const GemmSeg* __restrict__ segs, const uint32_t* __restrict__ koff,
                       int total_tiles, int conjA, int conjB, int ksplit,
                       double2* __restrict__ partial) {
-  constexpr int BK = 16;                 // complex k per tile
+  // complex k per tile (32 for the bf16 form measured slower once tiles are
+  // autotuned: its larger LDS stage halves the workgroups per CU)
+  constexpr int BK = 16;
   constexpr bool BF = (P == GRAFT_PREC_BF16X3);
+  static_assert(!PL || BF, "split-word operands are BF16x3 only");
   constexpr int NWN = BN / WN;
   constexpr int NT = 64 * (BM / WM) * NWN;
   constexpr int EA = BM * BK / NT;
@@ -154,7 +160,40 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
       colB[0] = (n < N) ? lp_decode((uint32_t)n, nrank, it->ndims, sg->b_nstr) : kBad;
     }
     double2 ra[EA], rb[EB];
+    uint2 wa[EA];
+    uint4 wb[EB];
+    const uint2* Aw = reinterpret_cast<const uint2*>(Abase) + sg->a_off;
+    const uint4* Bw = reinterpret_cast<const uint4*>(Bbase) + sg->b_off;
     auto fetch = [&](int k0) {
+      if constexpr (PL) {
+      if (akf) {
+        const int q = k0 + tid % BK;
+        const uint32_t ko = (q < kend) ? kA[q] : 0u;
+#pragma unroll
+        for (int e = 0; e < EA; ++e)
+          wa[e] = (q < kend && rowA[e] != kBad) ? Aw[(uint64_t)rowA[e] + ko] : uint2{0u, 0u};
+      } else {
+#pragma unroll
+        for (int e = 0; e < EA; ++e) {
+          const int q = k0 + tid / BM + e * (NT / BM);
+          wa[e] = (q < kend && rowA[0] != kBad) ? Aw[(uint64_t)rowA[0] + kA[q]] : uint2{0u, 0u};
+        }
+      }
+      if (bkf) {
+        const int q = k0 + tid % BK;
+        const uint32_t ko = (q < kend) ? kB[q] : 0u;
+#pragma unroll
+        for (int e = 0; e < EB; ++e)
+          wb[e] = (q < kend && colB[e] != kBad) ? Bw[(uint64_t)colB[e] + ko] : uint4{0u, 0u, 0u, 0u};
+      } else {
+#pragma unroll
+        for (int e = 0; e < EB; ++e) {
+          const int q = k0 + tid / BN + e * (NT / BN);
+          wb[e] = (q < kend && colB[0] != kBad) ? Bw[(uint64_t)colB[0] + kB[q]] : uint4{0u, 0u, 0u, 0u};
+        }
+      }
+            return;
+      }
       if (akf) {
         const int q = k0 + tid % BK;
         const uint32_t ko = (q < kend) ? kA[q] : 0u;
@@ -213,7 +252,40 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
         Bs[q * BSL + 2 * BN + p] = -bi;
       }
     };
+    // conjugation of a split-word A: the imaginary halves change sign
+    const uint32_t cmaskA = conjA ? 0x80000000u : 0u;
     auto stage = [&]() {
+      if constexpr (PL) {
+      if (akf) {
+#pragma unroll
+        for (int e = 0; e < EA; ++e) {
+          const int p = tid / BK + e * (NT / BK), q = tid % BK;
+          Ah[p * RW + q] = wa[e].x ^ cmaskA;
+          Al[p * RW + q] = wa[e].y ^ cmaskA;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < EA; ++e) {
+          const int p = tid % BM, q = tid / BM + e * (NT / BM);
+          Ah[p * RW + q] = wa[e].x ^ cmaskA;
+          Al[p * RW + q] = wa[e].y ^ cmaskA;
+        }
+      }
+      auto putw = [&](int q, int p, uint4 w) {
+        Brh[p * RW + q] = w.x;
+        Brl[p * RW + q] = w.y;
+        Bih[p * RW + q] = w.z;
+        Bil[p * RW + q] = w.w;
+      };
+      if (bkf) {
+#pragma unroll
+        for (int e = 0; e < EB; ++e) putw(tid % BK, tid / BK + e * (NT / BK), wb[e]);
+      } else {
+#pragma unroll
+        for (int e = 0; e < EB; ++e) putw(tid / BN + e * (NT / BN), tid % BN, wb[e]);
+      }
+            return;
+      }
       if (akf) {
 #pragma unroll
         for (int e = 0; e < EA; ++e) putA(tid / BK + e * (NT / BK), tid % BK, ra[e]);
@@ -238,21 +310,25 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
       __syncthreads();
       if (kt + 1 < nkt) fetch((kt + 1) * BK);
       if constexpr (BF) {
-        // One K' = 32 chunk: lane group kr holds complex k 4kr .. 4kr+3 (re, im pairs).
+#pragma unroll
+       for (int ch = 0; ch < BK / 16; ++ch) {
+        // One K' = 32 chunk (16 complex k): lane group kr holds complex k
+        // 16 ch + 4kr .. 16 ch + 4kr + 3 (re, im pairs).
+        const int kw = 16 * ch + 4 * kr;
         bf16x8 ah[FM], al[FM], brh[FN], brl[FN], bih[FN], bil[FN];
 #pragma unroll
         for (int a = 0; a < FM; ++a) {
           const int row = wm0 + a * 16 + li;
-          ah[a] = *reinterpret_cast<const bf16x8*>(Ah + row * RW + 4 * kr);
-          al[a] = *reinterpret_cast<const bf16x8*>(Al + row * RW + 4 * kr);
+          ah[a] = *reinterpret_cast<const bf16x8*>(Ah + row * RW + kw);
+          al[a] = *reinterpret_cast<const bf16x8*>(Al + row * RW + kw);
         }
 #pragma unroll
         for (int b = 0; b < FN; ++b) {
           const int col = wn0 + b * 16 + li;
-          brh[b] = *reinterpret_cast<const bf16x8*>(Brh + col * RW + 4 * kr);
-          brl[b] = *reinterpret_cast<const bf16x8*>(Brl + col * RW + 4 * kr);
-          bih[b] = *reinterpret_cast<const bf16x8*>(Bih + col * RW + 4 * kr);
-          bil[b] = *reinterpret_cast<const bf16x8*>(Bil + col * RW + 4 * kr);
+          brh[b] = *reinterpret_cast<const bf16x8*>(Brh + col * RW + kw);
+          brl[b] = *reinterpret_cast<const bf16x8*>(Brl + col * RW + kw);
+          bih[b] = *reinterpret_cast<const bf16x8*>(Bih + col * RW + kw);
+          bil[b] = *reinterpret_cast<const bf16x8*>(Bil + col * RW + kw);
         }
 #pragma unroll
         for (int a = 0; a < FM; ++a)
@@ -274,6 +350,7 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
               accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bih[b], accI[a][b], 0, 0, 0);
             }
           }
+       }
       } else {
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 2) {
@@ -348,18 +425,22 @@ template <int BM, int BN, int WM, int WN, int P>
 static hipError_t launch_lp(const void* A, const void* B, void* C, const GemmItem* items,
                             int nitems, const GemmSeg* segs, const uint32_t* koff, int total_tiles,
                             int conjA, int conjB, int ksplit, void* partial, bool tr,
-                            hipStream_t stream) {
+                            hipStream_t stream, bool planes) {
   constexpr int NT = 64 * (BM / WM) * (BN / WN);
-  if (tr)
-    hipLaunchKernelGGL((zgemm_lowp_kernel<BM, BN, WM, WN, true, P>), dim3(total_tiles * ksplit),
-                       dim3(NT), 0, stream, (const double2*)A, (const double2*)B, (double2*)C,
-                       items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit,
-                       (double2*)partial);
-  else
-    hipLaunchKernelGGL((zgemm_lowp_kernel<BM, BN, WM, WN, false, P>), dim3(total_tiles * ksplit),
-                       dim3(NT), 0, stream, (const double2*)A, (const double2*)B, (double2*)C,
-                       items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit,
-                       (double2*)partial);
+  constexpr bool BF = (P == GRAFT_PREC_BF16X3);
+  if (planes && (!BF || ksplit != 1)) return hipErrorInvalidValue;
+#define GRAFT_LP(TRV, PLV)                                                                          \
+  hipLaunchKernelGGL((zgemm_lowp_kernel<BM, BN, WM, WN, TRV, P, PLV>), dim3(total_tiles * ksplit),  \
+                     dim3(NT), 0, stream, (const double2*)A, (const double2*)B, (double2*)C, items, \
+                     nitems, segs, koff, total_tiles, conjA, conjB, ksplit, (double2*)partial)
+  if constexpr (BF) {
+    if (planes) {
+      if (tr) GRAFT_LP(true, true); else GRAFT_LP(false, true);
+      return hipGetLastError();
+    }
+  }
+  if (tr) GRAFT_LP(true, false); else GRAFT_LP(false, false);
+#undef GRAFT_LP
   return hipGetLastError();
 }
 
@@ -370,16 +451,17 @@ template <int P>
 static hipError_t launch_lp_config(int config, const void* A, const void* B, void* C,
                                    const GemmItem* items, int nitems, const GemmSeg* segs,
                                    const uint32_t* koff, int total_tiles, int conjA, int conjB,
-                                   int ksplit, void* partial, bool tr, hipStream_t stream) {
+                                   int ksplit, void* partial, bool tr, hipStream_t stream,
+                                   bool planes) {
   switch (config) {
-    case 0: return launch_lp<128, 128, 64, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 1: case 8: case 12: case 14: case 16: case 18: return launch_lp<128, 64, 64, 32, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 2: case 7: case 11: case 15: return launch_lp<64, 64, 32, 32, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 3: return launch_lp<256, 16, 64, 16, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 4: case 9: case 13: case 17: return launch_lp<64, 128, 32, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 10: return launch_lp<64, 256, 64, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 5: return launch_lp<32, 32, 16, 16, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
-    case 6: return launch_lp<16, 256, 16, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+    case 0: return launch_lp<128, 128, 64, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream, planes);
+    case 1: case 8: case 12: case 14: case 16: case 18: return launch_lp<128, 64, 64, 32, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream, planes);
+    case 2: case 7: case 11: case 15: return launch_lp<64, 64, 32, 32, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream, planes);
+    case 3: return launch_lp<256, 16, 64, 16, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream, planes);
+    case 4: case 9: case 13: case 17: return launch_lp<64, 128, 32, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream, planes);
+    case 10: return launch_lp<64, 256, 64, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream, planes);
+    case 5: return launch_lp<32, 32, 16, 16, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream, planes);
+    case 6: return launch_lp<16, 256, 16, 64, P>(A, B, C, items, nitems, segs, koff, total_tiles, conjA, conjB, ksplit, partial, tr, stream, planes);
     default: return hipErrorInvalidValue;
   }
 }
@@ -387,14 +469,15 @@ static hipError_t launch_lp_config(int config, const void* A, const void* B, voi
 hipError_t launch_zgemm_lowp(int config, int precision, const void* A, const void* B, void* C,
                              const GemmItem* items, int nitems, const GemmSeg* segs,
                              const uint32_t* koff, int total_tiles, int conjA, int conjB,
-                             int ksplit, void* partial, bool tr, hipStream_t stream) {
+                             int ksplit, void* partial, bool tr, hipStream_t stream, bool planes) {
   if (precision == GRAFT_PREC_FP32)
     return launch_lp_config<GRAFT_PREC_FP32>(config, A, B, C, items, nitems, segs, koff,
-                                             total_tiles, conjA, conjB, ksplit, partial, tr, stream);
+                                             total_tiles, conjA, conjB, ksplit, partial, tr, stream,
+                                             planes);
   if (precision == GRAFT_PREC_BF16X3)
     return launch_lp_config<GRAFT_PREC_BF16X3>(config, A, B, C, items, nitems, segs, koff,
                                                total_tiles, conjA, conjB, ksplit, partial, tr,
-                                               stream);
+                                               stream, planes);
   return hipErrorNotSupported;
 }
 

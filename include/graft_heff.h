@@ -138,6 +138,13 @@ int graft_ctx_set_fused_map_loads(graft_ctx* ctx, int32_t on);
  * summation order.  No reference counterpart (TensorKit multiplies block by
  * block). */
 int graft_ctx_set_lean_stream(graft_ctx* ctx, int32_t on);
+/* BF16X3 applies: 1 (default) a dense streaming step whose result is a
+ * tiled GEMM step's A operand writes it as split-bf16 word pairs (8 bytes per
+ * element) and the GEMM stages them, with its B operand split once per apply,
+ * without converting on every tile load; 0 the GEMM converts FP64 operands
+ * itself.  Same values.  No reference counterpart (the reference multiplies
+ * in FP64 only). */
+int graft_ctx_set_split_words(graft_ctx* ctx, int32_t on);
 int graft_synchronize(graft_ctx* ctx);
 const char* graft_last_error(void);
 const char* graft_version(void);

@@ -336,6 +336,49 @@ def test_reduced_precision_h2(g, prec):
     assert _rel(h2.with_precision("fp64")(DTensor.from_numpy(x)).to_numpy(), want) < TOL
 
 
+def test_split_words_bf16x3(g):
+    """BF16x3 applies in which a streaming step hands its result to the GEMM
+    that reads it as A in split-bf16 words (and the GEMM's B is split once per
+    apply) give the result of the GEMM converting FP64 operands itself, on
+    every map of a binary(3) Heisenberg network with the streaming kernel
+    forced wherever a step can take it; both within the BF16x3 tolerance of
+    the oracle."""
+    from graft_amd import _native as nat, contractions as C, trees as Tr
+    from graft_amd.tensors import DTensor
+    ctx = nat.default_context()
+    topo = Tr.binary_topology(3)
+    psi, H, host, Wh, t = _network(g, topo, "heisenberg", 16, 5, 1)
+    envs = ref.all_envs(t, host, Wh)
+    rng = np.random.default_rng(4)
+    handed = 0
+    try:
+        for n, m in [(1, 0), (3, 1)]:
+            _say("split words bf16x3, h2 edge", n, m)
+            h2 = C.eff_h2(C.EnvCache(topo), psi, H, n, m, precision="bf16x3")
+            steps = h2.native.step_info()
+            forced = []
+            for i in range(len(steps)):
+                try:
+                    h2.native.set_tile_config(i, 19)     # the streaming kernel, where eligible
+                    forced.append(i)
+                except nat.GraftError:
+                    pass
+            consumers = [i + 1 for i in forced if i + 1 < len(steps) and i + 1 not in forced]
+            handed += len(consumers)
+            x = ref.random_complex(rng, h2.xdims)
+            xd = DTensor.from_numpy(x)
+            ctx.set_split_words(0)
+            y0 = h2(xd).data.clone()
+            ctx.set_split_words(1)
+            y1 = h2(xd)
+            assert bool((y0 == y1.data).all()), float((y0 - y1.data).abs().max())
+            want = ref.h2_apply(t, Wh, envs, n, m, x)
+            assert _rel(y1.to_numpy(), want) < PREC_TOL["bf16x3"]
+    finally:
+        ctx.set_split_words(1)
+    assert handed > 0
+
+
 # --------------------------------------------------------------------------
 # Krylov vector kernels
 # --------------------------------------------------------------------------
