@@ -1222,8 +1222,15 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
       maxS = std::max(maxS, sz);
     }
   }
+  // Automatic mode streams only when the output blocks sum few block pairs
+  // (at most 4 on average): a block that sums dozens re-reads its rows once
+  // per pair, and the per-pair records make large plans slow to build; such
+  // steps run as fused per-charge GEMMs instead (measured on Hubbard sweeps:
+  // D=128, two sweeps 2.9 s against 3.8 s).
   const bool stream_first =
-      bigside != 0 && (fuse_mode == 3 || (fuse_mode == 1 && maxS <= kBlockStreamS));
+      bigside != 0 &&
+      (fuse_mode == 3 || (fuse_mode == 1 && maxS <= kBlockStreamS &&
+                          pairs.size() <= 4 * C.blocks.size()));
   // Fused sector path for steps made of many small blocks (decided from the
   // output blocks alone, before any per-block GEMM item is built).
   ex.fused = false;
@@ -1234,7 +1241,12 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
   if (!stream_first && (fuse_mode == 1 || fuse_mode == 2) && C.blocks.size() >= 2 &&
       canonical_blocks(A) && canonical_blocks(B) && canonical_blocks(C)) {
     const double avg = (double)C.total / (double)C.blocks.size();
-    if (fuse_mode == 2 || (avg < 64.0 * 64.0 && C.blocks.size() >= 8)) {
+    // fused when the output blocks are small, or when each sums several block
+    // pairs (per-block items would repeat the output tile once per pair and
+    // carry one record per pair: measured slower in sweeps, and plans with
+    // hundreds of thousands of pairs take tens of ms to build and upload)
+    const bool many_pairs = pairs.size() >= 4 * C.blocks.size();
+    if (fuse_mode == 2 || ((avg < 64.0 * 64.0 || many_pairs) && C.blocks.size() >= 8)) {
       double macs = 0.0;
       for (size_t cb = 0; cb < C.blocks.size(); ++cb) {
         for (int64_t i = pstart[cb]; i < pstart[cb + 1]; ++i) {
@@ -1255,6 +1267,7 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
   }
   // Build items (one per output block) and segments.  Merged index groups
   // use fixed-capacity records: no heap allocation per block or pair.
+  auto t_items = clk();
   size_t koff_total = 0;
   for (const auto& pr : pairs) {
     const Block& ab = A.blocks[pr.first];
@@ -1381,6 +1394,8 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
     it.seg_count = (int)ex.segs.size() - it.seg_begin;
     ex.items.push_back(it);
   }
+  const double ms_items = timing ? ms_since(t_items) : 0.0;
+  auto t_group = clk();
   // Output blocks grouped by the large operand's open sector tuple (the
   // blocks of one group read the same rows of the large operand).
   if (bigside != 0) {
@@ -1424,6 +1439,10 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
     ex.small_total = bigside == 1 ? B.total : A.total;
     build_lean(ex);
   }
+  if (timing)
+    std::fprintf(stderr, "  contrib %.2f ms (%zu keys), C blocks %.2f ms, items %.2f ms (%zu segs, koff %zu), "
+                 "groups+lean %.2f ms\n", t_contrib, nkeys, t_cblocks, ms_items, ex.segs.size(),
+                 ex.koff.size(), ms_since(t_group));
   // Orientation: decided by the largest output block's unit-stride leg.
   {
     size_t big = 0;
@@ -2191,12 +2210,14 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
   auto t_up = now();
   rc = upload_step_meta(p);
   if (rc) return rc;
+  auto t_maps = now();
   rc = build_maps(ctx, p);
   if (rc) return rc;
   if (timing)
-    std::fprintf(stderr, "plan create: %.2f ms total, %.2f ms upload\n",
+    std::fprintf(stderr, "plan create: %.2f ms total, %.2f ms upload (%.1f MB), %.2f ms maps (%.1f MB)\n",
                  std::chrono::duration<double, std::milli>(now() - t_start).count(),
-                 std::chrono::duration<double, std::milli>(now() - t_up).count());
+                 std::chrono::duration<double, std::milli>(t_maps - t_up).count(), p.meta_bytes / 1e6,
+                 std::chrono::duration<double, std::milli>(now() - t_maps).count(), p.map_bytes / 1e6);
   *out = plan.release();
   return GRAFT_OK;
 }
