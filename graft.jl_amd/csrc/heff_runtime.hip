@@ -760,26 +760,40 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
   auto holes = [&](bool side_a, std::vector<HoleRect>& out) {
     out.clear();
     const int n1 = side_a ? nr : nc, n2 = side_a ? nc : ns;
-    std::unordered_set<int64_t> have;
-    if (side_a) {
-      for (size_t i = 0; i < A.blocks.size(); ++i) have.insert((int64_t)aR[i] * n2 + aC[i]);
-    } else {
-      for (size_t i = 0; i < B.blocks.size(); ++i) have.insert((int64_t)bC[i] * n2 + bS[i]);
-    }
+    // position of every tuple id inside its component's tuple list, and a
+    // presence bitmap per component (tuple pairs with a stored block)
     std::vector<std::vector<int>> t1(comps.size()), t2(comps.size());
+    std::vector<int> p1(n1, -1), p2(n2, -1);
     for (int x = 0; x < n1; ++x) {
       const int gi = side_a ? rg[x] : cg[x];
-      if (gi >= 0) t1[gi].push_back(x);
+      if (gi >= 0) {
+        p1[x] = (int)t1[gi].size();
+        t1[gi].push_back(x);
+      }
     }
     for (int y = 0; y < n2; ++y) {
       const int gi = side_a ? cg[y] : sg[y];
-      if (gi >= 0) t2[gi].push_back(y);
+      if (gi >= 0) {
+        p2[y] = (int)t2[gi].size();
+        t2[gi].push_back(y);
+      }
+    }
+    std::vector<std::vector<char>> have(comps.size());
+    for (size_t gi = 0; gi < comps.size(); ++gi) have[gi].assign(t1[gi].size() * t2[gi].size(), 0);
+    const size_t nb = side_a ? A.blocks.size() : B.blocks.size();
+    for (size_t i = 0; i < nb; ++i) {
+      const int x = side_a ? aR[i] : bC[i], y = side_a ? aC[i] : bS[i];
+      const int gi = side_a ? rg[x] : cg[x];
+      if (gi < 0 || p1[x] < 0 || p2[y] < 0) continue;
+      have[gi][(size_t)p1[x] * t2[gi].size() + p2[y]] = 1;
     }
     for (size_t gi = 0; gi < comps.size(); ++gi) {
       const Comp& g = comps[gi];
-      for (int x : t1[gi])
-        for (int y : t2[gi]) {
-          if (have.count((int64_t)x * n2 + y)) continue;
+      const size_t w = t2[gi].size();
+      for (size_t ix = 0; ix < t1[gi].size(); ++ix)
+        for (size_t iy = 0; iy < w; ++iy) {
+          if (have[gi][ix * w + iy]) continue;
+          const int x = t1[gi][ix], y = t2[gi][iy];
           HoleRect h;
           if (side_a) {   // rows r, columns c of A_g: (m, k) at m*K + k or m + M*k
             h.rows = (int32_t)tuple_size(A, st.oindA, R.keys[x]);
@@ -1552,6 +1566,13 @@ static void update_workspace_size(graft_plan_impl& p) {
   p.ws_bytes = p.ws_planes + planes;
 }
 
+// Pinned staging buffer for plan metadata uploads (grown on demand, kept
+// until the process ends): a pageable copy of tens of MB costs several times
+// the pinned one, and sweeps compile a plan for nearly every bond update.
+static std::mutex g_stage_mu;
+static char* g_stage = nullptr;
+static size_t g_stage_bytes = 0;
+
 static int upload_step_meta(graft_plan_impl& p) {
   // one allocation: per step [items | segs | koff], 256-byte aligned pieces
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
@@ -1573,26 +1594,35 @@ static int upload_step_meta(graft_plan_impl& p) {
   }
   GRAFT_CHECK_HIP(hipMalloc(&p.d_meta, total));
   p.meta_bytes = total;
-  std::vector<char> host(total, 0);
+  std::lock_guard<std::mutex> stage_lock(g_stage_mu);
+  if (g_stage_bytes < total) {
+    if (g_stage) (void)hipHostFree(g_stage);
+    g_stage = nullptr;
+    g_stage_bytes = 0;
+    const size_t want = std::max<size_t>(total, 16u << 20);
+    GRAFT_CHECK_HIP(hipHostMalloc((void**)&g_stage, want, hipHostMallocDefault));
+    g_stage_bytes = want;
+  }
+  char* host = g_stage;   // padding bytes between pieces are never read
   size_t off = 0;
   for (auto& ex : p.steps) {
     size_t s1 = ex.items.size() * sizeof(GemmItem);
-    std::memcpy(host.data() + off, ex.items.data(), s1);
+    std::memcpy(host + off, ex.items.data(), s1);
     ex.d_items = (GemmItem*)((char*)p.d_meta + off);
     off += al(s1);
     size_t s2 = ex.segs.size() * sizeof(GemmSeg);
-    if (s2) std::memcpy(host.data() + off, ex.segs.data(), s2);
+    if (s2) std::memcpy(host + off, ex.segs.data(), s2);
     ex.d_segs = (GemmSeg*)((char*)p.d_meta + off);
     off += al(std::max<size_t>(1, ex.segs.size()) * sizeof(GemmSeg));
     size_t s3 = ex.koff.size() * sizeof(uint32_t);
-    if (s3) std::memcpy(host.data() + off, ex.koff.data(), s3);
+    if (s3) std::memcpy(host + off, ex.koff.data(), s3);
     ex.d_koff = (uint32_t*)((char*)p.d_meta + off);
     off += al(std::max<size_t>(1, ex.koff.size()) * sizeof(uint32_t));
     PermBlock** dp[3] = {&ex.d_pa, &ex.d_pb, &ex.d_pc};
     std::vector<PermBlock>* hp[3] = {&ex.pa, &ex.pb, &ex.pc};
     for (int i = 0; i < 3; ++i) {
       const size_t sz = hp[i]->size() * sizeof(PermBlock);
-      if (sz) std::memcpy(host.data() + off, hp[i]->data(), sz);
+      if (sz) std::memcpy(host + off, hp[i]->data(), sz);
       *dp[i] = sz ? (PermBlock*)((char*)p.d_meta + off) : nullptr;
       off += al(sz);
     }
@@ -1600,21 +1630,21 @@ static int upload_step_meta(graft_plan_impl& p) {
     std::vector<int32_t>* hc[3] = {&ex.ca, &ex.cb, &ex.cc};
     for (int i = 0; i < 3; ++i) {
       const size_t sz = hc[i]->size() * sizeof(int32_t);
-      if (sz) std::memcpy(host.data() + off, hc[i]->data(), sz);
+      if (sz) std::memcpy(host + off, hc[i]->data(), sz);
       *dc[i] = sz ? (int32_t*)((char*)p.d_meta + off) : nullptr;
       off += al(sz);
     }
     const size_t so = ex.sorder.size() * sizeof(int32_t);
-    if (so) std::memcpy(host.data() + off, ex.sorder.data(), so);
+    if (so) std::memcpy(host + off, ex.sorder.data(), so);
     ex.d_sorder = so ? (int32_t*)((char*)p.d_meta + off) : nullptr;
     off += al(so);
     const size_t sw = ex.swork.size() * sizeof(StreamWork);
-    if (sw) std::memcpy(host.data() + off, ex.swork.data(), sw);
+    if (sw) std::memcpy(host + off, ex.swork.data(), sw);
     ex.d_swork = sw ? (StreamWork*)((char*)p.d_meta + off) : nullptr;
     off += al(sw);
     auto put = [&](const void* src, size_t sz) -> void* {
       if (!sz) return nullptr;
-      std::memcpy(host.data() + off, src, sz);
+      std::memcpy(host + off, src, sz);
       void* d = (char*)p.d_meta + off;
       off += al(sz);
       return d;
@@ -1625,7 +1655,7 @@ static int upload_step_meta(graft_plan_impl& p) {
     ex.d_holes_a = (HoleRect*)put(ex.holes_a.data(), ex.holes_a.size() * sizeof(HoleRect));
     ex.d_holes_b = (HoleRect*)put(ex.holes_b.data(), ex.holes_b.size() * sizeof(HoleRect));
   }
-  GRAFT_CHECK_HIP(hipMemcpy(p.d_meta, host.data(), total, hipMemcpyHostToDevice));
+  GRAFT_CHECK_HIP(hipMemcpy(p.d_meta, host, total, hipMemcpyHostToDevice));
   return GRAFT_OK;
 }
 
@@ -2086,6 +2116,7 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
     have_out = true;
   }
   p.steps.resize(nsteps);
+  std::vector<int> slot_producer(nslots, -1);
   static const bool timing = std::getenv("GRAFT_PLAN_TIMING") != nullptr;
   auto now = [] { return std::chrono::steady_clock::now(); };
   auto t_start = now();
@@ -2107,12 +2138,20 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
                   "step leg counts out of range");
     const TStruct* fixed = (st.dst == output_slot && have_out) ? &outfixed : nullptr;
     TStruct C;
+    // hole rectangles are needed only for an operand a fused step can hand
+    // over (fused, every result element mapped: compiled before this step)
+    auto handable = [&](int slot) {
+      if (slot < ninputs || slot_producer[slot] < 0) return false;
+      const StepExec& e = p.steps[slot_producer[slot]];
+      return e.fused && e.c_complete;
+    };
     if (compile_step(st, p.slots[st.a], p.slots[st.b], fixed, C, p.steps[s], err,
-                     ctx->sector_fuse, st.a >= ninputs, st.b >= ninputs)) {
+                     ctx->sector_fuse, handable(st.a), handable(st.b))) {
       set_error("step " + std::to_string(s) + ": " + err);
       return GRAFT_E_INVALID;
     }
     p.slots[st.dst] = std::move(C);
+    slot_producer[st.dst] = s;
     defined[st.dst] = 1;
     born[st.dst] = s;
     if (st.a >= ninputs) dies[st.a] = s;
