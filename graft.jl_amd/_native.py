@@ -84,6 +84,10 @@ def lib():
         "graft_plan_create": [vp, ctypes.POINTER(PairStepC), i32, i32, i32,
                               ctypes.POINTER(TensorDescC), i32, ctypes.POINTER(TensorDescC),
                               ctypes.POINTER(vp)],
+        "graft_plan_create_coupled": [vp, ctypes.POINTER(PairStepC), i32, i32, i32,
+                                      ctypes.POINTER(TensorDescC), i32,
+                                      ctypes.POINTER(TensorDescC), ctypes.POINTER(i64),
+                                      ctypes.POINTER(i32), dp, ctypes.POINTER(vp)],
         "graft_plan_destroy": [vp],
         "graft_plan_info": [vp, ctypes.POINTER(u64), dp, ctypes.POINTER(u64)],
         "graft_plan_device_bytes": [vp, ctypes.POINTER(u64)],

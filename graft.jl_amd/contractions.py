@@ -5,8 +5,9 @@ env!, _build_env_spec, invalidation) and effective.jl (_h1_spec, _h2_spec,
 _h0_spec, eff_h1 / eff_h2 / eff_h0, two_site_tensor, split_two_site!) with
 0-based indices.  The label bookkeeping is the reference's; execution is a
 compiled plan in libgraft_heff.so.  Pivotal twists (envcache.jl:883-905) are
-the identity for trivial and bosonic abelian sectors, which is every space
-this engine handles, so none are applied.
+the identity for trivial, bosonic abelian and SU(2) sectors (SU(2) braids
+bosonically: every irrep's twist is 1), which is every space this engine
+handles, so none are applied.
 """
 from __future__ import annotations
 
@@ -134,6 +135,17 @@ def _legs_of(t):
     return None if t.blocks is None else t.blocks.legs
 
 
+def _is_su2(structure):
+    return structure is not None and hasattr(structure, "dense_from_flat")
+
+
+def _invariant_structure(legs):
+    if hasattr(legs[0], "isdual"):
+        from .su2 import SU2Structure
+        return SU2Structure.invariant(legs)
+    return BlockStructure.invariant(legs)
+
+
 def _native_output_structure(native, plan, legs, conjs):
     """BlockStructure of a sector plan's result: legs from the plan, blocks from the library."""
     sl = [conj_legs(l, c) for l, c in zip(legs, conjs)] + [None] * len(plan.steps)
@@ -216,14 +228,20 @@ class EnvCache:
         self.plan_misses += 1
         dims = [list(op.dims) for op in operands]
         legs = None
-        if any(op.blocks is not None for op in operands):
+        su2 = any(_is_su2(op.blocks) for op in operands)
+        if any(op.blocks is not None for op in operands) and not su2:
             legs = [_legs_of(op) for op in operands]
+        # SU(2) networks keep the planner's dense selection (non-unique
+        # fusion, planner.jl:566-583) and compile to coupled block-pair steps
         plan = plan_contraction(spec, dims, legs, optimize=optimize, memory_weight=memory_weight,
                                 sector_aware=sector_aware, memory_cap_bytes=memory_cap_bytes,
                                 fused_permutes=True, order_cache=_ORDER_CACHE)
         out_dims = slot_dims(plan, dims)[plan.output_slot]
         p = None
-        if (self.tune_orders and legs is None and optimize and memory_cap_bytes is None
+        if su2:
+            from .su2 import CoupledPlan
+            p = CoupledPlan(plan, operands, out_structure, out_dims)
+        elif (self.tune_orders and legs is None and optimize and memory_cap_bytes is None
                 and kind in _TUNED_KINDS):
             p = _measured_order(spec, dims, plan, operands, out_dims)
         if p is None:
@@ -259,6 +277,9 @@ class EnvCache:
         if c is None:
             if legs is None:
                 c = B.ones_tensor(dims)
+            elif hasattr(legs[0], "isdual"):
+                from .su2 import root_cap
+                c = root_cap(legs)
             else:
                 from .sectors import HostBlocks, invariant_blocks
                 blocks = {}
@@ -653,7 +674,7 @@ def eff_h0(cache, psi, H, n, m, optimize=True, memory_weight=1.0, sector_aware=T
         xl = [En.blocks.legs[0].dual(), Em.blocks.legs[0].dual()]
         if En.blocks.legs[2] != xl[0] or Em.blocks.legs[2] != xl[1]:
             raise ValueError("eff_h0: the environments' ket and bra legs are not one bond space")
-        blocks = BlockStructure.invariant(xl)
+        blocks = _invariant_structure(xl)
     proto = TensorProto((En.dims[0], Em.dims[0]), 1, blocks)
     native = cache.get_plan("h0", spec, [proto] + statics, optimize, memory_weight, sector_aware,
                             memory_cap_bytes, out_structure=blocks)
@@ -673,6 +694,9 @@ def two_site_tensor(psi, n, m):
     if t.parent[n] != m:
         raise ValueError("two_site_tensor: m must be the parent of n")
     if psi.tensors[n].blocks is not None:
+        if _is_su2(psi.tensors[n].blocks):
+            from . import su2
+            return su2.two_site_tensor(psi, n, m)
         from . import sectors
         return sectors.two_site_tensor(psi, n, m)
     A, Bm = psi.tensors[n], psi.tensors[m]

@@ -68,10 +68,14 @@ inline int xcd_grid(const XcdSplit& s) {
 // Element maps of a fused sector step read / written inside the GEMM: entry
 // i of a fused matrix lives at blocked element a[i] / b[i] / c[i]
 // (kMapZero: absent — zero when read, skipped when stored).
+// coef (coupled steps, single-buffered tiles only): a real factor per
+// segment, applied to A's values as they are staged — the recoupling
+// coefficient of a non-abelian block pair.
 struct FusedMaps {
   const uint32_t* a;
   const uint32_t* b;
   const uint32_t* c;
+  const double* coef = nullptr;
 };
 
 // Launch one grouped strided complex GEMM (precision GRAFT_PREC_*).

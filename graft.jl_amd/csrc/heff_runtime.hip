@@ -118,7 +118,11 @@ struct TStruct {
   int64_t total = 0;  // elements spanned by the buffer
 };
 
-static int build_struct(const graft_tensor_desc& d, TStruct& t, std::string& err) {
+// allow_dup: blocks may share a sector tuple (tensors of a coupled plan,
+// whose blocks are told apart by fusion-tree labels the descriptor does not
+// carry); such structures are addressed by block position, not by index.
+static int build_struct(const graft_tensor_desc& d, TStruct& t, std::string& err,
+                        bool allow_dup = false) {
   if (d.rank < 0 || d.rank > GRAFT_MAX_RANK) {
     err = "tensor rank out of range";
     return 1;
@@ -185,11 +189,13 @@ static int build_struct(const graft_tensor_desc& d, TStruct& t, std::string& err
     int64_t span = 1;
     for (int l = 0; l < d.rank; ++l) span += (blk.dims[l] - 1) * blk.strides[l];
     t.total = std::max(t.total, blk.offset + span);
-    if (t.index.count(blk.sec)) {
-      err = "duplicate block";
-      return 1;
+    if (!allow_dup) {
+      if (t.index.count(blk.sec)) {
+        err = "duplicate block";
+        return 1;
+      }
+      t.index[blk.sec] = (int)t.blocks.size();
     }
-    t.index[blk.sec] = (int)t.blocks.size();
     t.blocks.push_back(std::move(blk));
   }
   return 0;
@@ -276,6 +282,10 @@ struct StepExec {
   // split once per apply into the plane region (pl_from: the producer).
   int pl_to = -1, pl_from = -1;
   uint64_t bplane_bytes = 0;
+  // Coupled step (non-abelian blocks): one real factor per segment, applied
+  // to the A tile as it is staged; FP64 single-buffered tiles only.
+  std::vector<double> coef;
+  double* d_coef = nullptr;
 };
 
 struct graft_plan_impl {
@@ -1015,6 +1025,158 @@ static void build_lean(StepExec& ex) {
   ex.lean = true;
 }
 
+// One GEMM item per output block and one segment per contributing block
+// pair (the pairs of output block b are pairs[pstart[b] .. pstart[b + 1])).
+// Merged index groups use fixed-capacity records: no heap allocation per
+// block or pair.
+static int build_items(const graft_pair_step& st, const TStruct& A, const TStruct& B,
+                       const TStruct& C, const std::vector<int>& pos_of_out,
+                       const std::vector<std::pair<int, int>>& pairs,
+                       const std::vector<int64_t>& pstart, StepExec& ex, std::string& err) {
+  const int nOA = st.n_oindA, nCA = st.n_cindA, nOB = st.n_oindB;
+  size_t koff_total = 0;
+  for (const auto& pr : pairs) {
+    const Block& ab = A.blocks[pr.first];
+    int64_t K = 1;
+    for (int i = 0; i < nCA; ++i) K *= ab.dims[st.cindA[i]];
+    koff_total += 2 * (size_t)K;
+  }
+  ex.items.reserve(C.blocks.size());
+  ex.segs.reserve(pairs.size());
+  ex.koff.reserve(koff_total);
+  std::vector<SmallVec<int64_t>> am, bn;
+  for (int cb = 0; cb < (int)C.blocks.size(); ++cb) {
+    const Block& cblk = C.blocks[cb];
+    GemmItem it;
+    std::memset(&it, 0, sizeof(it));
+    it.c_off = cblk.offset;
+    it.seg_begin = (int)ex.segs.size();
+    const std::pair<int, int>* prs = pairs.data() + pstart[cb];
+    const size_t np = (size_t)(pstart[cb + 1] - pstart[cb]);
+    // m/n dims are fixed by the output block; strides per contribution.
+    SmallVec<int64_t> md, nd, cm0, cn0;
+    for (int p = 0; p < nOA; ++p) {
+      const int q = pos_of_out[p];
+      md.push_back(cblk.dims[q]);
+      cm0.push_back(cblk.strides[q]);
+    }
+    for (int p = 0; p < nOB; ++p) {
+      const int q = pos_of_out[nOA + p];
+      nd.push_back(cblk.dims[q]);
+      cn0.push_back(cblk.strides[q]);
+    }
+    int64_t M = 1, N = 1;
+    for (auto d : md) M *= d;
+    for (auto d : nd) N *= d;
+    // Merge decisions must hold for every contribution: one common merged
+    // layout over C, every A block and every B block of this output block.
+    if (am.size() < np) {
+      am.resize(np);
+      bn.resize(np);
+    }
+    for (size_t sgi = 0; sgi < np; ++sgi) {
+      const Block& ab = A.blocks[prs[sgi].first];
+      const Block& bb = B.blocks[prs[sgi].second];
+      am[sgi].n = 0;
+      bn[sgi].n = 0;
+      for (int p = 0; p < nOA; ++p) am[sgi].push_back(ab.strides[st.oindA[p]]);
+      for (int p = 0; p < nOB; ++p) bn[sgi].push_back(bb.strides[st.oindB[p]]);
+    }
+    merge_legs(md, cm0, am.data(), np);
+    merge_legs(nd, cn0, bn.data(), np);
+    if ((int)md.size() > GL || (int)nd.size() > GL) {
+      err = "too many unmerged legs in one GEMM index group";
+      return 1;
+    }
+    it.M = (int32_t)M;
+    it.N = (int32_t)N;
+    if (M >= (1ll << 31) || N >= (1ll << 31)) {
+      err = "GEMM extent exceeds 2^31";
+      return 1;
+    }
+    it.mrank = (int)md.size();
+    it.nrank = (int)nd.size();
+    for (int l = 0; l < it.mrank; ++l) {
+      it.mdims[l] = (int32_t)md[l];
+      it.c_mstr[l] = (uint32_t)cm0[l];
+    }
+    for (int l = 0; l < it.nrank; ++l) {
+      it.ndims[l] = (int32_t)nd[l];
+      it.c_nstr[l] = (uint32_t)cn0[l];
+    }
+    ex.maxM = std::max(ex.maxM, M);
+    ex.maxN = std::max(ex.maxN, N);
+    for (size_t sgi = 0; sgi < np; ++sgi) {
+      const Block& ab = A.blocks[prs[sgi].first];
+      const Block& bb = B.blocks[prs[sgi].second];
+      GemmSeg sg;
+      std::memset(&sg, 0, sizeof(sg));
+      sg.a_off = ab.offset;
+      sg.b_off = bb.offset;
+      for (int l = 0; l < it.mrank; ++l) sg.a_mstr[l] = (uint32_t)am[sgi][l];
+      for (int l = 0; l < it.nrank; ++l) sg.b_nstr[l] = (uint32_t)bn[sgi][l];
+      // contracted legs
+      SmallVec<int64_t> kd, ka, kb;
+      for (int i = 0; i < nCA; ++i) {
+        kd.push_back(ab.dims[st.cindA[i]]);
+        ka.push_back(ab.strides[st.cindA[i]]);
+        kb.push_back(bb.strides[st.cindB[i]]);
+      }
+      merge_legs(kd, ka, &kb, 1);
+      int64_t K = 1;
+      for (auto d : kd) K *= d;
+      if (K >= (1ll << 31)) {
+        err = "contracted extent exceeds 2^31";
+        return 1;
+      }
+      sg.K = (int32_t)K;
+      sg.koff_idx = (int32_t)ex.koff.size();
+      ex.koff.resize(ex.koff.size() + 2 * K);
+      for (int64_t q = 0; q < K; ++q) {
+        int64_t idx = q, oa = 0, ob = 0;
+        for (size_t l = 0; l < kd.size(); ++l) {
+          const int64_t i = idx % kd[l];
+          idx /= kd[l];
+          oa += i * ka[l];
+          ob += i * kb[l];
+        }
+        ex.koff[sg.koff_idx + q] = (uint32_t)oa;
+        ex.koff[sg.koff_idx + K + q] = (uint32_t)ob;
+      }
+      // unit-stride leg decides the load direction
+      int64_t minm = INT64_MAX, mink = INT64_MAX, minn = INT64_MAX, minkb = INT64_MAX;
+      for (int l = 0; l < it.mrank; ++l) minm = std::min(minm, am[sgi][l]);
+      for (int l = 0; l < it.nrank; ++l) minn = std::min(minn, bn[sgi][l]);
+      for (size_t l = 0; l < kd.size(); ++l) {
+        mink = std::min(mink, ka[l]);
+        minkb = std::min(minkb, kb[l]);
+      }
+      sg.a_kfast = (mink < minm) ? 1 : 0;
+      sg.b_kfast = (minkb < minn) ? 1 : 0;
+      ex.segs.push_back(sg);
+      ex.maxK = std::max(ex.maxK, K);
+      ex.macs += (double)M * (double)K * (double)N;
+    }
+    it.seg_count = (int)ex.segs.size() - it.seg_begin;
+    ex.items.push_back(it);
+  }
+  return 0;
+}
+
+// MFMA orientation decided by the largest output block's unit-stride leg.
+static void auto_orientation(StepExec& ex) {
+  if (ex.items.empty()) return;
+  size_t big = 0;
+  for (size_t i = 1; i < ex.items.size(); ++i)
+    if ((int64_t)ex.items[i].M * ex.items[i].N > (int64_t)ex.items[big].M * ex.items[big].N) big = i;
+  const GemmItem& it = ex.items[big];
+  uint32_t mm = UINT32_MAX, mn = UINT32_MAX;
+  for (int l = 0; l < it.mrank; ++l) mm = std::min(mm, it.c_mstr[l]);
+  for (int l = 0; l < it.nrank; ++l) mn = std::min(mn, it.c_nstr[l]);
+  ex.tr_auto = mm < mn;
+  ex.tr = ex.tr_auto;
+}
+
 static int compile_step(const graft_pair_step& st, const TStruct& A, const TStruct& B,
                         const TStruct* fixed_out, TStruct& C, StepExec& ex, std::string& err,
                         int fuse_mode, bool want_holes_a, bool want_holes_b) {
@@ -1279,135 +1441,8 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
       return 0;
     }
   }
-  // Build items (one per output block) and segments.  Merged index groups
-  // use fixed-capacity records: no heap allocation per block or pair.
   auto t_items = clk();
-  size_t koff_total = 0;
-  for (const auto& pr : pairs) {
-    const Block& ab = A.blocks[pr.first];
-    int64_t K = 1;
-    for (int i = 0; i < nCA; ++i) K *= ab.dims[st.cindA[i]];
-    koff_total += 2 * (size_t)K;
-  }
-  ex.items.reserve(C.blocks.size());
-  ex.segs.reserve(pairs.size());
-  ex.koff.reserve(koff_total);
-  std::vector<SmallVec<int64_t>> am, bn;
-  for (int cb = 0; cb < (int)C.blocks.size(); ++cb) {
-    const Block& cblk = C.blocks[cb];
-    GemmItem it;
-    std::memset(&it, 0, sizeof(it));
-    it.c_off = cblk.offset;
-    it.seg_begin = (int)ex.segs.size();
-    const std::pair<int, int>* prs = pairs.data() + pstart[cb];
-    const size_t np = (size_t)(pstart[cb + 1] - pstart[cb]);
-    // m/n dims are fixed by the output block; strides per contribution.
-    SmallVec<int64_t> md, nd, cm0, cn0;
-    for (int p = 0; p < nOA; ++p) {
-      const int q = pos_of_out[p];
-      md.push_back(cblk.dims[q]);
-      cm0.push_back(cblk.strides[q]);
-    }
-    for (int p = 0; p < nOB; ++p) {
-      const int q = pos_of_out[nOA + p];
-      nd.push_back(cblk.dims[q]);
-      cn0.push_back(cblk.strides[q]);
-    }
-    int64_t M = 1, N = 1;
-    for (auto d : md) M *= d;
-    for (auto d : nd) N *= d;
-    // Merge decisions must hold for every contribution: one common merged
-    // layout over C, every A block and every B block of this output block.
-    if (am.size() < np) {
-      am.resize(np);
-      bn.resize(np);
-    }
-    for (size_t sgi = 0; sgi < np; ++sgi) {
-      const Block& ab = A.blocks[prs[sgi].first];
-      const Block& bb = B.blocks[prs[sgi].second];
-      am[sgi].n = 0;
-      bn[sgi].n = 0;
-      for (int p = 0; p < nOA; ++p) am[sgi].push_back(ab.strides[st.oindA[p]]);
-      for (int p = 0; p < nOB; ++p) bn[sgi].push_back(bb.strides[st.oindB[p]]);
-    }
-    merge_legs(md, cm0, am.data(), np);
-    merge_legs(nd, cn0, bn.data(), np);
-    if ((int)md.size() > GL || (int)nd.size() > GL) {
-      err = "too many unmerged legs in one GEMM index group";
-      return 1;
-    }
-    it.M = (int32_t)M;
-    it.N = (int32_t)N;
-    if (M >= (1ll << 31) || N >= (1ll << 31)) {
-      err = "GEMM extent exceeds 2^31";
-      return 1;
-    }
-    it.mrank = (int)md.size();
-    it.nrank = (int)nd.size();
-    for (int l = 0; l < it.mrank; ++l) {
-      it.mdims[l] = (int32_t)md[l];
-      it.c_mstr[l] = (uint32_t)cm0[l];
-    }
-    for (int l = 0; l < it.nrank; ++l) {
-      it.ndims[l] = (int32_t)nd[l];
-      it.c_nstr[l] = (uint32_t)cn0[l];
-    }
-    ex.maxM = std::max(ex.maxM, M);
-    ex.maxN = std::max(ex.maxN, N);
-    for (size_t sgi = 0; sgi < np; ++sgi) {
-      const Block& ab = A.blocks[prs[sgi].first];
-      const Block& bb = B.blocks[prs[sgi].second];
-      GemmSeg sg;
-      std::memset(&sg, 0, sizeof(sg));
-      sg.a_off = ab.offset;
-      sg.b_off = bb.offset;
-      for (int l = 0; l < it.mrank; ++l) sg.a_mstr[l] = (uint32_t)am[sgi][l];
-      for (int l = 0; l < it.nrank; ++l) sg.b_nstr[l] = (uint32_t)bn[sgi][l];
-      // contracted legs
-      SmallVec<int64_t> kd, ka, kb;
-      for (int i = 0; i < nCA; ++i) {
-        kd.push_back(ab.dims[st.cindA[i]]);
-        ka.push_back(ab.strides[st.cindA[i]]);
-        kb.push_back(bb.strides[st.cindB[i]]);
-      }
-      merge_legs(kd, ka, &kb, 1);
-      int64_t K = 1;
-      for (auto d : kd) K *= d;
-      if (K >= (1ll << 31)) {
-        err = "contracted extent exceeds 2^31";
-        return 1;
-      }
-      sg.K = (int32_t)K;
-      sg.koff_idx = (int32_t)ex.koff.size();
-      ex.koff.resize(ex.koff.size() + 2 * K);
-      for (int64_t q = 0; q < K; ++q) {
-        int64_t idx = q, oa = 0, ob = 0;
-        for (size_t l = 0; l < kd.size(); ++l) {
-          const int64_t i = idx % kd[l];
-          idx /= kd[l];
-          oa += i * ka[l];
-          ob += i * kb[l];
-        }
-        ex.koff[sg.koff_idx + q] = (uint32_t)oa;
-        ex.koff[sg.koff_idx + K + q] = (uint32_t)ob;
-      }
-      // unit-stride leg decides the load direction
-      int64_t minm = INT64_MAX, mink = INT64_MAX, minn = INT64_MAX, minkb = INT64_MAX;
-      for (int l = 0; l < it.mrank; ++l) minm = std::min(minm, am[sgi][l]);
-      for (int l = 0; l < it.nrank; ++l) minn = std::min(minn, bn[sgi][l]);
-      for (size_t l = 0; l < kd.size(); ++l) {
-        mink = std::min(mink, ka[l]);
-        minkb = std::min(minkb, kb[l]);
-      }
-      sg.a_kfast = (mink < minm) ? 1 : 0;
-      sg.b_kfast = (minkb < minn) ? 1 : 0;
-      ex.segs.push_back(sg);
-      ex.maxK = std::max(ex.maxK, K);
-      ex.macs += (double)M * (double)K * (double)N;
-    }
-    it.seg_count = (int)ex.segs.size() - it.seg_begin;
-    ex.items.push_back(it);
-  }
+  if (build_items(st, A, B, C, pos_of_out, pairs, pstart, ex, err)) return 1;
   const double ms_items = timing ? ms_since(t_items) : 0.0;
   auto t_group = clk();
   // Output blocks grouped by the large operand's open sector tuple (the
@@ -1457,18 +1492,7 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
     std::fprintf(stderr, "  contrib %.2f ms (%zu keys), C blocks %.2f ms, items %.2f ms (%zu segs, koff %zu), "
                  "groups+lean %.2f ms\n", t_contrib, nkeys, t_cblocks, ms_items, ex.segs.size(),
                  ex.koff.size(), ms_since(t_group));
-  // Orientation: decided by the largest output block's unit-stride leg.
-  {
-    size_t big = 0;
-    for (size_t i = 1; i < ex.items.size(); ++i)
-      if ((int64_t)ex.items[i].M * ex.items[i].N > (int64_t)ex.items[big].M * ex.items[big].N) big = i;
-    const GemmItem& it = ex.items[big];
-    uint32_t mm = UINT32_MAX, mn = UINT32_MAX;
-    for (int l = 0; l < it.mrank; ++l) mm = std::min(mm, it.c_mstr[l]);
-    for (int l = 0; l < it.nrank; ++l) mn = std::min(mn, it.c_nstr[l]);
-    ex.tr_auto = mm < mn;
-    ex.tr = ex.tr_auto;
-  }
+  auto_orientation(ex);
   // Streaming-kernel eligibility: one dense block, short K, one small open side.
   ex.skinny = 0;
   if (ex.items.size() == 1 && ex.segs.size() == 1) {
@@ -1483,6 +1507,122 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
     else if (fits(it.M) && it.N > it.M)
       ex.skinny = 2;
   }
+  return 0;
+}
+
+// A coupled step: the caller lists every contributing block pair of every
+// output block with a real coefficient (non-abelian tensors: the recoupling
+// coefficient of the pair's fusion trees).  Blocks are addressed by their
+// position in the descriptors; a pair's A and B blocks must agree on the
+// contracted legs' sectors, and its output block must carry the pair's open
+// sectors.  The pairs of one output block are summed in the order given.
+static int compile_coupled_step(const graft_pair_step& st, const TStruct& A, const TStruct& B,
+                                const TStruct& C, const int32_t* triples, const double* coefs,
+                                int64_t npairs, StepExec& ex, std::string& err) {
+  const int nOA = st.n_oindA, nCA = st.n_cindA, nOB = st.n_oindB, nCB = st.n_cindB;
+  if (nCA != nCB || nOA + nCA != A.rank || nOB + nCB != B.rank) {
+    err = "pair step leg counts do not match operand ranks";
+    return 1;
+  }
+  const int nOut = st.n_out_codomain + st.n_out_domain;
+  if (nOut != nOA + nOB || C.rank != nOut) {
+    err = "output legs do not match the open legs";
+    return 1;
+  }
+  std::vector<int> seenA(A.rank, 0), seenB(B.rank, 0), seenO(nOut, 0);
+  for (int i = 0; i < nOA; ++i) seenA.at(st.oindA[i])++;
+  for (int i = 0; i < nCA; ++i) seenA.at(st.cindA[i])++;
+  for (int i = 0; i < nOB; ++i) seenB.at(st.oindB[i])++;
+  for (int i = 0; i < nCB; ++i) seenB.at(st.cindB[i])++;
+  for (int i = 0; i < nOut; ++i) seenO.at(st.out[i])++;
+  for (int v : seenA) if (v != 1) { err = "A legs are not a partition"; return 1; }
+  for (int v : seenB) if (v != 1) { err = "B legs are not a partition"; return 1; }
+  for (int v : seenO) if (v != 1) { err = "output permutation is not a permutation"; return 1; }
+  for (int i = 0; i < nCA; ++i)
+    if (A.secdims[st.cindA[i]] != B.secdims[st.cindB[i]]) {
+      err = "contracted legs have different degeneracies";
+      return 1;
+    }
+  std::vector<int> pos_of_out(nOut);
+  for (int q = 0; q < nOut; ++q) {
+    const int p = st.out[q];
+    pos_of_out[p] = q;
+    const auto& want = p < nOA ? A.secdims[st.oindA[p]] : B.secdims[st.oindB[p - nOA]];
+    if (C.secdims[q] != want) {
+      err = "output leg " + std::to_string(q) + " does not match its open leg";
+      return 1;
+    }
+  }
+  if (C.blocks.empty()) {
+    err = "coupled step without output blocks";
+    return 1;
+  }
+  const int64_t nA = (int64_t)A.blocks.size(), nB = (int64_t)B.blocks.size();
+  const int64_t nC = (int64_t)C.blocks.size();
+  std::vector<int64_t> pstart(nC + 1, 0);
+  for (int64_t i = 0; i < npairs; ++i) {
+    const int32_t c = triples[3 * i], a = triples[3 * i + 1], b = triples[3 * i + 2];
+    if (c < 0 || c >= nC || a < 0 || a >= nA || b < 0 || b >= nB) {
+      err = "block pair " + std::to_string(i) + " indexes a block out of range";
+      return 1;
+    }
+    const Block& ab = A.blocks[a];
+    const Block& bb = B.blocks[b];
+    const Block& cb = C.blocks[c];
+    for (int q = 0; q < nCA; ++q)
+      if (ab.sec[st.cindA[q]] != bb.sec[st.cindB[q]]) {
+        err = "block pair " + std::to_string(i) + " disagrees on a contracted sector";
+        return 1;
+      }
+    for (int p = 0; p < nOA; ++p)
+      if (cb.sec[pos_of_out[p]] != ab.sec[st.oindA[p]]) {
+        err = "block pair " + std::to_string(i) + " does not carry its output block's sectors";
+        return 1;
+      }
+    for (int p = 0; p < nOB; ++p)
+      if (cb.sec[pos_of_out[nOA + p]] != bb.sec[st.oindB[p]]) {
+        err = "block pair " + std::to_string(i) + " does not carry its output block's sectors";
+        return 1;
+      }
+    if (!std::isfinite(coefs[i])) {
+      err = "block pair " + std::to_string(i) + " has a non-finite coefficient";
+      return 1;
+    }
+    pstart[c + 1]++;
+  }
+  for (int64_t c = 0; c < nC; ++c) pstart[c + 1] += pstart[c];
+  std::vector<std::pair<int, int>> pairs((size_t)npairs);
+  std::vector<double> cf((size_t)npairs);
+  {
+    std::vector<int64_t> fill(pstart.begin(), pstart.end() - 1);
+    for (int64_t i = 0; i < npairs; ++i) {
+      const int64_t j = fill[triples[3 * i]]++;
+      pairs[j] = {triples[3 * i + 1], triples[3 * i + 2]};
+      cf[j] = coefs[i];
+    }
+  }
+  if ((uint64_t)A.total >= (1ull << 32) - 2 || (uint64_t)B.total >= (1ull << 32) - 2 ||
+      (uint64_t)C.total >= (1ull << 32) - 2) {
+    err = "tensor exceeds 2^32 elements (32-bit tile offsets)";
+    return 1;
+  }
+  ex.a = st.a;
+  ex.b = st.b;
+  ex.dst = st.dst;
+  ex.conjA = st.conjA;
+  ex.conjB = st.conjB;
+  ex.items.clear();
+  ex.segs.clear();
+  ex.koff.clear();
+  ex.macs = 0.0;
+  ex.fused = false;
+  ex.bstream = 0;
+  ex.bstream_pref = false;
+  ex.skinny = 0;
+  ex.lean = false;
+  if (build_items(st, A, B, C, pos_of_out, pairs, pstart, ex, err)) return 1;
+  ex.coef = std::move(cf);
+  auto_orientation(ex);
   return 0;
 }
 
@@ -1541,7 +1681,8 @@ static void assign_tiles(StepExec& ex) {
   // long contraction (environment closures contract D*D*d legs at once).
   ex.ksplit = 1;
   ex.partial_bytes = 0;
-  if (ex.items.size() == 1 && ex.segs.size() == 1 && t > 0 && t < kTargetBlocks) {
+  if (ex.items.size() == 1 && ex.segs.size() == 1 && t > 0 && t < kTargetBlocks &&
+      ex.coef.empty()) {   // coupled steps run whole tiles (mapped launch)
     const int64_t K = ex.segs[0].K;
     const int64_t ktiles = (K + c.bk - 1) / c.bk;
     int64_t ks = (kTargetBlocks + t - 1) / t;
@@ -1587,6 +1728,7 @@ static int upload_step_meta(graft_plan_impl& p) {
     total += al(ex.litems.size() * sizeof(LeanItem)) + al(ex.lsegs.size() * sizeof(LeanSeg)) +
              al(ex.lwork.size() * sizeof(LeanWork));
     total += al(ex.holes_a.size() * sizeof(HoleRect)) + al(ex.holes_b.size() * sizeof(HoleRect));
+    total += al(ex.coef.size() * sizeof(double));
   }
   if (p.d_meta) {
     (void)hipFree(p.d_meta);
@@ -1654,6 +1796,7 @@ static int upload_step_meta(graft_plan_impl& p) {
     ex.d_lwork = (LeanWork*)put(ex.lwork.data(), ex.lwork.size() * sizeof(LeanWork));
     ex.d_holes_a = (HoleRect*)put(ex.holes_a.data(), ex.holes_a.size() * sizeof(HoleRect));
     ex.d_holes_b = (HoleRect*)put(ex.holes_b.data(), ex.holes_b.size() * sizeof(HoleRect));
+    ex.d_coef = (double*)put(ex.coef.data(), ex.coef.size() * sizeof(double));
   }
   GRAFT_CHECK_HIP(hipMemcpy(p.d_meta, host, total, hipMemcpyHostToDevice));
   return GRAFT_OK;
@@ -1818,6 +1961,18 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     GRAFT_CHECK_HIP(launch_zgemm(ex.config, precision, ma ? A : Af, mb ? B : Bf, C, ex.d_items,
                                  (int)ex.items.size(), ex.d_segs, ex.d_koff, ex.total_tiles,
                                  ex.conjA, ex.conjB, 1, nullptr, ex.tr, ctx->stream, &maps));
+    return GRAFT_OK;
+  }
+  if (ex.d_coef != nullptr) {
+    // coupled step: every block pair scaled by its coefficient as A is staged
+    if (precision != GRAFT_PREC_FP64) {
+      set_error("coupled (non-abelian) steps multiply in FP64 only");
+      return GRAFT_E_UNSUPPORTED;
+    }
+    const FusedMaps maps{nullptr, nullptr, nullptr, ex.d_coef};
+    GRAFT_CHECK_HIP(launch_zgemm(ex.config, precision, A, B, C, ex.d_items, (int)ex.items.size(),
+                                 ex.d_segs, ex.d_koff, ex.total_tiles, ex.conjA, ex.conjB, 1,
+                                 nullptr, ex.tr, ctx->stream, &maps));
     return GRAFT_OK;
   }
   if (ex.fused) {
@@ -2080,9 +2235,19 @@ int graft_download(graft_ctx* ctx, void* host_dst, const void* src, uint64_t byt
   return GRAFT_OK;
 }
 
-int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nsteps,
-                      int32_t nslots, int32_t output_slot, const graft_tensor_desc* input_descs,
-                      int32_t ninputs, const graft_tensor_desc* out_desc, graft_plan** out) {
+// Explicit block pairs of a coupled plan (graft_plan_create_coupled).
+struct CoupledSpec {
+  const graft_tensor_desc* outs;   // per step: its result's block list
+  const int64_t* npairs;           // per step
+  const int32_t* pairs;            // (C block, A block, B block) triples, steps concatenated
+  const double* coefs;             // one per pair
+};
+
+static int plan_create_impl(graft_ctx* ctx, const graft_pair_step* steps, int32_t nsteps,
+                            int32_t nslots, int32_t output_slot,
+                            const graft_tensor_desc* input_descs, int32_t ninputs,
+                            const graft_tensor_desc* out_desc, const CoupledSpec* cp,
+                            graft_plan** out) {
   GRAFT_REQUIRE(ctx != nullptr && out != nullptr, "null argument");
   GRAFT_REQUIRE(steps != nullptr && nsteps >= 1, "plan without steps");
   GRAFT_REQUIRE(input_descs != nullptr && ninputs >= 1, "plan without inputs");
@@ -2100,7 +2265,7 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
   std::vector<int> defined(nslots, 0), consumed(nslots, 0), born(nslots, -1), dies(nslots, -1);
   std::string err;
   for (int i = 0; i < ninputs; ++i) {
-    if (build_struct(input_descs[i], p.slots[i], err)) {
+    if (build_struct(input_descs[i], p.slots[i], err, cp != nullptr)) {
       set_error("input " + std::to_string(i) + ": " + err);
       return GRAFT_E_INVALID;
     }
@@ -2108,7 +2273,7 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
   }
   TStruct outfixed;
   bool have_out = false;
-  if (out_desc) {
+  if (out_desc && !cp) {
     if (build_struct(*out_desc, outfixed, err)) {
       set_error("output descriptor: " + err);
       return GRAFT_E_INVALID;
@@ -2120,6 +2285,7 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
   static const bool timing = std::getenv("GRAFT_PLAN_TIMING") != nullptr;
   auto now = [] { return std::chrono::steady_clock::now(); };
   auto t_start = now();
+  int64_t pair_off = 0;
   for (int s = 0; s < nsteps; ++s) {
     const graft_pair_step& st = steps[s];
     auto t_step = now();
@@ -2145,8 +2311,17 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
       const StepExec& e = p.steps[slot_producer[slot]];
       return e.fused && e.c_complete;
     };
-    if (compile_step(st, p.slots[st.a], p.slots[st.b], fixed, C, p.steps[s], err,
-                     ctx->sector_fuse, handable(st.a), handable(st.b))) {
+    if (cp) {
+      if (cp->npairs[s] < 0) err = "negative pair count";
+      if (cp->npairs[s] < 0 || build_struct(cp->outs[s], C, err, true) ||
+          compile_coupled_step(st, p.slots[st.a], p.slots[st.b], C, cp->pairs + 3 * pair_off,
+                               cp->coefs + pair_off, cp->npairs[s], p.steps[s], err)) {
+        set_error("step " + std::to_string(s) + ": " + err);
+        return GRAFT_E_INVALID;
+      }
+      pair_off += cp->npairs[s];
+    } else if (compile_step(st, p.slots[st.a], p.slots[st.b], fixed, C, p.steps[s], err,
+                            ctx->sector_fuse, handable(st.a), handable(st.b))) {
       set_error("step " + std::to_string(s) + ": " + err);
       return GRAFT_E_INVALID;
     }
@@ -2259,6 +2434,30 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
                  std::chrono::duration<double, std::milli>(now() - t_maps).count(), p.map_bytes / 1e6);
   *out = plan.release();
   return GRAFT_OK;
+}
+
+int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nsteps,
+                      int32_t nslots, int32_t output_slot, const graft_tensor_desc* input_descs,
+                      int32_t ninputs, const graft_tensor_desc* out_desc, graft_plan** out) {
+  return plan_create_impl(ctx, steps, nsteps, nslots, output_slot, input_descs, ninputs, out_desc,
+                          nullptr, out);
+}
+
+int graft_plan_create_coupled(graft_ctx* ctx, const graft_pair_step* steps, int32_t nsteps,
+                              int32_t nslots, int32_t output_slot,
+                              const graft_tensor_desc* input_descs, int32_t ninputs,
+                              const graft_tensor_desc* step_out_descs, const int64_t* npairs,
+                              const int32_t* pairs, const double* coefs, graft_plan** out) {
+  GRAFT_REQUIRE(steps != nullptr && nsteps >= 1, "plan without steps");
+  GRAFT_REQUIRE(step_out_descs != nullptr && npairs != nullptr,
+                "coupled plan without step result descriptors or pair counts");
+  int64_t total = 0;
+  for (int s = 0; s < nsteps; ++s) total += npairs[s] > 0 ? npairs[s] : 0;
+  GRAFT_REQUIRE(total == 0 || (pairs != nullptr && coefs != nullptr),
+                "coupled plan without block pairs or coefficients");
+  const CoupledSpec cp{step_out_descs, npairs, pairs, coefs};
+  return plan_create_impl(ctx, steps, nsteps, nslots, output_slot, input_descs, ninputs, nullptr,
+                          &cp, out);
 }
 
 int graft_plan_destroy(graft_plan* plan) {

@@ -174,6 +174,9 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
       }
       return B[i];
     };
+    // recoupling factor of this block pair (coupled steps); 1 otherwise
+    const double ca = (MAP && maps.coef != nullptr) ? maps.coef[s] : 1.0;
+    const double caI = ca * sA;
     const int K = sg->K;
     const uint32_t* kA = koff + sg->koff_idx;
     const uint32_t* kB = kA + K;
@@ -246,17 +249,21 @@ __global__ void __launch_bounds__(64 * (BM / WM) * (BN / WN))
 #pragma unroll
         for (int e = 0; e < EA; ++e) {
           const int q = tid % BK, p = tid / BK + e * (NT / BK);
-          As[q * ASL + p] = ra[e].x;
-          As[q * ASL + BM + p] = sA * ra[e].y;
-          if constexpr (M3) As[q * ASL + 2 * BM + p] = ra[e].x + sA * ra[e].y;
+          const double re = MAP ? ca * ra[e].x : ra[e].x;
+          const double im = MAP ? caI * ra[e].y : sA * ra[e].y;
+          As[q * ASL + p] = re;
+          As[q * ASL + BM + p] = im;
+          if constexpr (M3) As[q * ASL + 2 * BM + p] = re + im;
         }
       } else {
 #pragma unroll
         for (int e = 0; e < EA; ++e) {
           const int p = tid % BM, q = tid / BM + e * (NT / BM);
-          As[q * ASL + p] = ra[e].x;
-          As[q * ASL + BM + p] = sA * ra[e].y;
-          if constexpr (M3) As[q * ASL + 2 * BM + p] = ra[e].x + sA * ra[e].y;
+          const double re = MAP ? ca * ra[e].x : ra[e].x;
+          const double im = MAP ? caI * ra[e].y : sA * ra[e].y;
+          As[q * ASL + p] = re;
+          As[q * ASL + BM + p] = im;
+          if constexpr (M3) As[q * ASL + 2 * BM + p] = re + im;
         }
       }
       if (bkf) {

@@ -1,0 +1,796 @@
+"""SU(2)-symmetric block tensors: the non-abelian sector blocks of the
+effective-Hamiltonian path.
+
+A TensorMap with SU(2) symmetry (TensorKit's ``Vect[SU2Irrep]`` spaces,
+re-exported by the reference in lib/GraftFoundation/src/Backend/Backend.jl:36)
+stores one dense block over the legs' degeneracies for every tuple of leg
+irreps that can fuse to the trivial irrep and every fusion tree of that
+tuple.  Here a tensor on legs V_0 .. V_{n-1} is
+
+    T = sum_t  T_t (x) X_t,
+
+t running over (irrep tuple, tree) pairs: T_t is the degeneracy block (what
+the device stores), X_t the structural tensor of tree t — the
+Clebsch-Gordan couplings ((j_0 j_1) k_1 j_2) k_2 ... j_{n-1} -> 0, left to
+right, with the map eps (eps U eps^-1 = conj(U)) applied on every dual leg.
+The X_t are real and orthonormal, so the inner product of two tensors is
+the sum of their blocks' inner products and the Krylov algebra runs on the
+flat block data unchanged.
+
+A pairwise contraction is, block pair by block pair,
+
+    C_tc += c(ta, tb, tc) * contract(op(A_ta), op(B_tb)),
+    c(ta, tb, tc) = < X_tc , contract(X_ta, X_tb) >,
+
+because contract(X_ta, X_tb) is an invariant tensor on C's legs and the X_tc
+are an orthonormal basis of those.  TensorKit reaches the same result by
+fusion-tree transformations (F-moves, R-moves) followed by one GEMM per
+coupled sector inside Backend.contract_pair! (Backend.jl:209-222); here the
+coefficients come straight from the Clebsch-Gordan tensors, so no 6j or
+phase convention is involved, and ``SU2Structure.to_dense`` (the embedding
+into the plain legs) checks every result.  The device runs every step of a
+contraction plan as one grouped GEMM launch over its block pairs, each pair
+scaled by its coefficient as the A tile is staged
+(``graft_plan_create_coupled``).  Planning uses the dense leg dimensions,
+as the reference's planner does for non-unique fusion
+(lib/GraftPlanning/src/Planning/planner.jl:566-583).
+"""
+from __future__ import annotations
+
+import ctypes
+import functools
+import itertools
+import math
+from fractions import Fraction
+
+import numpy as np
+
+from . import _native as nat
+
+# ---------------------------------------------------------------------------
+# Spin algebra (arguments are twice the spin: tj = 2j, tm = 2m)
+# ---------------------------------------------------------------------------
+
+
+def fusion(tj1, tj2):
+    """Irreps in j1 (x) j2: |j1 - j2| .. j1 + j2 in integer steps."""
+    return range(abs(tj1 - tj2), tj1 + tj2 + 1, 2)
+
+
+@functools.lru_cache(maxsize=None)
+def clebsch_gordan(tj1, tm1, tj2, tm2, tJ, tM):
+    """<j1 m1; j2 m2 | J M> (Condon-Shortley phases) by Racah's closed form,
+    evaluated in exact rational arithmetic and rounded once."""
+    if tm1 + tm2 != tM or (tj1 + tj2 + tJ) % 2 or not abs(tj1 - tj2) <= tJ <= tj1 + tj2:
+        return 0.0
+    for tj, tm in ((tj1, tm1), (tj2, tm2), (tJ, tM)):
+        if abs(tm) > tj or (tj + tm) % 2:
+            return 0.0
+    f = math.factorial
+    a = (tj1 + tj2 - tJ) // 2
+    b = (tj1 - tj2 + tJ) // 2
+    c = (-tj1 + tj2 + tJ) // 2
+    j1m, j1p = (tj1 - tm1) // 2, (tj1 + tm1) // 2
+    j2m, j2p = (tj2 - tm2) // 2, (tj2 + tm2) // 2
+    Jm, Jp = (tJ - tM) // 2, (tJ + tM) // 2
+    x1 = (tJ - tj2 + tm1) // 2
+    x2 = (tJ - tj1 - tm2) // 2
+    pre = Fraction((tJ + 1) * f(a) * f(b) * f(c), f((tj1 + tj2 + tJ) // 2 + 1))
+    pre *= f(Jp) * f(Jm) * f(j1m) * f(j1p) * f(j2m) * f(j2p)
+    s = Fraction(0)
+    for k in range(max(0, -x1, -x2), min(a, j1m, j2p) + 1):
+        s += Fraction((-1) ** k, f(k) * f(a - k) * f(j1m - k) * f(j2p - k) * f(x1 + k) * f(x2 + k))
+    if s == 0:
+        return 0.0
+    return math.copysign(math.sqrt(pre * s * s), s)
+
+
+@functools.lru_cache(maxsize=None)
+def cg_tensor(tj1, tj2, tJ):
+    """Clebsch-Gordan tensor [m1, m2, M]; index i of a spin-j axis is m = j - i."""
+    out = np.zeros((tj1 + 1, tj2 + 1, tJ + 1))
+    for i1 in range(tj1 + 1):
+        for i2 in range(tj2 + 1):
+            tM = (tj1 - 2 * i1) + (tj2 - 2 * i2)
+            if abs(tM) <= tJ:
+                out[i1, i2, (tJ - tM) // 2] = clebsch_gordan(tj1, tj1 - 2 * i1, tj2, tj2 - 2 * i2,
+                                                            tJ, tM)
+    out.setflags(write=False)
+    return out
+
+
+@functools.lru_cache(maxsize=None)
+def epsilon(tj):
+    """eps[m, m'] = (-1)^(j - m) delta(m, -m'): eps D(g) eps^-1 = conj(D(g))."""
+    e = np.zeros((tj + 1, tj + 1))
+    for i in range(tj + 1):
+        e[i, tj - i] = -1.0 if i % 2 else 1.0
+    e.setflags(write=False)
+    return e
+
+
+def spin_matrices(tj):
+    """(Jx, Jy, Jz) of spin j in the basis m = j, j-1, .., -j."""
+    m = np.array([(tj - 2 * i) / 2 for i in range(tj + 1)])
+    j = tj / 2
+    jp = np.zeros((tj + 1, tj + 1))
+    for i in range(1, tj + 1):
+        jp[i - 1, i] = math.sqrt((j - m[i]) * (j + m[i] + 1))
+    jm = jp.T
+    return (jp + jm) / 2, (jp - jm) / 2j, np.diag(m)
+
+
+@functools.lru_cache(maxsize=None)
+def fusion_trees(tjs):
+    """Internal labels of every left-to-right fusion tree of irreps ``tjs`` to
+    the trivial irrep: (k_1, .., k_{n-2}), k_l the coupled spin of legs 0..l."""
+    n = len(tjs)
+    if n == 0:
+        return ((),)
+    if n == 1:
+        return ((),) if tjs[0] == 0 else ()
+    # reach[l]: total spins legs l.. can fuse to
+    reach = [None] * (n + 1)
+    reach[n] = {0}
+    for l in range(n - 1, 0, -1):
+        reach[l] = {s for r in reach[l + 1] for s in fusion(tjs[l], r)}
+    out = []
+
+    def walk(l, k, labels):
+        if l == n - 1:
+            if k == tjs[n - 1]:
+                out.append(tuple(labels))
+            return
+        for kn in fusion(k, tjs[l]):
+            if kn in reach[l + 1]:
+                walk(l + 1, kn, labels + [kn])
+
+    if tjs[0] in reach[1]:
+        walk(1, tjs[0], [])
+    return tuple(out)
+
+
+@functools.lru_cache(maxsize=1 << 16)
+def tree_tensor(tjs, labels, duals):
+    """Structural tensor X_t (real, unit Frobenius norm) of one fusion tree."""
+    n = len(tjs)
+    if n == 0:
+        return np.ones(())
+    X = np.eye(tjs[0] + 1)
+    k = tjs[0]
+    for l in range(1, n):
+        kn = labels[l - 1] if l < n - 1 else 0
+        X = np.tensordot(X, cg_tensor(k, tjs[l], kn), axes=([X.ndim - 1], [0]))
+        k = kn
+    X = X[..., 0]
+    for l in range(n):
+        if duals[l]:
+            X = np.moveaxis(np.tensordot(epsilon(tjs[l]), X, axes=([1], [l])), 0, l)
+    X = np.ascontiguousarray(X)
+    X.setflags(write=False)
+    return X
+
+
+@functools.lru_cache(maxsize=1 << 14)
+def tree_basis(tjs, duals):
+    """(trees, matrix [ntrees, prod(2j+1)]) of all trees of one irrep tuple."""
+    trees = fusion_trees(tjs)
+    size = int(np.prod([t + 1 for t in tjs])) if tjs else 1
+    M = np.zeros((len(trees), size))
+    for i, tr in enumerate(trees):
+        M[i] = tree_tensor(tjs, tr, duals).reshape(-1)
+    M.setflags(write=False)
+    return trees, M
+
+
+# ---------------------------------------------------------------------------
+# Legs and block structures
+# ---------------------------------------------------------------------------
+
+
+class SU2Leg:
+    """One flat leg: irreps (twice the spin) with degeneracies; ``isdual``
+    marks a dual (domain / bra-side) leg, which transforms with conj(U)."""
+
+    __slots__ = ("irreps", "isdual", "_hash")
+
+    def __init__(self, irreps, isdual=False):
+        irr = tuple(sorted((int(tj), int(d)) for tj, d in irreps))
+        if not irr or any(d <= 0 or tj < 0 for tj, d in irr) or len({tj for tj, _ in irr}) != len(irr):
+            raise ValueError(f"SU2Leg: irreps must be distinct spins with positive degeneracy: {irreps}")
+        self.irreps = irr
+        self.isdual = bool(isdual)
+        self._hash = hash((self.irreps, self.isdual))
+
+    @property
+    def dim(self):
+        return sum(d * (tj + 1) for tj, d in self.irreps)
+
+    def dual(self):
+        return SU2Leg(self.irreps, not self.isdual)
+
+    def degs(self):
+        return [d for _, d in self.irreps]
+
+    def spins(self):
+        return [tj for tj, _ in self.irreps]
+
+    def offsets(self):
+        """Start of every irrep's (degeneracy x multiplet) range on the dense leg."""
+        out, o = [], 0
+        for tj, d in self.irreps:
+            out.append(o)
+            o += d * (tj + 1)
+        return out
+
+    def __eq__(self, other):
+        return isinstance(other, SU2Leg) and self.irreps == other.irreps and self.isdual == other.isdual
+
+    def __hash__(self):
+        return self._hash
+
+    def __repr__(self):
+        s = ", ".join(f"{tj}/2:{d}" if tj % 2 else f"{tj // 2}:{d}" for tj, d in self.irreps)
+        return f"SU2Leg({s}{', dual' if self.isdual else ''})"
+
+
+class SU2Structure:
+    """Block layout of an SU(2) tensor: one degeneracy block per (irrep index
+    tuple, fusion tree), packed in list order, each first leg fastest.
+
+    ``dims`` are the dense (embedded) leg dimensions; the device descriptor
+    carries the degeneracy extents, with one entry per block (blocks of one
+    irrep tuple and different trees repeat its sector tuple)."""
+
+    def __init__(self, legs, blocks):
+        self.legs = list(legs)
+        self.blocks = [(tuple(int(x) for x in ir), tuple(int(x) for x in tr)) for ir, tr in blocks]
+        self.index = {b: i for i, b in enumerate(self.blocks)}
+        if len(self.index) != len(self.blocks):
+            raise ValueError("SU2Structure: duplicate block")
+        degs = [l.degs() for l in self.legs]
+        self.block_dims = [tuple(degs[l][s] for l, s in enumerate(ir)) for ir, _ in self.blocks]
+        self.sizes = [int(np.prod(d)) if d else 1 for d in self.block_dims]
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)[:-1]]).astype(np.int64) \
+            if self.sizes else np.zeros(0, np.int64)
+        self.total = int(sum(self.sizes))
+
+    @property
+    def dims(self):
+        return tuple(l.dim for l in self.legs)
+
+    @property
+    def duals(self):
+        return tuple(l.isdual for l in self.legs)
+
+    def spins(self, b):
+        ir = self.blocks[b][0]
+        return tuple(self.legs[l].irreps[s][0] for l, s in enumerate(ir))
+
+    @staticmethod
+    def invariant(legs):
+        """Every (irrep tuple, tree) with a fusion channel to the trivial irrep."""
+        legs = list(legs)
+        blocks = []
+        for ir in itertools.product(*[range(len(l.irreps)) for l in legs]):
+            tjs = tuple(legs[l].irreps[s][0] for l, s in enumerate(ir))
+            for tr in fusion_trees(tjs):
+                blocks.append((ir, tr))
+        return SU2Structure(legs, sorted(blocks))
+
+    def key(self):
+        k = getattr(self, "_key", None)
+        if k is None:
+            k = self._key = ("su2", tuple(self.legs), tuple(self.blocks))
+        return k
+
+    def desc(self):
+        d = getattr(self, "_desc", None)
+        if d is None and not self.legs:
+            d = self._desc = nat.DescKeepalive([])       # a scalar: one block of one element
+        if d is None:
+            d = self._desc = nat.DescKeepalive(
+                [sum(l.degs()) for l in self.legs],
+                ([l.degs() for l in self.legs],
+                 np.asarray([ir for ir, _ in self.blocks], dtype=np.int32).reshape(len(self.blocks), -1),
+                 self.offsets, None))
+        return d
+
+    # -- dense embedding ------------------------------------------------------
+    def _slices(self, ir):
+        return tuple(slice(self.legs[l].offsets()[s],
+                           self.legs[l].offsets()[s] + self.legs[l].irreps[s][1] * (self.legs[l].irreps[s][0] + 1))
+                     for l, s in enumerate(ir))
+
+    def to_dense(self, flat):
+        """Dense tensor on the embedded legs (index d * (2j + 1) + i inside an irrep)."""
+        flat = np.asarray(flat)
+        out = np.zeros(self.dims, dtype=np.complex128)
+        n = len(self.legs)
+        for b, (ir, tr) in enumerate(self.blocks):
+            tjs = self.spins(b)
+            o = int(self.offsets[b])
+            T = flat[o:o + self.sizes[b]].reshape(self.block_dims[b], order="F")
+            X = tree_tensor(tjs, tr, self.duals)
+            full = np.einsum(T, list(range(0, 2 * n, 2)), X, list(range(1, 2 * n, 2)),
+                             list(range(2 * n)))
+            out[self._slices(ir)] += full.reshape([d * (tj + 1) for d, tj in zip(self.block_dims[b], tjs)])
+        return out
+
+    def from_dense(self, arr):
+        """Degeneracy blocks of the projection of a dense tensor onto this
+        structure's invariant subspace (exact for an invariant tensor)."""
+        arr = np.asarray(arr, dtype=np.complex128)
+        if arr.shape != self.dims:
+            raise ValueError(f"from_dense: expected shape {self.dims}, got {arr.shape}")
+        n = len(self.legs)
+        flat = np.zeros(self.total, dtype=np.complex128)
+        for b, (ir, tr) in enumerate(self.blocks):
+            tjs = self.spins(b)
+            shape = []
+            for d, tj in zip(self.block_dims[b], tjs):
+                shape += [d, tj + 1]
+            sub = np.ascontiguousarray(arr[self._slices(ir)]).reshape(shape)
+            X = tree_tensor(tjs, tr, self.duals)
+            T = np.einsum(sub, list(range(2 * n)), X, list(range(1, 2 * n, 2)),
+                          list(range(0, 2 * n, 2)))
+            o = int(self.offsets[b])
+            flat[o:o + self.sizes[b]] = np.asarray(T).reshape(-1, order="F")
+        return flat
+
+    def dense_from_flat(self, host):
+        return self.to_dense(host)
+
+    def __repr__(self):
+        return f"SU2Structure({len(self.legs)} legs, {len(self.blocks)} blocks, {self.total} elements)"
+
+
+def is_su2(structure):
+    return isinstance(structure, SU2Structure)
+
+
+def random_flat(rng, structure):
+    """Random complex degeneracy blocks (unit-variance entries)."""
+    n = structure.total
+    return (rng.standard_normal(n) + 1j * rng.standard_normal(n)) / math.sqrt(2.0)
+
+
+def to_device(structure, flat, ncod=None):
+    """A DTensor holding SU(2) degeneracy blocks (dims: the dense leg dims)."""
+    import torch
+
+    from .tensors import DTensor, _device
+    data = torch.from_numpy(np.ascontiguousarray(np.asarray(flat, dtype=np.complex128))).to(_device())
+    return DTensor(structure.dims, data, ncod, structure)
+
+
+# ---------------------------------------------------------------------------
+# Coupled pairwise steps
+# ---------------------------------------------------------------------------
+
+_COEF_CACHE = {}
+_COEF_CACHE_MAX = 1 << 18
+COEF_TOL = 1e-13
+
+
+def _pair_coefficients(tjsA, dualsA, tjsB, dualsB, oA, cA, oB, cB, perm, dualsC):
+    """c[ta, tb, tc] for every tree of A's irreps (ta), of B's (tb) and of the
+    output's (tc): <X_tc, contract(X_ta, X_tb)>.  Cached on the spins,
+    directions and step geometry."""
+    key = (tjsA, dualsA, tjsB, dualsB, oA, cA, oB, cB, perm, dualsC)
+    hit = _COEF_CACHE.get(key)
+    if hit is not None:
+        return hit
+    treesA, MA = tree_basis(tjsA, dualsA)
+    treesB, MB = tree_basis(tjsB, dualsB)
+    nA, nB = len(treesA), len(treesB)
+    inter = tuple(tjsA[i] for i in oA) + tuple(tjsB[i] for i in oB)
+    tjsC = tuple(inter[p] for p in perm)
+    treesC, MC = tree_basis(tjsC, dualsC)
+    if not (nA and nB and treesC):
+        hit = _COEF_CACHE[key] = (treesA, treesB, tjsC, treesC, np.zeros((nA, nB, len(treesC))))
+        return hit
+    shapeA = tuple(t + 1 for t in tjsA)
+    shapeB = tuple(t + 1 for t in tjsB)
+    XA = MA.reshape((nA,) + shapeA).transpose((0,) + tuple(1 + i for i in oA + cA))
+    XB = MB.reshape((nB,) + shapeB).transpose((0,) + tuple(1 + i for i in cB + oB))
+    kdim = int(np.prod([tjsA[i] + 1 for i in cA])) if cA else 1
+    ma = int(np.prod([tjsA[i] + 1 for i in oA])) if oA else 1
+    nb = int(np.prod([tjsB[i] + 1 for i in oB])) if oB else 1
+    Y = np.einsum("aik,bkj->abij", XA.reshape(nA, ma, kdim), XB.reshape(nB, kdim, nb))
+    Y = Y.reshape((nA, nB) + tuple(t + 1 for t in inter)).transpose((0, 1) + tuple(2 + p for p in perm))
+    coef = Y.reshape(nA, nB, -1) @ MC.T            # [nA, nB, nC]
+    coef[np.abs(coef) < COEF_TOL] = 0.0
+    if len(_COEF_CACHE) >= _COEF_CACHE_MAX:
+        _COEF_CACHE.clear()
+    hit = _COEF_CACHE[key] = (treesA, treesB, tjsC, treesC, coef)
+    return hit
+
+
+def compile_pair(SA, legsA, SB, legsB, oA, cA, oB, cB, perm, fixed_out=None):
+    """One pairwise step: the result's structure and its block pairs.
+
+    ``legsA`` / ``legsB`` are the operands' legs as the step sees them (dual
+    flipped where the operand is conjugated); ``SA`` / ``SB`` their stored
+    structures (the structural tensors are real, so conjugation changes the
+    legs' directions and not the coefficients).  Returns (C structure,
+    int32 array of (C block, A block, B block) triples grouped by C block,
+    float64 coefficients)."""
+    oA, cA, oB, cB, perm = tuple(oA), tuple(cA), tuple(oB), tuple(cB), tuple(perm)
+    for i, j in zip(cA, cB):
+        if legsA[i] != legsB[j].dual():
+            raise ValueError(f"coupled step: contracted legs {legsA[i]} and {legsB[j]} are not dual")
+    inter = [legsA[i] for i in oA] + [legsB[i] for i in oB]
+    out_legs = [inter[p] for p in perm]
+    dualsC = tuple(l.isdual for l in out_legs)
+    if fixed_out is not None and list(fixed_out.legs) != out_legs:
+        raise ValueError("coupled step: the given output structure has other legs")
+    groupsB = {}
+    for bi, (ir, _) in enumerate(SB.blocks):
+        groupsB.setdefault(tuple(ir[j] for j in cB), {}).setdefault(ir, []).append(bi)
+    groupsA = {}
+    for ai, (ir, _) in enumerate(SA.blocks):
+        groupsA.setdefault(ir, []).append(ai)
+    dA, dB = SA.duals, SB.duals
+    entries = []         # (C key, A block, B block, coefficient)
+    for irA, alist in groupsA.items():
+        ck = tuple(irA[i] for i in cA)
+        gB = groupsB.get(ck)
+        if not gB:
+            continue
+        tjsA = tuple(SA.legs[l].irreps[s][0] for l, s in enumerate(irA))
+        posA = {SA.blocks[ai][1]: ai for ai in alist}
+        for irB, blist in gB.items():
+            tjsB = tuple(SB.legs[l].irreps[s][0] for l, s in enumerate(irB))
+            treesA, treesB, tjsC, treesC, coef = _pair_coefficients(
+                tjsA, dA, tjsB, dB, oA, cA, oB, cB, perm, dualsC)
+            if not treesC:
+                continue
+            interir = tuple(irA[i] for i in oA) + tuple(irB[i] for i in oB)
+            irC = tuple(interir[p] for p in perm)
+            posB = {SB.blocks[bi][1]: bi for bi in blist}
+            nz = np.nonzero(coef)
+            for x, y, z in zip(*nz):
+                ai = posA.get(treesA[x])
+                bi = posB.get(treesB[y])
+                if ai is None or bi is None:
+                    continue
+                entries.append(((irC, treesC[z]), ai, bi, float(coef[x, y, z])))
+    if fixed_out is not None:
+        C = fixed_out
+    else:
+        keys = sorted({e[0] for e in entries})
+        if not keys:
+            raise ValueError("coupled step: no block pair contributes")
+        C = SU2Structure(out_legs, keys)
+    cidx = C.index
+    trip = np.empty((len(entries), 3), dtype=np.int32)
+    cf = np.empty(len(entries), dtype=np.float64)
+    for i, (k, ai, bi, c) in enumerate(entries):
+        ci = cidx.get(k)
+        if ci is None:
+            raise ValueError(f"coupled step: block {k} lies outside the given output structure")
+        trip[i] = (ci, ai, bi)
+        cf[i] = c
+    order = np.argsort(trip[:, 0], kind="stable") if len(entries) else np.zeros(0, np.int64)
+    return C, np.ascontiguousarray(trip[order]), np.ascontiguousarray(cf[order])
+
+
+def host_execute_pair(C, trip, coef, SA, flatA, conjA, SB, flatB, conjB, oA, cA, oB, cB, perm):
+    """The numpy form of one coupled step (tests and small host checks)."""
+    out = np.zeros(C.total, dtype=np.complex128)
+    nOA = len(oA)
+    for (ci, ai, bi), c in zip(trip, coef):
+        a = flatA[SA.offsets[ai]:SA.offsets[ai] + SA.sizes[ai]].reshape(SA.block_dims[ai], order="F")
+        b = flatB[SB.offsets[bi]:SB.offsets[bi] + SB.sizes[bi]].reshape(SB.block_dims[bi], order="F")
+        a = np.conj(a) if conjA else a
+        b = np.conj(b) if conjB else b
+        r = np.tensordot(a.transpose(tuple(oA) + tuple(cA)), b.transpose(tuple(cB) + tuple(oB)),
+                         axes=(list(range(nOA, nOA + len(cA))), list(range(len(cB)))))
+        r = np.asarray(r).transpose(tuple(perm))
+        o = int(C.offsets[ci])
+        out[o:o + C.sizes[ci]] += c * r.reshape(-1, order="F")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Device plans
+# ---------------------------------------------------------------------------
+
+
+class CoupledPlan:
+    """A ContractionPlan over SU(2) tensors compiled into libgraft_heff.so
+    (graft_plan_create_coupled): every step one grouped GEMM launch over its
+    block pairs.  Quacks like contractions.NativePlan for EnvCache and
+    EffectiveMap."""
+
+    def __init__(self, plan, operands, out_structure=None, out_dims=None):
+        self.plan = plan
+        self.ninputs = plan.ninputs
+        structs = [op.blocks for op in operands]
+        if not all(is_su2(s) for s in structs):
+            raise ValueError("CoupledPlan: every operand must be an SU(2) block tensor")
+        slot_struct = list(structs) + [None] * len(plan.steps)
+        self.step_structs, self.step_pairs, self.step_coefs = [], [], []
+        for s in plan.steps:
+            SA, SB = slot_struct[s.a], slot_struct[s.b]
+            legsA = [l.dual() for l in SA.legs] if s.conjA else list(SA.legs)
+            legsB = [l.dual() for l in SB.legs] if s.conjB else list(SB.legs)
+            perm = tuple(s.out[0]) + tuple(s.out[1])
+            fixed = out_structure if s.dst == plan.output_slot else None
+            C, trip, cf = compile_pair(SA, legsA, SB, legsB, s.oindA, s.cindA, s.oindB, s.cindB,
+                                       perm, fixed)
+            slot_struct[s.dst] = C
+            self.step_structs.append(C)
+            self.step_pairs.append(trip)
+            self.step_coefs.append(cf)
+        self.out_structure = slot_struct[plan.output_slot]
+        self.out_dims = out_dims if out_dims is not None else self.out_structure.dims
+        ctx = nat.default_context()
+        descs = [st.desc() for st in structs]
+        arr = (nat.TensorDescC * len(descs))(*[d.c for d in descs])
+        odescs = [C.desc() for C in self.step_structs]
+        oarr = (nat.TensorDescC * len(odescs))(*[d.c for d in odescs])
+        steps = (nat.PairStepC * len(plan.steps))(*[nat.make_step(s) for s in plan.steps])
+        npairs = np.asarray([len(c) for c in self.step_coefs], dtype=np.int64)
+        trip = np.ascontiguousarray(np.concatenate(self.step_pairs).reshape(-1).astype(np.int32)) \
+            if npairs.sum() else np.zeros(3, np.int32)
+        coefs = np.ascontiguousarray(np.concatenate(self.step_coefs)) if npairs.sum() \
+            else np.zeros(1)
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().graft_plan_create_coupled(
+            ctx.handle, steps, len(plan.steps), plan.nslots, plan.output_slot, arr, len(descs),
+            oarr, npairs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            trip.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            coefs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(h)))
+        self.handle = h
+        ws, macs, outn, db = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
+        nat.check(nat.lib().graft_plan_info(h, ctypes.byref(ws), ctypes.byref(macs), ctypes.byref(outn)))
+        nat.check(nat.lib().graft_plan_device_bytes(h, ctypes.byref(db)))
+        self.workspace_bytes = ws.value
+        self.macs = macs.value
+        self.output_elements = outn.value
+        self.device_bytes = db.value
+        self.npairs = int(npairs.sum())
+
+    # the rest of NativePlan's interface
+    from .contractions import NativePlan as _NP
+    step_info = _NP.step_info
+    step_gemms = _NP.step_gemms
+    execute = _NP.execute
+    execute_timed = _NP.execute_timed
+    set_tile_config = _NP.set_tile_config
+    set_orientation = _NP.set_orientation
+    close = _NP.close
+    __del__ = _NP.__del__
+    del _NP
+
+    def autotune(self, operands, out, precision=nat.PREC_FP64, repeats=2):
+        from .contractions import NativePlan
+        NativePlan.autotune(self, operands, out, precision, repeats)
+
+
+def root_cap(legs):
+    """Cap closing the root's unit legs (every leg the trivial irrep, once)."""
+    st = SU2Structure.invariant(legs)
+    if st.total != 1:
+        raise ValueError("SU(2) root cap: the root legs must each hold the trivial irrep once")
+    return to_device(st, np.ones(1, dtype=np.complex128), len(legs))
+
+
+# ---------------------------------------------------------------------------
+# Single contractions, gauge and model builders
+# ---------------------------------------------------------------------------
+
+_PAIR_PLANS = {}
+_PAIR_PLANS_MAX = 64
+
+
+class _Step:
+    """The fields of one planner PairStep (nat.make_step reads these)."""
+
+    def __init__(self, a, b, dst, oindA, cindA, oindB, cindB, conjA, conjB, out):
+        self.a, self.b, self.dst = a, b, dst
+        self.oindA, self.cindA, self.oindB, self.cindB = oindA, cindA, oindB, cindB
+        self.conjA, self.conjB, self.out = conjA, conjB, out
+
+
+class _OnePlan:
+    def __init__(self, step):
+        self.steps = [step]
+        self.nslots = 3
+        self.output_slot = 2
+        self.ninputs = 2
+
+
+def contract(A, cA, B, cB, conjA=False, conjB=False, perm=None, ncod=None, out_structure=None):
+    """contract_pair (Backend.jl:166-170) on SU(2) device tensors: A's legs
+    ``cA`` with B's legs ``cB``; the result's legs are A's open legs then B's
+    (in order), permuted by ``perm``.  Plans are cached per structure."""
+    cA, cB = tuple(cA), tuple(cB)
+    oA = tuple(i for i in range(A.rank) if i not in cA)
+    oB = tuple(i for i in range(B.rank) if i not in cB)
+    nout = len(oA) + len(oB)
+    perm = tuple(range(nout)) if perm is None else tuple(perm)
+    ncod = nout if ncod is None else int(ncod)
+    key = (A.blocks.key(), B.blocks.key(), cA, cB, bool(conjA), bool(conjB), perm, ncod,
+           None if out_structure is None else out_structure.key())
+    p = _PAIR_PLANS.get(key)
+    if p is None:
+        step = _Step(0, 1, 2, oA, cA, oB, cB, bool(conjA), bool(conjB), (perm[:ncod], perm[ncod:]))
+        p = CoupledPlan(_OnePlan(step), [A, B], out_structure)
+        if len(_PAIR_PLANS) >= _PAIR_PLANS_MAX:
+            _PAIR_PLANS.pop(next(iter(_PAIR_PLANS)))
+        _PAIR_PLANS[key] = p
+    from .tensors import DTensor
+    out = DTensor.empty(p.out_structure.dims, ncod, p.out_structure)
+    p.execute([A, B], out)
+    return out
+
+
+def fuse_multiplicities(m1, m2, maxspin=None):
+    """{spin: multiplicity} of the product of two {spin: multiplicity} spaces."""
+    out = {}
+    for a, x in m1.items():
+        for b, y in m2.items():
+            for k in fusion(a, b):
+                if maxspin is None or k <= maxspin:
+                    out[k] = out.get(k, 0) + x * y
+    return out
+
+
+def bond_legs(topo, tphys=1, maxdeg=None, maxspin=None):
+    """Bond leg (the child's space, not dual) of every edge child -> parent:
+    the irreps of the child's subtree (children bonds (x) physical spin),
+    each degeneracy capped at ``maxdeg`` and spins above ``maxspin`` dropped."""
+    from .trees import nnodes, postorder
+    legs = [None] * nnodes(topo)
+    for n in postorder(topo):
+        m = {tphys: 1}
+        for c in topo.children[n]:
+            m = fuse_multiplicities(m, dict(legs[c].irreps), maxspin)
+        if maxdeg is not None:
+            m = {k: min(v, maxdeg) for k, v in m.items()}
+        legs[n] = SU2Leg(sorted(m.items()))
+    return legs
+
+
+def site_legs(topo, n, bonds, tphys=1):
+    """Legs of TTNS tensor n: children bonds, physical spin, parent bond (dual)."""
+    legs = [bonds[c] for c in topo.children[n]] + [SU2Leg([(tphys, 1)])]
+    parent = bonds[n] if topo.parent[n] >= 0 else SU2Leg([(0, 1)])
+    return legs + [parent.dual()]
+
+
+def left_orthonormalize(structure, flat):
+    """Make a site tensor an isometry from its parent leg (the last leg, dual)
+    onto the others: per parent irrep J, the degeneracy blocks stacked over
+    (irrep tuple, tree, degeneracies) get orthonormal columns scaled by
+    sqrt(2J + 1), the norm the structural tensors give each column (their
+    partial trace over every other leg is 1 / (2J + 1) on the parent's
+    multiplet).  The thin QR of the stack (TensorKit's per-sector
+    leftorth, ttns.jl:181-203) keeps the tensor's span."""
+    flat = np.array(flat, dtype=np.complex128)
+    last = len(structure.legs) - 1
+    byJ = {}
+    for b, (ir, _) in enumerate(structure.blocks):
+        byJ.setdefault(ir[last], []).append(b)
+    for s, blist in byJ.items():
+        tJ, dJ = structure.legs[last].irreps[s]
+        rows = [structure.sizes[b] // dJ for b in blist]
+        M = np.concatenate([flat[structure.offsets[b]:structure.offsets[b] + structure.sizes[b]]
+                            .reshape(rows[i], dJ, order="F") for i, b in enumerate(blist)])
+        if M.shape[0] < dJ:
+            raise ValueError("left_orthonormalize: a parent irrep has more degeneracy than its span")
+        Q, R = np.linalg.qr(M)
+        Q = Q * np.sign(np.real(np.diag(R)) + (np.real(np.diag(R)) == 0))[None, :]
+        Q *= math.sqrt(tJ + 1)
+        o = 0
+        for i, b in enumerate(blist):
+            flat[structure.offsets[b]:structure.offsets[b] + structure.sizes[b]] = \
+                Q[o:o + rows[i]].reshape(-1, order="F")
+            o += rows[i]
+    return flat
+
+
+def random_ttns(rng, topo, tphys=1, maxdeg=None, maxspin=None, center=None):
+    """Random SU(2) tree state: every site spin tphys/2, bonds from
+    ``bond_legs``; every tensor but the centre's (the root by default) an
+    isometry towards its parent, the centre normalized."""
+    from .networks import TTNS
+    from .trees import nnodes
+    bonds = bond_legs(topo, tphys, maxdeg, maxspin)
+    root = next(i for i in range(nnodes(topo)) if topo.parent[i] < 0)
+    center = root if center is None else center
+    if center != root:
+        raise ValueError("random_ttns: the centre must be the root")
+    tensors = []
+    for n in range(nnodes(topo)):
+        legs = site_legs(topo, n, bonds, tphys)
+        st = SU2Structure.invariant(legs)
+        if st.total == 0:
+            raise ValueError(f"random_ttns: node {topo.ids[n]} admits no invariant block")
+        flat = random_flat(rng, st)
+        flat = flat / np.linalg.norm(flat) if n == center else left_orthonormalize(st, flat)
+        tensors.append(to_device(st, flat, len(legs) - 1))
+    return TTNS(topo, tensors, center, [True] * nnodes(topo))
+
+
+CHANNEL = SU2Leg([(0, 2), (2, 1)])     # idle, done (scalars), one open S vector (spin 1)
+
+
+def heisenberg_dense_site(K, isroot, J=1.0, tphys=1):
+    """Dense TTNO tensor of J sum_edges S_i . S_j (site spin tphys / 2) at a
+    node with K children, channels in CHANNEL's embedded basis: 0 idle,
+    1 done, 2..4 the spherical components q = +1, 0, -1 of an open term; legs
+    (children channels..., P_out, P_in, parent channel).  The root's parent
+    channel is 'done' only.  The node emits T_q (its spin's spherical
+    components: T_+1 = -S+/sqrt2, T_0 = Sz, T_-1 = S-/sqrt2) towards the
+    parent and closes a child's T_q with T_q^dagger:
+    S_i . S_j = sum_q T_q(i) T_q(j)^dagger."""
+    jx, jy, jz = spin_matrices(tphys)
+    sp, sm = jx + 1j * jy, jx - 1j * jy
+    T = {1: -sp / math.sqrt(2.0), 0: jz.astype(np.complex128), -1: sm / math.sqrt(2.0)}
+    d = tphys + 1
+    I = np.eye(d, dtype=np.complex128)
+    pdim = 1 if isroot else 5
+    W = np.zeros((5,) * K + (d, d, pdim), dtype=np.complex128)
+    idle = (0,) * K
+    done = 0 if isroot else 1
+    if not isroot:
+        W[idle + (slice(None), slice(None), 0)] = I
+        for i, q in enumerate((1, 0, -1)):
+            W[idle + (slice(None), slice(None), 2 + i)] = T[q]
+    for k in range(K):
+        ch = list(idle)
+        ch[k] = 1
+        W[tuple(ch) + (slice(None), slice(None), done)] += I
+        for i, q in enumerate((1, 0, -1)):
+            ch[k] = 2 + i
+            W[tuple(ch) + (slice(None), slice(None), done)] += J * T[q].conj().T
+    return W
+
+
+def heisenberg_ttno(topo, J=1.0, tphys=1):
+    """SU(2) TTNO of the spin-(tphys/2) Heisenberg model J sum_<ij> S_i . S_j on the
+    tree's edges (the dense tensors of ``heisenberg_dense_site`` projected onto
+    their invariant blocks; the projection is exact, which ``from_dense``
+    followed by ``to_dense`` confirms)."""
+    from .networks import TTNO
+    from .trees import nnodes
+    tensors = []
+    for n in range(nnodes(topo)):
+        K = len(topo.children[n])
+        isroot = topo.parent[n] < 0
+        legs = [CHANNEL] * K + [SU2Leg([(tphys, 1)]), SU2Leg([(tphys, 1)], True),
+                                (SU2Leg([(0, 1)]) if isroot else CHANNEL).dual()]
+        st = SU2Structure.invariant(legs)
+        Wd = heisenberg_dense_site(K, isroot, J, tphys)
+        flat = st.from_dense(Wd)
+        if np.abs(st.to_dense(flat) - Wd).max() > 1e-12:
+            raise RuntimeError("Heisenberg TTNO tensor is not SU(2) invariant")
+        tensors.append(to_device(st, flat, K + 1))
+    return TTNO(topo, tensors, ishermitian=True)
+
+
+def random_ttno(rng, topo, tphys=1, channel=CHANNEL):
+    """Random invariant TTNO tensors on the given channel leg (tests)."""
+    from .networks import TTNO
+    from .trees import nnodes
+    tensors = []
+    for n in range(nnodes(topo)):
+        K = len(topo.children[n])
+        isroot = topo.parent[n] < 0
+        legs = [channel] * K + [SU2Leg([(tphys, 1)]), SU2Leg([(tphys, 1)], True),
+                                (SU2Leg([(0, 1)]) if isroot else channel).dual()]
+        st = SU2Structure.invariant(legs)
+        tensors.append(to_device(st, random_flat(rng, st), K + 1))
+    return TTNO(topo, tensors)
+
+
+def two_site_tensor(psi, n, m):
+    """Theta = A_n contracted with A_m over their edge (effective.jl:827-843)."""
+    from .trees import childslot
+    A, Bm = psi.tensors[n], psi.tensors[m]
+    k = childslot(psi.topo, m, n)
+    return contract(A, (A.rank - 1,), Bm, (k,), ncod=A.rank - 1 + Bm.rank - 1)

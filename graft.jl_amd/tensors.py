@@ -196,6 +196,8 @@ class DTensor:
         host = self.data.detach().cpu().numpy()
         if self.blocks is None:
             return host.reshape(self.dims, order="F")
+        if hasattr(self.blocks, "dense_from_flat"):     # non-abelian structures embed themselves
+            return self.blocks.dense_from_flat(host)
         out = np.zeros(self.dims, dtype=np.complex128)
         starts = [np.concatenate([[0], np.cumsum(l.degs())[:-1]]) for l in self.blocks.legs]
         for i, b in enumerate(self.blocks.blocks):

@@ -18,6 +18,12 @@
  * Reference interfaces replaced (paths relative to the Graft.jl repo):
  *   graft_plan_create   <- Planning.PairStep / ContractionPlan
  *                          lib/GraftPlanning/src/Planning/types.jl:54-65, 94-115
+ *   graft_plan_create_coupled <- the same plans over SU(2) TensorMaps, whose
+ *                          pairwise steps TensorKit runs as fusion-tree
+ *                          recoupling + one GEMM per coupled sector inside
+ *                          Backend.contract_pair! (Backend.jl:209-222), with
+ *                          the planner's dense selection for non-unique
+ *                          fusion (lib/GraftPlanning/src/Planning/planner.jl:566-583)
  *   graft_plan_execute  <- Planning.execute / _execute_workspace
  *                          lib/GraftPlanning/src/Planning/executor.jl:16-45, 99-143
  *   graft_heff_apply    <- (::EffectiveMap)(x) / (::WorkspaceMap)(x)
@@ -163,6 +169,24 @@ int graft_plan_create(graft_ctx* ctx, const graft_pair_step* steps, int32_t nste
                       int32_t nslots, int32_t output_slot,
                       const graft_tensor_desc* input_descs, int32_t ninputs,
                       const graft_tensor_desc* out_desc, graft_plan** out);
+/*
+ * A plan over non-abelian (SU(2)) block tensors.  Every tensor is a list of
+ * degeneracy blocks (graft_tensor_desc; one block per sector tuple AND fusion
+ * tree, so blocks may repeat a sector tuple and are addressed by position).
+ * step_out_descs[s] lists the blocks of step s's result.  Step s multiplies
+ * npairs[s] block pairs: pairs holds (C block, A block, B block) index
+ * triples, the steps' lists concatenated, and coefs one real factor per pair,
+ *   C_blk[c] = sum over its pairs of coef * contract(op(A_blk[a]), op(B_blk[b])),
+ * the pairs of one output block summed in the order given; an output block
+ * without pairs is zero.  The caller computes the factors (recoupling
+ * coefficients of the blocks' fusion trees).  Steps run on the FP64 tile
+ * kernels; other precisions return GRAFT_E_UNSUPPORTED at execution.
+ */
+int graft_plan_create_coupled(graft_ctx* ctx, const graft_pair_step* steps, int32_t nsteps,
+                              int32_t nslots, int32_t output_slot,
+                              const graft_tensor_desc* input_descs, int32_t ninputs,
+                              const graft_tensor_desc* step_out_descs, const int64_t* npairs,
+                              const int32_t* pairs, const double* coefs, graft_plan** out);
 int graft_plan_destroy(graft_plan* plan);
 /* Workspace bytes the plan needs, algorithmic complex MACs (sum m*k*n over
  * all GEMM blocks), and the number of output elements. */
