@@ -153,10 +153,11 @@ constexpr int kBlockStreamWaves = 65536; // work entries above which one entry w
 struct StreamWork {
   int32_t begin, count, j0, pad;
 };
+// coef (coupled steps, else null): a real factor per segment.
 hipError_t launch_zstream_blocks(bool big_is_a, const void* A, const void* B, void* C,
                                  const GemmItem* items, const GemmSeg* segs, const uint32_t* koff,
                                  const int32_t* order, const StreamWork* work, int nwork, int conjA,
-                                 int conjB, hipStream_t stream);
+                                 int conjB, hipStream_t stream, const double* coef = nullptr);
 
 // Lean form of the per-block streaming step, used when the small operand
 // fits in LDS, every output block has at most kLeanMaxS small-side outputs
@@ -182,7 +183,8 @@ struct alignas(16) LeanItem {
 };
 struct alignas(16) LeanSeg {
   int64_t big_off;              // big block start
-  int32_t K, pad;
+  double coef;                  // factor of this block pair (1 but in coupled steps)
+  int32_t K, pad[3];
   uint32_t bstr[kLeanRank];     // big block strides of the big-side legs
   uint32_t sso[kLeanMaxS];      // small offset (block start included) of small-side index s
   uint32_t kb[kLeanInlineK];    // big and small offsets of every contracted index k

@@ -983,6 +983,7 @@ static void build_lean(StepExec& ex) {
         ls.sso[s0] = (uint32_t)o;
       }
       ls.K = sg.K;
+      ls.coef = ex.coef.empty() ? 1.0 : ex.coef[g];
       const uint32_t* kA = ex.koff.data() + sg.koff_idx;
       const uint32_t* kbig = bigA ? kA : kA + sg.K;
       const uint32_t* ksml = bigA ? kA + sg.K : kA;
@@ -1160,6 +1161,53 @@ static int build_items(const graft_pair_step& st, const TStruct& A, const TStruc
     it.seg_count = (int)ex.segs.size() - it.seg_begin;
     ex.items.push_back(it);
   }
+  return 0;
+}
+
+// Output blocks grouped by the large operand's open sector tuple (the
+// blocks of one group read the same rows of the large operand), and the lean
+// records of the per-block streaming kernel where they apply.
+static int build_stream_groups(StepExec& ex, const TStruct& C, int bigside, int nOA, int nOB,
+                               const std::vector<int>& pos_of_out, int64_t small_total,
+                               std::string& err) {
+  std::unordered_map<SecKey, int, SecHash> pid;
+  std::vector<std::vector<int32_t>> groups;
+  SecKey key;
+  for (int cb = 0; cb < (int)C.blocks.size(); ++cb) {
+    const Block& b = C.blocks[cb];
+    if (bigside == 1) {
+      key.assign(nOA, 0);
+      for (int p = 0; p < nOA; ++p) key[p] = b.sec[pos_of_out[p]];
+    } else {
+      key.assign(nOB, 0);
+      for (int p = 0; p < nOB; ++p) key[p] = b.sec[pos_of_out[nOA + p]];
+    }
+    auto f = pid.find(key);
+    int id;
+    if (f == pid.end()) {
+      id = (int)groups.size();
+      pid.emplace(key, id);
+      groups.emplace_back();
+    } else {
+      id = f->second;
+    }
+    groups[id].push_back(cb);
+  }
+  for (const auto& grp : groups) {
+    const int J0 = bigside == 1 ? ex.items[grp[0]].M : ex.items[grp[0]].N;
+    for (int cb : grp) {
+      if ((bigside == 1 ? ex.items[cb].M : ex.items[cb].N) != J0) {
+        err = "output blocks of one open sector tuple differ in extent";
+        return 1;
+      }
+    }
+    ex.spanel.push_back((int32_t)ex.sorder.size());
+    ex.sorder.insert(ex.sorder.end(), grp.begin(), grp.end());
+  }
+  ex.spanel.push_back((int32_t)ex.sorder.size());
+  ex.bstream = bigside;
+  ex.small_total = small_total;
+  build_lean(ex);
   return 0;
 }
 
@@ -1448,45 +1496,10 @@ static int compile_step(const graft_pair_step& st, const TStruct& A, const TStru
   // Output blocks grouped by the large operand's open sector tuple (the
   // blocks of one group read the same rows of the large operand).
   if (bigside != 0) {
-    std::unordered_map<SecKey, int, SecHash> pid;
-    std::vector<std::vector<int32_t>> groups;
-    SecKey key;
-    for (int cb = 0; cb < (int)C.blocks.size(); ++cb) {
-      const Block& b = C.blocks[cb];
-      if (bigside == 1) {
-        key.assign(nOA, 0);
-        for (int p = 0; p < nOA; ++p) key[p] = b.sec[pos_of_out[p]];
-      } else {
-        key.assign(nOB, 0);
-        for (int p = 0; p < nOB; ++p) key[p] = b.sec[pos_of_out[nOA + p]];
-      }
-      auto f = pid.find(key);
-      int id;
-      if (f == pid.end()) {
-        id = (int)groups.size();
-        pid.emplace(key, id);
-        groups.emplace_back();
-      } else {
-        id = f->second;
-      }
-      groups[id].push_back(cb);
-    }
-    for (const auto& grp : groups) {
-      const int J0 = bigside == 1 ? ex.items[grp[0]].M : ex.items[grp[0]].N;
-      for (int cb : grp) {
-        if ((bigside == 1 ? ex.items[cb].M : ex.items[cb].N) != J0) {
-          err = "output blocks of one open sector tuple differ in extent";
-          return 1;
-        }
-      }
-      ex.spanel.push_back((int32_t)ex.sorder.size());
-      ex.sorder.insert(ex.sorder.end(), grp.begin(), grp.end());
-    }
-    ex.spanel.push_back((int32_t)ex.sorder.size());
-    ex.bstream = bigside;
+    if (build_stream_groups(ex, C, bigside, nOA, nOB, pos_of_out,
+                            bigside == 1 ? B.total : A.total, err))
+      return 1;
     ex.bstream_pref = stream_first;
-    ex.small_total = bigside == 1 ? B.total : A.total;
-    build_lean(ex);
   }
   if (timing)
     std::fprintf(stderr, "  contrib %.2f ms (%zu keys), C blocks %.2f ms, items %.2f ms (%zu segs, koff %zu), "
@@ -1622,6 +1635,18 @@ static int compile_coupled_step(const graft_pair_step& st, const TStruct& A, con
   ex.lean = false;
   if (build_items(st, A, B, C, pos_of_out, pairs, pstart, ex, err)) return 1;
   ex.coef = std::move(cf);
+  // Per-block streaming where one operand adds few open elements to every
+  // output block (non-abelian steps are mostly such: many small block pairs
+  // with short contractions, latency-bound on the tile kernels).
+  if (C.blocks.size() >= 2) {
+    const int bigside = A.total >= B.total ? 1 : 2;
+    int64_t maxS = 0;
+    for (const auto& it : ex.items) maxS = std::max<int64_t>(maxS, bigside == 1 ? it.N : it.M);
+    if (build_stream_groups(ex, C, bigside, nOA, nOB, pos_of_out,
+                            bigside == 1 ? B.total : A.total, err))
+      return 1;
+    ex.bstream_pref = maxS <= kBlockStreamS;
+  }
   auto_orientation(ex);
   return 0;
 }
@@ -1963,7 +1988,7 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
                                  ex.conjA, ex.conjB, 1, nullptr, ex.tr, ctx->stream, &maps));
     return GRAFT_OK;
   }
-  if (ex.d_coef != nullptr) {
+  if (ex.d_coef != nullptr && ex.config < kNumTileConfigs) {
     // coupled step: every block pair scaled by its coefficient as A is staged
     if (precision != GRAFT_PREC_FP64) {
       set_error("coupled (non-abelian) steps multiply in FP64 only");
@@ -2012,7 +2037,8 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     // HBM-bound, FP64 whatever the precision (as the streaming kernel below)
     GRAFT_CHECK_HIP(launch_zstream_blocks(ex.bstream == 1, A, B, C, ex.d_items, ex.d_segs,
                                           ex.d_koff, ex.d_sorder, ex.d_swork,
-                                          (int)ex.swork.size(), ex.conjA, ex.conjB, ctx->stream));
+                                          (int)ex.swork.size(), ex.conjA, ex.conjB, ctx->stream,
+                                          ex.d_coef));
   } else if (ex.config == kSkinnyConfig) {
     // HBM-bound: the streaming kernel always multiplies in FP64, whatever
     // precision the compute-bound steps use.  One launch per item (a fused

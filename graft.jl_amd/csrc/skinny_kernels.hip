@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(kSkinnyThreads)
                           double2* __restrict__ C, const GemmItem* __restrict__ items,
                           const GemmSeg* __restrict__ segs, const uint32_t* __restrict__ koff,
                           const int32_t* __restrict__ order, const StreamWork* __restrict__ work,
-                          int nwork, int conjA, int conjB) {
+                          int nwork, int conjA, int conjB, const double* __restrict__ coef) {
   constexpr int kWaves = kSkinnyThreads / 64;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int w = (int)blockIdx.x * kWaves + wave;
@@ -245,6 +245,8 @@ __global__ void __launch_bounds__(kSkinnyThreads)
         for (int s = 0; s < kBlockStreamS; ++s) acc[s] = double2{0.0, 0.0};
         for (int g = 0; g < it->seg_count; ++g) {
           const GemmSeg* sg = segs + it->seg_begin + g;
+          const double cf = coef != nullptr ? coef[it->seg_begin + g] : 1.0;
+          const double cfi = cf * sgBig;
           const int K = sg->K;
           const uint32_t* kA = koff + sg->koff_idx;
           const uint32_t* kbig = BIG_IS_A ? kA : kA + K;
@@ -262,7 +264,8 @@ __global__ void __launch_bounds__(kSkinnyThreads)
 #pragma unroll 4
             for (int k = 0; k < K; ++k) {
               double2 x = big[(uint64_t)bo + kbig[k]];
-              x.y *= sgBig;
+              x.x *= cf;
+              x.y *= cfi;
               const uint32_t kk = ksml[k];
 #pragma unroll
               for (int s = 0; s < kBlockStreamS; ++s) {
@@ -377,6 +380,8 @@ __global__ void __launch_bounds__(kSkinnyThreads)
 #pragma unroll
         for (int q = 0; q < SM / 4; ++q) sso[q] = reinterpret_cast<const uint4*>(sg->sso)[q];
         const int K = sg->K;
+        const double cf = sg->coef;
+        const double cfi = cf * sgBig;
         const uint32_t kbv[8] = {kb0.x, kb0.y, kb0.z, kb0.w, kb1.x, kb1.y, kb1.z, kb1.w};
         const uint32_t ksv[8] = {ks0.x, ks0.y, ks0.z, ks0.w, ks1.x, ks1.y, ks1.z, ks1.w};
         const double2* bp[RR];
@@ -391,7 +396,8 @@ __global__ void __launch_bounds__(kSkinnyThreads)
 #pragma unroll
             for (int r = 0; r < RR; ++r) {
               x[r] = bp[r][kbv[k]];
-              x[r].y *= sgBig;
+              x[r].x *= cf;
+              x[r].y *= cfi;
             }
 #pragma unroll
             for (int s = 0; s < SM; ++s) {
@@ -454,18 +460,18 @@ hipError_t launch_zstream_lean(bool big_is_a, int smax, const void* A, const voi
 hipError_t launch_zstream_blocks(bool big_is_a, const void* A, const void* B, void* C,
                                  const GemmItem* items, const GemmSeg* segs, const uint32_t* koff,
                                  const int32_t* order, const StreamWork* work, int nwork, int conjA,
-                                 int conjB, hipStream_t stream) {
+                                 int conjB, hipStream_t stream, const double* coef) {
   if (nwork <= 0) return hipSuccess;
   constexpr int kWaves = kSkinnyThreads / 64;
   const int grid = (nwork + kWaves - 1) / kWaves;
   if (big_is_a)
     hipLaunchKernelGGL(zstream_blocks_kernel<true>, dim3(grid), dim3(kSkinnyThreads), 0, stream,
                        (const double2*)A, (const double2*)B, (double2*)C, items, segs, koff, order,
-                       work, nwork, conjA, conjB);
+                       work, nwork, conjA, conjB, coef);
   else
     hipLaunchKernelGGL(zstream_blocks_kernel<false>, dim3(grid), dim3(kSkinnyThreads), 0, stream,
                        (const double2*)A, (const double2*)B, (double2*)C, items, segs, koff, order,
-                       work, nwork, conjA, conjB);
+                       work, nwork, conjA, conjB, coef);
   return hipGetLastError();
 }
 

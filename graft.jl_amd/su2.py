@@ -789,8 +789,12 @@ def random_ttno(rng, topo, tphys=1, channel=CHANNEL):
 
 
 def two_site_tensor(psi, n, m):
-    """Theta = A_n contracted with A_m over their edge (effective.jl:827-843)."""
+    """Theta = A_n contracted with A_m over their edge (effective.jl:827-843),
+    on every invariant block of its legs: the two-site map reaches blocks
+    whose n-side coupled spin the bond between n and m does not carry."""
     from .trees import childslot
     A, Bm = psi.tensors[n], psi.tensors[m]
     k = childslot(psi.topo, m, n)
-    return contract(A, (A.rank - 1,), Bm, (k,), ncod=A.rank - 1 + Bm.rank - 1)
+    legs = A.blocks.legs[:-1] + [l for j, l in enumerate(Bm.blocks.legs) if j != k]
+    return contract(A, (A.rank - 1,), Bm, (k,), ncod=len(legs),
+                    out_structure=SU2Structure.invariant(legs))
