@@ -587,6 +587,60 @@ def hubbard_apply_bench(D, reps=20, sigma=1.6, seed=5, autotune=True, lanczos=Tr
             "lanczos": lanczos_runs}
 
 
+def su2_apply_bench(tphys=2, maxdeg=24, maxspin=10, reps=20, dense=True, seed=0):
+    """Root-edge eff_h2 of the spin-(tphys/2) Heisenberg model on the 16-site
+    tree with SU(2) blocks (bond multiplet degeneracies capped at ``maxdeg``,
+    spins at ``maxspin`` / 2), against the plain dense FP64 map of the same
+    tensors' dense embeddings."""
+    import torch
+    topo = hubbard_tree()
+    from graft_amd import contractions as C, networks as N, su2, tensors as T
+    rng = np.random.default_rng(seed)
+    psi = su2.random_ttns(rng, topo, tphys, maxdeg, maxspin)
+    H = su2.heisenberg_ttno(topo, 1.0, tphys)
+    cache = C.EnvCache(topo)
+    x = C.two_site_tensor(psi, 1, 0)
+    for w, v in ((3, 1), (4, 1), (2, 0)):
+        C.env_(cache, psi, H, w, v)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    h2 = C.eff_h2(cache, psi, H, 1, 0, out_structure=x.blocks)
+    t_plan = time.perf_counter() - t
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(m, v):
+        y = m(v)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            y = m(v, out=y)
+        e1.record()
+        torch.cuda.synchronize()
+        return y, e0.elapsed_time(e1) / reps
+
+    y, ms = timed(h2, x)
+    step_ms = h2.native.execute_timed(h2.operands(x), y)
+    r = {"tphys": tphys, "maxdeg": maxdeg, "maxspin": maxspin,
+         "bond_dims_dense": [psi.tensors[c].dims[-1] for c in range(1, 16)],
+         "theta_elements_su2": int(x.numel), "theta_elements_dense": int(np.prod(x.dims)),
+         "theta_blocks": len(x.blocks.blocks), "block_pairs": h2.native.npairs,
+         "plan_s": t_plan, "ms_per_apply": ms, "step_ms": step_ms,
+         "steps": [(s["m"], s["k"], s["n"], s["blocks"], s["config"]) for s in h2.native.step_info()]}
+    if dense:
+        psid = N.TTNS(topo, [T.DTensor.from_numpy(a.to_numpy(), a.ncod) for a in psi.tensors],
+                      psi.center, psi.hasphys)
+        Hd = N.TTNO(topo, [T.DTensor.from_numpy(w.to_numpy(), w.ncod) for w in H.tensors])
+        cd = C.EnvCache(topo)
+        xd = C.two_site_tensor(psid, 1, 0)
+        h2d = C.eff_h2(cd, psid, Hd, 1, 0)
+        yd, msd = timed(h2d, xd)
+        ref = yd.to_numpy()
+        r.update({"dense_ms_per_apply": msd, "dense_tflops": h2d.flops / msd / 1e9,
+                  "speedup_vs_dense": msd / ms,
+                  "rel_diff_vs_dense": float(np.linalg.norm(y.to_numpy() - ref) / np.linalg.norm(ref))})
+    return r
+
+
 def blas_threads():
     try:
         from threadpoolctl import threadpool_info
@@ -606,6 +660,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-one-D", type=int, default=64,
                     help="bond dimension of the one-core host apply")
+    ap.add_argument("--no-su2-apply", action="store_true",
+                    help="apply mode: skip the SU(2) block apply figure")
+    ap.add_argument("--su2-maxdeg", type=int, default=24)
     ap.add_argument("--no-sector-apply", action="store_true",
                     help="apply mode: skip the D=1024 sector-batched apply figure")
     ap.add_argument("--no-sweep", action="store_true",
@@ -615,7 +672,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the heuristic tile configurations (default: autotune every step)")
     ap.add_argument("--mode", choices=["apply", "sweep", "hubbard", "hubbard-apply", "hubbard-tdvp",
-                                       "precision"],
+                                       "precision", "su2-apply"],
                     default="apply")
     ap.add_argument("--prec-D", type=int, default=64)
     ap.add_argument("--prec-steps", type=int, default=10)
@@ -663,6 +720,14 @@ def main():
                           "unit": "ms", "higher_is_better": False, "n_gpus": 1,
                           "config": {"workload": f"eff_h2 root edge, Hubbard 16-site tree, U(1)xU(1), "
                                                  f"D={r['D']}"}, "detail": r}), flush=True)
+        return
+    if args.mode == "su2-apply":
+        r = su2_apply_bench(maxdeg=args.su2_maxdeg, dense=not args.no_dense)
+        print(json.dumps({"metric": "SU(2)-block eff_h2 ms per apply", "value": r["ms_per_apply"],
+                          "unit": "ms", "higher_is_better": False, "n_gpus": 1,
+                          "config": {"workload": f"eff_h2 root edge, spin-1 Heisenberg 16-site tree, "
+                                                 f"SU(2), multiplet degeneracy <= {r['maxdeg']}"},
+                          "detail": r}), flush=True)
         return
     if args.mode == "hubbard-tdvp":
         r = hubbard_tdvp_bench(args.hub_D, args.prec_steps, dense=not args.no_dense)
@@ -835,6 +900,20 @@ def main():
                   "flops_stored_blocks": r["flops"]}
         print(f"# sector apply {time.perf_counter() - t_sa:.1f} s", file=sys.stderr, flush=True)
     out["sector_apply"] = sector
+    su2r = None
+    if not args.no_su2_apply:
+        # SU(2) blocks: coupled block pairs (recoupling coefficients) per step
+        t_su = time.perf_counter()
+        r = su2_apply_bench(maxdeg=args.su2_maxdeg)
+        su2r = {"workload": "eff_h2 root edge, spin-1 Heisenberg 16-site tree, SU(2), "
+                            f"multiplet degeneracy <= {r['maxdeg']}",
+                "ms_per_apply": r["ms_per_apply"], "dense_ms_per_apply": r["dense_ms_per_apply"],
+                "speedup_vs_dense": r["speedup_vs_dense"], "rel_diff_vs_dense": r["rel_diff_vs_dense"],
+                "theta_elements_su2": r["theta_elements_su2"],
+                "theta_elements_dense": r["theta_elements_dense"], "block_pairs": r["block_pairs"],
+                "step_ms": r["step_ms"]}
+        print(f"# su2 apply {time.perf_counter() - t_su:.1f} s", file=sys.stderr, flush=True)
+    out["su2_apply"] = su2r
     print(json.dumps(out), flush=True)
 
 

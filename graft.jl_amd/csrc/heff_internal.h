@@ -159,8 +159,9 @@ hipError_t launch_zstream_blocks(bool big_is_a, const void* A, const void* B, vo
                                  const int32_t* order, const StreamWork* work, int nwork, int conjA,
                                  int conjB, hipStream_t stream, const double* coef = nullptr);
 
-// Lean form of the per-block streaming step, used when the small operand
-// fits in LDS, every output block has at most kLeanMaxS small-side outputs
+// Lean form of the per-block streaming step (the small operand staged in LDS
+// when it fits, else read at wave-uniform addresses through the caches),
+// used when every output block has at most kLeanMaxS small-side outputs
 // and kLeanRank big-side legs, and every segment contracts at most
 // kLeanInlineK indices.  Everything a lane needs beyond its row index is precomputed on
 // the host: row digits come from a multiply-high by a per-leg reciprocal

@@ -893,9 +893,9 @@ static int build_fused(const graft_pair_step& st, const TStruct& A, const TStruc
   return 0;
 }
 
-// Lean per-block streaming records (see LeanItem): built when the small
-// operand fits in LDS, every output block has at most kLeanMaxS small-side
-// outputs and at most kLeanRank big-side legs of extent > 1, every segment
+// Lean per-block streaming records (see LeanItem): built when every output
+// block has at most kLeanMaxS small-side outputs and at most kLeanRank
+// big-side legs of extent > 1, every segment
 // contracts at most kLeanInlineK indices, and every big-side digit decodes
 // exactly by a multiply-high (J * dim < 2^32).  Records are stored group by
 // group (sorder); work entries run rows outer, the blocks of one big-side
@@ -911,7 +911,7 @@ static void build_lean(StepExec& ex) {
     ex.lsegs.clear();
     ex.lwork.clear();
   };
-  if (ex.bstream == 0 || ex.small_total > kLeanSmallMax || ex.items.empty()) return;
+  if (ex.bstream == 0 || ex.items.empty()) return;
   const bool bigA = ex.bstream == 1;
   int smax = 1;
   for (const GemmItem& it : ex.items) {
@@ -2032,7 +2032,8 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
     // HBM-bound, FP64 whatever the precision (as the streaming kernels below)
     GRAFT_CHECK_HIP(launch_zstream_lean(ex.bstream == 1, ex.lean_smax, A, B, C, ex.d_litems,
                                         ex.d_lsegs, ex.d_lwork, (int)ex.lwork.size(),
-                                        (int)ex.small_total, ex.conjA, ex.conjB, ctx->stream));
+                                        (int)std::min<int64_t>(ex.small_total, INT32_MAX), ex.conjA,
+                                        ex.conjB, ctx->stream));
   } else if (ex.config == kBlockStreamConfig) {
     // HBM-bound, FP64 whatever the precision (as the streaming kernel below)
     GRAFT_CHECK_HIP(launch_zstream_blocks(ex.bstream == 1, A, B, C, ex.d_items, ex.d_segs,

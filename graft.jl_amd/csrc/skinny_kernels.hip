@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(kSkinnyThreads)
 // steps).  Row digits come from a multiply-high by the per-leg reciprocal
 // the host computed.  Item and segment records are fixed-width and read as
 // whole 16-byte vectors before any use, so the wave waits once per record.
-template <bool BIG_IS_A, int SM>
+template <bool BIG_IS_A, int SM, bool SMEM>
 __global__ void __launch_bounds__(kSkinnyThreads)
     zstream_lean_kernel(const double2* __restrict__ A, const double2* __restrict__ B,
                         double2* __restrict__ C, const LeanItem* __restrict__ items,
@@ -314,12 +314,14 @@ __global__ void __launch_bounds__(kSkinnyThreads)
   const double2* __restrict__ big = BIG_IS_A ? A : B;
   const double sgSml = (BIG_IS_A ? conjB : conjA) ? -1.0 : 1.0;
   const double sgBig = (BIG_IS_A ? conjA : conjB) ? -1.0 : 1.0;
-  for (int i = threadIdx.x; i < nsmall; i += kSkinnyThreads) {
-    double2 v = small[i];
-    v.y *= sgSml;
-    sm[i] = v;
+  if constexpr (SMEM) {
+    for (int i = threadIdx.x; i < nsmall; i += kSkinnyThreads) {
+      double2 v = small[i];
+      v.y *= sgSml;
+      sm[i] = v;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   const int lane = (int)(threadIdx.x & 63);
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int nw = (int)gridDim.x * kWaves;
@@ -403,7 +405,14 @@ __global__ void __launch_bounds__(kSkinnyThreads)
             for (int s = 0; s < SM; ++s) {
               if (s < S) {
                 const uint32_t so = (&sso[s / 4].x)[s % 4];
-                const double2 v = sm[ksv[k] + so];
+                double2 v;
+                if constexpr (SMEM) {
+                  v = sm[ksv[k] + so];
+                } else {
+                  // uniform address: one scalar load for the wave, cached
+                  v = small[(uint64_t)ksv[k] + so];
+                  v.y *= sgSml;
+                }
 #pragma unroll
                 for (int r = 0; r < RR; ++r) {
                   acc[r][s].x += x[r].x * v.x - x[r].y * v.y;
@@ -431,19 +440,28 @@ hipError_t launch_zstream_lean(bool big_is_a, int smax, const void* A, const voi
                                const LeanItem* items, const LeanSeg* segs, const LeanWork* work,
                                int nwork, int nsmall, int conjA, int conjB, hipStream_t stream) {
   if (nwork <= 0) return hipSuccess;
-  if (nsmall < 0 || nsmall > kLeanSmallMax || smax < 1 || smax > kLeanMaxS) return hipErrorInvalidValue;
+  if (nsmall < 0 || smax < 1 || smax > kLeanMaxS) return hipErrorInvalidValue;
+  // a small operand that fits in LDS is staged there; a larger one is read
+  // from global memory at wave-uniform addresses (cached scalar loads)
+  const bool smem = nsmall <= kLeanSmallMax;
   constexpr int kWaves = kSkinnyThreads / 64;
   // enough workgroups to fill the chip several times over; each stages the
   // small operand once, so fewer and longer-lived when it is large
-  const int cap = nsmall > 1024 ? 2048 : 8192;
+  const int cap = smem && nsmall > 1024 ? 2048 : 8192;
   const int grid = std::min((nwork + kWaves - 1) / kWaves, cap);
-  const size_t lds = (size_t)std::max(nsmall, 1) * sizeof(double2);
+  const size_t lds = smem ? (size_t)std::max(nsmall, 1) * sizeof(double2) : 0;
   const double2* a = (const double2*)A;
   const double2* b = (const double2*)B;
   double2* c = (double2*)C;
 #define GRAFT_LEAN(BIGA, SMV)                                                                      \
-  hipLaunchKernelGGL((zstream_lean_kernel<BIGA, SMV>), dim3(grid), dim3(kSkinnyThreads), lds,     \
-                     stream, a, b, c, items, segs, work, nwork, nsmall, conjA, conjB)
+  do {                                                                                             \
+    if (smem)                                                                                      \
+      hipLaunchKernelGGL((zstream_lean_kernel<BIGA, SMV, true>), dim3(grid), dim3(kSkinnyThreads), \
+                         lds, stream, a, b, c, items, segs, work, nwork, nsmall, conjA, conjB);    \
+    else                                                                                           \
+      hipLaunchKernelGGL((zstream_lean_kernel<BIGA, SMV, false>), dim3(grid), dim3(kSkinnyThreads),\
+                         lds, stream, a, b, c, items, segs, work, nwork, nsmall, conjA, conjB);    \
+  } while (0)
   if (big_is_a) {
     if (smax <= 4) GRAFT_LEAN(true, 4);
     else if (smax <= 8) GRAFT_LEAN(true, 8);

@@ -432,7 +432,7 @@ def compile_pair(SA, legsA, SB, legsB, oA, cA, oB, cB, perm, fixed_out=None):
     for ai, (ir, _) in enumerate(SA.blocks):
         groupsA.setdefault(ir, []).append(ai)
     dA, dB = SA.duals, SB.duals
-    entries = []         # (C key, A block, B block, coefficient)
+    chunks = []          # per irrep pattern pair: (C irreps, C trees, tc, A block, B block, coefficient)
     for irA, alist in groupsA.items():
         ck = tuple(irA[i] for i in cA)
         gB = groupsB.get(ck)
@@ -446,33 +446,39 @@ def compile_pair(SA, legsA, SB, legsB, oA, cA, oB, cB, perm, fixed_out=None):
                 tjsA, dA, tjsB, dB, oA, cA, oB, cB, perm, dualsC)
             if not treesC:
                 continue
-            interir = tuple(irA[i] for i in oA) + tuple(irB[i] for i in oB)
-            irC = tuple(interir[p] for p in perm)
+            x, y, z = np.nonzero(coef)
+            if not len(x):
+                continue
             posB = {SB.blocks[bi][1]: bi for bi in blist}
-            nz = np.nonzero(coef)
-            for x, y, z in zip(*nz):
-                ai = posA.get(treesA[x])
-                bi = posB.get(treesB[y])
-                if ai is None or bi is None:
-                    continue
-                entries.append(((irC, treesC[z]), ai, bi, float(coef[x, y, z])))
+            pa = np.asarray([posA.get(t, -1) for t in treesA], dtype=np.int64)[x]
+            pb = np.asarray([posB.get(t, -1) for t in treesB], dtype=np.int64)[y]
+            ok = (pa >= 0) & (pb >= 0)
+            if not ok.all():
+                x, y, z, pa, pb = x[ok], y[ok], z[ok], pa[ok], pb[ok]
+            interir = tuple(irA[i] for i in oA) + tuple(irB[i] for i in oB)
+            chunks.append((tuple(interir[p] for p in perm), treesC, z, pa, pb, coef[x, y, z]))
     if fixed_out is not None:
         C = fixed_out
     else:
-        keys = sorted({e[0] for e in entries})
+        keys = sorted({(irC, treesC[int(t)]) for irC, treesC, z, _, _, _ in chunks
+                       for t in np.unique(z)})
         if not keys:
             raise ValueError("coupled step: no block pair contributes")
         C = SU2Structure(out_legs, keys)
     cidx = C.index
-    trip = np.empty((len(entries), 3), dtype=np.int32)
-    cf = np.empty(len(entries), dtype=np.float64)
-    for i, (k, ai, bi, c) in enumerate(entries):
-        ci = cidx.get(k)
-        if ci is None:
-            raise ValueError(f"coupled step: block {k} lies outside the given output structure")
-        trip[i] = (ci, ai, bi)
-        cf[i] = c
-    order = np.argsort(trip[:, 0], kind="stable") if len(entries) else np.zeros(0, np.int64)
+    parts, cfs = [], []
+    for irC, treesC, z, pa, pb, cf in chunks:
+        ci = np.asarray([cidx.get((irC, t), -1) for t in treesC], dtype=np.int64)[z]
+        if (ci < 0).any():
+            bad = treesC[int(z[np.argmax(ci < 0)])]
+            raise ValueError(f"coupled step: block {(irC, bad)} lies outside the given output structure")
+        parts.append(np.stack([ci, pa, pb], axis=1))
+        cfs.append(cf)
+    if not parts:
+        return C, np.zeros((0, 3), np.int32), np.zeros(0)
+    trip = np.concatenate(parts).astype(np.int32)
+    cf = np.concatenate(cfs).astype(np.float64)
+    order = np.argsort(trip[:, 0], kind="stable")
     return C, np.ascontiguousarray(trip[order]), np.ascontiguousarray(cf[order])
 
 
