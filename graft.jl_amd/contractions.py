@@ -713,6 +713,10 @@ def split_two_site_(psi, theta, n, m, trunc=B.NO_TRUNCATION, center_on="n"):
 
     Returns (psi, info) with the discarded norm and the full singular spectrum."""
     if theta.blocks is not None:
+        if _is_su2(theta.blocks):
+            from . import su2
+            return su2.split_two_site_(psi, theta, n, m, lambda s: B.truncation_rank(s, trunc),
+                                       center_on)
         from . import sectors
         return sectors.split_two_site_(psi, theta, n, m, lambda s: B.truncation_rank(s, trunc),
                                        center_on)

@@ -90,6 +90,9 @@ def normalize_(psi):
 def _move_center_edge_(psi, m, cache):
     """ttns.jl:181-203 (QR up- and down-moves)."""
     if psi.tensors[psi.center].blocks is not None:
+        if hasattr(psi.tensors[psi.center].blocks, "dense_from_flat"):
+            from . import su2
+            return su2.move_center_edge_(psi, m, cache)
         from . import sectors
         return sectors.move_center_edge_(psi, m, cache)
     t = psi.topo

@@ -804,3 +804,165 @@ def two_site_tensor(psi, n, m):
     legs = A.blocks.legs[:-1] + [l for j, l in enumerate(Bm.blocks.legs) if j != k]
     return contract(A, (A.rank - 1,), Bm, (k,), ncod=len(legs),
                     out_structure=SU2Structure.invariant(legs))
+
+
+# ---------------------------------------------------------------------------
+# Two-site split and gauge moves (per coupled sector)
+# ---------------------------------------------------------------------------
+
+
+def _scale_axis(structure, flat, axis, scales):
+    """Multiply every block along ``axis`` (a degeneracy index of that leg's
+    irrep s) by scales[s][index]."""
+    out = np.array(flat, dtype=np.complex128)
+    for b, (ir, _) in enumerate(structure.blocks):
+        sc = scales.get(ir[axis])
+        if sc is None:
+            continue
+        o = int(structure.offsets[b])
+        blk = out[o:o + structure.sizes[b]].reshape(structure.block_dims[b], order="F")
+        shape = [1] * blk.ndim
+        shape[axis] = len(sc)
+        blk *= np.asarray(sc).reshape(shape)
+        out[o:o + structure.sizes[b]] = blk.reshape(-1, order="F")
+    return out
+
+
+def split_two_site_(psi, theta, n, m, trunc_rank, center_on="n"):
+    """split_two_site! (effective.jl:852-876) for SU(2) states.
+
+    Theta's fusion trees pass through the coupled spin J of the n-side legs
+    (its label after the first pn legs), so Theta is block diagonal in J:
+    M_J rows = (n-side irreps, n-side tree, degeneracies), columns = the
+    m-side.  Its dense embedding has the singular values sigma(M_J) /
+    sqrt(2J + 1), each (2J + 1) times: the spectrum the truncation rules see
+    (``trunc_rank`` of the descending dense spectrum); whole multiplets are
+    kept, as many as fit.  A_n = sqrt(2J + 1) U_J per J (a dense isometry
+    onto the new bond); A_m = conj(A_n) contracted with Theta over the
+    n-side legs (the coupled contraction recouples it into A_m's own trees),
+    with the bond at child slot k.  With the centre on n, A_n takes the dense
+    singular values and A_m = conj(A_n) contracted with Theta's polar factor
+    (every kept singular value one)."""
+    from .trees import childslot
+    t = psi.topo
+    k = childslot(t, m, n)
+    An, Am = psi.tensors[n], psi.tensors[m]
+    pn = An.ncod
+    st = theta.blocks
+    host = theta.data.detach().cpu().numpy()
+    rows, cols, pieces = {}, {}, {}
+    for b, (ir, tr) in enumerate(st.blocks):
+        tjs = st.spins(b)
+        J = tr[pn - 2] if pn >= 2 else tjs[0]
+        rk, ck = (ir[:pn], tr[:max(pn - 2, 0)]), (ir[pn:], tr[pn - 1:] if pn >= 2 else tr)
+        nr = int(np.prod(st.block_dims[b][:pn]))
+        nc = int(np.prod(st.block_dims[b][pn:])) if len(ir) > pn else 1
+        rows.setdefault(J, {}).setdefault(rk, nr)
+        cols.setdefault(J, {}).setdefault(ck, nc)
+        o = int(st.offsets[b])
+        pieces[(J, rk, ck)] = host[o:o + st.sizes[b]].reshape(nr, nc, order="F")
+    svd = {}
+    spectrum = []
+    for J in rows:
+        rkeys, ckeys = list(rows[J]), list(cols[J])
+        roff = np.concatenate([[0], np.cumsum([rows[J][r] for r in rkeys])])
+        coff = np.concatenate([[0], np.cumsum([cols[J][c] for c in ckeys])])
+        M = np.zeros((roff[-1], coff[-1]), dtype=np.complex128)
+        for i, r in enumerate(rkeys):
+            for j, c in enumerate(ckeys):
+                p = pieces.get((J, r, c))
+                if p is not None:
+                    M[roff[i]:roff[i + 1], coff[j]:coff[j + 1]] = p
+        U, s, Vh = np.linalg.svd(M, full_matrices=False)
+        sd = s / math.sqrt(J + 1)
+        svd[J] = (rkeys, roff, U, sd, ckeys, coff, Vh)
+        spectrum += [(x, J) for x in sd]
+    spectrum.sort(key=lambda v: -v[0])
+    dense = np.repeat([v[0] for v in spectrum], [v[1] + 1 for v in spectrum]) if spectrum \
+        else np.zeros(0)
+    r = int(trunc_rank(dense))
+    keep, used = {}, 0
+    for x, J in spectrum:
+        if used + J + 1 > r:
+            break
+        if x <= 0.0:
+            break
+        keep[J] = keep.get(J, 0) + 1
+        used += J + 1
+    if not keep:                         # keep the leading multiplet, whatever its size
+        x, J = spectrum[0]
+        keep[J] = 1
+    kept_sq = sum(float(np.sum(svd[J][3][:d] ** 2)) * (J + 1) for J, d in keep.items())
+    disc = math.sqrt(max(0.0, float(np.sum(dense ** 2)) - kept_sq))
+    bond = SU2Leg(sorted(keep.items()))
+    bidx = {tj: i for i, (tj, _) in enumerate(bond.irreps)}
+    # A_n on every invariant block of (n-side legs, bond*)
+    nlegs = list(An.blocks.legs[:pn]) + [bond.dual()]
+    Sn = SU2Structure.invariant(nlegs)
+    fn = np.zeros(Sn.total, dtype=np.complex128)
+    for b, (ir, tr) in enumerate(Sn.blocks):
+        J = bond.irreps[ir[-1]][0]
+        rkeys, roff, U = svd[J][:3]
+        rk = (ir[:pn], tr[:max(pn - 2, 0)])
+        if rk not in rkeys:
+            continue
+        i = rkeys.index(rk)
+        d = keep[J]
+        blk = U[roff[i]:roff[i + 1], :d] * math.sqrt(J + 1)
+        o = int(Sn.offsets[b])
+        fn[o:o + Sn.sizes[b]] = blk.reshape(-1, order="F")
+    An_iso = to_device(Sn, fn, pn)
+    # A_m = conj(A_n) . Theta, the bond moved to child slot k
+    rank_m = Am.rank
+    perm = tuple(0 if j == k else 1 + (j if j < k else j - 1) for j in range(rank_m))
+    mlegs = [bond] + list(theta.blocks.legs[pn:])
+    mlegs = [mlegs[p] for p in perm]
+    sig = {bidx[J]: svd[J][3][:d] for J, d in keep.items()}
+    if center_on == "n":
+        # the isometric side from Theta's polar factor sqrt(2J+1) U_J Vh_J
+        # (every kept dense singular value set to one): conj(A_n) . Theta' =
+        # Vh exactly, with no division by small singular values
+        fp = np.zeros(st.total, dtype=np.complex128)
+        for b, (ir, tr) in enumerate(st.blocks):
+            tjs = st.spins(b)
+            J = tr[pn - 2] if pn >= 2 else tjs[0]
+            d = keep.get(J)
+            if not d:
+                continue
+            rkeys, roff, U, _, ckeys, coff, Vh = svd[J]
+            i = rkeys.index((ir[:pn], tr[:max(pn - 2, 0)]))
+            j = ckeys.index((ir[pn:], tr[pn - 1:] if pn >= 2 else tr))
+            blk = math.sqrt(J + 1) * (U[roff[i]:roff[i + 1], :d] @ Vh[:d, coff[j]:coff[j + 1]])
+            o = int(st.offsets[b])
+            fp[o:o + st.sizes[b]] = blk.reshape(-1, order="F")
+        polar = to_device(st, fp, theta.ncod)
+        psi.tensors[m] = contract(An_iso, tuple(range(pn)), polar, tuple(range(pn)), conjA=True,
+                                  perm=perm, ncod=Am.ncod, out_structure=SU2Structure.invariant(mlegs))
+        psi.tensors[n] = to_device(Sn, _scale_axis(Sn, fn, pn, sig), pn)
+    else:
+        Amn = contract(An_iso, tuple(range(pn)), theta, tuple(range(pn)), conjA=True, perm=perm,
+                       ncod=Am.ncod, out_structure=SU2Structure.invariant(mlegs))
+        psi.tensors[n] = An_iso
+        psi.tensors[m] = Amn
+    psi.center = n if center_on == "n" else m
+    return psi, dict(discarded=disc, spectrum=dense, kept=bond.dim)
+
+
+def move_center_edge_(psi, m, cache=None):
+    """One gauge move of the centre to the neighbour m (ttns.jl:181-203): the
+    two-site tensor of the edge split again with every nonzero multiplet kept
+    and the centre on m (an SVD where the reference takes a QR; exact)."""
+    t = psi.topo
+    n = psi.center
+    child, parent = (n, m) if t.parent[n] == m else (m, n)
+    theta = two_site_tensor(psi, child, parent)
+
+    def keep_nonzero(s):
+        if len(s) == 0:
+            return 0
+        return int(np.sum(s > 1e-14 * s[0]))
+
+    split_two_site_(psi, theta, child, parent, keep_nonzero, "n" if m == child else "m")
+    if cache is not None:
+        cache.invalidate_edge_(n, m)
+    return psi

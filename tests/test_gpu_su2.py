@@ -154,3 +154,62 @@ def test_su2_heisenberg_lanczos_gives_the_singlet_ground_energy(g, name, tphys):
     h2 = C.eff_h2(cache, psi, H, child, root, out_structure=theta.blocks)
     vals, vecs, info = K.eigsolve(h2, theta, 1, "SR", krylovdim=30, tol=1e-12)
     assert abs(vals[0] - e_singlet) < 1e-10, (vals[0], e_singlet)
+
+
+def test_su2_split_and_center_moves_keep_the_state(g):
+    """The SU(2) two-site split without truncation and the centre moves keep
+    the dense state (1e-12); the split's factors are dense isometries on the
+    non-centre side, and its spectrum is the dense Theta's singular values."""
+    from graft_amd import contractions as C, networks as N, su2, trees as Tr
+    topo = Tr.binary_topology(2)
+    t = _tree(topo)
+    _say("su2 split and centre moves, binary2 spin 1, maxdeg 3")
+    rng = np.random.default_rng(23)
+    psi = su2.random_ttns(rng, topo, 2, 3)
+    v0 = ref.ttns_to_dense(t, [a.to_numpy() for a in psi.tensors], list(psi.hasphys))
+    child = topo.children[psi.center][0]
+    theta = C.two_site_tensor(psi, child, psi.center)
+    th = theta.to_numpy()
+    pn = psi.tensors[child].ncod
+    sv = np.linalg.svd(th.reshape(int(np.prod(th.shape[:pn])), -1, order="F"), compute_uv=False)
+    for center_on in ("n", "m"):
+        p = N.TTNS(topo, list(psi.tensors), psi.center, psi.hasphys)
+        p, info = C.split_two_site_(p, theta, child, topo.parent[child], center_on=center_on)
+        v = ref.ttns_to_dense(t, [a.to_numpy() for a in p.tensors], list(p.hasphys))
+        assert _rel(v, v0) < 1e-12, center_on
+        nz = np.sort(sv[sv > 1e-12 * sv[0]])[::-1]
+        got = np.sort(info["spectrum"][info["spectrum"] > 1e-12 * sv[0]])[::-1]
+        assert len(got) == len(nz) and np.abs(got - nz).max() < 1e-12 * sv[0]
+        iso = child if center_on == "m" else topo.parent[child]
+        A = p.tensors[iso].to_numpy()
+        if iso == child:
+            Mx = A.reshape(-1, A.shape[-1], order="F")
+        else:
+            k = topo.children[topo.parent[child]].index(child)
+            Mx = np.moveaxis(A, k, -1).reshape(-1, A.shape[k], order="F")
+        assert np.abs(Mx.conj().T @ Mx - np.eye(Mx.shape[1])).max() < 1e-12
+    p = N.TTNS(topo, list(psi.tensors), psi.center, psi.hasphys)
+    for target in (5, 2, 6, 0, 3):
+        N.move_center_(p, target)
+        v = ref.ttns_to_dense(t, [a.to_numpy() for a in p.tensors], list(p.hasphys))
+        assert _rel(v, v0) < 1e-12, target
+        assert p.center == target
+
+
+@pytest.mark.parametrize("name,tphys", [("star31", 1), ("chain6", 1), ("binary2", 2)])
+def test_su2_dmrg2_reaches_the_singlet_ground_energy(g, name, tphys):
+    """dmrg2_ on SU(2) states from small bonds reaches the lowest singlet
+    energy of exact diagonalization (1e-10)."""
+    from graft_amd import backend as B, groundstate as GS, su2, trees as Tr
+    topo = {"star31": lambda: Tr.star_topology(3, 1), "chain6": lambda: Tr.mps_topology(6),
+            "binary2": lambda: Tr.binary_topology(2)}[name]()
+    t = _tree(topo)
+    _say("su2 dmrg2", name, "spin", tphys, "/2")
+    rng = np.random.default_rng(29)
+    psi = su2.random_ttns(rng, topo, tphys, 1)
+    H = su2.heisenberg_ttno(topo, 1.0, tphys)
+    n = len(topo.ids)
+    Hd = ref.ttno_to_dense(t, [w.to_numpy() for w in H.tensors])
+    e_singlet = np.linalg.eigvalsh(Hd + 50.0 * _total_spin_squared(n, tphys))[0]
+    _, Es = GS.dmrg2_(psi, H, trunc=B.TruncationScheme(maxdim=4096), nsweeps=6)
+    assert abs(Es[-1] - e_singlet) < 1e-10, (Es, e_singlet)
