@@ -577,11 +577,23 @@ class CoupledPlan:
 
 
 def root_cap(legs):
-    """Cap closing the root's unit legs (every leg the trivial irrep, once)."""
+    """Cap closing the root's parent legs (bra, operator channel, ket): the
+    identity between the bra and ket legs on the operator's trivial channel.
+    With a trivial root leg this is the ones tensor of the reference's
+    _root_cap! (envcache.jl:626-655); a root leg carrying the target total
+    spin S is closed by the identity on its multiplet, so every map acts as
+    H on each of the 2S + 1 states."""
     st = SU2Structure.invariant(legs)
-    if st.total != 1:
-        raise ValueError("SU(2) root cap: the root legs must each hold the trivial irrep once")
-    return to_device(st, np.ones(1, dtype=np.complex128), len(legs))
+    dims = st.dims
+    if len(dims) != 3 or dims[1] != 1 or dims[0] != dims[2]:
+        raise ValueError("SU(2) root cap: expected (S, trivial, S*) root legs")
+    dense = np.zeros(dims, dtype=np.complex128)
+    for i in range(dims[0]):
+        dense[i, 0, i] = 1.0
+    flat = st.from_dense(dense)
+    if np.abs(st.to_dense(flat) - dense).max() > 1e-12:
+        raise ValueError("SU(2) root cap: the root legs do not close invariantly")
+    return to_device(st, flat, len(legs))
 
 
 # ---------------------------------------------------------------------------
@@ -661,10 +673,11 @@ def bond_legs(topo, tphys=1, maxdeg=None, maxspin=None):
     return legs
 
 
-def site_legs(topo, n, bonds, tphys=1):
-    """Legs of TTNS tensor n: children bonds, physical spin, parent bond (dual)."""
+def site_legs(topo, n, bonds, tphys=1, target=0):
+    """Legs of TTNS tensor n: children bonds, physical spin, parent bond
+    (dual); the root's parent leg carries the target total spin (twice it)."""
     legs = [bonds[c] for c in topo.children[n]] + [SU2Leg([(tphys, 1)])]
-    parent = bonds[n] if topo.parent[n] >= 0 else SU2Leg([(0, 1)])
+    parent = bonds[n] if topo.parent[n] >= 0 else SU2Leg([(target, 1)])
     return legs + [parent.dual()]
 
 
@@ -699,10 +712,10 @@ def left_orthonormalize(structure, flat):
     return flat
 
 
-def random_ttns(rng, topo, tphys=1, maxdeg=None, maxspin=None, center=None):
-    """Random SU(2) tree state: every site spin tphys/2, bonds from
-    ``bond_legs``; every tensor but the centre's (the root by default) an
-    isometry towards its parent, the centre normalized."""
+def random_ttns(rng, topo, tphys=1, maxdeg=None, maxspin=None, center=None, target=0):
+    """Random SU(2) tree state of total spin target / 2: every site spin
+    tphys/2, bonds from ``bond_legs``; every tensor but the centre's (the root
+    by default) an isometry towards its parent, the centre normalized."""
     from .networks import TTNS
     from .trees import nnodes
     bonds = bond_legs(topo, tphys, maxdeg, maxspin)
@@ -712,7 +725,7 @@ def random_ttns(rng, topo, tphys=1, maxdeg=None, maxspin=None, center=None):
         raise ValueError("random_ttns: the centre must be the root")
     tensors = []
     for n in range(nnodes(topo)):
-        legs = site_legs(topo, n, bonds, tphys)
+        legs = site_legs(topo, n, bonds, tphys, target)
         st = SU2Structure.invariant(legs)
         if st.total == 0:
             raise ValueError(f"random_ttns: node {topo.ids[n]} admits no invariant block")

@@ -213,3 +213,19 @@ def test_su2_dmrg2_reaches_the_singlet_ground_energy(g, name, tphys):
     e_singlet = np.linalg.eigvalsh(Hd + 50.0 * _total_spin_squared(n, tphys))[0]
     _, Es = GS.dmrg2_(psi, H, trunc=B.TruncationScheme(maxdim=4096), nsweeps=6)
     assert abs(Es[-1] - e_singlet) < 1e-10, (Es, e_singlet)
+
+
+def test_su2_dmrg2_with_a_target_spin(g):
+    """A root leg of total spin S: on star(3,1) with spin 1/2 the ground state
+    is the triplet, E = -5/4 J exactly (the root spin against leaf total spin
+    3/2), and dmrg2_ at S = 1 reaches it; S = 0 gives -3/4 J."""
+    from graft_amd import backend as B, contractions as C, groundstate as GS, su2, trees as Tr
+    topo = Tr.star_topology(3, 1)
+    for target, e_exact in ((2, -1.25), (0, -0.75)):
+        _say("su2 dmrg2 star31 target spin", target, "/2")
+        rng = np.random.default_rng(31)
+        psi = su2.random_ttns(rng, topo, 1, 1, target=target)
+        H = su2.heisenberg_ttno(topo)
+        _, Es = GS.dmrg2_(psi, H, trunc=B.TruncationScheme(maxdim=64), nsweeps=4)
+        assert abs(Es[-1] - e_exact) < 1e-10, (target, Es)
+        assert abs(C.expect(psi, H) - e_exact) < 1e-10
