@@ -641,6 +641,42 @@ def su2_apply_bench(tphys=2, maxdeg=24, maxspin=10, reps=20, dense=True, seed=0)
     return r
 
 
+def su2_sweep_bench(L=32, D=256, nsweeps=3, dense=True, seed=1):
+    """dmrg2_ sweeps of the spin-1/2 Heisenberg chain of L sites (singlet
+    ground state) at maxdim D: SU(2) blocks against plain dense FP64 maps,
+    the same number of sweeps from random states; seconds per sweep."""
+    import torch
+    import __graft_entry__
+    __graft_entry__.load_package()
+    from graft_amd import backend as B, groundstate as GS, networks as N, operators as O, su2
+    from graft_amd import trees as Tr
+    topo = Tr.mps_topology(L)
+    out = {"L": L, "D": D}
+    for kind in (("su2", "dense") if dense else ("su2",)):
+        rng = np.random.default_rng(seed)
+        if kind == "su2":
+            psi = su2.random_ttns(rng, topo, 1, 1, maxspin=4)
+            H = su2.heisenberg_ttno(topo)
+        else:
+            psi = N.random_ttns(rng, topo, 2, 16, center=topo.root)
+            H = O.heisenberg_ttno(topo)
+            H.ishermitian = True
+        secs, es = [], []
+        for _ in range(nsweeps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            _, Es = GS.dmrg2_(psi, H, trunc=B.TruncationScheme(maxdim=D), nsweeps=1)
+            torch.cuda.synchronize()
+            secs.append(time.perf_counter() - t)
+            es.append(Es[-1])
+        out[kind] = {"sweep_s": secs, "energies": es,
+                     "max_bond": max(psi.tensors[c].dims[-1] for c in range(L) if topo.parent[c] >= 0)}
+    if dense:
+        out["speedup_last_sweep"] = out["dense"]["sweep_s"][-1] / out["su2"]["sweep_s"][-1]
+        out["energy_diff"] = abs(out["dense"]["energies"][-1] - out["su2"]["energies"][-1])
+    return out
+
+
 def blas_threads():
     try:
         from threadpoolctl import threadpool_info
@@ -672,7 +708,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the heuristic tile configurations (default: autotune every step)")
     ap.add_argument("--mode", choices=["apply", "sweep", "hubbard", "hubbard-apply", "hubbard-tdvp",
-                                       "precision", "su2-apply"],
+                                       "precision", "su2-apply", "su2-sweep"],
                     default="apply")
     ap.add_argument("--prec-D", type=int, default=64)
     ap.add_argument("--prec-steps", type=int, default=10)
@@ -727,6 +763,14 @@ def main():
                           "unit": "ms", "higher_is_better": False, "n_gpus": 1,
                           "config": {"workload": f"eff_h2 root edge, spin-1 Heisenberg 16-site tree, "
                                                  f"SU(2), multiplet degeneracy <= {r['maxdeg']}"},
+                          "detail": r}), flush=True)
+        return
+    if args.mode == "su2-sweep":
+        r = su2_sweep_bench(D=args.sweep_D, nsweeps=args.sweeps, dense=not args.no_dense)
+        print(json.dumps({"metric": "dmrg2 seconds per sweep, SU(2) blocks vs plain dense FP64",
+                          "value": r["su2"]["sweep_s"][-1], "unit": "s", "higher_is_better": False,
+                          "n_gpus": 1, "config": {"workload": f"dmrg2_ sweeps, spin-1/2 Heisenberg chain, "
+                                                              f"L={r['L']}, maxdim {r['D']}"},
                           "detail": r}), flush=True)
         return
     if args.mode == "hubbard-tdvp":
@@ -914,6 +958,16 @@ def main():
                 "step_ms": r["step_ms"]}
         print(f"# su2 apply {time.perf_counter() - t_su:.1f} s", file=sys.stderr, flush=True)
     out["su2_apply"] = su2r
+    su2s = None
+    if not args.no_su2_apply:
+        t_su = time.perf_counter()
+        r = su2_sweep_bench(D=256, nsweeps=3)
+        su2s = {"workload": "dmrg2_ sweeps, spin-1/2 Heisenberg chain, L=32, maxdim 256, 3 sweeps each",
+                "su2_sweep_s": r["su2"]["sweep_s"], "plain_dense_fp64_sweep_s": r["dense"]["sweep_s"],
+                "speedup_last_sweep": r["speedup_last_sweep"], "energy_su2": r["su2"]["energies"][-1],
+                "energy_dense": r["dense"]["energies"][-1], "energy_diff": r["energy_diff"]}
+        print(f"# su2 sweeps {time.perf_counter() - t_su:.1f} s", file=sys.stderr, flush=True)
+    out["su2_sweep"] = su2s
     print(json.dumps(out), flush=True)
 
 
