@@ -238,6 +238,8 @@ struct StepExec {
   // Lean form of the per-block streaming step (small operand staged in LDS)
   bool lean = false;
   int lean_smax = 0;                  // largest small-side extent of an output block
+  int lean_rank = 0;                  // most big-side legs of extent > 1 of an output block
+  const char* lean_why = "";          // why a streaming step has no lean records (plan timing)
   int64_t small_total = 0;            // elements of the small operand
   std::vector<LeanItem> litems;
   std::vector<LeanSeg> lsegs;
@@ -906,23 +908,26 @@ static void build_lean(StepExec& ex) {
   ex.litems.clear();
   ex.lsegs.clear();
   ex.lwork.clear();
-  auto fail = [&ex]() {
+  auto fail = [&ex](const char* why) {
     ex.litems.clear();
     ex.lsegs.clear();
     ex.lwork.clear();
+    ex.lean_why = why;
   };
+  ex.lean_why = "";
   if (ex.bstream == 0 || ex.items.empty()) return;
   const bool bigA = ex.bstream == 1;
   int smax = 1;
   for (const GemmItem& it : ex.items) {
     const int S = bigA ? it.N : it.M;
-    if (S > kLeanMaxS) return;
+    if (S > kLeanMaxS) return fail("small side > 16");
     smax = std::max(smax, S);
   }
   for (const GemmSeg& sg : ex.segs)
-    if (sg.K > kLeanInlineK) return;
+    if (sg.K > kLeanInlineK) return fail("K > 8");
   ex.litems.reserve(ex.items.size());
   ex.lsegs.reserve(ex.segs.size());
+  int lrank = 0;
   for (int32_t item_id : ex.sorder) {
     const GemmItem& it = ex.items[item_id];
     LeanItem li;
@@ -942,10 +947,12 @@ static void build_lean(StepExec& ex) {
     int keep[GL], nk = 0;
     for (int l = 0; l < brank; ++l) {
       if (bdims[l] == 1) continue;
-      if ((uint64_t)J * (uint64_t)bdims[l] >= (1ull << 32) || nk == kLeanRank) return fail();
+      if ((uint64_t)J * (uint64_t)bdims[l] >= (1ull << 32) || nk == kLeanRank)
+        return fail("big-side legs");
       keep[nk++] = l;
     }
     li.rank = nk;
+    lrank = std::max(lrank, nk);
     for (int q = 0; q < nk; ++q) {
       const uint32_t d = (uint32_t)bdims[keep[q]];
       li.dims[q] = d;
@@ -995,34 +1002,44 @@ static void build_lean(StepExec& ex) {
     }
     ex.litems.push_back(li);
   }
-  // every block of a group has the big-side legs of its first block, so one
-  // row decode serves the group
+  // Work entries share one row decode, so the blocks of an entry must have
+  // the big-side legs of its first block: a group whose blocks merged their
+  // legs differently (the merge follows each block's own contributions) is
+  // cut into runs of equal legs.
+  std::vector<int32_t> lpanel;
   for (size_t g = 0; g + 1 < ex.spanel.size(); ++g) {
     const int b = ex.spanel[g], e = ex.spanel[g + 1];
-    const LeanItem& f = ex.litems[b];
+    int f = b;
+    lpanel.push_back(b);
     for (int c = b + 1; c < e; ++c) {
+      const LeanItem& x = ex.litems[f];
       const LeanItem& o = ex.litems[c];
-      bool same = o.J == f.J && o.rank == f.rank;
-      for (int q = 0; same && q < f.rank; ++q) same = o.dims[q] == f.dims[q];
-      if (!same) return fail();
+      bool same = o.J == x.J && o.rank == x.rank;
+      for (int q = 0; same && q < x.rank; ++q) same = o.dims[q] == x.dims[q];
+      if (!same) {
+        lpanel.push_back(c);
+        f = c;
+      }
     }
   }
+  lpanel.push_back((int32_t)ex.litems.size());
   // One block per entry unless that makes more than kLeanWalkAbove entries:
-  // walking the blocks of a group one after another trades memory-level
+  // walking the blocks of a run one after another trades memory-level
   // parallelism for cache reuse, which pays only on very large steps.
   int64_t single = 0;
-  for (size_t g = 0; g + 1 < ex.spanel.size(); ++g)
-    single += (int64_t)((ex.litems[ex.spanel[g]].J + 64 * kLeanRows - 1) / (64 * kLeanRows)) *
-              (ex.spanel[g + 1] - ex.spanel[g]);
+  for (size_t g = 0; g + 1 < lpanel.size(); ++g)
+    single += (int64_t)((ex.litems[lpanel[g]].J + 64 * kLeanRows - 1) / (64 * kLeanRows)) *
+              (lpanel[g + 1] - lpanel[g]);
   const int panel = (int)std::max<int64_t>(1, (single + kLeanWalkAbove - 1) / kLeanWalkAbove);
-  for (size_t g = 0; g + 1 < ex.spanel.size(); ++g) {
-    const int b = ex.spanel[g], e = ex.spanel[g + 1];
+  for (size_t g = 0; g + 1 < lpanel.size(); ++g) {
+    const int b = lpanel[g], e = lpanel[g + 1];
     const int J = ex.litems[b].J;
     for (int j0 = 0; j0 < J; j0 += 64 * kLeanRows)
       for (int c = b; c < e; c += panel) ex.lwork.push_back({c, std::min(panel, e - c), j0, 0});
   }
-  if (ex.lwork.size() >= (1u << 30)) return fail();
+  if (ex.lwork.size() >= (1u << 30)) return fail("work entries");
   ex.lean_smax = smax;
+  ex.lean_rank = lrank;
   ex.lean = true;
 }
 
@@ -2369,10 +2386,12 @@ static int plan_create_impl(graft_ctx* ctx, const graft_pair_step* steps, int32_
     }
     p.macs += ex.macs;
     if (timing)
-      std::fprintf(stderr, "plan step %d: %.2f ms, blocks A %zu B %zu C %zu, items %zu, fused %d\n", s,
-                   std::chrono::duration<double, std::milli>(now() - t_step).count(),
-                   p.slots[st.a].blocks.size(), p.slots[st.b].blocks.size(),
-                   p.slots[st.dst].blocks.size(), ex.items.size(), (int)ex.fused);
+      std::fprintf(stderr, "plan step %d: %.2f ms, blocks A %zu B %zu C %zu, items %zu, fused %d, "
+                 "stream %d lean %d (rank %d) %s\n", s,
+                 std::chrono::duration<double, std::milli>(now() - t_step).count(),
+                 p.slots[st.a].blocks.size(), p.slots[st.b].blocks.size(),
+                 p.slots[st.dst].blocks.size(), ex.items.size(), (int)ex.fused, ex.bstream,
+                 (int)ex.lean, ex.lean_rank, ex.lean_why);
   }
   GRAFT_REQUIRE(defined[output_slot], "output slot never produced");
   for (int s = ninputs; s < nslots; ++s)
