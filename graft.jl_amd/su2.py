@@ -979,3 +979,179 @@ def move_center_edge_(psi, m, cache=None):
     if cache is not None:
         cache.invalidate_edge_(n, m)
     return psi
+
+
+# ---------------------------------------------------------------------------
+# Link splits of the one-site TDVP sweep
+# ---------------------------------------------------------------------------
+
+
+def _identity_block_scale(tj, duals):
+    """X of the two-leg tree (j, j) is c * identity; returns c."""
+    X = tree_tensor((tj, tj), (), duals)
+    c = X[0, 0]
+    if np.abs(X - c * np.eye(tj + 1)).max() > 1e-14:
+        raise RuntimeError("two-leg fusion tree is not proportional to the identity")
+    return c
+
+
+def identity(leg):
+    """The identity on ``leg`` as a two-leg tensor on (leg*, leg)."""
+    st = SU2Structure.invariant([leg.dual(), leg])
+    flat = np.zeros(st.total, dtype=np.complex128)
+    for b, (ir, _) in enumerate(st.blocks):
+        tj, d = leg.irreps[ir[1]]
+        o = int(st.offsets[b])
+        flat[o:o + st.sizes[b]] = (np.eye(d) / _identity_block_scale(tj, st.duals)).reshape(-1, order="F")
+    return to_device(st, flat, 1)
+
+
+def _move_leg(A, src, dst):
+    """A with leg ``src`` moved to position ``dst`` (a contraction with the identity)."""
+    if src == dst:
+        return A
+    r = A.rank
+    I = identity(A.blocks.legs[src])
+    order = [j for j in range(r) if j != src]
+    order.insert(dst, None)
+    # intermediate legs: A's legs but src in order, then the identity's leg
+    perm = tuple(r - 1 if j is None else (j if j < src else j - 1) for j in order)
+    legs = [A.blocks.legs[j] for j in range(r) if j != src] + [A.blocks.legs[src]]
+    legs = [legs[p] for p in perm]
+    return contract(A, (src,), I, (0,), perm=perm, ncod=A.ncod, out_structure=SU2Structure.invariant(legs))
+
+
+def _qr_last(A):
+    """A = Q . (R (x) 1) on its last leg: per irrep J of that leg the blocks
+    stacked over (irrep tuple, tree, degeneracies) are QR-factored; Q takes
+    sqrt(2J + 1) times the orthonormal factor (a dense isometry onto the new
+    bond, which keeps the old leg's irreps and direction) and R the triangular
+    factor over sqrt(2J + 1), so that densely A = Q (R (x) 1).  Returns (Q, R
+    by irrep of the new bond, new bond leg)."""
+    st = A.blocks
+    host = A.data.detach().cpu().numpy()
+    last = len(st.legs) - 1
+    byJ = {}
+    for b, (ir, _) in enumerate(st.blocks):
+        byJ.setdefault(ir[last], []).append(b)
+    Qs, Rs, newirr = {}, {}, []
+    for s in sorted(byJ):
+        tj, d = st.legs[last].irreps[s]
+        blist = byJ[s]
+        rows = [st.sizes[b] // d for b in blist]
+        M = np.concatenate([host[st.offsets[b]:st.offsets[b] + st.sizes[b]].reshape(rows[i], d, order="F")
+                            for i, b in enumerate(blist)])
+        Q, R = np.linalg.qr(M)
+        Qs[tj] = (blist, rows, Q * math.sqrt(tj + 1))
+        Rs[tj] = R / math.sqrt(tj + 1)           # the blocks' sqrt(2J + 1) moved into Q
+        newirr.append((tj, Q.shape[1]))
+    old = st.legs[last]
+    new = SU2Leg(newirr, old.isdual)
+    legs = list(st.legs[:last]) + [new]
+    Sq = SU2Structure.invariant(legs)
+    fq = np.zeros(Sq.total, dtype=np.complex128)
+    for b, (ir, tr) in enumerate(Sq.blocks):
+        tj = new.irreps[ir[last]][0]
+        blist, rows, Q = Qs[tj]
+        key = (ir[:last], tr)
+        src = [i for i, bb in enumerate(blist) if (st.blocks[bb][0][:last], st.blocks[bb][1]) == key]
+        if not src:
+            continue
+        i = src[0]
+        o = int(Sq.offsets[b])
+        r0 = sum(rows[:i])
+        fq[o:o + Sq.sizes[b]] = Q[r0:r0 + rows[i]].reshape(-1, order="F")
+    return to_device(Sq, fq, A.ncod), Rs, new
+
+
+def _link_tensor(structure, mats):
+    """Two-leg tensor on ``structure`` whose block of irrep j is mats[j] / c_j
+    (dense: mats[j] (x) identity on the multiplet)."""
+    flat = np.zeros(structure.total, dtype=np.complex128)
+    for b, (ir, _) in enumerate(structure.blocks):
+        tj = structure.legs[0].irreps[ir[0]][0]
+        m = mats.get(tj)
+        if m is None:
+            continue
+        o = int(structure.offsets[b])
+        blk = m / _identity_block_scale(tj, structure.duals)
+        if blk.shape != structure.block_dims[b]:
+            raise ValueError("link tensor block does not match the map's structure")
+        flat[o:o + structure.sizes[b]] = blk.reshape(-1, order="F")
+    return to_device(structure, flat, 1)
+
+
+def _link_matrices(C):
+    """The per-irrep matrices of a two-leg tensor (inverse of _link_tensor)."""
+    st = C.blocks
+    host = C.data.detach().cpu().numpy()
+    out = {}
+    for b, (ir, _) in enumerate(st.blocks):
+        tj = st.legs[0].irreps[ir[0]][0]
+        o = int(st.offsets[b])
+        out[tj] = host[o:o + st.sizes[b]].reshape(st.block_dims[b], order="F") * \
+            _identity_block_scale(tj, st.duals)
+    return out
+
+
+def _absorb_on_leg(P, k, mats):
+    """Leg k of P multiplied by mats[j] on the degeneracy index of irrep j
+    (new index first): P'[.., i, ..] = sum_d mats[j][i, d] P[.., d, ..]."""
+    st = P.blocks
+    host = P.data.detach().cpu().numpy()
+    leg = st.legs[k]
+    newirr = [(tj, mats[tj].shape[0]) for tj, _ in leg.irreps if tj in mats]
+    nleg = SU2Leg(newirr, leg.isdual)
+    legs = list(st.legs)
+    legs[k] = nleg
+    Sn = SU2Structure.invariant(legs)
+    fn = np.zeros(Sn.total, dtype=np.complex128)
+    for b, (ir, tr) in enumerate(Sn.blocks):
+        tj = nleg.irreps[ir[k]][0]
+        old_s = [s for s, (t, _) in enumerate(leg.irreps) if t == tj][0]
+        src = st.index.get((ir[:k] + (old_s,) + ir[k + 1:], tr))
+        if src is None:
+            continue
+        o = int(st.offsets[src])
+        blk = host[o:o + st.sizes[src]].reshape(st.block_dims[src], order="F")
+        blk = np.moveaxis(np.tensordot(mats[tj], blk, axes=([1], [k])), 0, k)
+        on = int(Sn.offsets[b])
+        fn[on:on + Sn.sizes[b]] = blk.reshape(-1, order="F")
+    return to_device(Sn, fn, P.ncod)
+
+
+def evolve_link_and_move_(ev, psi, H, u, v, dz, herm, exponentiate):
+    """The link step of the one-site TDVP sweep (tdvp.jl:257-301) on SU(2)
+    states: the centre tensor is QR-split on the edge's leg per irrep (moved to
+    the last position by a contraction with the identity when it is a child
+    slot), the link tensor evolves backwards under the zero-site map and
+    multiplies into the neighbour."""
+    from .contractions import eff_h0
+    from .trees import childslot
+    t = psi.topo
+    cache = ev.cache
+    kw = dict(ishermitian=herm, krylovdim=ev.krylovdim, tol=ev.tol, eager=ev.eager)
+    A = psi.tensors[u]
+    if t.parent[u] == v:
+        Q, R, _ = _qr_last(A)                              # A = Q . (R (x) 1) on the parent leg
+        psi.tensors[u] = Q
+        cache.invalidate_node_(u)
+        k0 = eff_h0(cache, psi, H, u, v, precision=ev.precision)
+        C = _link_tensor(k0.blocks, R)                     # (new, old*)
+        C, _ = exponentiate(k0, -dz, C, **kw)
+        psi.tensors[v] = _absorb_on_leg(psi.tensors[v], childslot(t, v, u), _link_matrices(C))
+    else:
+        k = childslot(t, u, v)
+        Al = _move_leg(A, k, A.rank - 1)
+        Ql, R, _ = _qr_last(Al)
+        psi.tensors[u] = _move_leg(Ql, Ql.rank - 1, k)
+        cache.invalidate_node_(u)
+        psi.center = u
+        k0 = eff_h0(cache, psi, H, v, u, precision=ev.precision)
+        C = _link_tensor(k0.blocks, {tj: r.T for tj, r in R.items()})     # (old, new*)
+        C, _ = exponentiate(k0, -dz, C, **kw)
+        P = psi.tensors[v]
+        psi.tensors[v] = _absorb_on_leg(P, P.rank - 1, {tj: m.T for tj, m in _link_matrices(C).items()})
+    psi.center = v
+    cache.invalidate_node_(v)
+    return psi

@@ -229,3 +229,87 @@ def test_su2_dmrg2_with_a_target_spin(g):
         _, Es = GS.dmrg2_(psi, H, trunc=B.TruncationScheme(maxdim=64), nsweeps=4)
         assert abs(Es[-1] - e_exact) < 1e-10, (target, Es)
         assert abs(C.expect(psi, H) - e_exact) < 1e-10
+
+
+def test_su2_link_split_pieces(g):
+    """Moving a leg (a contraction with the identity) permutes the dense tensor;
+    the per-irrep QR of the last leg gives a dense isometry and rebuilds it."""
+    from graft_amd import su2
+    _say("su2 leg move and per-irrep QR")
+    rng = np.random.default_rng(37)
+    legs = [_leg([(1, 2), (3, 1)]), _leg([(1, 1)]), _leg([(0, 2), (2, 3)], True)]
+    st = su2.SU2Structure.invariant(legs)
+    A = su2.to_device(st, su2.random_flat(rng, st), 2)
+    D = A.to_numpy()
+    for src, dst in ((0, 2), (2, 0), (1, 2)):
+        M = su2._move_leg(A, src, dst)
+        assert _rel(M.to_numpy(), np.moveaxis(D, src, dst)) < 1e-13, (src, dst)
+    Q, R, new = su2._qr_last(A)
+    Qd = Q.to_numpy()
+    Qm = Qd.reshape(-1, Qd.shape[-1], order="F")
+    assert np.abs(Qm.conj().T @ Qm - np.eye(Qm.shape[1])).max() < 1e-12
+    # A = Q . (R (x) 1) on the last leg
+    Rd = np.zeros((new.dim, legs[2].dim), dtype=complex)
+    ro, co = new.offsets(), legs[2].dual().offsets()
+    for s, (tj, dn) in enumerate(new.irreps):
+        so = [i for i, (t, _) in enumerate(legs[2].irreps) if t == tj][0]
+        Rd[ro[s]:ro[s] + dn * (tj + 1), co[so]:co[so] + legs[2].irreps[so][1] * (tj + 1)] = \
+            np.kron(R[tj], np.eye(tj + 1))
+    assert _rel(np.tensordot(Qd, Rd, axes=([2], [0])), D) < 1e-12
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_su2_tdvp1_is_exact_at_full_bonds(g, order):
+    """TDVP1 on SU(2) states (link splits by per-irrep QR): on star(3,1) at full
+    bonds every leaf step and the link step after it cancel and the root step
+    is the whole propagator, so the state equals exp(dz H) of the dense
+    Heisenberg matrix on the embedded vector (1e-9), norm and energy kept."""
+    from graft_amd import contractions as C, evolution as EV, su2, trees as Tr
+    topo = Tr.star_topology(3, 1)
+    t = _tree(topo)
+    _say("su2 TDVP1 order", order, "star31 full bonds")
+    rng = np.random.default_rng(41)
+    psi = su2.random_ttns(rng, topo, 1)
+    H = su2.heisenberg_ttno(topo)
+    H.ishermitian = True
+    Hd = ref.ttno_to_dense(t, [w.to_numpy() for w in H.tensors])
+    v0 = ref.ttns_to_dense(t, [a.to_numpy() for a in psi.tensors], list(psi.hasphys))
+    vex = ref.exact_evolve(Hd, v0, -0.15j)
+    assert np.linalg.norm(vex - v0) > 1e-2
+    e0 = C.expect(psi, H)
+    ev = EV.TDVP1(order=order)
+    for _ in range(3):
+        EV.step_(ev, psi, H, -0.05j)
+    v = ref.ttns_to_dense(t, [a.to_numpy() for a in psi.tensors], list(psi.hasphys))
+    assert np.linalg.norm(v - vex) < 1e-9, np.linalg.norm(v - vex)
+    assert abs(np.linalg.norm(v) - np.linalg.norm(v0)) < 1e-10
+    assert abs(C.expect(psi, H) - e0) < 1e-9
+
+
+def test_su2_tdvp1_matches_dense_tdvp1_at_truncated_bonds(g):
+    """Truncated bonds (depth-2 binary tree, spin 1, degeneracies <= 2): the
+    SU(2) TDVP1 and the plain dense TDVP1 on the embedded tensors follow the
+    same trajectory (H is SU(2) invariant, so the dense tangent-space flow
+    stays in the symmetric states), to 1e-9; norm and energy are conserved."""
+    from graft_amd import contractions as C, evolution as EV, networks as N, su2, tensors as T
+    from graft_amd import trees as Tr
+    topo = Tr.binary_topology(2)
+    t = _tree(topo)
+    _say("su2 vs dense TDVP1 binary2 spin 1 maxdeg 2")
+    rng = np.random.default_rng(43)
+    psi = su2.random_ttns(rng, topo, 2, 2)
+    H = su2.heisenberg_ttno(topo, 1.0, 2)
+    H.ishermitian = True
+    psid = N.TTNS(topo, [T.DTensor.from_numpy(a.to_numpy(), a.ncod) for a in psi.tensors], psi.center,
+                  psi.hasphys)
+    Hd = N.TTNO(topo, [T.DTensor.from_numpy(w.to_numpy(), w.ncod) for w in H.tensors], ishermitian=True)
+    e0 = C.expect(psi, H)
+    for p, h in ((psi, H), (psid, Hd)):
+        ev = EV.TDVP1(order=2)
+        for _ in range(2):
+            EV.step_(ev, p, h, -0.05j)
+    v = ref.ttns_to_dense(t, [a.to_numpy() for a in psi.tensors], list(psi.hasphys))
+    vd = ref.ttns_to_dense(t, [a.to_numpy() for a in psid.tensors], list(psid.hasphys))
+    assert np.linalg.norm(v - vd) < 1e-9, np.linalg.norm(v - vd)
+    assert abs(np.linalg.norm(v) - 1.0) < 1e-10
+    assert abs(C.expect(psi, H) - e0) < 1e-9
