@@ -150,7 +150,9 @@ def fusion_trees(tjs):
     return tuple(out)
 
 
-@functools.lru_cache(maxsize=1 << 16)
+# Structural tensors hold prod(2j + 1) entries: the caches are bounded by
+# count (large spins on many legs make single entries large).
+@functools.lru_cache(maxsize=4096)
 def tree_tensor(tjs, labels, duals):
     """Structural tensor X_t (real, unit Frobenius norm) of one fusion tree."""
     n = len(tjs)
@@ -171,7 +173,7 @@ def tree_tensor(tjs, labels, duals):
     return X
 
 
-@functools.lru_cache(maxsize=1 << 14)
+@functools.lru_cache(maxsize=1024)
 def tree_basis(tjs, duals):
     """(trees, matrix [ntrees, prod(2j+1)]) of all trees of one irrep tuple."""
     trees = fusion_trees(tjs)
@@ -369,7 +371,7 @@ def to_device(structure, flat, ncod=None):
 # ---------------------------------------------------------------------------
 
 _COEF_CACHE = {}
-_COEF_CACHE_MAX = 1 << 18
+_COEF_CACHE_MAX = 1 << 16
 COEF_TOL = 1e-13
 
 
