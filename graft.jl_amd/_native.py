@@ -271,6 +271,16 @@ class Context:
 _default_ctx = None
 
 
+def skinny_config():
+    """Configuration number of the single-block streaming kernel (after the tiles)."""
+    return lib().graft_num_tile_configs() - 2
+
+
+def block_stream_config():
+    """Configuration number of the per-block streaming kernels."""
+    return lib().graft_num_tile_configs() - 1
+
+
 def default_context():
     """Process-wide context on the current device and torch stream."""
     global _default_ctx

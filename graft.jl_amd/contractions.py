@@ -311,7 +311,7 @@ def _step_signature(info):
     rounded to a power of two.  Sweeps meet a new block structure on nearly
     every bond update; steps of the same class take the same tiles."""
     lg = lambda v: int(round(math.log2(max(int(v), 1))))
-    return (info["config"] if info["config"] >= 19 else -1, lg(info["m"]), lg(info["k"]),
+    return (info["config"] if info["config"] >= nat.skinny_config() else -1, lg(info["m"]), lg(info["k"]),
             lg(info["n"]), lg(info["blocks"]), lg(info["elements"][2]))
 
 

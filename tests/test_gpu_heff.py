@@ -359,7 +359,7 @@ def test_split_words_bf16x3(g):
             forced = []
             for i in range(len(steps)):
                 try:
-                    h2.native.set_tile_config(i, 19)     # the streaming kernel, where eligible
+                    h2.native.set_tile_config(i, nat.skinny_config())   # the streaming kernel, where eligible
                     forced.append(i)
                 except nat.GraftError:
                     pass

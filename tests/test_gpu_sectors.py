@@ -365,7 +365,7 @@ def test_lean_stream_equals_general_kernel(g):
             _say("lean streaming vs general kernel, fusion mode", mode, "16-site D=48")
             ctx.set_sector_fusion(mode)
             h2 = C.eff_h2(C.EnvCache(topo), psi, Hs, 1, 0, out_structure=theta.blocks)
-            assert any(s["config"] == 20 for s in h2.native.step_info())
+            assert any(s["config"] == nat.block_stream_config() for s in h2.native.step_info())
             ctx.set_lean_stream(0)
             y0 = h2(theta).data.clone()
             ctx.set_lean_stream(1)
