@@ -933,11 +933,13 @@ def main():
     sector = None
     if not args.no_sector_apply:
         # the sector-batched apply at the batching configuration's size (D=1024,
-        # U(1) x U(1)), heuristic tiles, reference-model FLOPs
+        # U(1) x U(1)): heuristic tiles, then the library's autotuner;
+        # reference-model FLOPs
         t_sa = time.perf_counter()
-        r = hubbard_apply_bench(1024, autotune=False, lanczos=False)
+        r = hubbard_apply_bench(1024, autotune=True, lanczos=False)
         sector = {"workload": "eff_h2 root edge, Hubbard 16-site tree, U(1)xU(1), D=1024",
-                  "ms_per_apply": r["ms_per_apply"], "step_ms": r["step_ms"],
+                  "ms_per_apply": r["ms_per_apply"], "ms_per_apply_heuristic_tiles": r["ms_per_apply_untuned"],
+                  "autotune_s": r["autotune_s"], "step_ms": r["step_ms"],
                   "flops_reference_model": r["flops_reference_model"],
                   "tflops_reference_model": r["tflops_reference_model"],
                   "fp64_mfma_fraction_reference_model": r["fp64_mfma_fraction_reference_model"],
