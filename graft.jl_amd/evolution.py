@@ -511,11 +511,10 @@ def _evolve_link_and_move_(ev, psi, H, u, v, dz, herm):
     """tdvp.jl:257-288 with the QR splits of tdvp.jl:291-301."""
     if psi.tensors[u].blocks is not None:
         if hasattr(psi.tensors[u].blocks, "dense_from_flat"):
-            if isinstance(ev, TDVP1_CBE) and ev.enabled:
-                raise NotImplementedError("TDVP1_CBE: bond expansion of SU(2) states is not built")
             from . import su2
             return su2.evolve_link_and_move_(ev, psi, H, u, v, dz, herm,
-                                             evolution_exponentiate_backend)
+                                             evolution_exponentiate_backend,
+                                             cbe=isinstance(ev, TDVP1_CBE) and ev.enabled)
         return _evolve_sector_link_(ev, psi, H, u, v, dz, herm)
     t = psi.topo
     cache = ev.cache

@@ -843,6 +843,32 @@ def _scale_axis(structure, flat, axis, scales):
     return out
 
 
+def _cut_key(st, b, pn):
+    """Block b of ``st`` seen across the cut after its first pn legs (whose
+    fusion trees pass through the coupled spin J of those legs): (J, row key =
+    (their irreps, their internal tree labels), column key, rows, columns)."""
+    ir, tr = st.blocks[b]
+    J = tr[pn - 2] if pn >= 2 else st.spins(b)[0]
+    rk = (ir[:pn], tr[:max(pn - 2, 0)])
+    ck = (ir[pn:], tr[pn - 1:] if pn >= 2 else tr)
+    nr = int(np.prod(st.block_dims[b][:pn]))
+    nc = int(np.prod(st.block_dims[b][pn:])) if len(ir) > pn else 1
+    return J, rk, ck, nr, nc
+
+
+def _cut_pieces(st, host, pn):
+    """Every block of a tensor as its matrix across the cut after pn legs:
+    {(J, row key, column key): matrix}, with the row and column sizes per J."""
+    rows, cols, pieces = {}, {}, {}
+    for b in range(len(st.blocks)):
+        J, rk, ck, nr, nc = _cut_key(st, b, pn)
+        rows.setdefault(J, {}).setdefault(rk, nr)
+        cols.setdefault(J, {}).setdefault(ck, nc)
+        o = int(st.offsets[b])
+        pieces[(J, rk, ck)] = host[o:o + st.sizes[b]].reshape(nr, nc, order="F")
+    return rows, cols, pieces
+
+
 def split_two_site_(psi, theta, n, m, trunc_rank, center_on="n"):
     """split_two_site! (effective.jl:852-876) for SU(2) states.
 
@@ -865,17 +891,7 @@ def split_two_site_(psi, theta, n, m, trunc_rank, center_on="n"):
     pn = An.ncod
     st = theta.blocks
     host = theta.data.detach().cpu().numpy()
-    rows, cols, pieces = {}, {}, {}
-    for b, (ir, tr) in enumerate(st.blocks):
-        tjs = st.spins(b)
-        J = tr[pn - 2] if pn >= 2 else tjs[0]
-        rk, ck = (ir[:pn], tr[:max(pn - 2, 0)]), (ir[pn:], tr[pn - 1:] if pn >= 2 else tr)
-        nr = int(np.prod(st.block_dims[b][:pn]))
-        nc = int(np.prod(st.block_dims[b][pn:])) if len(ir) > pn else 1
-        rows.setdefault(J, {}).setdefault(rk, nr)
-        cols.setdefault(J, {}).setdefault(ck, nc)
-        o = int(st.offsets[b])
-        pieces[(J, rk, ck)] = host[o:o + st.sizes[b]].reshape(nr, nc, order="F")
+    rows, cols, pieces = _cut_pieces(st, host, pn)
     svd = {}
     spectrum = []
     for J in rows:
@@ -938,15 +954,14 @@ def split_two_site_(psi, theta, n, m, trunc_rank, center_on="n"):
         # (every kept dense singular value set to one): conj(A_n) . Theta' =
         # Vh exactly, with no division by small singular values
         fp = np.zeros(st.total, dtype=np.complex128)
-        for b, (ir, tr) in enumerate(st.blocks):
-            tjs = st.spins(b)
-            J = tr[pn - 2] if pn >= 2 else tjs[0]
+        for b in range(len(st.blocks)):
+            J, rk, ck, _, _ = _cut_key(st, b, pn)
             d = keep.get(J)
             if not d:
                 continue
             rkeys, roff, U, _, ckeys, coff, Vh = svd[J]
-            i = rkeys.index((ir[:pn], tr[:max(pn - 2, 0)]))
-            j = ckeys.index((ir[pn:], tr[pn - 1:] if pn >= 2 else tr))
+            i = rkeys.index(rk)
+            j = ckeys.index(ck)
             blk = math.sqrt(J + 1) * (U[roff[i]:roff[i + 1], :d] @ Vh[:d, coff[j]:coff[j + 1]])
             o = int(st.offsets[b])
             fp[o:o + st.sizes[b]] = blk.reshape(-1, order="F")
@@ -1096,21 +1111,27 @@ def _link_matrices(C):
     return out
 
 
-def _absorb_on_leg(P, k, mats):
+def _absorb_on_leg(P, k, mats, irreps=None):
     """Leg k of P multiplied by mats[j] on the degeneracy index of irrep j
-    (new index first): P'[.., i, ..] = sum_d mats[j][i, d] P[.., d, ..]."""
+    (new index first): P'[.., i, ..] = sum_d mats[j][i, d] P[.., d, ..].
+    ``irreps`` (spin, degeneracy) of the new leg, when it carries irreps the
+    old one lacks (their blocks are zero); by default those of ``mats``."""
     st = P.blocks
     host = P.data.detach().cpu().numpy()
     leg = st.legs[k]
-    newirr = [(tj, mats[tj].shape[0]) for tj, _ in leg.irreps if tj in mats]
-    nleg = SU2Leg(newirr, leg.isdual)
+    if irreps is None:
+        irreps = [(tj, mats[tj].shape[0]) for tj, _ in leg.irreps if tj in mats]
+    nleg = SU2Leg(irreps, leg.isdual)
+    old_of = {tj: s for s, (tj, _) in enumerate(leg.irreps)}
     legs = list(st.legs)
     legs[k] = nleg
     Sn = SU2Structure.invariant(legs)
     fn = np.zeros(Sn.total, dtype=np.complex128)
     for b, (ir, tr) in enumerate(Sn.blocks):
         tj = nleg.irreps[ir[k]][0]
-        old_s = [s for s, (t, _) in enumerate(leg.irreps) if t == tj][0]
+        old_s = old_of.get(tj)
+        if old_s is None or tj not in mats:
+            continue
         src = st.index.get((ir[:k] + (old_s,) + ir[k + 1:], tr))
         if src is None:
             continue
@@ -1122,12 +1143,157 @@ def _absorb_on_leg(P, k, mats):
     return to_device(Sn, fn, P.ncod)
 
 
-def evolve_link_and_move_(ev, psi, H, u, v, dz, herm, exponentiate):
+def _permute(T, perm):
+    """T with its legs reordered (leg perm[i] becomes leg i): a contraction of
+    leg 0 with the identity, which recouples the trees."""
+    r = T.rank
+    p = tuple(r - 1 if j == 0 else j - 1 for j in perm)
+    legs = [T.blocks.legs[j] for j in perm]
+    return contract(T, (0,), identity(T.blocks.legs[0]), (0,), perm=p, ncod=T.ncod,
+                    out_structure=SU2Structure.invariant(legs))
+
+
+def cbe_link_split(ev, psi, H, u, v, dz):
+    """_cbe_split for SU(2) states: the predictor selector's exact sector-aware
+    fallback for graded maps (cbe.jl:536-560, 389-399) with the state-preserving
+    splice (cbe.jl:575-584).
+
+    The edge's leg of the centre tensor A_u is moved last (Al).  The projected
+    core Z = (1 - Q Q^H) H_eff2 Theta (1 - conj(P) P^T) is materialized: the
+    two-site map applied to Theta = A_u A_v, the partner's projection as two
+    coupled contractions, and the active side's per coupled spin J of A_u's
+    external legs, where the core is block diagonal (rows = those legs' irreps
+    and trees, the same row basis as Al's blocks).  Its dense spectrum is
+    sigma(Z_J) / sqrt(2J + 1), each (2J + 1) times; whole multiplets are
+    spawned in descending order while they fit the budget min(room, max_add,
+    neigs) in dense dimensions and reach spawn_threshold * ||A_u|| / |dz|.
+    Per J, U_J = [Qh_J, E_J] (Qh_J: the orthonormal QR factor of Al's rows,
+    E_J the selected left vectors, projected against Qh_J again and
+    orthonormalized), stored as sqrt(2J + 1) U_J (a dense isometry); R_J =
+    [R_qr_J; 0] / sqrt(2J + 1), so densely A = U (R (x) 1) and the state is
+    unchanged.  Spins new to the bond have no R block (zero weight).
+
+    Returns (U in Al's leg order, {spin: R_J}, the new bond leg)."""
+    from .contractions import eff_h2
+    from .trees import childslot
+    t = psi.topo
+    A = psi.tensors[u]
+    up = t.parent[u] == v
+    k = A.rank - 1 if up else childslot(t, u, v)
+    Al = A if up else _move_leg(A, k, A.rank - 1)
+    pu = Al.rank - 1
+    ast = Al.blocks
+    arows, _, apieces = _cut_pieces(ast, Al.data.detach().cpu().numpy(), pu)
+    old = ast.legs[-1]
+    strat = ev.cbe
+    qdim = sum((tj + 1) * min(sum(arows[tj].values()), d) for tj, d in old.irreps if tj in arows)
+    room = max(0, ev.trunc.maxdim - qdim)
+    budget = min(room, strat.max_add, strat.neigs)
+    info = {"selected": 0, "budget": budget, "solver": "exact_svd_sector"}
+    ev.last_cbe_info = info
+    # per spin: Al's rows and the QR of its (rows x degeneracy) matrix
+    qr = {}
+    for s, (tj, d) in enumerate(old.irreps):
+        if tj not in arows:
+            continue
+        rk = list(arows[tj])
+        M = np.concatenate([apieces[(tj, r, ((s,), ()))] for r in rk])
+        Qh, R = np.linalg.qr(M)
+        qr[tj] = (rk, [arows[tj][r] for r in rk], Qh, R)
+    zrows, zsvd = {}, {}
+    if budget > 0 and dz != 0:
+        n, m = (u, v) if up else (v, u)
+        theta = two_site_tensor(psi, n, m)
+        Y = eff_h2(ev.cache, psi, H, n, m, out_structure=theta.blocks)(theta)
+        P = psi.tensors[v]
+        kv = childslot(t, v, u) if up else P.rank - 1
+        pv = P.rank - 1
+        if not up:                                   # Theta's legs: v's first
+            Y = _permute(Y, tuple(range(pv, pv + pu)) + tuple(range(pv)))
+        pext = tuple(j for j in range(P.rank) if j != kv)
+        T1 = contract(Y, tuple(range(pu, pu + pv)), P, pext, conjB=True, ncod=pu)
+        T2 = contract(T1, (pu,), P, (kv,), ncod=Y.ncod, out_structure=Y.blocks)
+        zh = (Y.data - T2.data).detach().cpu().numpy()
+        zr, zc, zp = _cut_pieces(Y.blocks, zh, pu)
+        cand = []
+        for J in zr:
+            # rows: Al's keys first (in QR order), then the core's others
+            size = dict(zip(qr[J][0], qr[J][1])) if J in qr else {}
+            size.update(zr[J])
+            rk = list(qr[J][0]) if J in qr else []
+            rk += [r for r in zr[J] if r not in rk]
+            rsz = [size[r] for r in rk]
+            ck = list(zc[J])
+            roff = np.concatenate([[0], np.cumsum(rsz)]).astype(int)
+            coff = np.concatenate([[0], np.cumsum([zc[J][c] for c in ck])]).astype(int)
+            Z = np.zeros((roff[-1], coff[-1]), dtype=np.complex128)
+            for i, r in enumerate(rk):
+                for j, c in enumerate(ck):
+                    pc = zp.get((J, r, c))
+                    if pc is not None:
+                        Z[roff[i]:roff[i + 1], coff[j]:coff[j + 1]] = pc
+            if J in qr:                              # the active side's projection, per J
+                Qh = np.zeros((roff[-1], qr[J][2].shape[1]), dtype=np.complex128)
+                Qh[:qr[J][2].shape[0]] = qr[J][2]
+                Z = Z - Qh @ (Qh.conj().T @ Z)
+            U, sv, _ = np.linalg.svd(Z, full_matrices=False)
+            zrows[J] = (rk, rsz)
+            zsvd[J] = U
+            cand += [(-float(x) / math.sqrt(J + 1), J, i) for i, x in enumerate(sv)]
+        norm_a = float(np.linalg.norm(A.data.detach().cpu().numpy()))
+        cutoff = strat.spawn_threshold * max(norm_a, np.finfo(float).eps) / abs(complex(dz))
+        cand.sort()
+        info["sigma_max"] = -cand[0][0] if cand else 0.0
+        sel, used = {}, 0
+        for negx, J, i in cand:
+            if -negx < cutoff or used + J + 1 > budget:
+                break
+            sel.setdefault(J, []).append(i)
+            used += J + 1
+        info["selected"] = used
+    else:
+        sel = {}
+    # U_J = [Qh_J, E_J] on the rows of both; R_J = [R_qr_J; 0]
+    cols, Rs = {}, {}
+    for J in sorted(set(qr) | set(sel)):
+        rk, rsz = zrows[J] if J in zrows else (qr[J][0], qr[J][1])
+        nrow = int(np.sum(rsz))
+        Qh = np.zeros((nrow, qr[J][2].shape[1] if J in qr else 0), dtype=np.complex128)
+        if J in qr:
+            Qh[:qr[J][2].shape[0]] = qr[J][2]
+        Uj = Qh
+        if J in sel:
+            E = zsvd[J][:, sel[J]]
+            E = E - Qh @ (Qh.conj().T @ E)
+            E, _ = np.linalg.qr(E)
+            Uj = np.concatenate([Qh, E], axis=1)
+        cols[J] = (rk, np.concatenate([[0], np.cumsum(rsz)]).astype(int), Uj)
+        if J in qr:
+            R = qr[J][3]
+            Rs[J] = np.concatenate([R, np.zeros((Uj.shape[1] - R.shape[0], R.shape[1]))]) / math.sqrt(J + 1)
+    new = SU2Leg([(J, cols[J][2].shape[1]) for J in sorted(cols) if cols[J][2].shape[1] > 0],
+                 old.isdual)
+    Su = SU2Structure.invariant(list(ast.legs[:-1]) + [new])
+    fu = np.zeros(Su.total, dtype=np.complex128)
+    for b in range(len(Su.blocks)):
+        J, rk, _, nr, _ = _cut_key(Su, b, pu)
+        rkeys, roff, Uj = cols[J]
+        if rk not in rkeys:
+            continue
+        i = rkeys.index(rk)
+        o = int(Su.offsets[b])
+        fu[o:o + Su.sizes[b]] = (math.sqrt(J + 1) * Uj[roff[i]:roff[i + 1]]).reshape(-1, order="F")
+    return to_device(Su, fu, Al.ncod), Rs, new
+
+
+def evolve_link_and_move_(ev, psi, H, u, v, dz, herm, exponentiate, cbe=False):
     """The link step of the one-site TDVP sweep (tdvp.jl:257-301) on SU(2)
     states: the centre tensor is QR-split on the edge's leg per irrep (moved to
     the last position by a contraction with the identity when it is a child
     slot), the link tensor evolves backwards under the zero-site map and
-    multiplies into the neighbour."""
+    multiplies into the neighbour.  With bond expansion (TDVP1_CBE,
+    cbe.jl:240-261) the split is cbe_link_split's: the neighbour's leg takes
+    the new bond, zero on the spawned directions."""
     from .contractions import eff_h0
     from .trees import childslot
     t = psi.topo
@@ -1135,17 +1301,23 @@ def evolve_link_and_move_(ev, psi, H, u, v, dz, herm, exponentiate):
     kw = dict(ishermitian=herm, krylovdim=ev.krylovdim, tol=ev.tol, eager=ev.eager)
     A = psi.tensors[u]
     if t.parent[u] == v:
-        Q, R, _ = _qr_last(A)                              # A = Q . (R (x) 1) on the parent leg
+        if cbe:                                            # A = U . (R (x) 1), U = [Q, E]
+            Q, R, new = cbe_link_split(ev, psi, H, u, v, dz)
+        else:
+            Q, R, new = _qr_last(A)                        # A = Q . (R (x) 1) on the parent leg
         psi.tensors[u] = Q
         cache.invalidate_node_(u)
         k0 = eff_h0(cache, psi, H, u, v, precision=ev.precision)
         C = _link_tensor(k0.blocks, R)                     # (new, old*)
         C, _ = exponentiate(k0, -dz, C, **kw)
-        psi.tensors[v] = _absorb_on_leg(psi.tensors[v], childslot(t, v, u), _link_matrices(C))
+        psi.tensors[v] = _absorb_on_leg(psi.tensors[v], childslot(t, v, u), _link_matrices(C),
+                                        new.irreps)
     else:
         k = childslot(t, u, v)
-        Al = _move_leg(A, k, A.rank - 1)
-        Ql, R, _ = _qr_last(Al)
+        if cbe:
+            Ql, R, new = cbe_link_split(ev, psi, H, u, v, dz)
+        else:
+            Ql, R, new = _qr_last(_move_leg(A, k, A.rank - 1))
         psi.tensors[u] = _move_leg(Ql, Ql.rank - 1, k)
         cache.invalidate_node_(u)
         psi.center = u
@@ -1153,7 +1325,8 @@ def evolve_link_and_move_(ev, psi, H, u, v, dz, herm, exponentiate):
         C = _link_tensor(k0.blocks, {tj: r.T for tj, r in R.items()})     # (old, new*)
         C, _ = exponentiate(k0, -dz, C, **kw)
         P = psi.tensors[v]
-        psi.tensors[v] = _absorb_on_leg(P, P.rank - 1, {tj: m.T for tj, m in _link_matrices(C).items()})
+        psi.tensors[v] = _absorb_on_leg(P, P.rank - 1, {tj: m.T for tj, m in _link_matrices(C).items()},
+                                        new.irreps)
     psi.center = v
     cache.invalidate_node_(v)
     return psi

@@ -313,3 +313,106 @@ def test_su2_tdvp1_matches_dense_tdvp1_at_truncated_bonds(g):
     assert np.linalg.norm(v - vd) < 1e-9, np.linalg.norm(v - vd)
     assert abs(np.linalg.norm(v) - 1.0) < 1e-10
     assert abs(C.expect(psi, H) - e0) < 1e-9
+
+
+def _dense_copies(psi, H):
+    from graft_amd import networks as N, tensors as T
+    psid = N.TTNS(psi.topo, [T.DTensor.from_numpy(a.to_numpy(), a.ncod) for a in psi.tensors],
+                  psi.center, psi.hasphys)
+    Hd = N.TTNO(H.topo, [T.DTensor.from_numpy(w.to_numpy(), w.ncod) for w in H.tensors],
+                ishermitian=True)
+    return psid, Hd
+
+
+def _dense_link(R, new, old):
+    """The dense (new x old) matrix of the per-irrep blocks R[j] (x) identity."""
+    Rd = np.zeros((new.dim, old.dim), dtype=complex)
+    ro, co = new.offsets(), old.offsets()
+    for s, (tj, dn) in enumerate(new.irreps):
+        so = [i for i, (t, _) in enumerate(old.irreps) if t == tj]
+        if tj in R and so:
+            Rd[ro[s]:ro[s] + dn * (tj + 1), co[so[0]]:co[so[0]] + old.irreps[so[0]][1] * (tj + 1)] = \
+                np.kron(R[tj], np.eye(tj + 1))
+    return Rd
+
+
+@pytest.mark.parametrize("u,v,maxspin", [(1, 0, None), (0, 2, None), (1, 0, 2), (0, 1, 2)])
+def test_su2_cbe_split_matches_the_dense_selector(g, u, v, maxspin):
+    """The SU(2) bond-expansion split of edge u -> v (depth-2 binary tree, spin
+    1): U is a dense isometry with U (R (x) 1) = A (the state is unchanged),
+    and the spawned directions span the same space as the plain dense
+    selector's on the embedded tensors with the same number of directions;
+    the leading singular value of the projected core agrees.  With bond spins
+    capped at 1 the split spawns spins the bond did not carry.  (An edge into
+    a leaf has a zero core here: the leaf's bond spans its whole space.)"""
+    from graft_amd import backend as B, contractions as C, evolution as EV, networks as N, su2
+    from graft_amd import trees as Tr
+    topo = Tr.binary_topology(2)
+    _say("su2 CBE split", u, "->", v, "maxspin", maxspin)
+    rng = np.random.default_rng(53 + u)
+    H = su2.heisenberg_ttno(topo, 1.0, 2)
+    H.ishermitian = True
+    psi = su2.random_ttns(rng, topo, 2, 1, maxspin)
+    N.move_center_(psi, u)
+    psid, Hd = _dense_copies(psi, H)
+    t = topo
+    k = psi.tensors[u].rank - 1 if t.parent[u] == v else Tr.childslot(t, u, v)
+    old = psi.tensors[u].blocks.legs[k]
+    ev = EV.TDVP1_CBE(cbe=EV.PredictorCBE(max_add=24, neigs=24, krylovdim=24),
+                      trunc=B.TruncationScheme(maxdim=10 ** 4))
+    ev.cache = C.EnvCache(topo)
+    U, R, new = su2.cbe_link_split(ev, psi, H, u, v, -0.05j)
+    info = dict(ev.last_cbe_info)
+    _say("selected", info["selected"], "sigma_max", info["sigma_max"], "old", old.irreps, "new", new.irreps)
+    Ud = U.to_numpy()
+    Um = Ud.reshape(-1, Ud.shape[-1], order="F")
+    assert np.abs(Um.conj().T @ Um - np.eye(Um.shape[1])).max() < 1e-12
+    Ad = np.moveaxis(psi.tensors[u].to_numpy(), k, -1)
+    assert _rel(np.tensordot(Ud, _dense_link(R, new, old), axes=([Ud.ndim - 1], [0])), Ad) < 1e-12
+    s = info["selected"]
+    assert s > 0 and new.dim == old.dim + s
+    if maxspin is not None:
+        assert {tj for tj, _ in new.irreps} - {tj for tj, _ in old.irreps}
+    evd = EV.TDVP1_CBE(cbe=EV.PredictorCBE(max_add=s, neigs=s, krylovdim=max(s, 4)),
+                       trunc=B.TruncationScheme(maxdim=10 ** 4))
+    evd.cache = C.EnvCache(topo)
+    Ut, _ = EV._cbe_split(evd, psid, Hd, u, v, -0.05j)
+    assert evd.last_cbe_info["selected"] == s
+    assert abs(evd.last_cbe_info["sigma_max"] - info["sigma_max"]) <= 1e-10 * info["sigma_max"]
+    Dt = np.moveaxis(Ut.to_numpy(), k, -1)
+    Dm = Dt.reshape(-1, Dt.shape[-1], order="F")
+    assert Dm.shape == Um.shape
+    assert np.abs(Um @ Um.conj().T - Dm @ Dm.conj().T).max() < 1e-9
+
+
+def test_su2_tdvp1_cbe_matches_dense_tdvp1_cbe(g):
+    """TDVP1_CBE on SU(2) states from small bonds (depth-2 binary tree, spin 1,
+    degeneracies 1, spins up to 1 on the bonds) and the plain dense TDVP1_CBE
+    on the embedded tensors, both spawning every direction above the
+    threshold: the bonds grow alike (dense dimensions) and the states agree to
+    1e-9; norm and energy are conserved."""
+    from graft_amd import backend as B, contractions as C, evolution as EV, su2, trees as Tr
+    topo = Tr.binary_topology(2)
+    t = _tree(topo)
+    _say("su2 vs dense TDVP1_CBE binary2 spin 1 maxdeg 1 maxspin 2")
+    rng = np.random.default_rng(59)
+    psi = su2.random_ttns(rng, topo, 2, 1, 2)
+    H = su2.heisenberg_ttno(topo, 1.0, 2)
+    H.ishermitian = True
+    psid, Hd = _dense_copies(psi, H)
+    e0 = C.expect(psi, H)
+    dims0 = [a.to_numpy().shape for a in psi.tensors]
+    for p, h in ((psi, H), (psid, Hd)):
+        ev = EV.TDVP1_CBE(cbe=EV.PredictorCBE(max_add=500, neigs=500, krylovdim=500,
+                                              spawn_threshold=1e-3),
+                          trunc=B.TruncationScheme(maxdim=500))
+        for _ in range(2):
+            EV.step_(ev, p, h, -0.05j)
+    dims = [a.to_numpy().shape for a in psi.tensors]
+    assert dims == [a.to_numpy().shape for a in psid.tensors]
+    assert dims != dims0
+    v = ref.ttns_to_dense(t, [a.to_numpy() for a in psi.tensors], list(psi.hasphys))
+    vd = ref.ttns_to_dense(t, [a.to_numpy() for a in psid.tensors], list(psid.hasphys))
+    assert np.linalg.norm(v - vd) < 1e-9, np.linalg.norm(v - vd)
+    assert abs(np.linalg.norm(v) - 1.0) < 1e-10
+    assert abs(C.expect(psi, H) - e0) < 1e-9
