@@ -74,7 +74,7 @@ def _null_enrich_split(A, maxdim):
     variational updates choose which directions acquire weight).  Returns
     (U as a site tensor, R :: new <- old) or (A, None) when there is no room."""
     if A.blocks is not None:
-        raise NotImplementedError("the physless-root bootstrap is built for dense states")
+        raise NotImplementedError("the physless-root bootstrap is built for dense and SU(2) states")
     M = B.matrix_view(A, A.rank - 1)
     rows, old = M.shape
     room = min(maxdim, rows) - old
@@ -106,11 +106,20 @@ def _bootstrap_physless_root_(psi, cache, targets):
         if psi.tensors[n].dims[-1] >= target:
             continue
         move_center_(psi, n, cache)
-        U, R = _null_enrich_split(psi.tensors[n], target)
-        if R is None:
-            continue
-        psi.tensors[n] = U
-        psi.tensors[r] = B.absorb_on_leg(psi.tensors[r], R, childslot(t, r, n))
+        A = psi.tensors[n]
+        if A.blocks is not None and hasattr(A.blocks, "dense_from_flat"):
+            from . import su2                     # SU(2) blocks: per-spin completion
+            U, R, new = su2.null_enrich_split(A, target)
+            if R is None:
+                continue
+            psi.tensors[n] = U
+            psi.tensors[r] = su2._absorb_on_leg(psi.tensors[r], childslot(t, r, n), R, new.irreps)
+        else:
+            U, R = _null_enrich_split(A, target)
+            if R is None:
+                continue
+            psi.tensors[n] = U
+            psi.tensors[r] = B.absorb_on_leg(psi.tensors[r], R, childslot(t, r, n))
         psi.center = r
         cache.invalidate_edge_(n, r)
     return psi

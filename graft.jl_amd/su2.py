@@ -714,10 +714,12 @@ def left_orthonormalize(structure, flat):
     return flat
 
 
-def random_ttns(rng, topo, tphys=1, maxdeg=None, maxspin=None, center=None, target=0):
+def random_ttns(rng, topo, tphys=1, maxdeg=None, maxspin=None, center=None, target=0,
+                root_phys=True):
     """Random SU(2) tree state of total spin target / 2: every site spin
-    tphys/2, bonds from ``bond_legs``; every tensor but the centre's (the root
-    by default) an isometry towards its parent, the centre normalized."""
+    tphys/2 (the root without a site when ``root_phys`` is false), bonds from
+    ``bond_legs``; every tensor but the centre's (the root by default) an
+    isometry towards its parent, the centre normalized."""
     from .networks import TTNS
     from .trees import nnodes
     bonds = bond_legs(topo, tphys, maxdeg, maxspin)
@@ -726,21 +728,24 @@ def random_ttns(rng, topo, tphys=1, maxdeg=None, maxspin=None, center=None, targ
     if center != root:
         raise ValueError("random_ttns: the centre must be the root")
     tensors = []
+    hasphys = [root_phys or n != root for n in range(nnodes(topo))]
     for n in range(nnodes(topo)):
         legs = site_legs(topo, n, bonds, tphys, target)
+        if not hasphys[n]:
+            del legs[len(topo.children[n])]
         st = SU2Structure.invariant(legs)
         if st.total == 0:
             raise ValueError(f"random_ttns: node {topo.ids[n]} admits no invariant block")
         flat = random_flat(rng, st)
         flat = flat / np.linalg.norm(flat) if n == center else left_orthonormalize(st, flat)
         tensors.append(to_device(st, flat, len(legs) - 1))
-    return TTNS(topo, tensors, center, [True] * nnodes(topo))
+    return TTNS(topo, tensors, center, hasphys)
 
 
 CHANNEL = SU2Leg([(0, 2), (2, 1)])     # idle, done (scalars), one open S vector (spin 1)
 
 
-def heisenberg_dense_site(K, isroot, J=1.0, tphys=1):
+def heisenberg_dense_site(K, isroot, J=1.0, tphys=1, phys=True):
     """Dense TTNO tensor of J sum_edges S_i . S_j (site spin tphys / 2) at a
     node with K children, channels in CHANNEL's embedded basis: 0 idle,
     1 done, 2..4 the spherical components q = +1, 0, -1 of an open term; legs
@@ -748,7 +753,24 @@ def heisenberg_dense_site(K, isroot, J=1.0, tphys=1):
     channel is 'done' only.  The node emits T_q (its spin's spherical
     components: T_+1 = -S+/sqrt2, T_0 = Sz, T_-1 = S-/sqrt2) towards the
     parent and closes a child's T_q with T_q^dagger:
-    S_i . S_j = sum_q T_q(i) T_q(j)^dagger."""
+    S_i . S_j = sum_q T_q(i) T_q(j)^dagger.  A root without a physical leg
+    (``phys`` false; legs: children channels..., parent channel) couples the
+    open terms of every pair of its children, S_k . S_l = sum_q (-1)^q
+    T_q(k) T_-q(l) (T_q^dagger = (-1)^q T_-q), and passes a child's 'done'."""
+    if not phys:
+        if not isroot:
+            raise ValueError("heisenberg_dense_site: only the root may lack a physical leg")
+        W = np.zeros((5,) * K + (1,), dtype=np.complex128)
+        for k in range(K):
+            ch = [0] * K
+            ch[k] = 1
+            W[tuple(ch) + (0,)] = 1.0
+            for l in range(k + 1, K):
+                for i, q in enumerate((1, 0, -1)):
+                    ch = [0] * K
+                    ch[k], ch[l] = 2 + i, 2 + (2 - i)
+                    W[tuple(ch) + (0,)] = J * (-1) ** abs(q)
+        return W
     jx, jy, jz = spin_matrices(tphys)
     sp, sm = jx + 1j * jy, jx - 1j * jy
     T = {1: -sp / math.sqrt(2.0), 0: jz.astype(np.complex128), -1: sm / math.sqrt(2.0)}
@@ -772,25 +794,27 @@ def heisenberg_dense_site(K, isroot, J=1.0, tphys=1):
     return W
 
 
-def heisenberg_ttno(topo, J=1.0, tphys=1):
+def heisenberg_ttno(topo, J=1.0, tphys=1, root_phys=True):
     """SU(2) TTNO of the spin-(tphys/2) Heisenberg model J sum_<ij> S_i . S_j on the
     tree's edges (the dense tensors of ``heisenberg_dense_site`` projected onto
     their invariant blocks; the projection is exact, which ``from_dense``
-    followed by ``to_dense`` confirms)."""
+    followed by ``to_dense`` confirms).  With ``root_phys`` false the root has
+    no site: its children's spins are coupled pairwise through it instead."""
     from .networks import TTNO
     from .trees import nnodes
     tensors = []
     for n in range(nnodes(topo)):
         K = len(topo.children[n])
         isroot = topo.parent[n] < 0
-        legs = [CHANNEL] * K + [SU2Leg([(tphys, 1)]), SU2Leg([(tphys, 1)], True),
-                                (SU2Leg([(0, 1)]) if isroot else CHANNEL).dual()]
+        phys = root_phys or not isroot
+        site = [SU2Leg([(tphys, 1)]), SU2Leg([(tphys, 1)], True)] if phys else []
+        legs = [CHANNEL] * K + site + [(SU2Leg([(0, 1)]) if isroot else CHANNEL).dual()]
         st = SU2Structure.invariant(legs)
-        Wd = heisenberg_dense_site(K, isroot, J, tphys)
+        Wd = heisenberg_dense_site(K, isroot, J, tphys, phys)
         flat = st.from_dense(Wd)
         if np.abs(st.to_dense(flat) - Wd).max() > 1e-12:
             raise RuntimeError("Heisenberg TTNO tensor is not SU(2) invariant")
-        tensors.append(to_device(st, flat, K + 1))
+        tensors.append(to_device(st, flat, K + len(site) // 2))
     return TTNO(topo, tensors, ishermitian=True)
 
 
@@ -1271,9 +1295,19 @@ def cbe_link_split(ev, psi, H, u, v, dz):
         if J in qr:
             R = qr[J][3]
             Rs[J] = np.concatenate([R, np.zeros((Uj.shape[1] - R.shape[0], R.shape[1]))]) / math.sqrt(J + 1)
+    U, new = _isometry_from_columns(ast.legs[:-1], old, cols, Al.ncod)
+    return U, Rs, new
+
+
+def _isometry_from_columns(ext, old, cols, ncod):
+    """The tensor on (ext legs, new bond) whose blocks of bond spin J are
+    sqrt(2J + 1) times the rows of cols[J] = (row keys, row offsets, U_J)
+    (orthonormal columns: a dense isometry); the new bond has U_J's column
+    count for every J and old's direction."""
+    pu = len(ext)
     new = SU2Leg([(J, cols[J][2].shape[1]) for J in sorted(cols) if cols[J][2].shape[1] > 0],
                  old.isdual)
-    Su = SU2Structure.invariant(list(ast.legs[:-1]) + [new])
+    Su = SU2Structure.invariant(list(ext) + [new])
     fu = np.zeros(Su.total, dtype=np.complex128)
     for b in range(len(Su.blocks)):
         J, rk, _, nr, _ = _cut_key(Su, b, pu)
@@ -1283,7 +1317,66 @@ def cbe_link_split(ev, psi, H, u, v, dz):
         i = rkeys.index(rk)
         o = int(Su.offsets[b])
         fu[o:o + Su.sizes[b]] = (math.sqrt(J + 1) * Uj[roff[i]:roff[i + 1]]).reshape(-1, order="F")
-    return to_device(Su, fu, Al.ncod), Rs, new
+    return to_device(Su, fu, ncod), new
+
+
+def null_enrich_split(A, maxdim):
+    """_null_enrich_split (expansion.jl:560-575) for SU(2) states: A (bond
+    last) == U (R (x) 1) with U a dense isometry of dense bond dimension up to
+    ``maxdim``: per spin J of the external legs' fusion, the QR factor of A's
+    rows completed by orthonormal directions of its complement (zero weight:
+    the state is unchanged; spins the bond lacks enter with complement
+    columns only).  Whole multiplets fill the room, smallest spins first (the
+    reference's choice among these equal-weight directions is arbitrary as
+    well).  Returns (U, {spin: R_J}, new bond leg), or (A, None, None) when
+    there is no room."""
+    st = A.blocks
+    pu = A.rank - 1
+    old = st.legs[-1]
+    ext = st.legs[:-1]
+    mult = {}
+    for l in ext:
+        mult = fuse_multiplicities(mult, dict(l.irreps)) if mult else dict(l.irreps)
+    room = min(maxdim, int(np.prod(A.dims[:-1]))) - old.dim
+    if room <= 0:
+        return A, None, None
+    # every row key of every spin the external legs couple to
+    full = SU2Structure.invariant(list(ext) + [SU2Leg([(J, 1) for J in mult], old.isdual)])
+    rows = {}
+    for b in range(len(full.blocks)):
+        J, rk, _, nr, _ = _cut_key(full, b, pu)
+        rows.setdefault(J, {}).setdefault(rk, nr)
+    _, _, pieces = _cut_pieces(st, A.data.detach().cpu().numpy(), pu)
+    cols, Rs, left = {}, {}, room
+    for J in sorted(rows):
+        rk = list(rows[J])
+        rsz = [rows[J][r] for r in rk]
+        roff = np.concatenate([[0], np.cumsum(rsz)]).astype(int)
+        s = [i for i, (tj, _) in enumerate(old.irreps) if tj == J]
+        if s:
+            d = old.irreps[s[0]][1]
+            M = np.zeros((roff[-1], d), dtype=np.complex128)
+            for i, r in enumerate(rk):
+                pc = pieces.get((J, r, ((s[0],), ())))
+                if pc is not None:
+                    M[roff[i]:roff[i + 1]] = pc
+            Qc, R = np.linalg.qr(M, mode="complete")
+            q = min(roff[-1], d)
+            Q, comp = Qc[:, :q], Qc[:, q:]
+            Rs[J] = R[:q]
+        else:
+            Q, comp = np.zeros((roff[-1], 0), dtype=np.complex128), np.eye(roff[-1], dtype=np.complex128)
+        e = min(comp.shape[1], left // (J + 1))
+        left -= e * (J + 1)
+        Uj = np.concatenate([Q, comp[:, :e]], axis=1)
+        if Uj.shape[1]:
+            cols[J] = (rk, roff, Uj)
+        if J in Rs:
+            Rs[J] = np.concatenate([Rs[J], np.zeros((e, Rs[J].shape[1]))]) / math.sqrt(J + 1)
+    if left == room:
+        return A, None, None
+    U, new = _isometry_from_columns(ext, old, cols, A.ncod)
+    return U, Rs, new
 
 
 def evolve_link_and_move_(ev, psi, H, u, v, dz, herm, exponentiate, cbe=False):

@@ -416,3 +416,57 @@ def test_su2_tdvp1_cbe_matches_dense_tdvp1_cbe(g):
     assert np.linalg.norm(v - vd) < 1e-9, np.linalg.norm(v - vd)
     assert abs(np.linalg.norm(v) - 1.0) < 1e-10
     assert abs(C.expect(psi, H) - e0) < 1e-9
+
+
+def test_su2_dmrg2_physless_binary_root_bootstrap(g):
+    """test/runtests.jl:781-804 on SU(2) blocks: depth-2 binary tree whose root
+    carries no site (spin-1/2 sites on the other six nodes, Heisenberg on the
+    tree's edges and between the root's two children through the root), from
+    bonds of spin 1/2 only (degeneracy 1: dense 2 on each root bond).  The
+    exact ground state (a singlet: the graph is bipartite with equal halves)
+    has Schmidt rank 6 across each root edge, a spin-1/2 and a spin-3/2
+    multiplet (8 is not reached: it is even under exchanging the two leaves
+    of each child).  A root-edge split can only keep spins the other root
+    bond carries, so spin 3/2 needs the per-spin completion: the bootstrap
+    alone keeps the state (1e-12), opens both root bonds to 8 and leaves
+    isometries; dmrg2_ then reaches the exact ground energy (1e-10) with root
+    bonds at the exact Schmidt rank."""
+    from graft_amd import backend as B, contractions as C, groundstate as GS, su2, trees as Tr
+    topo = Tr.binary_topology(2)
+    t = _tree(topo)
+    _say("su2 dmrg2 physless binary(2) root bootstrap")
+    flags = [n != topo.root for n in range(7)]
+    H = su2.heisenberg_ttno(topo, 1.0, 1, root_phys=False)
+    H.ishermitian = True
+    w, V = np.linalg.eigh(ref.ttno_to_dense(t, [h.to_numpy() for h in H.tensors], flags))
+    kids = topo.children[topo.root]
+    sites = [n for n in range(7) if flags[n]]
+    ranks = []
+    for k in kids:                           # Schmidt rank of the ground state across edge k
+        sub = [k] + list(topo.children[k])
+        v = np.moveaxis(V[:, 0].reshape([2] * 6, order="F"), [sites.index(x) for x in sub], [0, 1, 2])
+        sv = np.linalg.svd(v.reshape(8, 8, order="F"), compute_uv=False)
+        ranks.append(int(np.sum(sv > 1e-10 * sv[0])))
+    assert ranks == [6, 6]
+
+    def fresh():
+        return su2.random_ttns(np.random.default_rng(61), topo, 1, 1, 1, root_phys=False)
+
+    boot = fresh()
+    assert [boot.tensors[k].blocks.legs[-1].irreps for k in kids] == [((1, 1),)] * 2
+    assert GS._physless_root_two_site_targets(boot, B.TruncationScheme()) == [(k, 8) for k in kids]
+    v0 = ref.ttns_to_dense(t, [a.to_numpy() for a in boot.tensors], flags)
+    GS._bootstrap_physless_root_(boot, C.EnvCache(topo),
+                                 GS._physless_root_two_site_targets(boot, B.TruncationScheme(maxdim=8)))
+    assert [boot.tensors[k].dims[-1] for k in kids] == [8, 8]
+    boot.check_arrows()
+    for k in kids:
+        D = boot.tensors[k].to_numpy()
+        M = D.reshape(-1, D.shape[-1], order="F")
+        assert np.abs(M.conj().T @ M - np.eye(M.shape[1])).max() < 1e-12
+    v1 = ref.ttns_to_dense(t, [a.to_numpy() for a in boot.tensors], flags)
+    assert np.linalg.norm(v1 - v0) < 1e-12
+    psi = fresh()
+    _, Es = GS.dmrg2_(psi, H, trunc=B.TruncationScheme(maxdim=8), nsweeps=4)
+    assert abs(Es[-1] - w[0]) < 1e-10, (Es, w[0])
+    assert [psi.tensors[k].dims[-1] for k in kids] == ranks

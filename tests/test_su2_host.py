@@ -190,3 +190,35 @@ def test_left_orthonormalize_gives_a_dense_isometry():
     f = su2.left_orthonormalize(st, su2.random_flat(np.random.default_rng(2), st))
     D = st.to_dense(f).reshape(-1, st.dims[-1], order="F")
     assert np.abs(D.conj().T @ D - np.eye(st.dims[-1])).max() < 1e-12
+
+
+def test_physless_root_heisenberg_is_the_spin_model():
+    """With no site at the root, the TTNO's dense embedding is J sum S_i . S_j
+    over the tree's edges between sites plus the pair of the root's children,
+    built from spin matrices (site i of the dense vector is axis i, first
+    site fastest), and every tensor is exactly SU(2) invariant."""
+    from graft_amd import trees as Tr
+    topo = Tr.binary_topology(2)
+    t = ref.Tree(list(topo.ids), list(topo.parent), [list(c) for c in topo.children])
+    flags = [n != topo.root for n in range(7)]
+    Wd = [su2.heisenberg_dense_site(len(topo.children[n]), topo.parent[n] < 0, 0.8, 1, flags[n])
+          for n in range(7)]
+    for n in range(7):
+        K = len(topo.children[n])
+        site = [su2.SU2Leg([(1, 1)]), su2.SU2Leg([(1, 1)], True)] if flags[n] else []
+        legs = [su2.CHANNEL] * K + site + [(su2.SU2Leg([(0, 1)]) if n == topo.root else su2.CHANNEL).dual()]
+        st = su2.SU2Structure.invariant(legs)
+        assert np.abs(st.to_dense(st.from_dense(Wd[n])) - Wd[n]).max() < 1e-13
+    sites = [n for n in range(7) if flags[n]]
+    S = su2.spin_matrices(1)
+
+    def op(n, M):
+        out = np.ones((1, 1))
+        for s in reversed(sites):
+            out = np.kron(out, M if s == n else np.eye(2))
+        return out
+
+    edges = [(c, topo.parent[c]) for c in sites if flags[topo.parent[c]]] + [tuple(topo.children[topo.root])]
+    assert len(edges) == 5
+    H = sum(0.8 * op(a, S[k]) @ op(b, S[k]) for a, b in edges for k in range(3))
+    assert np.abs(ref.ttno_to_dense(t, Wd, flags) - H).max() < 1e-13
