@@ -90,6 +90,9 @@ hipError_t launch_zgemm(int config, int precision, const void* A, const void* B,
                         int ksplit, void* partial, bool tr, hipStream_t stream,
                         const FusedMaps* maps = nullptr, int kfast = -1,
                         const XcdSplit* xs = nullptr);
+// C[e] = sum of nslices consecutive slices of ``total`` complex elements.
+hipError_t launch_sum_slices(const void* partial, int nslices, int64_t total, void* C,
+                             hipStream_t stream);
 int tiles_for(int config, int M, int N);
 // FP32 / BF16x3 products with the same addressing (zgemm_lowp_kernels.hip).
 // Double-buffered three-product FP64 tiles (configurations 11-18), operands

@@ -248,6 +248,12 @@ struct StepExec {
   LeanSeg* d_lsegs = nullptr;
   LeanWork* d_lwork = nullptr;
   uint64_t partial_bytes = 0; // workspace for the split-K partial tiles
+  // Coupled steps with few output tiles: every output block's pair list cut
+  // into cslices chunks, chunk s storing into slice s of the partial buffer
+  // (ctot elements per slice), the slices summed into C afterwards.
+  std::vector<GemmItem> items_base;   // the per-block items before the cut
+  int cslices = 1;
+  int64_t ctot = 0;
   // Fused sector path: the GEMM items above run on per-charge dense matrices
   // gathered from A and B and scattered into C.
   bool fused = false;
@@ -1670,6 +1676,10 @@ static int compile_coupled_step(const graft_pair_step& st, const TStruct& A, con
 
 // Target number of workgroups for one launch (two per CU on 256 CUs).
 static constexpr int kTargetBlocks = 512;
+// Coupled steps: output tiles wanted before a block's pair list is cut, and
+// the most chunks per block.
+static constexpr int kCoupledTargetTiles = 4096;
+static constexpr int kCoupledMaxSlices = 16;
 
 static int default_config(const StepExec& ex) {
   if (ex.bstream_pref) return kBlockStreamConfig;
@@ -1677,6 +1687,14 @@ static int default_config(const StepExec& ex) {
 }
 
 static void assign_tiles(StepExec& ex) {
+  // coupled steps: start from the per-block items (the streaming kernels and
+  // their work lists index them by output block)
+  ex.cslices = 1;
+  if (!ex.coef.empty()) {
+    if (ex.items_base.empty()) ex.items_base = ex.items;
+    else ex.items = ex.items_base;
+    ex.xcd_config = -1;
+  }
   if (ex.config == kBlockStreamConfig) {
     // One output block per entry while that keeps the wave count moderate
     // (small steps are latency-bound: they need every wave they can get);
@@ -1719,10 +1737,62 @@ static void assign_tiles(StepExec& ex) {
     t += ((it.M + c.bm - 1) / c.bm) * it.tiles_n;
   }
   ex.total_tiles = t;
+  if (!ex.coef.empty() && t > 0 && t < kCoupledTargetTiles) {
+    // A coupled output block sums its block pairs one after another in one
+    // workgroup: with few output tiles the longest pair list sets the time.
+    // Cut every block's list into S chunks of equal k work (k-tiles + 1 per
+    // pair); chunk s of every block forms the s-th copy of the item list,
+    // storing into slice s of the partial buffer.
+    int maxseg = 0;
+    int64_t ctot = 0;
+    for (const auto& it : ex.items) {
+      maxseg = std::max(maxseg, it.seg_count);
+      ctot = std::max<int64_t>(ctot, it.c_off + (int64_t)it.M * it.N);
+    }
+    const int S = (int)std::min<int64_t>({(kCoupledTargetTiles + t - 1) / t, (int64_t)maxseg,
+                                          (int64_t)kCoupledMaxSlices});
+    if (S >= 2) {
+      std::vector<GemmItem> cut;
+      cut.reserve(ex.items.size() * S);
+      std::vector<int32_t> bounds(S + 1);
+      for (int sl = 0; sl < S; ++sl) {
+        for (const auto& it : ex.items) {
+          // chunk boundaries of this block: first pair whose preceding work
+          // reaches sl / S of the total (recomputed per slice: blocks are few)
+          int64_t W = 0;
+          for (int q = it.seg_begin; q < it.seg_begin + it.seg_count; ++q)
+            W += (ex.segs[q].K + c.bk - 1) / c.bk + 1;
+          int32_t lo = it.seg_begin + it.seg_count, hi = lo;
+          int64_t cum = 0;
+          for (int q = it.seg_begin; q < it.seg_begin + it.seg_count; ++q) {
+            const int64_t w = (ex.segs[q].K + c.bk - 1) / c.bk + 1;
+            if (lo == it.seg_begin + it.seg_count && cum * S >= W * sl) lo = q;
+            if (cum * S >= W * (sl + 1)) { hi = q; break; }
+            cum += w;
+          }
+          if (sl == S - 1) hi = it.seg_begin + it.seg_count;
+          GemmItem ci = it;
+          ci.c_off = it.c_off + (int64_t)sl * ctot;
+          ci.seg_begin = lo;
+          ci.seg_count = std::max(0, hi - lo);
+          cut.push_back(ci);
+        }
+      }
+      ex.items = std::move(cut);
+      t = 0;
+      for (auto& it : ex.items) {
+        it.tile_begin = t;
+        t += ((it.M + c.bm - 1) / c.bm) * it.tiles_n;
+      }
+      ex.total_tiles = t;
+      ex.cslices = S;
+      ex.ctot = ctot;
+    }
+  }
   // Split K when one output block has too few tiles to fill the chip and a
   // long contraction (environment closures contract D*D*d legs at once).
   ex.ksplit = 1;
-  ex.partial_bytes = 0;
+  ex.partial_bytes = ex.cslices > 1 ? (uint64_t)ex.cslices * ex.ctot * 16 : 0;
   if (ex.items.size() == 1 && ex.segs.size() == 1 && t > 0 && t < kTargetBlocks &&
       ex.coef.empty()) {   // coupled steps run whole tiles (mapped launch)
     const int64_t K = ex.segs[0].K;
@@ -2012,9 +2082,12 @@ static int run_step(graft_ctx* ctx, graft_plan_impl& p, StepExec& ex, const void
       return GRAFT_E_UNSUPPORTED;
     }
     const FusedMaps maps{nullptr, nullptr, nullptr, ex.d_coef};
-    GRAFT_CHECK_HIP(launch_zgemm(ex.config, precision, A, B, C, ex.d_items, (int)ex.items.size(),
+    void* dst = ex.cslices > 1 ? (void*)(ws + p.ws_partial) : C;
+    GRAFT_CHECK_HIP(launch_zgemm(ex.config, precision, A, B, dst, ex.d_items, (int)ex.items.size(),
                                  ex.d_segs, ex.d_koff, ex.total_tiles, ex.conjA, ex.conjB, 1,
                                  nullptr, ex.tr, ctx->stream, &maps));
+    if (ex.cslices > 1)
+      GRAFT_CHECK_HIP(launch_sum_slices(dst, ex.cslices, ex.ctot, C, ctx->stream));
     return GRAFT_OK;
   }
   if (ex.fused) {

@@ -510,6 +510,32 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// C[e] = sum over s of slice s (total elements each): the chunk partials of
+// a coupled step whose output blocks' pair lists were cut.
+__global__ void __launch_bounds__(256)
+    sum_slices_kernel(const double2* __restrict__ partial, int nslices, int64_t total,
+                      double2* __restrict__ C) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < nslices; ++s) {
+      const double2 v = partial[(size_t)s * total + e];
+      re += v.x;
+      im += v.y;
+    }
+    C[e] = double2{re, im};
+  }
+}
+
+hipError_t launch_sum_slices(const void* partial, int nslices, int64_t total, void* C,
+                             hipStream_t stream) {
+  if (total <= 0) return hipSuccess;
+  const int grid = (int)std::min<int64_t>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(sum_slices_kernel, dim3(grid), dim3(256), 0, stream,
+                     (const double2*)partial, nslices, total, (double2*)C);
+  return hipGetLastError();
+}
+
 hipError_t launch_zgemm(int config, int precision, const void* A, const void* B, void* C,
                         const GemmItem* items, int nitems, const GemmSeg* segs,
                         const uint32_t* koff, int total_tiles, int conjA, int conjB,
