@@ -2615,7 +2615,7 @@ int graft_plan_step_info(graft_plan* plan, int32_t step, int64_t* mkn3, int64_t*
     mkn3[1] = ex.maxK;
     mkn3[2] = ex.maxN;
   }
-  if (nblocks) *nblocks = (int64_t)ex.items.size();
+  if (nblocks) *nblocks = (int64_t)(ex.cslices > 1 ? ex.items_base.size() : ex.items.size());
   if (macs) *macs = ex.macs;
   if (tile_config) *tile_config = ex.config;
   return GRAFT_OK;
@@ -2628,7 +2628,7 @@ int graft_plan_step_launch(graft_plan* plan, int32_t step, int32_t* config, int3
   const StepExec& ex = plan->impl.steps[step];
   if (config) *config = ex.config;
   if (orientation) *orientation = ex.tr ? 1 : 0;
-  if (ksplit) *ksplit = ex.ksplit;
+  if (ksplit) *ksplit = ex.cslices > 1 ? ex.cslices : ex.ksplit;
   return GRAFT_OK;
 }
 

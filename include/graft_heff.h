@@ -201,7 +201,8 @@ int graft_plan_device_bytes(graft_plan* plan, uint64_t* bytes);
 int graft_plan_step_info(graft_plan* plan, int32_t step, int64_t* mkn3, int64_t* nblocks,
                          double* macs, int32_t* tile_config);
 /* Launch shape chosen for one step: tile configuration (numbered as in
- * graft_plan_set_tile_config), MFMA orientation, split-K factor. */
+ * graft_plan_set_tile_config), MFMA orientation, split-K factor (for a
+ * coupled step: the chunks every output block's pair list is cut into). */
 int graft_plan_step_launch(graft_plan* plan, int32_t step, int32_t* config, int32_t* orientation,
                            int32_t* ksplit);
 /* Stored complex elements of one step's operands and result (A, B, C): the

@@ -58,6 +58,7 @@ def test_coupled_contract_matches_host_and_dense(g):
          (2,), (0,), (3, 0, 2, 1), False, False),
     ]
     rng = np.random.default_rng(5)
+    splits = []
     for i, (legsA, legsB, cA, cB, perm, conjA, conjB) in enumerate(cases):
         _say("coupled contract", i, "conj", conjA, conjB)
         SA, SB = su2.SU2Structure.invariant(legsA), su2.SU2Structure.invariant(legsB)
@@ -79,6 +80,21 @@ def test_coupled_contract_matches_host_and_dense(g):
                              axes=(list(range(len(oA), len(oA) + nc)), list(range(nc)))).transpose(perm)
         assert _rel(C.to_numpy(), dense) < TOL
         assert p.npairs > len(C.blocks.blocks)        # blocks sum several pairs
+        splits.append(_step_split(p))
+    # a tile step with few output tiles cuts its blocks' pair lists into
+    # chunks summed afterwards: exercised here, and checked like the rest
+    assert max(splits) > 1, splits
+
+
+def _step_split(p):
+    """Split factor of a one-step plan (chunks per output block's pair list
+    for a coupled tile step)."""
+    import ctypes
+    from graft_amd import _native as nat
+    cfg, orient, ks = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    nat.check(nat.lib().graft_plan_step_launch(p.handle, 0, ctypes.byref(cfg), ctypes.byref(orient),
+                                               ctypes.byref(ks)))
+    return ks.value
 
 
 @pytest.mark.parametrize("name,tphys,maxdeg", [("star31", 1, None), ("binary2", 2, 2)])
