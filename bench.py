@@ -677,6 +677,45 @@ def su2_sweep_bench(L=32, D=256, nsweeps=3, dense=True, seed=1):
     return out
 
 
+def su2_tdvp_bench(L=32, D=128, nsteps=4, dt=0.05, dense=True, seed=3):
+    """Real-time TDVP1_CBE (order 2) of the spin-1/2 Heisenberg chain of L sites
+    from a random singlet state with small bonds (spins up to 2, degeneracy 1),
+    bond cap D: SU(2) blocks against the plain dense FP64 maps on the state's
+    dense embedding, same PredictorCBE settings; seconds per step."""
+    import torch
+    import __graft_entry__
+    __graft_entry__.load_package()
+    from graft_amd import backend as B, contractions as C, evolution as EV, networks as N, su2
+    from graft_amd import tensors as T, trees as Tr
+    topo = Tr.mps_topology(L)
+    psi = su2.random_ttns(np.random.default_rng(seed), topo, 1, 1, maxspin=4)
+    H = su2.heisenberg_ttno(topo)
+    H.ishermitian = True
+    runs = [("su2", psi, H)]
+    if dense:
+        runs.append(("dense", N.TTNS(topo, [T.DTensor.from_numpy(a.to_numpy(), a.ncod) for a in psi.tensors],
+                                     psi.center, psi.hasphys),
+                     N.TTNO(topo, [T.DTensor.from_numpy(w.to_numpy(), w.ncod) for w in H.tensors],
+                            ishermitian=True)))
+    out = {"L": L, "D": D, "dt": dt}
+    for kind, p, h in runs:
+        ev = EV.TDVP1_CBE(cbe=EV.PredictorCBE(max_add=16, neigs=16, krylovdim=16),
+                          trunc=B.TruncationScheme(maxdim=D))
+        e0 = C.expect(p, h)
+        secs = []
+        for _ in range(nsteps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            EV.step_(ev, p, h, -1j * dt)
+            torch.cuda.synchronize()
+            secs.append(time.perf_counter() - t)
+        out[kind] = {"step_s": secs, "energy_drift": abs(C.expect(p, h) - e0),
+                     "max_bond": max(p.tensors[c].dims[-1] for c in range(L) if topo.parent[c] >= 0)}
+    if dense:
+        out["speedup_last_step"] = out["dense"]["step_s"][-1] / out["su2"]["step_s"][-1]
+    return out
+
+
 def blas_threads():
     try:
         from threadpoolctl import threadpool_info
@@ -708,7 +747,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the heuristic tile configurations (default: autotune every step)")
     ap.add_argument("--mode", choices=["apply", "sweep", "hubbard", "hubbard-apply", "hubbard-tdvp",
-                                       "precision", "su2-apply", "su2-sweep"],
+                                       "precision", "su2-apply", "su2-sweep", "su2-tdvp"],
                     default="apply")
     ap.add_argument("--prec-D", type=int, default=64)
     ap.add_argument("--prec-steps", type=int, default=10)
@@ -763,6 +802,14 @@ def main():
                           "unit": "ms", "higher_is_better": False, "n_gpus": 1,
                           "config": {"workload": f"eff_h2 root edge, spin-1 Heisenberg 16-site tree, "
                                                  f"SU(2), multiplet degeneracy <= {r['maxdeg']}"},
+                          "detail": r}), flush=True)
+        return
+    if args.mode == "su2-tdvp":
+        r = su2_tdvp_bench(D=args.sweep_D, nsteps=args.prec_steps, dense=not args.no_dense)
+        print(json.dumps({"metric": "TDVP1_CBE seconds per time step, SU(2) blocks vs plain dense FP64",
+                          "value": r["su2"]["step_s"][-1], "unit": "s", "higher_is_better": False,
+                          "n_gpus": 1, "config": {"workload": f"TDVP1_CBE, spin-1/2 Heisenberg chain, "
+                                                              f"L={r['L']}, maxdim {r['D']}"},
                           "detail": r}), flush=True)
         return
     if args.mode == "su2-sweep":
